@@ -1,0 +1,150 @@
+/*
+ * tk_hip.h — C ABI of the MI355X block-sparse DMRG hot path.
+ *
+ * This is the drop-in boundary for tensorkraft's linear-algebra backend traits
+ * (tk-linalg/src/traits.rs:25-205) and the H_eff producer of tk-dmrg
+ * (tk-dmrg/src/environments/mod.rs:491-602).  A Rust `DeviceHip` would bind
+ * these symbols from build.rs; the C++ host classes in
+ * tensorkraft-design_amd/csrc/backend.hpp implement the mirrored traits on top
+ * of them.
+ *
+ * Conventions
+ *   - Every function returns a tk_status (0 = TK_OK); the message of the last
+ *     failure on a context is returned by tk_last_error(ctx) (or
+ *     tk_last_error(NULL) for calls that take no context).
+ *   - Data pointers named *_dev are device pointers owned by the caller; all
+ *     device work is enqueued on the context's stream and is asynchronous
+ *     unless stated otherwise.  tk_ctx_sync waits for it.
+ *   - One context per host thread.  Plans are owned by their creator and must
+ *     be destroyed before their context.
+ *   - Block-sparse data use FlatBlockStorage order
+ *     (tk-symmetry/src/flat_storage.rs:95-140): blocks sorted by ascending
+ *     packed sector key (tk-symmetry/src/sector_key.rs:31-55), each block
+ *     row-major, blocks back to back.
+ *   - Quantum numbers are passed as pairs (a, b): U1 uses (q, 0), U1U1 uses
+ *     (N_up, N_dn).
+ */
+#ifndef TK_HIP_H
+#define TK_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t tk_status;
+enum {
+  TK_OK = 0,
+  TK_ERR_INVALID_ARG = 1,        /* LinAlgError::DimensionMismatch and argument errors (tk-linalg/src/error.rs:16-21) */
+  TK_ERR_SVD_CONVERGENCE = 2,    /* LinAlgError::SvdConvergence / SvdConvergenceError (tk-linalg/src/error.rs:6-9, results.rs:47) */
+  TK_ERR_HIP = 3,                /* device runtime failure (the CudaError{op,detail} slot, tk-linalg/src/error.rs:29-32) */
+  TK_ERR_NOT_CONVERGED = 4,      /* eigensolver did not reach its tolerance */
+  TK_ERR_CANCELLED = 5,          /* DmrgError::Cancelled (tk-dmrg/src/error.rs:53) */
+  TK_ERR_SYMMETRY = 6,           /* SymmetryError: flux rule, sector not found, QN range */
+  TK_ERR_CAPACITY = 7,           /* caller-provided output array too small */
+  TK_ERR_INTERNAL = 8
+};
+enum { TK_PREC_FP64 = 0, TK_PREC_FP32 = 1, TK_PREC_BF16X3 = 2 };
+enum { TK_SYM_U1 = 1, TK_SYM_U1U1 = 2 };
+enum { TK_DIR_IN = 0, TK_DIR_OUT = 1 }; /* LegDirection (tk-symmetry/src/quantum_number.rs) */
+
+typedef struct tk_ctx tk_ctx;
+typedef struct tk_heff_plan tk_heff_plan;
+
+/* QIndex (tk-symmetry/src/sector_key.rs:95-158): sectors need not be sorted;
+ * they are sorted by quantum number on use, as QIndex::new does. */
+typedef struct {
+  int32_t n_sectors;
+  const int32_t* qn;   /* n_sectors x 2 */
+  const int64_t* dims; /* n_sectors, all > 0 */
+} tk_qindex;
+
+/* MPO site tensor W[s_in][s_out][w_l][w_r], dense over the full index ranges
+ * (tk-dmrg/src/mpo/mod.rs:36-93, leg order of mpo/mod.rs:3).  Nonzeros must
+ * obey q(s_in) + q(w_l) = q(s_out) + q(w_r). */
+typedef struct {
+  tk_qindex phys;
+  tk_qindex wl;
+  tk_qindex wr;
+  const double* w; /* host, phys^2 * wl * wr values */
+} tk_mpo_site;
+
+/* Block-sparse environment (bra, w, ket) with every flux-allowed block stored:
+ * q(bra) = q(ket) - q(w).  Replaces the dense Environment of
+ * tk-dmrg/src/environments/mod.rs:17-37 (layout env:90-95 per block). */
+typedef struct {
+  tk_qindex bond; /* bra and ket basis */
+  tk_qindex w;    /* MPO bond basis */
+  const double* data_dev;
+} tk_env;
+
+/* ---- version, context, plumbing ---------------------------------------- */
+const char* tk_version(void);
+/* One HIP stream + device memory pool.  Replaces the backend value
+ * (DeviceFaer, tk-linalg/src/device/faer.rs) that every trait call carries. */
+tk_status tk_ctx_create(int32_t device, tk_ctx** out);
+tk_status tk_ctx_destroy(tk_ctx* ctx);
+tk_status tk_ctx_sync(tk_ctx* ctx);
+const char* tk_last_error(tk_ctx* ctx);
+tk_status tk_device_alloc(tk_ctx* ctx, int64_t bytes, void** out_dev);
+tk_status tk_device_free(tk_ctx* ctx, void* p_dev);
+/* Synchronous copies (return after the data has arrived). */
+tk_status tk_memcpy_h2d(tk_ctx* ctx, void* dst_dev, const void* src_host, int64_t bytes);
+tk_status tk_memcpy_d2h(tk_ctx* ctx, void* dst_host, const void* src_dev, int64_t bytes);
+
+/* ---- sector tables (host only) ------------------------------------------ */
+/* QIndex::new / iter_sectors / total_dim (tk-symmetry/src/sector_key.rs:108-158):
+ * sectors sorted by quantum number, prefix offsets.  Outputs have n_sectors
+ * entries (qn_out: 2 per sector). */
+tk_status tk_qindex_layout(int32_t sym, const tk_qindex* q, int32_t* qn_out, int64_t* dims_out,
+                           int64_t* offsets_out, int64_t* total_out);
+/* PackedSectorKey::pack (tk-symmetry/src/sector_key.rs:31-46) over `rank`
+ * quantum numbers: leg i at bits [i*W, (i+1)*W), W = 8 (U1) or 16 (U1U1). */
+tk_status tk_pack_key(int32_t sym, int32_t rank, const int32_t* qn, uint64_t* key_out);
+/* enumerate_valid_sectors (tk-symmetry/src/flux.rs:43-68) + BlockSparseTensor::zeros
+ * key sort (tk-symmetry/src/block_sparse.rs:40-84).  Writes up to `capacity`
+ * blocks: keys, element offsets, per-block sector positions (rank entries,
+ * indices into each leg's sorted QIndex) and the total element count. */
+tk_status tk_block_table(int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs,
+                         const int32_t* flux, int64_t capacity, int64_t* n_blocks, uint64_t* keys,
+                         int64_t* offsets, int32_t* sectors, int64_t* total);
+/* fuse_legs offset map (tk-symmetry/src/block_sparse.rs:444-611): the fused
+ * QIndex (sorted) and, per sub-sector combination in leg-by-leg order with the
+ * later leg fastest, the fused sector position and the offset inside it. */
+tk_status tk_fuse_map(int32_t sym, int32_t n_legs, const tk_qindex* legs, const int32_t* dirs,
+                      int64_t capacity, int32_t* n_fused, int32_t* fused_qn, int64_t* fused_dims,
+                      int64_t* n_combos, int32_t* combo_fused_sector, int64_t* combo_offset);
+
+/* ---- two-site effective Hamiltonian ------------------------------------- */
+/* build_heff_two_site (tk-dmrg/src/environments/mod.rs:491-602;
+ * techspec/6_tech-spec_tk-dmrg.md:561-572).  psi layout (s_i, s_j, a_L, a_R)
+ * with dirs [Out, Out, Out, In], flux 0.  The plan keeps pointers to the
+ * environment data; they must stay valid while the plan is used. */
+tk_status tk_heff_plan_create(tk_ctx* ctx, int32_t prec, int32_t sym, const tk_env* left, const tk_env* right,
+                              const tk_mpo_site* wi, const tk_mpo_site* wj, tk_heff_plan** out);
+tk_status tk_heff_plan_destroy(tk_heff_plan* plan);
+/* The psi block table the plan uses (keys ascending, offsets). */
+tk_status tk_heff_psi_table(tk_heff_plan* plan, int64_t capacity, int64_t* n_blocks, uint64_t* keys,
+                            int64_t* offsets, int64_t* total);
+/* y = H_eff x (the matvec closure, environments/mod.rs:592-599). */
+tk_status tk_heff_apply(tk_heff_plan* plan, const double* x_dev, double* y_dev);
+/* y = H_eff x with FP64 contractions whatever the plan precision (refinement). */
+tk_status tk_heff_apply_fp64(tk_heff_plan* plan, const double* x_dev, double* y_dev);
+/* d = diag(H_eff), the Davidson preconditioner (eigensolver/mod.rs:425-435). */
+tk_status tk_heff_diag(tk_heff_plan* plan, double* d_dev);
+/* Algorithmic flops and HBM bytes of one apply; counts[0..5] = psi blocks,
+ * dim, T1 elements, stage-1 items, stage-3 items, stage-3 terms. */
+tk_status tk_heff_stats(tk_heff_plan* plan, double* flops, double* bytes, int64_t* counts);
+/* Per-launch algorithmic work: out[0..3] = stage-1 flops, stage-1 bytes,
+ * stage-3 flops, stage-3 bytes. */
+tk_status tk_heff_stage_stats(tk_heff_plan* plan, double* out);
+/* Times `reps` applies with HIP events on the plan's stream; returns the mean
+ * per apply in microseconds and the mean per launch of each stage. */
+tk_status tk_heff_time(tk_heff_plan* plan, const double* x_dev, double* y_dev, int32_t reps, double* us_apply,
+                       double* us_stage1, double* us_stage3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TK_HIP_H */

@@ -1,0 +1,245 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/build/libtk_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  See
+oracle/tk_oracle.hpp for the reference file each function restates and the
+checks that pin it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libtk_oracle.so")
+
+
+class or_qindex(C.Structure):
+    _fields_ = [("n_sectors", C.c_int32), ("qn", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64))]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+class Q:
+    """Keeps a qindex's arrays alive; sectors as ((a, b), dim) or (q, dim)."""
+
+    def __init__(self, sectors):
+        self.sectors = [((q, 0) if np.isscalar(q) else tuple(q), int(d)) for q, d in sectors]
+        self.qn = np.array([c for q, _ in self.sectors for c in q], dtype=np.int32)
+        self.dims = np.array([d for _, d in self.sectors], dtype=np.int64)
+        self.c = or_qindex(len(self.sectors), self.qn.ctypes.data_as(C.POINTER(C.c_int32)),
+                           self.dims.ctypes.data_as(C.POINTER(C.c_int64)))
+
+    def sorted(self):
+        return sorted(self.sectors)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def pack_key(sym, qns):
+    a = np.array([c for q in qns for c in ((q, 0) if np.isscalar(q) else q)], dtype=np.int32)
+    k = C.c_uint64(0)
+    st = lib().or_pack_key(sym, len(qns), _p(a, C.c_int32), C.byref(k))
+    if st:
+        raise ValueError("pack failed")
+    return k.value
+
+
+def unpack_key(sym, key, rank):
+    """PackedSectorKey::unpack (sector_key.rs:48-55), pure Python."""
+    w = 8 if sym == 1 else 16
+    out = []
+    for i in range(rank):
+        bits = (int(key) >> (i * w)) & ((1 << w) - 1)
+        a = bits & 0xFF
+        a = a - 256 if a >= 128 else a
+        b = (bits >> 8) & 0xFF if sym == 2 else 0
+        b = b - 256 if b >= 128 else b
+        out.append((a, b))
+    return out
+
+
+def block_table(sym, legs, dirs, flux=(0, 0)):
+    qs = [Q(l) for l in legs]
+    arr = (or_qindex * len(qs))(*[q.c for q in qs])
+    d = np.array(dirs, dtype=np.int32)
+    f = np.array(flux, dtype=np.int32)
+    n = C.c_int64(0)
+    tot = C.c_int64(0)
+    lib().or_block_table(sym, len(qs), arr, _p(d, C.c_int32), _p(f, C.c_int32), 0, C.byref(n), None, None,
+                         C.byref(tot))
+    keys = np.zeros(n.value, dtype=np.uint64)
+    offs = np.zeros(n.value, dtype=np.int64)
+    st = lib().or_block_table(sym, len(qs), arr, _p(d, C.c_int32), _p(f, C.c_int32), n.value, C.byref(n),
+                              _p(keys, C.c_uint64), _p(offs, C.c_int64), C.byref(tot))
+    if st:
+        raise RuntimeError(f"or_block_table failed {st}")
+    return keys, offs, tot.value
+
+
+def fuse_map(sym, legs, dirs):
+    qs = [Q(l) for l in legs]
+    arr = (or_qindex * len(qs))(*[q.c for q in qs])
+    d = np.array(dirs, dtype=np.int32)
+    cap = int(np.prod([len(l) for l in legs]))
+    nf = C.c_int32(0)
+    nc = C.c_int64(0)
+    fq = np.zeros(2 * cap, dtype=np.int32)
+    fd = np.zeros(cap, dtype=np.int64)
+    cs = np.zeros(cap, dtype=np.int32)
+    co = np.zeros(cap, dtype=np.int64)
+    st = lib().or_fuse_map(sym, len(qs), arr, _p(d, C.c_int32), cap, C.byref(nf), _p(fq, C.c_int32),
+                           _p(fd, C.c_int64), C.byref(nc), _p(cs, C.c_int32), _p(co, C.c_int64))
+    if st:
+        raise RuntimeError("or_fuse_map failed")
+    fused = [((int(fq[2 * i]), int(fq[2 * i + 1])), int(fd[i])) for i in range(nf.value)]
+    return fused, cs[:nc.value].copy(), co[:nc.value].copy()
+
+
+def block_gemm(sym, a_rows, a_cols, a_flux, a_data, b_rows, b_cols, b_flux, b_data):
+    qa_r, qa_c, qb_r, qb_c = Q(a_rows), Q(a_cols), Q(b_rows), Q(b_cols)
+    fa = np.array(a_flux, dtype=np.int32)
+    fb = np.array(b_flux, dtype=np.int32)
+    ad = np.ascontiguousarray(a_data, dtype=np.float64)
+    bd = np.ascontiguousarray(b_data, dtype=np.float64)
+    cap = 4096
+    keys = np.zeros(cap, dtype=np.uint64)
+    offs = np.zeros(cap, dtype=np.int64)
+    cdat = np.zeros(1 << 20)
+    n = C.c_int64(0)
+    tot = C.c_int64(0)
+    st = lib().or_block_gemm(sym, C.byref(qa_r.c), C.byref(qa_c.c), _p(fa, C.c_int32), _p(ad, C.c_double),
+                             C.byref(qb_r.c), C.byref(qb_c.c), _p(fb, C.c_int32), _p(bd, C.c_double), cap,
+                             C.byref(n), _p(keys, C.c_uint64), _p(offs, C.c_int64), _p(cdat, C.c_double),
+                             C.byref(tot))
+    if st:
+        raise RuntimeError(f"or_block_gemm failed {st}")
+    return keys[:n.value].copy(), offs[:n.value].copy(), cdat[:tot.value].copy()
+
+
+def _heff_args(p):
+    W = p["W"]
+    keep = [Q(W.phys), Q(W.phys), Q(p["bond_l"]), Q(p["bond_r"]), Q(W.wl), Q(W.wr), Q(W.wr)]
+    return keep
+
+
+def heff_apply(p, x, nthreads=0, Wi=None, Wj=None):
+    Wi = Wi if Wi is not None else p["W"]
+    Wj = Wj if Wj is not None else p["W"]
+    keep = [Q(Wi.phys), Q(Wj.phys), Q(p["bond_l"]), Q(p["bond_r"]), Q(Wi.wl), Q(Wi.wr), Q(Wj.wr)]
+    wi = np.ascontiguousarray(Wi.w, dtype=np.float64)
+    wj = np.ascontiguousarray(Wj.w, dtype=np.float64)
+    L = np.ascontiguousarray(p["L"])
+    R = np.ascontiguousarray(p["R"])
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    st = lib().or_heff_apply(p["sym"], *[C.byref(k.c) for k in keep], _p(wi, C.c_double), _p(wj, C.c_double),
+                             _p(L, C.c_double), _p(R, C.c_double), _p(x, C.c_double), _p(y, C.c_double),
+                             int(nthreads))
+    if st:
+        raise RuntimeError(f"or_heff_apply failed {st}")
+    return y
+
+
+def heff_diag(p, Wi=None, Wj=None):
+    Wi = Wi if Wi is not None else p["W"]
+    Wj = Wj if Wj is not None else p["W"]
+    keep = [Q(Wi.phys), Q(Wj.phys), Q(p["bond_l"]), Q(p["bond_r"]), Q(Wi.wl), Q(Wi.wr), Q(Wj.wr)]
+    wi = np.ascontiguousarray(Wi.w, dtype=np.float64)
+    wj = np.ascontiguousarray(Wj.w, dtype=np.float64)
+    L = np.ascontiguousarray(p["L"])
+    R = np.ascontiguousarray(p["R"])
+    n = len(p["x"])
+    d = np.zeros(n)
+    st = lib().or_heff_diag(p["sym"], *[C.byref(k.c) for k in keep], _p(wi, C.c_double), _p(wj, C.c_double),
+                            _p(L, C.c_double), _p(R, C.c_double), _p(d, C.c_double))
+    if st:
+        raise RuntimeError(f"or_heff_diag failed {st}")
+    return d
+
+
+def heff_dense_reference(p, x_dense, Wi=None, Wj=None):
+    Wi = Wi if Wi is not None else p["W"]
+    Wj = Wj if Wj is not None else p["W"]
+    keep = [Q(Wi.phys), Q(Wj.phys), Q(p["bond_l"]), Q(p["bond_r"]), Q(Wi.wl), Q(Wi.wr), Q(Wj.wr)]
+    wi = np.ascontiguousarray(Wi.w, dtype=np.float64)
+    wj = np.ascontiguousarray(Wj.w, dtype=np.float64)
+    L = np.ascontiguousarray(p["L"])
+    R = np.ascontiguousarray(p["R"])
+    xd = np.ascontiguousarray(x_dense, dtype=np.float64)
+    yd = np.zeros_like(xd)
+    st = lib().or_heff_dense_reference(p["sym"], *[C.byref(k.c) for k in keep], _p(wi, C.c_double),
+                                       _p(wj, C.c_double), _p(L, C.c_double), _p(R, C.c_double),
+                                       _p(xd, C.c_double), _p(yd, C.c_double))
+    if st:
+        raise RuntimeError(f"or_heff_dense_reference failed {st}")
+    return yd
+
+
+def psi_to_dense(sym, phys_i, phys_j, bond_l, bond_r, flat):
+    """Scatter a FlatBlockStorage two-site vector into the dense full-index
+    (s_i, s_j, a_L, a_R) array of the reference layout (environments/mod.rs:489-490)."""
+    legs = [sorted(Q(l).sectors) for l in (phys_i, phys_j, bond_l, bond_r)]
+    offs = []
+    for l in legs:
+        o, acc = {}, 0
+        for q, d in l:
+            o[q] = (acc, d)
+            acc += d
+        offs.append((o, acc))
+    keys, boffs, tot = block_table(sym, [phys_i, phys_j, bond_l, bond_r], [1, 1, 1, 0])
+    dense = np.zeros(tuple(t for _, t in offs))
+    for k, off in zip(keys, boffs):
+        qs = unpack_key(sym, k, 4)
+        sl = [offs[i][0][qs[i]] for i in range(4)]
+        shape = tuple(d for _, d in sl)
+        n = int(np.prod(shape))
+        dense[tuple(slice(o, o + d) for o, d in sl)] = flat[off:off + n].reshape(shape)
+    return dense
+
+
+def dense_to_psi(sym, phys_i, phys_j, bond_l, bond_r, dense):
+    legs = [sorted(Q(l).sectors) for l in (phys_i, phys_j, bond_l, bond_r)]
+    offs = []
+    for l in legs:
+        o, acc = {}, 0
+        for q, d in l:
+            o[q] = (acc, d)
+            acc += d
+        offs.append(o)
+    keys, boffs, tot = block_table(sym, [phys_i, phys_j, bond_l, bond_r], [1, 1, 1, 0])
+    flat = np.zeros(tot)
+    for k, off in zip(keys, boffs):
+        qs = unpack_key(sym, k, 4)
+        sl = [offs[i][qs[i]] for i in range(4)]
+        blk = dense[tuple(slice(o, o + d) for o, d in sl)]
+        flat[off:off + blk.size] = blk.ravel()
+    return flat
+
+
+def qindex_layout(sectors):
+    """QIndex::new + iter_sectors offsets (sector_key.rs:108-158)."""
+    q = Q(sectors)
+    n = len(q.sectors)
+    qn = np.zeros(2 * n, dtype=np.int32)
+    dims = np.zeros(n, dtype=np.int64)
+    offs = np.zeros(n, dtype=np.int64)
+    tot = C.c_int64(0)
+    if lib().or_qindex_layout(C.byref(q.c), _p(qn, C.c_int32), _p(dims, C.c_int64), _p(offs, C.c_int64),
+                              C.byref(tot)):
+        raise RuntimeError("or_qindex_layout failed")
+    return [((int(qn[2 * i]), int(qn[2 * i + 1])), int(dims[i])) for i in range(n)], offs, tot.value

@@ -1,0 +1,389 @@
+// extern "C" entry points declared in include/tk_hip.h.
+#include "../../include/tk_hip.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "device.hpp"
+#include "heff.hpp"
+#include "sym.hpp"
+
+struct tk_ctx {
+  std::unique_ptr<tk::Context> c;
+};
+struct tk_heff_plan {
+  tk_ctx* ctx;
+  std::unique_ptr<tk::HeffPlan> p;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+void set_error(tk_ctx* ctx, const std::string& msg) {
+  g_error = msg;
+  if (ctx && ctx->c) ctx->c->last_error = msg;
+}
+
+template <typename F>
+tk_status guard(tk_ctx* ctx, F&& f) {
+  try {
+    f();
+    return TK_OK;
+  } catch (const tk::SymError& e) {
+    set_error(ctx, e.what());
+    return TK_ERR_SYMMETRY;
+  } catch (const tk::ArgError& e) {
+    set_error(ctx, e.what());
+    return TK_ERR_INVALID_ARG;
+  } catch (const tk::HipError& e) {
+    set_error(ctx, e.what());
+    return TK_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    set_error(ctx, "host allocation failed");
+    return TK_ERR_INTERNAL;
+  } catch (const std::exception& e) {
+    set_error(ctx, e.what());
+    return TK_ERR_INTERNAL;
+  }
+}
+
+tk::Sym to_sym(int32_t s) {
+  if (s == TK_SYM_U1) return tk::Sym::U1;
+  if (s == TK_SYM_U1U1) return tk::Sym::U1U1;
+  throw tk::ArgError("unknown symmetry id");
+}
+tk::Prec to_prec(int32_t p) {
+  if (p == TK_PREC_FP64) return tk::Prec::FP64;
+  if (p == TK_PREC_FP32) return tk::Prec::FP32;
+  if (p == TK_PREC_BF16X3) return tk::Prec::BF16X3;
+  throw tk::ArgError("unknown precision id");
+}
+tk::Dir to_dir(int32_t d) {
+  if (d == TK_DIR_IN) return tk::Dir::In;
+  if (d == TK_DIR_OUT) return tk::Dir::Out;
+  throw tk::ArgError("unknown leg direction");
+}
+
+tk::QIndex to_qindex(tk::Sym sym, const tk_qindex& q) {
+  if (q.n_sectors < 0 || (q.n_sectors > 0 && (!q.qn || !q.dims))) throw tk::ArgError("bad tk_qindex");
+  std::vector<std::pair<tk::QN, int64_t>> s;
+  for (int i = 0; i < q.n_sectors; ++i) {
+    tk::QN x{q.qn[2 * i], q.qn[2 * i + 1]};
+    (void)tk::pack_qn(sym, x);  // range / component check
+    s.push_back({x, q.dims[i]});
+  }
+  return tk::QIndex(std::move(s));
+}
+
+tk::MpoSite to_mpo(tk::Sym sym, const tk_mpo_site* w) {
+  if (!w || !w->w) throw tk::ArgError("null MPO site");
+  tk::MpoSite m;
+  m.phys = to_qindex(sym, w->phys);
+  m.wl = to_qindex(sym, w->wl);
+  m.wr = to_qindex(sym, w->wr);
+  // Callers give QIndex sectors in any order; dense W indices follow the
+  // caller's order, so re-map them onto the sorted QIndex order.
+  auto perm = [&](const tk_qindex& q, const tk::QIndex& sorted) {
+    std::vector<int64_t> p;  // caller full index -> sorted full index
+    for (int i = 0; i < q.n_sectors; ++i) {
+      int s = sorted.find(tk::QN{q.qn[2 * i], q.qn[2 * i + 1]});
+      for (int64_t x = 0; x < q.dims[i]; ++x) p.push_back(sorted.off[s] + x);
+    }
+    return p;
+  };
+  auto pp = perm(w->phys, m.phys), pl = perm(w->wl, m.wl), pr = perm(w->wr, m.wr);
+  const int64_t d = m.phys.total, a = m.wl.total, b = m.wr.total;
+  m.w.assign(size_t(d * d * a * b), 0.0);
+  for (int64_t si = 0; si < d; ++si)
+    for (int64_t so = 0; so < d; ++so)
+      for (int64_t x = 0; x < a; ++x)
+        for (int64_t y = 0; y < b; ++y)
+          m.w[size_t(((pp[si] * d + pp[so]) * a + pl[x]) * b + pr[y])] = w->w[((si * d + so) * a + x) * b + y];
+  return m;
+}
+
+tk::BlockTable env_table(tk::Sym sym, const tk_env* e) {
+  if (!e) throw tk::ArgError("null environment");
+  tk::QIndex bond = to_qindex(sym, e->bond);
+  tk::QIndex w = to_qindex(sym, e->w);
+  return tk::make_block_table(sym, {bond, w, bond}, {tk::Dir::In, tk::Dir::In, tk::Dir::Out}, tk::QN{});
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tk_version(void) { return "tk-hip 0.1.0 (gfx950)"; }
+
+tk_status tk_ctx_create(int32_t device, tk_ctx** out) {
+  return guard(nullptr, [&] {
+    if (!out) throw tk::ArgError("null output");
+    auto c = std::make_unique<tk_ctx>();
+    c->c = std::make_unique<tk::Context>(device);
+    *out = c.release();
+  });
+}
+
+tk_status tk_ctx_destroy(tk_ctx* ctx) {
+  return guard(nullptr, [&] { delete ctx; });
+}
+
+tk_status tk_ctx_sync(tk_ctx* ctx) {
+  return guard(ctx, [&] {
+    if (!ctx) throw tk::ArgError("null context");
+    ctx->c->sync();
+  });
+}
+
+const char* tk_last_error(tk_ctx* ctx) {
+  if (ctx && ctx->c) return ctx->c->last_error.c_str();
+  return g_error.c_str();
+}
+
+tk_status tk_device_alloc(tk_ctx* ctx, int64_t bytes, void** out) {
+  return guard(ctx, [&] {
+    if (!ctx || !out || bytes < 0) throw tk::ArgError("bad alloc arguments");
+    *out = ctx->c->pool.alloc(size_t(bytes));
+  });
+}
+
+tk_status tk_device_free(tk_ctx* ctx, void* p) {
+  return guard(ctx, [&] {
+    if (!ctx) throw tk::ArgError("null context");
+    ctx->c->sync();
+    ctx->c->pool.free(p);
+  });
+}
+
+tk_status tk_memcpy_h2d(tk_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  return guard(ctx, [&] {
+    if (!ctx || bytes < 0) throw tk::ArgError("bad copy arguments");
+    if (bytes) TK_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyHostToDevice, ctx->c->stream));
+    ctx->c->sync();
+  });
+}
+
+tk_status tk_memcpy_d2h(tk_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  return guard(ctx, [&] {
+    if (!ctx || bytes < 0) throw tk::ArgError("bad copy arguments");
+    if (bytes) TK_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToHost, ctx->c->stream));
+    ctx->c->sync();
+  });
+}
+
+tk_status tk_pack_key(int32_t sym, int32_t rank, const int32_t* qn, uint64_t* key_out) {
+  return guard(nullptr, [&] {
+    if (rank < 0 || rank > 8 || (rank && !qn) || !key_out) throw tk::ArgError("bad pack arguments");
+    tk::QN qs[8];
+    for (int i = 0; i < rank; ++i) qs[i] = tk::QN{qn[2 * i], qn[2 * i + 1]};
+    *key_out = tk::pack_key(to_sym(sym), qs, rank);
+  });
+}
+
+tk_status tk_qindex_layout(int32_t sym, const tk_qindex* q, int32_t* qn_out, int64_t* dims_out,
+                           int64_t* offsets_out, int64_t* total_out) {
+  return guard(nullptr, [&] {
+    if (!q) throw tk::ArgError("null qindex");
+    tk::QIndex x = to_qindex(to_sym(sym), *q);
+    for (int i = 0; i < x.n(); ++i) {
+      if (qn_out) {
+        qn_out[2 * i] = x.q[i].a;
+        qn_out[2 * i + 1] = x.q[i].b;
+      }
+      if (dims_out) dims_out[i] = x.dim[i];
+      if (offsets_out) offsets_out[i] = x.off[i];
+    }
+    if (total_out) *total_out = x.total;
+  });
+}
+
+tk_status tk_block_table(int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs, const int32_t* flux,
+                         int64_t capacity, int64_t* n_blocks, uint64_t* keys, int64_t* offsets, int32_t* sectors,
+                         int64_t* total) {
+  return guard(nullptr, [&] {
+    if (rank <= 0 || rank > 8 || !legs || !dirs || !flux || !n_blocks) throw tk::ArgError("bad block-table arguments");
+    tk::Sym s = to_sym(sym);
+    std::vector<tk::QIndex> ql;
+    std::vector<tk::Dir> dl;
+    for (int i = 0; i < rank; ++i) {
+      ql.push_back(to_qindex(s, legs[i]));
+      dl.push_back(to_dir(dirs[i]));
+    }
+    tk::QN fq{flux[0], flux[1]};
+    (void)tk::pack_qn(s, fq);
+    tk::BlockTable t = tk::make_block_table(s, ql, dl, fq);
+    *n_blocks = t.n_blocks();
+    if (total) *total = t.total;
+    if (t.n_blocks() > capacity) throw tk::ArgError("capacity");
+    for (int64_t b = 0; b < t.n_blocks(); ++b) {
+      if (keys) keys[b] = t.keys[b];
+      if (offsets) offsets[b] = t.offset[b];
+      if (sectors)
+        for (int l = 0; l < rank; ++l) sectors[b * rank + l] = t.s(b, l);
+    }
+  });
+}
+
+tk_status tk_fuse_map(int32_t sym, int32_t n_legs, const tk_qindex* legs, const int32_t* dirs, int64_t capacity,
+                      int32_t* n_fused, int32_t* fused_qn, int64_t* fused_dims, int64_t* n_combos,
+                      int32_t* combo_fused_sector, int64_t* combo_offset) {
+  return guard(nullptr, [&] {
+    if (n_legs <= 0 || !legs || !dirs || !n_fused || !n_combos) throw tk::ArgError("bad fuse-map arguments");
+    tk::Sym s = to_sym(sym);
+    std::vector<tk::QIndex> ql;
+    std::vector<tk::Dir> dl;
+    for (int i = 0; i < n_legs; ++i) {
+      ql.push_back(to_qindex(s, legs[i]));
+      dl.push_back(to_dir(dirs[i]));
+    }
+    tk::FuseMap m = tk::make_fuse_map(ql, dl);
+    *n_fused = m.fused.n();
+    *n_combos = int64_t(m.combo_offset.size());
+    if (m.fused.n() > capacity || int64_t(m.combo_offset.size()) > capacity) throw tk::ArgError("capacity");
+    for (int i = 0; i < m.fused.n(); ++i) {
+      if (fused_qn) {
+        fused_qn[2 * i] = m.fused.q[i].a;
+        fused_qn[2 * i + 1] = m.fused.q[i].b;
+      }
+      if (fused_dims) fused_dims[i] = m.fused.dim[i];
+    }
+    for (size_t c = 0; c < m.combo_offset.size(); ++c) {
+      if (combo_fused_sector) combo_fused_sector[c] = m.combo_fused_sector[c];
+      if (combo_offset) combo_offset[c] = m.combo_offset[c];
+    }
+  });
+}
+
+tk_status tk_heff_plan_create(tk_ctx* ctx, int32_t prec, int32_t sym, const tk_env* left, const tk_env* right,
+                              const tk_mpo_site* wi, const tk_mpo_site* wj, tk_heff_plan** out) {
+  return guard(ctx, [&] {
+    if (!ctx || !out) throw tk::ArgError("null argument");
+    tk::Sym s = to_sym(sym);
+    tk::BlockTable el = env_table(s, left), er = env_table(s, right);
+    tk::MpoSite mi = to_mpo(s, wi), mj = to_mpo(s, wj);
+    auto h = std::make_unique<tk_heff_plan>();
+    h->ctx = ctx;
+    h->p = std::make_unique<tk::HeffPlan>(ctx->c.get(), to_prec(prec), el, left->data_dev, er, right->data_dev, mi,
+                                          mj);
+    *out = h.release();
+  });
+}
+
+tk_status tk_heff_plan_destroy(tk_heff_plan* plan) {
+  return guard(nullptr, [&] {
+    if (plan) plan->ctx->c->sync();
+    delete plan;
+  });
+}
+
+tk_status tk_heff_psi_table(tk_heff_plan* plan, int64_t capacity, int64_t* n_blocks, uint64_t* keys,
+                            int64_t* offsets, int64_t* total) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !n_blocks) throw tk::ArgError("null argument");
+    const tk::BlockTable& t = plan->p->psi();
+    *n_blocks = t.n_blocks();
+    if (total) *total = t.total;
+    if (t.n_blocks() > capacity) throw tk::ArgError("capacity");
+    for (int64_t b = 0; b < t.n_blocks(); ++b) {
+      if (keys) keys[b] = t.keys[b];
+      if (offsets) offsets[b] = t.offset[b];
+    }
+  });
+}
+
+tk_status tk_heff_apply(tk_heff_plan* plan, const double* x, double* y) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !x || !y) throw tk::ArgError("null argument");
+    plan->p->apply(x, y);
+  });
+}
+
+tk_status tk_heff_apply_fp64(tk_heff_plan* plan, const double* x, double* y) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !x || !y) throw tk::ArgError("null argument");
+    plan->p->apply_fp64(x, y);
+  });
+}
+
+tk_status tk_heff_diag(tk_heff_plan* plan, double* d) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !d) throw tk::ArgError("null argument");
+    plan->p->diagonal(d);
+  });
+}
+
+tk_status tk_heff_stats(tk_heff_plan* plan, double* flops, double* bytes, int64_t* counts) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan) throw tk::ArgError("null plan");
+    const tk::HeffStats& s = plan->p->stats();
+    if (flops) *flops = s.flops;
+    if (bytes) *bytes = s.bytes;
+    if (counts) {
+      counts[0] = s.n_psi_blocks;
+      counts[1] = s.dim;
+      counts[2] = s.t1_elems;
+      counts[3] = s.stage1_items;
+      counts[4] = s.stage3_items;
+      counts[5] = s.stage3_terms;
+    }
+  });
+}
+
+tk_status tk_heff_stage_stats(tk_heff_plan* plan, double* out) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !out) throw tk::ArgError("null argument");
+    const tk::HeffStats& s = plan->p->stats();
+    out[0] = s.s1_flops;
+    out[1] = s.s1_bytes;
+    out[2] = s.s3_flops;
+    out[3] = s.s3_bytes;
+  });
+}
+
+tk_status tk_heff_time(tk_heff_plan* plan, const double* x, double* y, int32_t reps, double* us_apply,
+                       double* us_stage1, double* us_stage3) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !x || !y || reps <= 0) throw tk::ArgError("bad timing arguments");
+    tk::Context* c = plan->ctx->c.get();
+    hipEvent_t e0, e1, e2;
+    TK_HIP(hipEventCreate(&e0));
+    TK_HIP(hipEventCreate(&e1));
+    TK_HIP(hipEventCreate(&e2));
+    double s1 = 0, s3 = 0;
+    hipEvent_t a0, a1;
+    TK_HIP(hipEventCreate(&a0));
+    TK_HIP(hipEventCreate(&a1));
+    // whole-apply timing without interleaved events
+    TK_HIP(hipEventRecord(a0, c->stream));
+    for (int r = 0; r < reps; ++r) plan->p->apply(x, y);
+    TK_HIP(hipEventRecord(a1, c->stream));
+    TK_HIP(hipEventSynchronize(a1));
+    float ms_all = 0;
+    TK_HIP(hipEventElapsedTime(&ms_all, a0, a1));
+    for (int r = 0; r < reps; ++r) {
+      TK_HIP(hipEventRecord(e0, c->stream));
+      plan->p->stage1(x);
+      TK_HIP(hipEventRecord(e1, c->stream));
+      plan->p->stage3(y);
+      TK_HIP(hipEventRecord(e2, c->stream));
+      TK_HIP(hipEventSynchronize(e2));
+      float m1 = 0, m3 = 0;
+      TK_HIP(hipEventElapsedTime(&m1, e0, e1));
+      TK_HIP(hipEventElapsedTime(&m3, e1, e2));
+      s1 += m1;
+      s3 += m3;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(a0);
+    (void)hipEventDestroy(a1);
+    if (us_apply) *us_apply = 1000.0 * ms_all / reps;
+    if (us_stage1) *us_stage1 = 1000.0 * s1 / reps;
+    if (us_stage3) *us_stage3 = 1000.0 * s3 / reps;
+  });
+}
+
+}  // extern "C"

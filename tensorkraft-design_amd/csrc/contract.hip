@@ -1,0 +1,180 @@
+// Persistent grouped block-contraction kernel on CDNA4 MFMA.
+//
+// Each wavefront owns one 32x32 output tile at a time (2x2 MFMA 16x16 tiles)
+// and walks the LPT-ordered item list with a grid stride, so one launch covers
+// every sector block of a contraction stage whatever its shape.  Operands are
+// read straight into MFMA operand registers; the A operand of a term is the
+// coefficient-weighted sum of its sources, formed in registers (this is where
+// the MPO W_i W_j application is fused).  FP64 uses v_mfma_f64_16x16x4f64,
+// FP32 v_mfma_f32_16x16x4f32; split-BF16 lives in contract_bf16.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+
+#include "contract.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace tk {
+
+void ContractList::finalize() {
+  items.clear();
+  flops = 0;
+  for (size_t o = 0; o < outs.size(); ++o) {
+    const GOut& g = outs[o];
+    int64_t work = 0;
+    for (int t = g.t_begin; t < g.t_end; ++t) {
+      const GTerm& tm = terms[t];
+      int ns = tm.s_end - tm.s_begin;
+      work += int64_t(tm.k) * ns;
+      flops += 2.0 * g.m * g.n * tm.k + 2.0 * g.m * tm.k * (ns - 1);
+    }
+    for (int m0 = 0; m0 < g.m; m0 += 32)
+      for (int n0 = 0; n0 < g.n; n0 += 32) {
+        int64_t c = std::max<int64_t>(1, work * std::min(32, g.m - m0) * std::min(32, g.n - n0) / 64);
+        items.push_back({int32_t(o), m0, n0, int32_t(std::min<int64_t>(c, INT32_MAX))});
+      }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const GItem& a, const GItem& b) { return a.cost > b.cost; });
+}
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  using acc_t = v4d;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  using acc_t = v4f;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename TI, typename TO, bool TA, bool TB, bool TC>
+__global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__ items, int n_items,
+                                                       const GOut* __restrict__ outs,
+                                                       const GTerm* __restrict__ terms,
+                                                       const GSrc* __restrict__ srcs,
+                                                       const TI* __restrict__ A, const TI* __restrict__ B,
+                                                       TO* __restrict__ C) {
+  using M = Mfma<TI>;
+  using acc_t = typename M::acc_t;
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  for (int it = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < n_items; it += nwaves) {
+    const GItem item = items[it];
+    const GOut o = outs[item.out];
+    const bool two_m = item.m0 + 16 < o.m;  // wave-uniform: second row tile live
+    const bool two_n = item.n0 + 16 < o.n;
+    acc_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    const int r0 = item.m0 + li, r1 = r0 + 16;
+    const int q0 = item.n0 + li, q1 = q0 + 16;
+    for (int t = o.t_begin; t < o.t_end; ++t) {
+      const GTerm tm = terms[t];
+      const TI* __restrict__ Bp = B + tm.b_off;
+      for (int k0 = 0; k0 < tm.k; k0 += 4) {
+        const int kk = k0 + lk;
+        TI a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+        if (kk < tm.k) {
+          for (int s = tm.s_begin; s < tm.s_end; ++s) {
+            const GSrc sr = srcs[s];
+            const TI* __restrict__ Ap = A + sr.a_off;
+            const TI cf = TI(sr.coef);
+            if (r0 < o.m) a0 += cf * (TA ? Ap[int64_t(kk) * sr.lda + r0] : Ap[int64_t(r0) * sr.lda + kk]);
+            if (r1 < o.m) a1 += cf * (TA ? Ap[int64_t(kk) * sr.lda + r1] : Ap[int64_t(r1) * sr.lda + kk]);
+          }
+          if (q0 < o.n) b0 = TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0];
+          if (q1 < o.n) b1 = TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1];
+        }
+        c00 = M::run(a0, b0, c00);
+        if (two_n) c01 = M::run(a0, b1, c01);
+        if (two_m) {
+          c10 = M::run(a1, b0, c10);
+          if (two_n) c11 = M::run(a1, b1, c11);
+        }
+      }
+    }
+    TO* __restrict__ Cp = C + o.c_off;
+    auto put = [&](int row, int col, TI v) {
+      if (row < o.m && col < o.n) {
+        if (TC) Cp[int64_t(col) * o.ldc + row] = TO(v);
+        else Cp[int64_t(row) * o.ldc + col] = TO(v);
+      }
+    };
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = item.m0 + 4 * lk + r;
+      const int col = item.n0 + li;
+      put(row, col, c00[r]);
+      put(row, col + 16, c01[r]);
+      put(row + 16, col, c10[r]);
+      put(row + 16, col + 16, c11[r]);
+    }
+  }
+}
+
+template <typename TI, typename TO>
+void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
+  if (L.n_items == 0) return;
+  const int waves_per_block = 4;
+  int blocks = (L.n_items + waves_per_block - 1) / waves_per_block;
+  blocks = std::min(blocks, ctx->n_cu * 8);
+  dim3 grid(blocks), block(64 * waves_per_block);
+#define TK_CASE(F)                                                                                     \
+  case F:                                                                                              \
+    hipLaunchKernelGGL((contract_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
+                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C); \
+    break;
+  switch (L.flags & 7) {
+    TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+  }
+#undef TK_CASE
+  TK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void DeviceContractList::upload(Context* ctx, const ContractList& h) {
+  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+  size_t o_items = 0;
+  size_t o_outs = align(o_items + h.items.size() * sizeof(GItem));
+  size_t o_terms = align(o_outs + h.outs.size() * sizeof(GOut));
+  size_t o_srcs = align(o_terms + h.terms.size() * sizeof(GTerm));
+  size_t total = align(o_srcs + h.srcs.size() * sizeof(GSrc));
+  host.assign(total, 0);
+  std::copy_n(reinterpret_cast<const char*>(h.items.data()), h.items.size() * sizeof(GItem), host.data() + o_items);
+  std::copy_n(reinterpret_cast<const char*>(h.outs.data()), h.outs.size() * sizeof(GOut), host.data() + o_outs);
+  std::copy_n(reinterpret_cast<const char*>(h.terms.data()), h.terms.size() * sizeof(GTerm), host.data() + o_terms);
+  std::copy_n(reinterpret_cast<const char*>(h.srcs.data()), h.srcs.size() * sizeof(GSrc), host.data() + o_srcs);
+  buf.reset(ctx, total);
+  buf.upload(host.data(), total);
+  char* base = buf.get();
+  items = reinterpret_cast<GItem*>(base + o_items);
+  outs = reinterpret_cast<GOut*>(base + o_outs);
+  terms = reinterpret_cast<GTerm*>(base + o_terms);
+  srcs = reinterpret_cast<GSrc*>(base + o_srcs);
+  n_items = int(h.items.size());
+  flags = h.flags;
+  flops = h.flops;
+}
+
+void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C) {
+  launch_typed<double, double>(ctx, L, A, B, C);
+}
+void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C) {
+  launch_typed<float, float>(ctx, L, A, B, C);
+}
+void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C) {
+  launch_typed<float, double>(ctx, L, A, B, C);
+}
+
+}  // namespace tk

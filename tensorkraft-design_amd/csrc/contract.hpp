@@ -1,0 +1,82 @@
+// Grouped block contraction: the work list format shared by the host planner
+// and the persistent MFMA kernel.
+//
+// One launch computes a set of output blocks
+//     C_o = sum_{t in terms(o)} ( sum_{s in srcs(t)} coef_s * op(A_s) ) * op(B_t)
+// where every operand is a row-major slice (base offset + leading dimension)
+// of one of three device buffers.  This covers, in one format,
+//   * the rank-2 block_gemm of SparseLinAlgBackend (tk-linalg/src/device/faer.rs:390-575),
+//     with one term and one source per sector GEMM task;
+//   * the two grouped-GEMM stages of H_eff*psi with the MPO W_i W_j application
+//     fused into the A-operand of the second stage (the A-side source sum);
+//   * the environment updates (tk-dmrg/src/environments/mod.rs:225-483).
+// Accumulation order is fixed by the term/source order, so results are
+// deterministic; each output tile is owned by exactly one wavefront.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace tk {
+
+struct GOut {
+  int64_t c_off;
+  int32_t m, n, ldc;
+  int32_t t_begin, t_end;
+  int32_t pad;
+};
+struct GTerm {
+  int64_t b_off;
+  int32_t ldb, k;
+  int32_t s_begin, s_end;
+};
+struct GSrc {
+  int64_t a_off;
+  int32_t lda, pad;
+  double coef;
+};
+// One wavefront work item: a 32x32 tile of output block `out`.
+struct GItem {
+  int32_t out, m0, n0, cost;
+};
+
+// Operand transposition flags of one launch.
+enum ContractFlags : int { kTransA = 1, kTransB = 2, kTransC = 4 };
+
+// Host-side list builder.
+struct ContractList {
+  std::vector<GOut> outs;
+  std::vector<GTerm> terms;
+  std::vector<GSrc> srcs;
+  std::vector<GItem> items;
+  int flags = 0;
+  double flops = 0;  // algorithmic flops of the list (2*m*n*k per term + 2*m*k per extra source)
+
+  int begin_out(int64_t c_off, int m, int n, int ldc) {
+    outs.push_back({c_off, m, n, ldc, int32_t(terms.size()), int32_t(terms.size()), 0});
+    return int(outs.size()) - 1;
+  }
+  void begin_term(int64_t b_off, int ldb, int k) {
+    terms.push_back({b_off, ldb, k, int32_t(srcs.size()), int32_t(srcs.size())});
+    outs.back().t_end = int32_t(terms.size());
+  }
+  void add_src(int64_t a_off, int lda, double coef) {
+    srcs.push_back({a_off, lda, 0, coef});
+    terms.back().s_end = int32_t(srcs.size());
+  }
+  // Drop a term that ended up with no sources.
+  void drop_empty_term() {
+    if (!terms.empty() && terms.back().s_begin == terms.back().s_end) {
+      terms.pop_back();
+      outs.back().t_end = int32_t(terms.size());
+    }
+  }
+  // Tile every output into 32x32 wavefront items, longest first (LPT order,
+  // tk-linalg/src/tasks.rs:172-174) with a stable tie-break on position.
+  void finalize();
+  void clear() {
+    outs.clear(); terms.clear(); srcs.clear(); items.clear();
+    flops = 0;
+  }
+};
+
+}  // namespace tk

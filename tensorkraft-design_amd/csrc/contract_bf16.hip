@@ -1,0 +1,145 @@
+// Split-BF16 variant of the grouped block contraction.
+//
+// Every FP32 operand value v is split as v = hi + lo with hi = bf16(v) and
+// lo = bf16(v - hi); the product is formed from three bf16 MFMAs
+// (hi*hi + hi*lo + lo*hi) with FP32 accumulation on gfx950's
+// v_mfma_f32_16x16x32_bf16.  The dropped lo*lo term bounds the relative error
+// of each product near 2^-16; the Krylov solver wins FP64 accuracy back by
+// FP64 residual refinement.  Work-list format and tiling are those of
+// contract.hip (one wavefront per 32x32 output tile).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace tk {
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v4f mfma3(const bf16x8& ah, const bf16x8& al, const bf16x8& bh, const bf16x8& bl,
+                                     v4f c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    __bf16 h = __bf16(v[i]);
+    hi[i] = h;
+    lo[i] = __bf16(v[i] - float(h));
+  }
+}
+
+template <typename TO, bool TA, bool TB, bool TC>
+__global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __restrict__ items, int n_items,
+                                                              const GOut* __restrict__ outs,
+                                                              const GTerm* __restrict__ terms,
+                                                              const GSrc* __restrict__ srcs,
+                                                              const float* __restrict__ A,
+                                                              const float* __restrict__ B, TO* __restrict__ C) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  for (int it = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < n_items; it += nwaves) {
+    const GItem item = items[it];
+    const GOut o = outs[item.out];
+    const bool two_m = item.m0 + 16 < o.m;
+    const bool two_n = item.n0 + 16 < o.n;
+    v4f c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    const int r0 = item.m0 + li, r1 = r0 + 16;
+    const int q0 = item.n0 + li, q1 = q0 + 16;
+    for (int t = o.t_begin; t < o.t_end; ++t) {
+      const GTerm tm = terms[t];
+      const float* __restrict__ Bp = B + tm.b_off;
+      for (int k0 = 0; k0 < tm.k; k0 += 32) {
+        float a0[8], a1[8], b0[8], b1[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a0[i] = a1[i] = b0[i] = b1[i] = 0.f;
+        for (int s = tm.s_begin; s < tm.s_end; ++s) {
+          const GSrc sr = srcs[s];
+          const float* __restrict__ Ap = A + sr.a_off;
+          const float cf = float(sr.coef);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int kk = k0 + 8 * lk + i;
+            if (kk < tm.k) {
+              if (r0 < o.m) a0[i] += cf * (TA ? Ap[int64_t(kk) * sr.lda + r0] : Ap[int64_t(r0) * sr.lda + kk]);
+              if (r1 < o.m) a1[i] += cf * (TA ? Ap[int64_t(kk) * sr.lda + r1] : Ap[int64_t(r1) * sr.lda + kk]);
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int kk = k0 + 8 * lk + i;
+          if (kk < tm.k) {
+            if (q0 < o.n) b0[i] = TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0];
+            if (q1 < o.n) b1[i] = TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1];
+          }
+        }
+        bf16x8 a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l;
+        split8(a0, a0h, a0l);
+        split8(b0, b0h, b0l);
+        c00 = mfma3(a0h, a0l, b0h, b0l, c00);
+        if (two_n) {
+          split8(b1, b1h, b1l);
+          c01 = mfma3(a0h, a0l, b1h, b1l, c01);
+        }
+        if (two_m) {
+          split8(a1, a1h, a1l);
+          c10 = mfma3(a1h, a1l, b0h, b0l, c10);
+          if (two_n) c11 = mfma3(a1h, a1l, b1h, b1l, c11);
+        }
+      }
+    }
+    TO* __restrict__ Cp = C + o.c_off;
+    auto put = [&](int row, int col, float v) {
+      if (row < o.m && col < o.n) {
+        if (TC) Cp[int64_t(col) * o.ldc + row] = TO(v);
+        else Cp[int64_t(row) * o.ldc + col] = TO(v);
+      }
+    };
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = item.m0 + 4 * lk + r;
+      const int col = item.n0 + li;
+      put(row, col, c00[r]);
+      put(row, col + 16, c01[r]);
+      put(row + 16, col, c10[r]);
+      put(row + 16, col + 16, c11[r]);
+    }
+  }
+}
+
+template <typename TO>
+void launch_bf16(Context* ctx, const DeviceContractList& L, const float* A, const float* B, TO* C) {
+  if (L.n_items == 0) return;
+  int blocks = std::min((L.n_items + 3) / 4, ctx->n_cu * 8);
+  dim3 grid(blocks), block(256);
+#define TK_CASE(F)                                                                                         \
+  case F:                                                                                                  \
+    hipLaunchKernelGGL((contract_bf16x3_kernel<TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
+                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C);   \
+    break;
+  switch (L.flags & 7) {
+    TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+  }
+#undef TK_CASE
+  TK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C) {
+  launch_bf16<float>(ctx, L, A, B, C);
+}
+void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C) {
+  launch_bf16<double>(ctx, L, A, B, C);
+}
+
+}  // namespace tk

@@ -1,0 +1,165 @@
+// Device context: one HIP stream, a caching device allocator and the last
+// error string.  One context per host thread (the C ABI's contract).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace tk {
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define TK_HIP(call)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::tk::HipError(e_, std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// Size-class caching allocator: freed blocks are kept for reuse until the
+// context is destroyed, so the sweep's per-step buffers cost no hipMalloc.
+class DevicePool {
+ public:
+  DevicePool() = default;
+  ~DevicePool() { release_all(); }
+  void* alloc(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    size_t cls = size_class(bytes);
+    std::lock_guard<std::mutex> g(mu_);
+    auto& fl = free_[cls];
+    if (!fl.empty()) {
+      void* p = fl.back();
+      fl.pop_back();
+      return p;
+    }
+    void* p = nullptr;
+    TK_HIP(hipMalloc(&p, cls));
+    size_of_[p] = cls;
+    held_ += cls;
+    return p;
+  }
+  void free(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = size_of_.find(p);
+    if (it == size_of_.end()) return;
+    free_[it->second].push_back(p);
+  }
+  size_t held_bytes() const { return held_; }
+  void release_all() {
+    std::lock_guard<std::mutex> g(mu_);
+    for (auto& [p, s] : size_of_) (void)hipFree(p);
+    size_of_.clear();
+    free_.clear();
+    held_ = 0;
+  }
+
+ private:
+  static size_t size_class(size_t b) {
+    // 256-byte granularity below 1 MiB, then powers of two in quarter steps
+    if (b <= (1u << 20)) return (b + 255) & ~size_t(255);
+    size_t p = 1;
+    while (p < b) p <<= 1;
+    size_t q = p >> 2;
+    size_t c = p >> 1;
+    while (c < b) c += q;
+    return c;
+  }
+  std::mutex mu_;
+  std::map<size_t, std::vector<void*>> free_;
+  std::map<void*, size_t> size_of_;
+  size_t held_ = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevicePool pool;
+  std::string last_error;
+  int n_cu = 256;
+  // Stream-ordered scratch for reduction partials; reused across launches.
+  void* scratch_p = nullptr;
+  size_t scratch_n = 0;
+  void* scratch(size_t bytes) {
+    if (bytes > scratch_n) {
+      if (scratch_p) pool.free(scratch_p);
+      scratch_p = pool.alloc(bytes);
+      scratch_n = bytes;
+    }
+    return scratch_p;
+  }
+  explicit Context(int dev) : device(dev) {
+    TK_HIP(hipSetDevice(dev));
+    TK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    TK_HIP(hipGetDeviceProperties(&prop, dev));
+    n_cu = prop.multiProcessorCount;
+  }
+  ~Context() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    pool.release_all();
+  }
+  void sync() { TK_HIP(hipStreamSynchronize(stream)); }
+};
+
+// RAII device buffer drawn from the context pool.
+template <typename T>
+class DBuf {
+ public:
+  DBuf() = default;
+  DBuf(Context* c, size_t n) { reset(c, n); }
+  ~DBuf() { release(); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept { *this = std::move(o); }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      ctx_ = o.ctx_; p_ = o.p_; n_ = o.n_;
+      o.ctx_ = nullptr; o.p_ = nullptr; o.n_ = 0;
+    }
+    return *this;
+  }
+  void reset(Context* c, size_t n) {
+    release();
+    ctx_ = c;
+    n_ = n;
+    p_ = static_cast<T*>(c->pool.alloc(n * sizeof(T)));
+  }
+  void release() {
+    if (p_ && ctx_) ctx_->pool.free(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+  void upload(const T* h, size_t n) {
+    if (n > n_) throw ArgError("upload larger than buffer");
+    if (n) TK_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, ctx_->stream));
+  }
+  void download(T* h, size_t n) const {
+    if (n > n_) throw ArgError("download larger than buffer");
+    if (n) TK_HIP(hipMemcpyAsync(h, p_, n * sizeof(T), hipMemcpyDeviceToHost, ctx_->stream));
+  }
+
+ private:
+  Context* ctx_ = nullptr;
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+}  // namespace tk

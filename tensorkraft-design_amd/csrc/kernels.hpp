@@ -1,0 +1,50 @@
+// Host-callable launchers for the HIP kernels of the library.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "contract.hpp"
+#include "device.hpp"
+
+namespace tk {
+
+// A ContractList resident on the device (one packed allocation).
+struct DeviceContractList {
+  DBuf<char> buf;
+  std::vector<char> host;  // kept alive until the list is released
+  const GItem* items = nullptr;
+  const GOut* outs = nullptr;
+  const GTerm* terms = nullptr;
+  const GSrc* srcs = nullptr;
+  int n_items = 0;
+  int flags = 0;
+  double flops = 0;
+  void upload(Context* ctx, const ContractList& h);
+};
+
+void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C);
+void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C);
+void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C);
+// split-BF16 (hi+lo, three products, FP32 accumulation) on FP32 operands
+void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C);
+void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C);
+
+// ---- vector kernels (vecops.hip) ----
+void convert_f64_to_f32(Context* ctx, const double* x, float* y, int64_t n);
+void convert_f32_to_f64(Context* ctx, const float* x, double* y, int64_t n);
+void fill_f64(Context* ctx, double* x, double v, int64_t n);
+// out[i] = sum_j V[i*ldv + j] * x[j] for i < nvec (deterministic two-pass reduction)
+void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double* x, int64_t n, double* out);
+// y[j] = beta*y[j] + sum_i c[i] * V[i*ldv + j]; c on the device
+void combine(Context* ctx, const double* V, int64_t ldv, int nvec, const double* c, double beta, double* y, int64_t n);
+// y[j] -= sum_i h[i] * V[i*ldv + j]; h on the device
+void subtract_projection(Context* ctx, const double* V, int64_t ldv, int nvec, const double* h, double* y, int64_t n);
+// x *= 1/sqrt(*nrm2) read on the device (no-op guard when the norm is zero)
+void scale_by_inv_sqrt(Context* ctx, double* x, const double* nrm2, int64_t n);
+void scale(Context* ctx, double* x, double a, int64_t n);
+void axpy(Context* ctx, double a, const double* x, double* y, int64_t n);
+// r = ay - theta*y (all device), optionally t = r / (theta - diag) where |theta-diag| > 1e-14
+void residual(Context* ctx, const double* ay, const double* y, double theta, double* r, int64_t n);
+void precondition(Context* ctx, const double* r, const double* diag, double theta, double* t, int64_t n);
+
+}  // namespace tk

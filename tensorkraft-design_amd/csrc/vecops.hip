@@ -1,0 +1,194 @@
+// FP64 Krylov-vector kernels for the device-resident Davidson/Lanczos and the
+// precision conversions of the mixed matvec.  All reductions are two-pass with
+// a fixed summation order, so results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace tk {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 8;
+constexpr int kChunk = kThreads * kPerThread;
+constexpr int kMaxVec = 128;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __restrict__ V, int64_t ldv, int nvec,
+                                                               const double* __restrict__ x, int64_t n,
+                                                               double* __restrict__ partial, int nchunks) {
+  __shared__ double red[kThreads / 64];
+  const int64_t base = int64_t(blockIdx.x) * kChunk + threadIdx.x;
+  double xr[kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; ++j) {
+    int64_t idx = base + int64_t(j) * kThreads;
+    xr[j] = idx < n ? x[idx] : 0.0;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = 0; i < nvec; ++i) {
+    const double* __restrict__ v = V + int64_t(i) * ldv;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      int64_t idx = base + int64_t(j) * kThreads;
+      if (idx < n) s += v[idx] * xr[j];
+    }
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+      partial[int64_t(i) * nchunks + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_rows_kernel(const double* __restrict__ partial, int nchunks,
+                                                               double* __restrict__ out) {
+  __shared__ double red[kThreads / 64];
+  const double* row = partial + int64_t(blockIdx.x) * nchunks;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < nchunks; c += kThreads) s += row[c];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+    out[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void combine_kernel(const double* __restrict__ V, int64_t ldv, int nvec,
+                                                           const double* __restrict__ c, double beta, double sign,
+                                                           double* __restrict__ y, int64_t n) {
+  __shared__ double cs[kMaxVec];
+  for (int i = threadIdx.x; i < nvec; i += kThreads) cs[i] = c[i];
+  __syncthreads();
+  for (int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x; j < n; j += int64_t(gridDim.x) * kThreads) {
+    double s = 0.0;
+    for (int i = 0; i < nvec; ++i) s += cs[i] * V[int64_t(i) * ldv + j];
+    y[j] = (beta == 0.0 ? 0.0 : beta * y[j]) + sign * s;
+  }
+}
+
+__global__ void scale_inv_sqrt_kernel(double* __restrict__ x, const double* __restrict__ nrm2, int64_t n) {
+  const double s = *nrm2;
+  const double f = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    x[j] *= f;
+}
+
+__global__ void scale_kernel(double* __restrict__ x, double a, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    x[j] *= a;
+}
+__global__ void axpy_kernel(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    y[j] += a * x[j];
+}
+__global__ void residual_kernel(const double* __restrict__ ay, const double* __restrict__ y, double theta,
+                                double* __restrict__ r, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    r[j] = ay[j] - theta * y[j];
+}
+__global__ void precond_kernel(const double* __restrict__ r, const double* __restrict__ d, double theta,
+                               double* __restrict__ t, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x) {
+    double den = theta - d[j];
+    t[j] = fabs(den) > 1e-14 ? r[j] / den : r[j];
+  }
+}
+__global__ void f64_to_f32_kernel(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    y[j] = float(x[j]);
+}
+__global__ void f32_to_f64_kernel(const float* __restrict__ x, double* __restrict__ y, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    y[j] = double(x[j]);
+}
+__global__ void fill_kernel(double* __restrict__ x, double v, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    x[j] = v;
+}
+
+int grid_for(Context* ctx, int64_t n) {
+  int64_t g = (n + kThreads - 1) / kThreads;
+  return int(std::max<int64_t>(1, std::min<int64_t>(g, int64_t(ctx->n_cu) * 16)));
+}
+
+double* scratch(Context* ctx, size_t n) { return static_cast<double*>(ctx->scratch(n * sizeof(double))); }
+
+}  // namespace
+
+void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double* x, int64_t n, double* out) {
+  if (nvec <= 0) return;
+  if (nvec > kMaxVec) throw ArgError("dot_many: too many vectors");
+  int nchunks = int(std::max<int64_t>(1, (n + kChunk - 1) / kChunk));
+  double* part = scratch(ctx, size_t(nchunks) * nvec);
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(nchunks), dim3(kThreads), 0, ctx->stream, V, ldv, nvec, x, n, part,
+                     nchunks);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(nvec), dim3(kThreads), 0, ctx->stream, part, nchunks, out);
+  TK_HIP(hipGetLastError());
+}
+
+void combine(Context* ctx, const double* V, int64_t ldv, int nvec, const double* c, double beta, double* y,
+             int64_t n) {
+  if (nvec > kMaxVec) throw ArgError("combine: too many vectors");
+  hipLaunchKernelGGL(combine_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, V, ldv, nvec, c, beta,
+                     1.0, y, n);
+  TK_HIP(hipGetLastError());
+}
+
+void subtract_projection(Context* ctx, const double* V, int64_t ldv, int nvec, const double* h, double* y,
+                         int64_t n) {
+  if (nvec > kMaxVec) throw ArgError("subtract_projection: too many vectors");
+  hipLaunchKernelGGL(combine_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, V, ldv, nvec, h, 1.0,
+                     -1.0, y, n);
+  TK_HIP(hipGetLastError());
+}
+
+void scale_by_inv_sqrt(Context* ctx, double* x, const double* nrm2, int64_t n) {
+  hipLaunchKernelGGL(scale_inv_sqrt_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, x, nrm2, n);
+  TK_HIP(hipGetLastError());
+}
+void scale(Context* ctx, double* x, double a, int64_t n) {
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, x, a, n);
+  TK_HIP(hipGetLastError());
+}
+void axpy(Context* ctx, double a, const double* x, double* y, int64_t n) {
+  hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, a, x, y, n);
+  TK_HIP(hipGetLastError());
+}
+void residual(Context* ctx, const double* ay, const double* y, double theta, double* r, int64_t n) {
+  hipLaunchKernelGGL(residual_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, ay, y, theta, r, n);
+  TK_HIP(hipGetLastError());
+}
+void precondition(Context* ctx, const double* r, const double* diag, double theta, double* t, int64_t n) {
+  hipLaunchKernelGGL(precond_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, r, diag, theta, t, n);
+  TK_HIP(hipGetLastError());
+}
+void convert_f64_to_f32(Context* ctx, const double* x, float* y, int64_t n) {
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, x, y, n);
+  TK_HIP(hipGetLastError());
+}
+void convert_f32_to_f64(Context* ctx, const float* x, double* y, int64_t n) {
+  hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, x, y, n);
+  TK_HIP(hipGetLastError());
+}
+void fill_f64(Context* ctx, double* x, double v, int64_t n) {
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, x, v, n);
+  TK_HIP(hipGetLastError());
+}
+
+}  // namespace tk

@@ -1,0 +1,472 @@
+"""ctypes binding of libtk_hip.so (the C ABI declared in include/tk_hip.h).
+
+The library is the product: every call here runs the HIP path.  Loading fails
+loudly when the shared object has not been built; there is no fallback.
+
+Also holds the workload builders used by the benchmark and the tests:
+Hubbard / Heisenberg MPO site tensors (basis and Jordan-Wigner conventions of
+tk-dsl/src/operators.rs:31-200) and the synthetic U(1)xU(1) bond of the single
+H_eff*psi microbenchmark.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtk_hip.so")
+
+TK_OK = 0
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+SYM_U1, SYM_U1U1 = 1, 2
+DIR_IN, DIR_OUT = 0, 1
+STATUS = {0: "TK_OK", 1: "TK_ERR_INVALID_ARG", 2: "TK_ERR_SVD_CONVERGENCE", 3: "TK_ERR_HIP",
+          4: "TK_ERR_NOT_CONVERGED", 5: "TK_ERR_CANCELLED", 6: "TK_ERR_SYMMETRY", 7: "TK_ERR_CAPACITY",
+          8: "TK_ERR_INTERNAL"}
+
+
+class TkError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class tk_qindex(C.Structure):
+    _fields_ = [("n_sectors", C.c_int32), ("qn", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int64))]
+
+
+class tk_mpo_site(C.Structure):
+    _fields_ = [("phys", tk_qindex), ("wl", tk_qindex), ("wr", tk_qindex), ("w", C.POINTER(C.c_double))]
+
+
+class tk_env(C.Structure):
+    _fields_ = [("bond", tk_qindex), ("w", tk_qindex), ("data_dev", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libtk_hip.so once; raise if it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, U64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+    pI32, pI64, pU64, pD = C.POINTER(I32), C.POINTER(I64), C.POINTER(U64), C.POINTER(D)
+    sig = {
+        "tk_version": (C.c_char_p, []),
+        "tk_ctx_create": (I32, [I32, C.POINTER(P)]),
+        "tk_ctx_destroy": (I32, [P]),
+        "tk_ctx_sync": (I32, [P]),
+        "tk_last_error": (C.c_char_p, [P]),
+        "tk_device_alloc": (I32, [P, I64, C.POINTER(P)]),
+        "tk_device_free": (I32, [P, P]),
+        "tk_memcpy_h2d": (I32, [P, P, P, I64]),
+        "tk_memcpy_d2h": (I32, [P, P, P, I64]),
+        "tk_pack_key": (I32, [I32, I32, pI32, pU64]),
+        "tk_qindex_layout": (I32, [I32, C.POINTER(tk_qindex), pI32, pI64, pI64, pI64]),
+        "tk_block_table": (I32, [I32, I32, C.POINTER(tk_qindex), pI32, pI32, I64, pI64, pU64, pI64, pI32, pI64]),
+        "tk_fuse_map": (I32, [I32, I32, C.POINTER(tk_qindex), pI32, I64, pI32, pI32, pI64, pI64, pI32, pI64]),
+        "tk_heff_plan_create": (I32, [P, I32, I32, C.POINTER(tk_env), C.POINTER(tk_env), C.POINTER(tk_mpo_site),
+                                      C.POINTER(tk_mpo_site), C.POINTER(P)]),
+        "tk_heff_plan_destroy": (I32, [P]),
+        "tk_heff_psi_table": (I32, [P, I64, pI64, pU64, pI64, pI64]),
+        "tk_heff_apply": (I32, [P, P, P]),
+        "tk_heff_apply_fp64": (I32, [P, P, P]),
+        "tk_heff_diag": (I32, [P, P]),
+        "tk_heff_stats": (I32, [P, pD, pD, pI64]),
+        "tk_heff_stage_stats": (I32, [P, pD]),
+        "tk_heff_time": (I32, [P, P, P, I32, pD, pD, pD]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_last_error", "tk_device_alloc",
+            "tk_device_free", "tk_memcpy_h2d", "tk_memcpy_d2h", "tk_pack_key", "tk_qindex_layout", "tk_block_table", "tk_fuse_map",
+            "tk_heff_plan_create", "tk_heff_plan_destroy", "tk_heff_psi_table", "tk_heff_apply",
+            "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time"]
+
+
+def _check(st, ctx=None):
+    if st != TK_OK:
+        msg = lib().tk_last_error(ctx).decode()
+        raise TkError(st, msg)
+
+
+# ---------------------------------------------------------------- qindex helpers
+class QIndexArg:
+    """Keeps the numpy arrays of a tk_qindex alive."""
+
+    def __init__(self, sectors):
+        self.sectors = [(tuple(q) if isinstance(q, (tuple, list)) else (int(q), 0), int(d)) for q, d in sectors]
+        self.qn = np.ascontiguousarray(np.array([c for q, _ in self.sectors for c in q], dtype=np.int32))
+        self.dims = np.ascontiguousarray(np.array([d for _, d in self.sectors], dtype=np.int64))
+        self.c = tk_qindex(len(self.sectors), self.qn.ctypes.data_as(C.POINTER(C.c_int32)),
+                           self.dims.ctypes.data_as(C.POINTER(C.c_int64)))
+
+    @property
+    def total(self):
+        return int(self.dims.sum())
+
+
+def sorted_sectors(sectors):
+    """QIndex::new order (sector_key.rs:108-117): by quantum number."""
+    return sorted(((tuple(q) if isinstance(q, (tuple, list)) else (int(q), 0)), int(d)) for q, d in sectors)
+
+
+def qindex_layout(sym, sectors):
+    """Product QIndex::new: sorted (qn, dim) list, offsets, total."""
+    q = QIndexArg(sectors)
+    n = len(q.sectors)
+    qn = np.zeros(2 * n, dtype=np.int32)
+    dims = np.zeros(n, dtype=np.int64)
+    offs = np.zeros(n, dtype=np.int64)
+    tot = C.c_int64(0)
+    _check(lib().tk_qindex_layout(sym, C.byref(q.c), qn.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  dims.ctypes.data_as(C.POINTER(C.c_int64)), offs.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  C.byref(tot)))
+    return [((int(qn[2 * i]), int(qn[2 * i + 1])), int(dims[i])) for i in range(n)], offs, tot.value
+
+
+def block_table(sym, legs, dirs, flux=(0, 0)):
+    """Product block table: (keys, offsets, sector positions, total)."""
+    qs = [QIndexArg(l) for l in legs]
+    arr = (tk_qindex * len(qs))(*[q.c for q in qs])
+    d = np.array(dirs, dtype=np.int32)
+    f = np.array(flux, dtype=np.int32)
+    n = C.c_int64(0)
+    tot = C.c_int64(0)
+    L = lib()
+    st = L.tk_block_table(sym, len(qs), arr, d.ctypes.data_as(C.POINTER(C.c_int32)),
+                          f.ctypes.data_as(C.POINTER(C.c_int32)), 0, C.byref(n), None, None, None, C.byref(tot))
+    if st not in (TK_OK, 1):
+        _check(st)
+    cap = n.value
+    keys = np.zeros(cap, dtype=np.uint64)
+    offs = np.zeros(cap, dtype=np.int64)
+    secs = np.zeros(cap * len(qs), dtype=np.int32)
+    _check(L.tk_block_table(sym, len(qs), arr, d.ctypes.data_as(C.POINTER(C.c_int32)),
+                            f.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(n),
+                            keys.ctypes.data_as(C.POINTER(C.c_uint64)), offs.ctypes.data_as(C.POINTER(C.c_int64)),
+                            secs.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(tot)))
+    return keys, offs, secs.reshape(cap, len(qs)), tot.value
+
+
+def pack_key(sym, qns):
+    a = np.array([c for q in qns for c in ((q, 0) if np.isscalar(q) else q)], dtype=np.int32)
+    k = C.c_uint64(0)
+    _check(lib().tk_pack_key(sym, len(qns), a.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(k)))
+    return k.value
+
+
+def fuse_map(sym, legs, dirs):
+    qs = [QIndexArg(l) for l in legs]
+    arr = (tk_qindex * len(qs))(*[q.c for q in qs])
+    d = np.array(dirs, dtype=np.int32)
+    nf = C.c_int32(0)
+    nc = C.c_int64(0)
+    cap = 1
+    for l in legs:
+        cap *= len(l)
+    fq = np.zeros(2 * cap, dtype=np.int32)
+    fd = np.zeros(cap, dtype=np.int64)
+    cs = np.zeros(cap, dtype=np.int32)
+    co = np.zeros(cap, dtype=np.int64)
+    _check(lib().tk_fuse_map(sym, len(qs), arr, d.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(nf),
+                             fq.ctypes.data_as(C.POINTER(C.c_int32)), fd.ctypes.data_as(C.POINTER(C.c_int64)),
+                             C.byref(nc), cs.ctypes.data_as(C.POINTER(C.c_int32)),
+                             co.ctypes.data_as(C.POINTER(C.c_int64))))
+    nfv = nf.value
+    fused = [((int(fq[2 * i]), int(fq[2 * i + 1])), int(fd[i])) for i in range(nfv)]
+    return fused, cs[:nc.value].copy(), co[:nc.value].copy()
+
+
+# ---------------------------------------------------------------- device context
+class Context:
+    """tk_ctx: one HIP stream + device pool on `device`."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        _check(self.L.tk_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.tk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(self.L.tk_ctx_sync(self.h), self.h)
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        _check(self.L.tk_device_alloc(self.h, int(nbytes), C.byref(p)), self.h)
+        return p.value
+
+    def free(self, p):
+        _check(self.L.tk_device_free(self.h, C.c_void_p(p)), self.h)
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(max(arr.nbytes, 8))
+        _check(self.L.tk_memcpy_h2d(self.h, C.c_void_p(p), arr.ctypes.data_as(C.c_void_p), arr.nbytes), self.h)
+        return DeviceArray(self, p, arr.shape, arr.dtype)
+
+    def empty(self, n, dtype=np.float64):
+        dt = np.dtype(dtype)
+        return DeviceArray(self, self.alloc(max(n * dt.itemsize, 8)), (n,), dt)
+
+
+class DeviceArray:
+    def __init__(self, ctx, ptr, shape, dtype):
+        self.ctx, self.ptr, self.shape, self.dtype = ctx, ptr, tuple(shape), np.dtype(dtype)
+
+    @property
+    def nbytes(self):
+        return int(np.prod(self.shape)) * self.dtype.itemsize
+
+    def get(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.ctx.L.tk_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                        self.nbytes), self.ctx.h)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.free(self.ptr)
+            self.ptr = None
+
+
+# ---------------------------------------------------------------- H_eff plan
+class HeffPlan:
+    """tk_heff_plan: H_eff*psi for one bond shape (build_heff_two_site)."""
+
+    def __init__(self, ctx, prec, sym, bond_l, w_l, L_dev, bond_r, w_r, R_dev, mpo_i, mpo_j):
+        self.ctx = ctx
+        self._keep = []
+        ql, wl, qr, wr = QIndexArg(bond_l), QIndexArg(w_l), QIndexArg(bond_r), QIndexArg(w_r)
+        self._keep += [ql, wl, qr, wr]
+        el = tk_env(ql.c, wl.c, C.c_void_p(L_dev.ptr))
+        er = tk_env(qr.c, wr.c, C.c_void_p(R_dev.ptr))
+        mi, mj = self._mpo(mpo_i), self._mpo(mpo_j)
+        h = C.c_void_p()
+        _check(ctx.L.tk_heff_plan_create(ctx.h, PREC[prec], sym, C.byref(el), C.byref(er), C.byref(mi),
+                                         C.byref(mj), C.byref(h)), ctx.h)
+        self.h = h
+
+    def _mpo(self, m):
+        p, a, b = QIndexArg(m.phys), QIndexArg(m.wl), QIndexArg(m.wr)
+        w = np.ascontiguousarray(m.w, dtype=np.float64)
+        self._keep += [p, a, b, w]
+        return tk_mpo_site(p.c, a.c, b.c, w.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def close(self):
+        if self.h:
+            self.ctx.L.tk_heff_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def psi_table(self):
+        n = C.c_int64(0)
+        tot = C.c_int64(0)
+        self.ctx.L.tk_heff_psi_table(self.h, 0, C.byref(n), None, None, C.byref(tot))
+        keys = np.zeros(n.value, dtype=np.uint64)
+        offs = np.zeros(n.value, dtype=np.int64)
+        _check(self.ctx.L.tk_heff_psi_table(self.h, n.value, C.byref(n), keys.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                            offs.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(tot)), self.ctx.h)
+        return keys, offs, tot.value
+
+    def apply(self, x_dev, y_dev):
+        _check(self.ctx.L.tk_heff_apply(self.h, C.c_void_p(x_dev.ptr), C.c_void_p(y_dev.ptr)), self.ctx.h)
+
+    def apply_fp64(self, x_dev, y_dev):
+        _check(self.ctx.L.tk_heff_apply_fp64(self.h, C.c_void_p(x_dev.ptr), C.c_void_p(y_dev.ptr)), self.ctx.h)
+
+    def diag(self, d_dev):
+        _check(self.ctx.L.tk_heff_diag(self.h, C.c_void_p(d_dev.ptr)), self.ctx.h)
+
+    def stats(self):
+        f = C.c_double(0)
+        b = C.c_double(0)
+        cnt = np.zeros(8, dtype=np.int64)
+        _check(self.ctx.L.tk_heff_stats(self.h, C.byref(f), C.byref(b), cnt.ctypes.data_as(C.POINTER(C.c_int64))),
+               self.ctx.h)
+        return {"flops": f.value, "bytes": b.value, "psi_blocks": int(cnt[0]), "dim": int(cnt[1]),
+                "t1_elems": int(cnt[2]), "stage1_items": int(cnt[3]), "stage3_items": int(cnt[4]),
+                "stage3_terms": int(cnt[5])}
+
+    def stage_stats(self):
+        out = np.zeros(4)
+        _check(self.ctx.L.tk_heff_stage_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), self.ctx.h)
+        return {"s1_flops": out[0], "s1_bytes": out[1], "s3_flops": out[2], "s3_bytes": out[3]}
+
+    def time(self, x_dev, y_dev, reps):
+        a, s1, s3 = C.c_double(0), C.c_double(0), C.c_double(0)
+        _check(self.ctx.L.tk_heff_time(self.h, C.c_void_p(x_dev.ptr), C.c_void_p(y_dev.ptr), int(reps), C.byref(a),
+                                       C.byref(s1), C.byref(s3)), self.ctx.h)
+        return a.value, s1.value, s3.value
+
+
+# ---------------------------------------------------------------- MPO builders
+@dataclass
+class MpoSite:
+    """Dense W[s_in][s_out][w_l][w_r] with the caller's QIndex sector order."""
+    phys: list
+    wl: list
+    wr: list
+    w: np.ndarray
+
+
+def _op(mat):
+    return np.array(mat, dtype=np.float64).reshape(len(mat) if np.ndim(mat) == 2 else int(math.isqrt(len(mat))), -1)
+
+
+# Fermion basis |0>, |up>, |dn>, |updn> (tk-dsl/src/operators.rs:88, 117-200);
+# FermionOp matrices omit every Jordan-Wigner string: c_dn|updn> = +|up>.
+F_CDAG_UP = _op([[0, 0, 0, 0], [1, 0, 0, 0], [0, 0, 0, 0], [0, 0, 1, 0]])
+F_C_UP = F_CDAG_UP.T.copy()
+F_CDAG_DN = _op([[0, 0, 0, 0], [0, 0, 0, 0], [1, 0, 0, 0], [0, 1, 0, 0]])
+F_C_DN = F_CDAG_DN.T.copy()
+F_ID = np.eye(4)
+F_PARITY = np.diag([1.0, -1.0, -1.0, 1.0])   # (-1)^(n_up + n_dn)
+F_UP_PARITY = np.diag([1.0, -1.0, 1.0, -1.0])  # (-1)^n_up, the on-site string of c_dn
+F_NUPNDN = np.diag([0.0, 0.0, 0.0, 1.0])
+# charges (N_up, N_dn) of the four states
+HUBBARD_PHYS = [((0, 0), 1), ((1, 0), 1), ((0, 1), 1), ((1, 1), 1)]
+
+
+def hubbard_w(t=1.0, U=4.0, position="bulk"):
+    """Hubbard-chain MPO site tensor, bond dim 6, U(1)xU(1) (N_up, N_dn).
+
+    H = -t sum_{i,s} (c+_{i s} c_{i+1 s} + h.c.) + U sum_i n_{i up} n_{i dn}.
+    With on-site c_dn = F_up * c_dn(ref) and the inter-site string F, the
+    hopping c+_i c_{i+1} = (c+_i F_i) (x) c_{i+1} and c+_{i+1} c_i = (F_i c_i) (x) c+_{i+1}.
+    MPO states (caller order, grouped by charge):
+      0 start (0,0), 1 done (0,0), 2 after c+_up F (-1,0), 3 after F c_up (+1,0),
+      4 after c+_dn F (0,-1), 5 after F c_dn (0,+1).
+    """
+    cu, cdu = F_C_UP, F_CDAG_UP
+    cd = F_UP_PARITY @ F_C_DN
+    cdd = cd.T.copy()
+    ops = {}
+    ops[(0, 0)] = F_ID
+    ops[(1, 1)] = F_ID
+    ops[(0, 1)] = U * F_NUPNDN
+    ops[(0, 2)] = cdu @ F_PARITY
+    ops[(0, 3)] = F_PARITY @ cu
+    ops[(0, 4)] = cdd @ F_PARITY
+    ops[(0, 5)] = F_PARITY @ cd
+    ops[(2, 1)] = -t * cu
+    ops[(3, 1)] = -t * cdu
+    ops[(4, 1)] = -t * cd
+    ops[(5, 1)] = -t * cdd
+    wq = [((0, 0), 2), ((-1, 0), 1), ((1, 0), 1), ((0, -1), 1), ((0, 1), 1)]
+    return _assemble(ops, 4, 6, HUBBARD_PHYS, wq, position, start=0, done=1)
+
+
+# Spin-1/2 basis (|up>, |dn>), S+ = [[0,1],[0,0]] (operators.rs:31-58); U(1) charge 2*Sz.
+S_PLUS = _op([[0, 1], [0, 0]])
+S_MINUS = _op([[0, 0], [1, 0]])
+S_Z = _op([[0.5, 0], [0, -0.5]])
+S_ID = np.eye(2)
+SPIN_PHYS = [((1, 0), 1), ((-1, 0), 1)]
+
+
+def heisenberg_w(J=1.0, position="bulk"):
+    """Heisenberg-chain MPO site tensor, bond dim 5, U(1) 2*Sz.
+
+    H = J sum_i (S+_i S-_{i+1}/2 + S-_i S+_{i+1}/2 + Sz_i Sz_{i+1}).
+    States: 0 start, 1 done, 2 after Sz (all charge 0), 3 after S+ (-2), 4 after S- (+2).
+    """
+    ops = {(0, 0): S_ID, (1, 1): S_ID, (0, 2): S_Z, (0, 3): S_PLUS, (0, 4): S_MINUS,
+           (2, 1): J * S_Z, (3, 1): 0.5 * J * S_MINUS, (4, 1): 0.5 * J * S_PLUS}
+    wq = [((0, 0), 3), ((-2, 0), 1), ((2, 0), 1)]
+    return _assemble(ops, 2, 5, SPIN_PHYS, wq, position, start=0, done=1)
+
+
+def _assemble(ops, d, w, phys, wq, position, start, done):
+    full = np.zeros((d, d, w, w))
+    for (a, b), O in ops.items():
+        # W[s_in][s_out][a][b] = <s_out| O |s_in> = O[s_out, s_in]
+        full[:, :, a, b] = O.T
+    if position == "bulk":
+        return MpoSite(phys, wq, wq, full)
+    if position == "left":
+        return MpoSite(phys, [((0, 0), 1)], wq, full[:, :, start:start + 1, :].copy())
+    if position == "right":
+        return MpoSite(phys, wq, [((0, 0), 1)], full[:, :, :, done:done + 1].copy())
+    if position == "single":
+        return MpoSite(phys, [((0, 0), 1)], [((0, 0), 1)], full[:, :, start:start + 1, done:done + 1].copy())
+    raise ValueError(position)
+
+
+# ---------------------------------------------------------------- synthetic bond
+def synthetic_uu_bond(total=1024, rmax=6, l1max=9, width=2.6):
+    """U(1)xU(1) bond of the single H_eff*psi microbenchmark.
+
+    Charges (a, b) with |a|,|b| <= rmax and |a|+|b| <= l1max (145 sectors for
+    the defaults); dims proportional to exp(-(a^2+b^2)/(2 width^2)), every
+    sector at least 1, rounded to `total` by largest remainder.
+    """
+    qs = [(a, b) for a in range(-rmax, rmax + 1) for b in range(-rmax, rmax + 1) if abs(a) + abs(b) <= l1max]
+    w = np.array([math.exp(-(a * a + b * b) / (2 * width * width)) for a, b in qs])
+    n = len(qs)
+    raw = (total - n) * w / w.sum()  # one state reserved per sector
+    base = np.floor(raw).astype(np.int64)
+    rem = int(total - n - base.sum())
+    frac = raw - base
+    order = sorted(range(n), key=lambda i: (-frac[i], qs[i]))
+    for i in order[:rem]:
+        base[i] += 1
+    dims = base + 1
+    assert int(dims.sum()) == total
+    return [(q, int(d)) for q, d in zip(qs, dims)]
+
+
+def env_table(sym, bond, wq):
+    """Block table of an environment (bond, w, bond), q(bra) = q(ket) - q(w)."""
+    return block_table(sym, [bond, wq, bond], [DIR_IN, DIR_IN, DIR_OUT])
+
+
+def c2_problem(seed=2024, total=1024):
+    """Inputs of the single H_eff*psi microbenchmark (U(1)xU(1) Hubbard, d=4, w=6).
+
+    Left bond: synthetic_uu_bond(total); right bond: the same shifted by (1,1).
+    L, R and psi block entries are N(0,1) from numpy's PCG64 seeded with
+    `seed`, drawn in FlatBlockStorage order (L, then R, then psi).
+    """
+    bl = synthetic_uu_bond(total)
+    br = [((a + 1, b + 1), d) for (a, b), d in bl]
+    W = hubbard_w(1.0, 4.0)
+    _, _, _, l_tot = env_table(SYM_U1U1, bl, W.wl)
+    _, _, _, r_tot = env_table(SYM_U1U1, br, W.wr)
+    _, _, _, p_tot = block_table(SYM_U1U1, [W.phys, W.phys, bl, br], [DIR_OUT, DIR_OUT, DIR_OUT, DIR_IN])
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal(l_tot)
+    R = rng.standard_normal(r_tot)
+    x = rng.standard_normal(p_tot)
+    return {"bond_l": bl, "bond_r": br, "W": W, "L": L, "R": R, "x": x, "sym": SYM_U1U1}
