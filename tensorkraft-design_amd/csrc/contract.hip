@@ -39,6 +39,40 @@ void ContractList::finalize() {
   std::stable_sort(items.begin(), items.end(), [](const GItem& a, const GItem& b) { return a.cost > b.cost; });
 }
 
+void ContractList::validate(int64_t a_elems, int64_t b_elems, int64_t c_elems) const {
+  const bool ta = flags & kTransA, tb = flags & kTransB, tc = flags & kTransC;
+  auto span = [](int64_t off, int64_t rows, int64_t cols, int64_t ld) {
+    // last element touched by a rows x cols row-major slice with leading dim ld
+    return off + (rows - 1) * ld + cols;
+  };
+  for (const GItem& it : items) {
+    if (it.out < 0 || it.out >= int32_t(outs.size())) throw std::out_of_range("contract: item out of range");
+  }
+  for (const GOut& o : outs) {
+    if (o.m <= 0 || o.n <= 0) throw std::out_of_range("contract: empty output block");
+    int64_t end = tc ? span(o.c_off, o.n, o.m, o.ldc) : span(o.c_off, o.m, o.n, o.ldc);
+    if (o.c_off < 0 || end > c_elems || (tc ? o.ldc < o.m : o.ldc < o.n))
+      throw std::out_of_range("contract: output slice outside its buffer");
+    if (o.t_begin < 0 || o.t_end > int32_t(terms.size()) || o.t_begin > o.t_end)
+      throw std::out_of_range("contract: bad term range");
+    for (int t = o.t_begin; t < o.t_end; ++t) {
+      const GTerm& tm = terms[t];
+      if (tm.k <= 0) throw std::out_of_range("contract: empty inner dimension");
+      int64_t bend = tb ? span(tm.b_off, o.n, tm.k, tm.ldb) : span(tm.b_off, tm.k, o.n, tm.ldb);
+      if (tm.b_off < 0 || bend > b_elems || (tb ? tm.ldb < tm.k : tm.ldb < o.n))
+        throw std::out_of_range("contract: B slice outside its buffer");
+      if (tm.s_begin < 0 || tm.s_end > int32_t(srcs.size()) || tm.s_begin >= tm.s_end)
+        throw std::out_of_range("contract: bad source range");
+      for (int s = tm.s_begin; s < tm.s_end; ++s) {
+        const GSrc& sr = srcs[s];
+        int64_t aend = ta ? span(sr.a_off, tm.k, o.m, sr.lda) : span(sr.a_off, o.m, tm.k, sr.lda);
+        if (sr.a_off < 0 || aend > a_elems || (ta ? sr.lda < o.m : sr.lda < tm.k))
+          throw std::out_of_range("contract: A slice outside its buffer");
+      }
+    }
+  }
+}
+
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));

@@ -73,6 +73,9 @@ struct ContractList {
   // Tile every output into 32x32 wavefront items, longest first (LPT order,
   // tk-linalg/src/tasks.rs:172-174) with a stable tie-break on position.
   void finalize();
+  // Checks every operand slice against the element counts of the three
+  // buffers (throws std::out_of_range); run before a list is first launched.
+  void validate(int64_t a_elems, int64_t b_elems, int64_t c_elems) const;
   void clear() {
     outs.clear(); terms.clear(); srcs.clear(); items.clear();
     flops = 0;

@@ -199,6 +199,14 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
     h1_.add_src(r.a_off, r.lda, 1.0);
   }
   h1_.finalize();
+  h1_.validate(l_total_, psi_.total, t1_total);
+  h3_.validate(t1_total, r_total_, psi_.total);
+  for (const DiagSlice& s : dslices_)
+    for (int t = s.t_begin; t < s.t_end; ++t) {
+      const DiagTerm& d = dterms_[t];
+      if (d.l_off + int64_t(s.m - 1) * d.l_step >= l_total_ || d.r_off + int64_t(s.n - 1) * d.r_step >= r_total_)
+        throw std::out_of_range("heff diag term outside the environments");
+    }
 
   d1_.upload(ctx_, h1_);
   d3_.upload(ctx_, h3_);
