@@ -84,12 +84,16 @@ template <> struct Mfma<double> {
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // accumulator element r of lane group g holds row g + 4r
+  static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
 };
 template <> struct Mfma<float> {
   using acc_t = v4f;
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // accumulator element r of lane group g holds row 4g + r
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 template <typename TI, typename TO, bool TA, bool TB, bool TC>
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
     };
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = item.m0 + 4 * lk + r;
+      const int row = item.m0 + M::row(lk, r);
       const int col = item.n0 + li;
       put(row, col, c00[r]);
       put(row, col + 16, c01[r]);

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -202,9 +203,16 @@ class Context:
         h = C.c_void_p()
         _check(self.L.tk_ctx_create(device, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()
+
+    def adopt(self, child):
+        """Register an object that must be closed before the context."""
+        self._children.add(child)
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch.close()
             self.L.tk_ctx_destroy(self.h)
             self.h = None
 
@@ -272,6 +280,7 @@ class HeffPlan:
         _check(ctx.L.tk_heff_plan_create(ctx.h, PREC[prec], sym, C.byref(el), C.byref(er), C.byref(mi),
                                          C.byref(mj), C.byref(h)), ctx.h)
         self.h = h
+        ctx.adopt(self)
 
     def _mpo(self, m):
         p, a, b = QIndexArg(m.phys), QIndexArg(m.wl), QIndexArg(m.wr)
@@ -280,9 +289,9 @@ class HeffPlan:
         return tk_mpo_site(p.c, a.c, b.c, w.ctypes.data_as(C.POINTER(C.c_double)))
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.tk_heff_plan_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
