@@ -98,20 +98,117 @@ def bench_heff(args):
     return rec
 
 
+def warmup_schedule(d0, d_max, n):
+    """BondDimensionSchedule::warmup (tk-dmrg/src/truncation/mod.rs:137-151):
+    round(d0 * r^i) half away from zero, r = (d_max/d0)^(1/n), clamped."""
+    import math
+
+    r = (d_max / d0) ** (1.0 / n)
+    out = []
+    for i in range(n + 1):
+        v = d0 * r ** i
+        out.append(max(1, min(d_max, int(math.floor(v + 0.5)))))
+    return out
+
+
+def run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, n_sweeps, prec, tol, log):
+    d = tk_hip.Dmrg(ctx, mpo, sym, prec=prec, schedule=sched, davidson_tol=tol, svd_cutoff=1e-12)
+    d.set_state(bonds, data)
+    t0 = time.perf_counter()
+    d.prepare()
+    log(f"[{prec}] prepared in {time.perf_counter() - t0:.2f}s")
+    out = []
+    for k in range(n_sweeps):
+        s = d.sweep()
+        out.append(s)
+        log(f"[{prec}] sweep {k + 1}/{n_sweeps} D={sched[min(k, len(sched) - 1)]} E={s['energy']:.12f} "
+            f"t={s['seconds']:.3f}s (plan {s['t_plan']:.2f} eig {s['t_eig']:.2f} svd {s['t_svd']:.2f} "
+            f"env {s['t_env']:.2f}) matvecs={s['matvecs']} fp64={s['matvecs_fp64']} maxD={s['max_bond_dim']} "
+            f"trunc={s['max_truncation_error']:.2e}")
+    d.close()
+    return out
+
+
+def bench_sweep(args):
+    import tk_hip
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    n = args.sites
+    sched = warmup_schedule(args.d0, args.dmax, args.ramp)
+    n_sweeps = args.warmup + args.steps
+    mpo = tk_hip.chain_mpo("hubbard", n, t=1.0, U=4.0)
+    sym = tk_hip.SYM_U1U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (n // 2, n // 2), sched[0], seed=0)
+    ctx = tk_hip.Context(0)
+    res = {}
+    for prec in ("fp64", args.mixed):
+        res[prec] = run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, n_sweeps, prec, args.tol, log)
+    timed64 = res["fp64"][args.warmup:]
+    timedlo = res[args.mixed][args.warmup:]
+    t64 = sum(s["seconds"] for s in timed64) / len(timed64)
+    tlo = sum(s["seconds"] for s in timedlo) / len(timedlo)
+    de = abs(res[args.mixed][-1]["energy"] - res["fp64"][-1]["energy"])
+    rec = {
+        "metric": METRIC,
+        "value": round(tlo, 4),
+        "unit": "s/sweep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * tlo, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32-mfma+f64-refine", "bf16x3": "bf16x3-f32acc+f64-refine"}[args.mixed],
+        "data": "synthetic",
+        "config": {"workload": "dmrg_sweep_hubbard", "sites": n, "t": 1.0, "U": 4.0, "filling": "half, Sz=0",
+                   "schedule": sched, "davidson_tol": args.tol, "timed_sweep_bond_dim": sched[-1],
+                   "mixed_precision": args.mixed},
+        "fp64_seconds_per_sweep": round(t64, 4),
+        "speedup_mixed_vs_fp64": round(t64 / tlo, 3),
+        "energy_fp64": res["fp64"][-1]["energy"],
+        "energy_mixed": res[args.mixed][-1]["energy"],
+        "abs_delta_e": de,
+        "delta_e_ok": de <= 1e-10,
+        "max_truncation_error": res[args.mixed][-1]["max_truncation_error"],
+        "phases_mixed": {k: round(sum(s[k] for s in timedlo) / len(timedlo), 4) for k in ("t_plan", "t_eig", "t_svd", "t_env")},
+        "phases_fp64": {k: round(sum(s[k] for s in timed64) / len(timed64), 4) for k in ("t_plan", "t_eig", "t_svd", "t_env")},
+        "matvecs_mixed": sum(s["matvecs"] for s in timedlo), "matvecs_mixed_fp64": sum(s["matvecs_fp64"] for s in timedlo),
+        "matvecs_fp64": sum(s["matvecs"] for s in timed64),
+    }
+    ctx.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="heff", choices=["heff"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "heff"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--sites", type=int, default=64)
+    ap.add_argument("--d0", type=int, default=256)
+    ap.add_argument("--dmax", type=int, default=2048)
+    ap.add_argument("--ramp", type=int, default=7)
+    ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
-    rec = bench_heff(args)
+    if args.workload == "heff":
+        args.steps = 200 if args.steps is None else args.steps
+        args.warmup = 20 if args.warmup is None else args.warmup
+        rec = bench_heff(args)
+    else:
+        args.steps = 1 if args.steps is None else args.steps
+        args.warmup = args.ramp if args.warmup is None else args.warmup
+        rec = bench_sweep(args)
     print(json.dumps(rec))
 
 

@@ -144,6 +144,61 @@ tk_status tk_heff_stage_stats(tk_heff_plan* plan, double* out);
 tk_status tk_heff_time(tk_heff_plan* plan, const double* x_dev, double* y_dev, int32_t reps, double* us_apply,
                        double* us_stage1, double* us_stage3);
 
+/* ---- two-site DMRG ------------------------------------------------------ */
+typedef struct tk_dmrg tk_dmrg;
+
+/* DMRGConfig + TruncationConfig + DavidsonSolver (tk-dmrg/src/sweep/mod.rs:124-162,
+ * truncation/mod.rs:12-30, eigensolver/mod.rs:300-323). */
+typedef struct {
+  int32_t prec;                 /* TK_PREC_*: precision of the H_eff contractions */
+  int32_t sym;                  /* TK_SYM_* */
+  int32_t n_sites;
+  int32_t n_schedule;
+  const int64_t* bond_schedule; /* max bond dimension per sweep (BondDimensionSchedule, truncation/mod.rs:131-172) */
+  double svd_cutoff;            /* keep sigma >= cutoff * sigma_max */
+  int64_t min_bond_dim;
+  double degeneracy_tol;        /* no cut inside |s_k^2 - s_{k+1}^2| <= tol * s_max^2 */
+  double davidson_tol;          /* absolute residual 2-norm */
+  int32_t max_subspace;
+  int32_t max_iter;
+  int32_t precondition;         /* diagonal preconditioner on/off */
+  double switch_rel;            /* low-precision -> FP64 refinement switch */
+} tk_dmrg_config;
+
+typedef struct {
+  double energy;                /* last step's eigenvalue (sweep/mod.rs:301) */
+  double max_truncation_error;
+  double seconds;               /* wall time of the sweep, stream-synchronised */
+  double t_plan, t_eig, t_svd, t_env;
+  int64_t max_bond_dim;
+  int32_t matvecs, matvecs_fp64, steps, unconverged;
+} tk_sweep_stats;
+
+typedef struct {
+  int32_t site, left_to_right, iterations, matvecs, matvecs_fp64, converged;
+  double energy, truncation_error, residual;
+  int64_t bond_dim;
+} tk_step_stats;
+
+/* DMRGEngine::new (sweep/mod.rs:234-265) with the MPO given site by site
+ * (mpo[i].wr must equal mpo[i+1].wl; the ends have dimension 1). */
+tk_status tk_dmrg_create(tk_ctx* ctx, const tk_dmrg_config* cfg, const tk_mpo_site* mpo, tk_dmrg** out);
+tk_status tk_dmrg_destroy(tk_dmrg* d);
+/* Initial MPS: n_sites+1 bond bases and per-site data (host, block-table order
+ * of (s, a_L, a_R), dirs [Out, Out, In]).  Bond 0 must be {0:1}, bond N the
+ * total charge with dimension 1. */
+tk_status tk_dmrg_set_state(tk_dmrg* d, const tk_qindex* bonds, const double* const* site_data);
+/* Right-canonicalise and build the right environments (Environments::build_from_scratch,
+ * environments/mod.rs:117-155, on a canonical MPS). */
+tk_status tk_dmrg_prepare(tk_dmrg* d);
+/* One left-to-right + right-to-left sweep (SweepSchedule::standard, sweep/mod.rs:59-81). */
+tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out);
+/* Per-step statistics of the last sweep (StepResult, sweep/mod.rs:35-50). */
+tk_status tk_dmrg_steps(tk_dmrg* d, int32_t capacity, int32_t* n, tk_step_stats* out);
+/* Current bond basis (sorted) and site tensor data. */
+tk_status tk_dmrg_bond(tk_dmrg* d, int32_t bond, int32_t capacity, int32_t* n_sectors, int32_t* qn, int64_t* dims);
+tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, double* data);
+
 #ifdef __cplusplus
 }
 #endif

@@ -243,3 +243,114 @@ def qindex_layout(sectors):
                               C.byref(tot)):
         raise RuntimeError("or_qindex_layout failed")
     return [((int(qn[2 * i]), int(qn[2 * i + 1])), int(dims[i])) for i in range(n)], offs, tot.value
+
+
+def jacobi_small(h):
+    """symmetric_eigen_small (eigensolver/mod.rs:532-601): (eigenvalues, eigenvectors as columns)."""
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    n = h.shape[0]
+    ev = np.zeros(n)
+    vv = np.zeros(n * n)
+    lib().or_jacobi_small(n, _p(h, C.c_double), _p(ev, C.c_double), _p(vv, C.c_double))
+    return ev, vv.reshape(n, n)
+
+
+def davidson_dense(h, diag=None, x0=None, tol=1e-10, max_subspace=60):
+    """Davidson on a dense symmetric matrix; x0=None uses the reference start vector."""
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    n = h.shape[0]
+    d = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
+    x = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    ev = C.c_double(0)
+    out = np.zeros(n)
+    nmv = C.c_int32(0)
+    lib().or_davidson_dense(n, _p(h, C.c_double), None if d is None else _p(d, C.c_double),
+                            None if x is None else _p(x, C.c_double), C.c_double(tol), max_subspace, C.byref(ev),
+                            _p(out, C.c_double), C.byref(nmv))
+    return ev.value, out, nmv.value
+
+
+def svd(a):
+    """One-sided Jacobi SVD: (U m x r, s, V n x r), s descending, r = min(m, n)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    m, n = a.shape
+    r = min(m, n)
+    u = np.zeros(m * r)
+    s = np.zeros(r)
+    v = np.zeros(n * r)
+    lib().or_svd(C.c_int64(m), C.c_int64(n), _p(a, C.c_double), _p(u, C.c_double), _p(s, C.c_double),
+                 _p(v, C.c_double))
+    return u.reshape(m, r), s, v.reshape(n, r)
+
+
+class Dmrg:
+    """The oracle's two-site DMRG (CPU, FP64) on the same MPO and initial state."""
+
+    def __init__(self, mpo_sites, sym, schedule=(64,), svd_cutoff=1e-10, davidson_tol=1e-10, max_subspace=60,
+                 max_iter=1000, precondition=True, degeneracy_tol=1e-12, nthreads=0):
+        L = lib()
+        L.or_dmrg_create.restype = C.c_void_p
+        L.or_dmrg_destroy.argtypes = [C.c_void_p]
+        L.or_dmrg_set_state.argtypes = [C.c_void_p, C.POINTER(or_qindex), C.POINTER(C.POINTER(C.c_double))]
+        L.or_dmrg_prepare.argtypes = [C.c_void_p]
+        L.or_dmrg_sweep.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.or_dmrg_bond.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int64)]
+        self.n = len(mpo_sites)
+        self._keep = []
+        ph, wl, wr, ws = [], [], [], []
+        for m in mpo_sites:
+            p, a, b = Q(m.phys), Q(m.wl), Q(m.wr)
+            w = np.ascontiguousarray(m.w, dtype=np.float64)
+            self._keep += [p, a, b, w]
+            ph.append(p.c)
+            wl.append(a.c)
+            wr.append(b.c)
+            ws.append(w.ctypes.data_as(C.POINTER(C.c_double)))
+        sched = np.array(schedule, dtype=np.int64)
+        self._keep.append(sched)
+        L.or_dmrg_create.argtypes = [C.c_int32, C.c_int32, C.POINTER(or_qindex), C.POINTER(or_qindex),
+                                     C.POINTER(or_qindex), C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64),
+                                     C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_int32]
+        n = self.n
+        self.h = L.or_dmrg_create(sym, n, (or_qindex * n)(*ph), (or_qindex * n)(*wl), (or_qindex * n)(*wr),
+                                  (C.POINTER(C.c_double) * n)(*ws), _p(sched, C.c_int64), len(sched), svd_cutoff,
+                                  degeneracy_tol, davidson_tol, max_subspace, max_iter, 1 if precondition else 0,
+                                  int(nthreads))
+        if not self.h:
+            raise RuntimeError("or_dmrg_create failed")
+
+    def close(self):
+        if self.h:
+            lib().or_dmrg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_state(self, bonds, data):
+        qs = [Q(b) for b in bonds]
+        ds = [np.ascontiguousarray(d, dtype=np.float64) for d in data]
+        ptrs = (C.POINTER(C.c_double) * len(ds))(*[_p(d, C.c_double) for d in ds])
+        if lib().or_dmrg_set_state(self.h, (or_qindex * len(qs))(*[q.c for q in qs]), ptrs):
+            raise RuntimeError("or_dmrg_set_state failed")
+
+    def prepare(self):
+        if lib().or_dmrg_prepare(self.h):
+            raise RuntimeError("or_dmrg_prepare failed")
+
+    def sweep(self):
+        out = np.zeros(8)
+        if lib().or_dmrg_sweep(self.h, _p(out, C.c_double)):
+            raise RuntimeError("or_dmrg_sweep failed")
+        keys = ["energy", "max_truncation_error", "max_bond_dim", "seconds", "matvecs", "t_matvec", "t_svd", "t_env"]
+        return dict(zip(keys, out.tolist()))
+
+    def bond(self, i):
+        n = C.c_int32(0)
+        qn = np.zeros(8192, dtype=np.int32)
+        dims = np.zeros(4096, dtype=np.int64)
+        if lib().or_dmrg_bond(self.h, i, 4096, C.byref(n), _p(qn, C.c_int32), _p(dims, C.c_int64)):
+            raise RuntimeError("or_dmrg_bond failed")
+        return [((int(qn[2 * k]), int(qn[2 * k + 1])), int(dims[k])) for k in range(n.value)]

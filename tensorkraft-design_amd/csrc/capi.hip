@@ -6,6 +6,7 @@
 #include <string>
 
 #include "device.hpp"
+#include "dmrg.hpp"
 #include "heff.hpp"
 #include "sym.hpp"
 
@@ -15,6 +16,10 @@ struct tk_ctx {
 struct tk_heff_plan {
   tk_ctx* ctx;
   std::unique_ptr<tk::HeffPlan> p;
+};
+struct tk_dmrg {
+  tk_ctx* ctx;
+  std::unique_ptr<tk::DmrgEngine> e;
 };
 
 namespace {
@@ -383,6 +388,134 @@ tk_status tk_heff_time(tk_heff_plan* plan, const double* x, double* y, int32_t r
     if (us_apply) *us_apply = 1000.0 * ms_all / reps;
     if (us_stage1) *us_stage1 = 1000.0 * s1 / reps;
     if (us_stage3) *us_stage3 = 1000.0 * s3 / reps;
+  });
+}
+
+tk_status tk_dmrg_create(tk_ctx* ctx, const tk_dmrg_config* cfg, const tk_mpo_site* mpo, tk_dmrg** out) {
+  return guard(ctx, [&] {
+    if (!ctx || !cfg || !mpo || !out) throw tk::ArgError("null argument");
+    if (cfg->n_sites < 2 || cfg->n_schedule < 1 || !cfg->bond_schedule) throw tk::ArgError("bad DMRG config");
+    tk::DmrgConfig c;
+    c.sym = to_sym(cfg->sym);
+    c.prec = to_prec(cfg->prec);
+    for (int i = 0; i < cfg->n_sites; ++i) c.mpo.push_back(to_mpo(c.sym, &mpo[i]));
+    c.bond_schedule.assign(cfg->bond_schedule, cfg->bond_schedule + cfg->n_schedule);
+    for (int64_t d : c.bond_schedule)
+      if (d < 1) throw tk::ArgError("bond dimension must be positive");
+    c.svd_cutoff = cfg->svd_cutoff;
+    c.min_bond_dim = std::max<int64_t>(1, cfg->min_bond_dim);
+    c.degeneracy_tol = cfg->degeneracy_tol;
+    c.davidson.tol = cfg->davidson_tol;
+    c.davidson.max_subspace = cfg->max_subspace > 0 ? cfg->max_subspace : 60;
+    c.davidson.max_iter = cfg->max_iter > 0 ? cfg->max_iter : 1000;
+    c.davidson.precondition = cfg->precondition != 0;
+    c.davidson.switch_rel = cfg->switch_rel > 0 ? cfg->switch_rel : 1e-6;
+    auto d = std::make_unique<tk_dmrg>();
+    d->ctx = ctx;
+    d->e = std::make_unique<tk::DmrgEngine>(ctx->c.get(), std::move(c));
+    *out = d.release();
+  });
+}
+
+tk_status tk_dmrg_destroy(tk_dmrg* d) {
+  return guard(nullptr, [&] {
+    if (d) d->ctx->c->sync();
+    delete d;
+  });
+}
+
+tk_status tk_dmrg_set_state(tk_dmrg* d, const tk_qindex* bonds, const double* const* site_data) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d || !bonds || !site_data) throw tk::ArgError("null argument");
+    const int n = d->e->n_sites();
+    std::vector<tk::QIndex> b;
+    for (int i = 0; i <= n; ++i) b.push_back(to_qindex(d->e->sym(), bonds[i]));
+    if (b.front().total != 1 || b.back().total != 1) throw tk::ArgError("boundary bonds must have dimension 1");
+    std::vector<std::vector<double>> data(n);
+    for (int i = 0; i < n; ++i) {
+      tk::BlockTable t = tk::make_block_table(d->e->sym(), {d->e->phys(i), b[i], b[i + 1]},
+                                              {tk::Dir::Out, tk::Dir::Out, tk::Dir::In}, tk::QN{});
+      if (!site_data[i] && t.total) throw tk::ArgError("null site data");
+      data[i].assign(site_data[i], site_data[i] + t.total);
+    }
+    d->e->set_state(b, data);
+  });
+}
+
+tk_status tk_dmrg_prepare(tk_dmrg* d) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d) throw tk::ArgError("null argument");
+    d->e->prepare();
+  });
+}
+
+tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d) throw tk::ArgError("null argument");
+    tk::SweepStats s = d->e->sweep();
+    if (out) {
+      out->energy = s.energy;
+      out->max_truncation_error = s.max_truncation_error;
+      out->seconds = s.seconds;
+      out->t_plan = s.t_plan;
+      out->t_eig = s.t_eig;
+      out->t_svd = s.t_svd;
+      out->t_env = s.t_env;
+      out->max_bond_dim = s.max_bond_dim;
+      out->matvecs = s.matvecs;
+      out->matvecs_fp64 = s.matvecs_fp64;
+      out->steps = s.steps;
+      out->unconverged = s.unconverged;
+    }
+  });
+}
+
+tk_status tk_dmrg_steps(tk_dmrg* d, int32_t capacity, int32_t* n, tk_step_stats* out) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d || !n) throw tk::ArgError("null argument");
+    const auto& st = d->e->last_steps();
+    *n = int32_t(st.size());
+    if (int32_t(st.size()) > capacity) throw tk::ArgError("capacity");
+    for (size_t k = 0; k < st.size(); ++k) {
+      out[k].site = st[k].site;
+      out[k].left_to_right = st[k].left_to_right;
+      out[k].iterations = st[k].iterations;
+      out[k].matvecs = st[k].matvecs;
+      out[k].matvecs_fp64 = st[k].matvecs_fp64;
+      out[k].converged = st[k].converged;
+      out[k].energy = st[k].energy;
+      out[k].truncation_error = st[k].truncation_error;
+      out[k].residual = st[k].residual;
+      out[k].bond_dim = st[k].bond_dim;
+    }
+  });
+}
+
+tk_status tk_dmrg_bond(tk_dmrg* d, int32_t bond, int32_t capacity, int32_t* n_sectors, int32_t* qn, int64_t* dims) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d || !n_sectors) throw tk::ArgError("null argument");
+    if (bond < 0 || bond > d->e->n_sites()) throw tk::ArgError("bond out of range");
+    const tk::QIndex& q = d->e->bonds()[bond];
+    *n_sectors = q.n();
+    if (q.n() > capacity) throw tk::ArgError("capacity");
+    for (int i = 0; i < q.n(); ++i) {
+      if (qn) {
+        qn[2 * i] = q.q[i].a;
+        qn[2 * i + 1] = q.q[i].b;
+      }
+      if (dims) dims[i] = q.dim[i];
+    }
+  });
+}
+
+tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, double* data) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d || !n) throw tk::ArgError("null argument");
+    if (site < 0 || site >= d->e->n_sites()) throw tk::ArgError("site out of range");
+    std::vector<double> h = d->e->site_data(site);
+    *n = int64_t(h.size());
+    if (int64_t(h.size()) > capacity) throw tk::ArgError("capacity");
+    if (data) std::copy(h.begin(), h.end(), data);
   });
 }
 

@@ -1,0 +1,218 @@
+// Davidson eigensolver with device-resident FP64 Krylov vectors (see davidson.hpp).
+#include "davidson.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "eig.hpp"
+#include "kernels.hpp"
+
+namespace tk {
+namespace {
+
+// P (n x n) <- symmetric matrix whose lower triangle is stored row-wise in H (ld).
+__global__ void pack_sym_kernel(const double* __restrict__ H, int ld, int n, double* __restrict__ P) {
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int i = e / n, j = e - i * n;
+    P[e] = j <= i ? H[i * ld + j] : H[j * ld + i];
+  }
+}
+// lowest Ritz pair of an ascending eigen-decomposition: c = column 0, theta = W[0]
+__global__ void ritz_kernel(const double* __restrict__ W, const double* __restrict__ E, int n,
+                            double* __restrict__ c, double* __restrict__ theta) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = E[i * n];
+  if (threadIdx.x == 0) *theta = W[0];
+}
+__global__ void residual_dev_kernel(const double* __restrict__ ay, const double* __restrict__ y,
+                                    const double* __restrict__ theta, double* __restrict__ r, int64_t n) {
+  const double th = *theta;
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    r[j] = ay[j] - th * y[j];
+}
+__global__ void precond_dev_kernel(const double* __restrict__ r, const double* __restrict__ d,
+                                   const double* __restrict__ theta, double* __restrict__ t, int64_t n) {
+  const double th = *theta;
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x) {
+    const double den = th - d[j];
+    t[j] = fabs(den) > 1e-14 ? r[j] / den : r[j];
+  }
+}
+__global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    b[j] = a[j];
+}
+
+int grid1(Context* ctx, int64_t n) {
+  return int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, int64_t(ctx->n_cu) * 16)));
+}
+
+struct Workspace {
+  DBuf<double> V, AV, H, P, Wv, E, Vws, c, h, scal, y, ay, r, diag;
+  DBuf<EigJob> job;
+};
+
+}  // namespace
+
+DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user, const double* diag_dev,
+                                bool low_precision, double* x, const DavidsonOpts& opts) {
+  DavidsonResult res;
+  if (n <= 0) throw ArgError("davidson: empty problem");
+  const int m = int(std::max<int64_t>(1, std::min<int64_t>({int64_t(opts.max_subspace), n, 64})));
+  hipStream_t s = ctx->stream;
+  Workspace w;
+  w.V.reset(ctx, size_t(m) * n);
+  w.AV.reset(ctx, size_t(m) * n);
+  w.H.reset(ctx, size_t(m) * m);
+  w.P.reset(ctx, size_t(m) * m);
+  w.Wv.reset(ctx, m);
+  w.E.reset(ctx, size_t(m) * m);
+  w.Vws.reset(ctx, size_t(m) * m);
+  w.c.reset(ctx, m);
+  w.h.reset(ctx, m);
+  w.scal.reset(ctx, 8);
+  w.y.reset(ctx, n);
+  w.ay.reset(ctx, n);
+  w.r.reset(ctx, n);
+  w.job.reset(ctx, 1);
+  double* V = w.V.get();
+  double* AV = w.AV.get();
+  double* H = w.H.get();
+  double* theta = w.scal.get();
+  double* rn2 = w.scal.get() + 1;
+  double* tn2 = w.scal.get() + 2;
+  bool low = low_precision;
+
+  auto matvec = [&](const double* in, double* out) {
+    mv(in, out, low ? 0 : 1, user);
+    ++res.matvecs;
+    if (!low) ++res.matvecs_fp64;
+  };
+  // basis <- {v / |v|}, AV0 = A v0, H00
+  auto start_from = [&](const double* v) {
+    if (v != V) hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, v, V, n);
+    dot_many(ctx, V, n, 1, V, n, tn2);
+    scale_by_inv_sqrt(ctx, V, tn2, n);
+    matvec(V, AV);
+    dot_many(ctx, V, n, 1, AV, n, H);
+  };
+  start_from(x);
+  int nsub = 1;
+  double best = INFINITY;
+  int stall = 0;
+  double host[3] = {0, 0, 0};
+  for (int it = 0; it < opts.max_iter; ++it) {
+    // Ritz pair of the projected matrix (Jacobi on the device, ascending)
+    hipLaunchKernelGGL(pack_sym_kernel, dim3(1), dim3(256), 0, s, H, m, nsub, w.P.get());
+    EigJob jb{0, 0, 0, nsub, 0};
+    w.job.upload(&jb, 1);
+    launch_eig(ctx, w.job.get(), 1, w.P.get(), w.Vws.get(), w.Wv.get(), w.E.get(), false);
+    hipLaunchKernelGGL(ritz_kernel, dim3(1), dim3(64), 0, s, w.Wv.get(), w.E.get(), nsub, w.c.get(), theta);
+    combine(ctx, V, n, nsub, w.c.get(), 0.0, w.y.get(), n);
+    combine(ctx, AV, n, nsub, w.c.get(), 0.0, w.ay.get(), n);
+    hipLaunchKernelGGL(residual_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), w.y.get(), theta,
+                       w.r.get(), n);
+    dot_many(ctx, w.r.get(), n, 1, w.r.get(), n, rn2);
+    TK_HIP(hipMemcpyAsync(host, w.scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    TK_HIP(hipStreamSynchronize(s));
+    TK_HIP(hipGetLastError());
+    res.iterations = it + 1;
+    res.eigenvalue = host[0];
+    res.residual = std::sqrt(std::max(0.0, host[1]));
+    if (low) {
+      if (res.residual < best * 0.7) {
+        best = res.residual;
+        stall = 0;
+      } else {
+        ++stall;
+      }
+      if (res.residual < opts.switch_rel * std::max(1.0, std::fabs(res.eigenvalue)) || stall >= 3 ||
+          res.residual < opts.tol) {
+        low = false;  // FP64 residual refinement from the current Ritz vector
+        start_from(w.y.get());
+        nsub = 1;
+        continue;
+      }
+    } else if (res.residual < opts.tol) {
+      res.converged = true;
+      break;
+    }
+    if (nsub >= n) break;  // the basis spans the space: exact
+    // correction vector in the next basis slot (or slot 1 after a restart)
+    if (nsub == m) {
+      // restart from the Ritz vector; A y is carried over
+      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.y.get(), V, n);
+      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), AV, n);
+      dot_many(ctx, V, n, 1, AV, n, H);
+      nsub = 1;
+    }
+    double* slot = V + int64_t(nsub) * n;
+    if (opts.precondition && diag_dev)
+      hipLaunchKernelGGL(precond_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), diag_dev, theta, slot, n);
+    else
+      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), slot, n);
+    for (int pass = 0; pass < 2; ++pass) {
+      dot_many(ctx, V, n, nsub, slot, n, w.h.get());
+      subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n);
+    }
+    dot_many(ctx, slot, n, 1, slot, n, tn2);
+    TK_HIP(hipMemcpyAsync(host + 2, tn2, sizeof(double), hipMemcpyDeviceToHost, s));
+    TK_HIP(hipStreamSynchronize(s));
+    if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
+      // the preconditioned correction collapsed into the basis: raw residual
+      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), slot, n);
+      for (int pass = 0; pass < 2; ++pass) {
+        dot_many(ctx, V, n, nsub, slot, n, w.h.get());
+        subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n);
+      }
+      dot_many(ctx, slot, n, 1, slot, n, tn2);
+      TK_HIP(hipMemcpyAsync(host + 2, tn2, sizeof(double), hipMemcpyDeviceToHost, s));
+      TK_HIP(hipStreamSynchronize(s));
+      if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
+        if (low) {  // stagnated in low precision: refine in FP64
+          low = false;
+          start_from(w.y.get());
+          nsub = 1;
+          continue;
+        }
+        break;
+      }
+    }
+    scale_by_inv_sqrt(ctx, slot, tn2, n);
+    matvec(slot, AV + int64_t(nsub) * n);
+    dot_many(ctx, V, n, nsub + 1, AV + int64_t(nsub) * n, n, H + int64_t(nsub) * m);
+    ++nsub;
+  }
+  // eigenvector: normalised Ritz vector
+  dot_many(ctx, w.y.get(), n, 1, w.y.get(), n, tn2);
+  scale_by_inv_sqrt(ctx, w.y.get(), tn2, n);
+  hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.y.get(), x, n);
+  TK_HIP(hipGetLastError());
+  ctx->sync();
+  return res;
+}
+
+namespace {
+struct PlanUser {
+  HeffPlan* plan;
+};
+void plan_matvec(const double* x, double* y, int fp64, void* user) {
+  HeffPlan* p = static_cast<PlanUser*>(user)->plan;
+  if (fp64) p->apply_fp64(x, y);
+  else p->apply(x, y);
+}
+}  // namespace
+
+DavidsonResult davidson(Context* ctx, HeffPlan& plan, double* x, const DavidsonOpts& opts) {
+  const int64_t n = plan.psi().total;
+  DBuf<double> diag;
+  if (opts.precondition) {
+    diag.reset(ctx, n);
+    plan.diagonal(diag.get());
+  }
+  PlanUser u{&plan};
+  return davidson_generic(ctx, n, plan_matvec, &u, opts.precondition ? diag.get() : nullptr,
+                          plan.prec() != Prec::FP64, x, opts);
+}
+
+}  // namespace tk

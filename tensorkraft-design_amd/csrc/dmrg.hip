@@ -1,0 +1,657 @@
+// Two-site DMRG engine (see dmrg.hpp).
+#include "dmrg.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <numeric>
+#include <tuple>
+
+#include "eig.hpp"
+#include "kernels.hpp"
+
+namespace tk {
+namespace {
+
+__global__ void copy2d_kernel(const CopyJob* __restrict__ jobs, int n_jobs, const double* __restrict__ src,
+                              double* __restrict__ dst) {
+  for (int j = blockIdx.x; j < n_jobs; j += gridDim.x) {
+    const CopyJob c = jobs[j];
+    const int total = c.m * c.n;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+      const int i = e / c.n, k = e - i * c.n;
+      const double v = c.trans ? src[c.src + int64_t(k) * c.src_ld + i] : src[c.src + int64_t(i) * c.src_ld + k];
+      dst[c.dst + int64_t(i) * c.dst_ld + k] = v;
+    }
+  }
+}
+
+__global__ void fill_kernel2(double* __restrict__ p, double v, int64_t n) {
+  for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n; j += int64_t(gridDim.x) * blockDim.x)
+    p[j] = v;
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool same_qindex(const QIndex& a, const QIndex& b) {
+  if (a.n() != b.n()) return false;
+  for (int i = 0; i < a.n(); ++i)
+    if (a.q[i] != b.q[i] || a.dim[i] != b.dim[i]) return false;
+  return true;
+}
+
+// Upload a work list and launch one contraction.
+template <typename TA, typename TB, typename TC>
+void run_list(Context* ctx, ContractList& cl, int64_t a_n, int64_t b_n, int64_t c_n, const TA* A, const TB* B,
+              TC* C) {
+  cl.finalize();
+  cl.validate(a_n, b_n, c_n);
+  DeviceContractList d;
+  d.upload(ctx, cl);
+  launch_contract(ctx, d, A, B, C);
+  ctx->sync();  // the list's host staging and device buffer end here
+}
+
+}  // namespace
+
+void launch_copies(Context* ctx, const std::vector<CopyJob>& jobs, const double* src, int64_t src_n, double* dst,
+                   int64_t dst_n) {
+  if (jobs.empty()) return;
+  for (const CopyJob& c : jobs) {
+    if (c.m <= 0 || c.n <= 0) throw std::out_of_range("copy: empty job");
+    const int64_t s_end = c.trans ? c.src + int64_t(c.n - 1) * c.src_ld + c.m : c.src + int64_t(c.m - 1) * c.src_ld + c.n;
+    const int64_t d_end = c.dst + int64_t(c.m - 1) * c.dst_ld + c.n;
+    if (c.src < 0 || c.dst < 0 || s_end > src_n || d_end > dst_n || c.dst_ld < c.n)
+      throw std::out_of_range("copy: job outside its buffers");
+  }
+  DBuf<CopyJob> d(ctx, jobs.size());
+  d.upload(jobs.data(), jobs.size());
+  int grid = int(std::min<size_t>(jobs.size(), size_t(ctx->n_cu) * 8));
+  hipLaunchKernelGGL(copy2d_kernel, dim3(grid), dim3(256), 0, ctx->stream, d.get(), int(jobs.size()), src, dst);
+  TK_HIP(hipGetLastError());
+  ctx->sync();
+}
+
+void launch_fill(Context* ctx, double* p, double v, int64_t n) {
+  if (n <= 0) return;
+  int grid = int(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, int64_t(ctx->n_cu) * 16)));
+  hipLaunchKernelGGL(fill_kernel2, dim3(grid), dim3(256), 0, ctx->stream, p, v, n);
+  TK_HIP(hipGetLastError());
+}
+
+DmrgEngine::DmrgEngine(Context* ctx, DmrgConfig cfg) : ctx_(ctx), cfg_(std::move(cfg)) {
+  n_ = int(cfg_.mpo.size());
+  if (n_ < 2) throw ArgError("DMRG needs at least two sites");
+  if (cfg_.mpo.front().wl.total != 1 || cfg_.mpo.back().wr.total != 1)
+    throw ArgError("boundary MPO bonds must have dimension 1");
+  for (int i = 0; i < n_; ++i) {
+    check_mpo_symmetry(cfg_.mpo[i]);
+    if (i + 1 < n_ && !same_qindex(cfg_.mpo[i].wr, cfg_.mpo[i + 1].wl))
+      throw ArgError("MPO bond " + std::to_string(i + 1) + " differs between neighbouring sites");
+  }
+  if (cfg_.bond_schedule.empty()) throw ArgError("empty bond-dimension schedule");
+}
+
+QIndex DmrgEngine::trivial_bond(QN q) const { return QIndex({{q, 1}}); }
+
+void DmrgEngine::set_state(const std::vector<QIndex>& bonds, const std::vector<std::vector<double>>& data) {
+  if (int(bonds.size()) != n_ + 1 || int(data.size()) != n_) throw ArgError("state size mismatch");
+  bonds_ = bonds;
+  site_.clear();
+  site_.resize(n_);
+  for (int i = 0; i < n_; ++i) {
+    site_[i].t = make_block_table(cfg_.sym, {cfg_.mpo[i].phys, bonds_[i], bonds_[i + 1]}, {Dir::Out, Dir::Out, Dir::In},
+                                  QN{});
+    if (int64_t(data[i].size()) != site_[i].t.total)
+      throw ArgError("site " + std::to_string(i) + " data size " + std::to_string(data[i].size()) + " != table " +
+                     std::to_string(site_[i].t.total));
+    site_[i].d.reset(ctx_, size_t(std::max<int64_t>(1, site_[i].t.total)));
+    site_[i].d.upload(data[i].data(), data[i].size());
+  }
+  ctx_->sync();
+  L_.clear();
+  R_.clear();
+  L_.resize(n_ + 1);
+  R_.resize(n_ + 1);
+  sweeps_ = 0;
+}
+
+std::vector<double> DmrgEngine::site_data(int i) const {
+  std::vector<double> h(size_t(site_[i].t.total));
+  site_[i].d.download(h.data(), h.size());
+  ctx_->sync();
+  return h;
+}
+
+DmrgEngine::Env DmrgEngine::boundary_env(const QIndex& bond, const QIndex& w) {
+  Env e;
+  e.t = make_block_table(cfg_.sym, {bond, w, bond}, {Dir::In, Dir::In, Dir::Out}, QN{});
+  if (e.t.total != 1) throw ArgError("boundary environment must be 1x1x1");
+  e.d.reset(ctx_, 1);
+  launch_fill(ctx_, e.d.get(), 1.0, 1);
+  return e;
+}
+
+void DmrgEngine::prepare() {
+  if (site_.empty()) throw ArgError("set_state first");
+  // right-canonicalise with exact (rank-only) splits, centre ends at site 0
+  for (int i = n_ - 2; i >= 0; --i) {
+    BlockTable psi = make_block_table(cfg_.sym, {cfg_.mpo[i].phys, cfg_.mpo[i + 1].phys, bonds_[i], bonds_[i + 2]},
+                                      {Dir::Out, Dir::Out, Dir::Out, Dir::In}, QN{});
+    DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total)));
+    contract_two_site(i, psi, x.get());
+    StepStats st;
+    split_two_site(i, false, psi, x.get(), INT64_MAX, 0.0, 1e-15, &st);
+  }
+  L_[0] = boundary_env(bonds_[0], cfg_.mpo[0].wl);
+  R_[n_] = boundary_env(bonds_[n_], cfg_.mpo[n_ - 1].wr);
+  for (int i = n_ - 1; i >= 2; --i) grow_right(i);
+  ctx_->sync();
+}
+
+SweepStats DmrgEngine::sweep() {
+  SweepStats s;
+  cur_ = &s;
+  steps_.clear();
+  const double t0 = now();
+  for (int i = 0; i + 1 < n_; ++i) steps_.push_back(step(i, true));
+  for (int i = n_ - 1; i >= 1; --i) steps_.push_back(step(i, false));
+  ctx_->sync();
+  s.seconds = now() - t0;
+  for (const StepStats& st : steps_) {
+    if (st.iterations == 0) continue;
+    ++s.steps;
+    s.max_truncation_error = std::max(s.max_truncation_error, st.truncation_error);
+    s.max_bond_dim = std::max(s.max_bond_dim, st.bond_dim);
+    s.matvecs += st.matvecs;
+    s.matvecs_fp64 += st.matvecs_fp64;
+    if (!st.converged) ++s.unconverged;
+  }
+  // sweep energy = the last step's eigenvalue (sweep/mod.rs:301, 313-314)
+  for (auto it = steps_.rbegin(); it != steps_.rend(); ++it)
+    if (it->iterations > 0) {
+      energy_ = it->energy;
+      break;
+    }
+  s.energy = energy_;
+  ++sweeps_;
+  cur_ = nullptr;
+  return s;
+}
+
+StepStats DmrgEngine::step(int i, bool lr) {
+  StepStats st;
+  st.site = i;
+  st.left_to_right = lr;
+  if (i + 1 >= n_) {  // right-to-left sweep's first site: no bond to its right (sweep/mod.rs:354-364)
+    st.energy = energy_;
+    st.bond_dim = bonds_[std::min(i, n_)].total;
+    return st;
+  }
+  const int64_t max_bond =
+      cfg_.bond_schedule[std::min<size_t>(size_t(sweeps_), cfg_.bond_schedule.size() - 1)];
+  double t0 = now();
+  HeffPlan plan(ctx_, cfg_.prec, L_[i].t, L_[i].d.get(), R_[i + 2].t, R_[i + 2].d.get(), cfg_.mpo[i],
+                cfg_.mpo[i + 1]);
+  const BlockTable& psi = plan.psi();
+  DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total)));
+  contract_two_site(i, psi, x.get());
+  ctx_->sync();
+  double t1 = now();
+  DavidsonResult dr = davidson(ctx_, plan, x.get(), cfg_.davidson);
+  double t2 = now();
+  st.energy = dr.eigenvalue;
+  st.iterations = dr.iterations;
+  st.matvecs = dr.matvecs;
+  st.matvecs_fp64 = dr.matvecs_fp64;
+  st.converged = dr.converged;
+  st.residual = dr.residual;
+  const double floor = std::max(cfg_.svd_cutoff * cfg_.svd_cutoff, 1e-15);
+  split_two_site(i, lr, psi, x.get(), max_bond, cfg_.svd_cutoff, floor, &st);
+  double t3 = now();
+  if (lr) {
+    grow_left(i);
+  } else {
+    grow_right(i + 1);
+  }
+  ctx_->sync();
+  double t4 = now();
+  if (cur_) {
+    cur_->t_plan += t1 - t0;
+    cur_->t_eig += t2 - t1;
+    cur_->t_svd += t3 - t2;
+    cur_->t_env += t4 - t3;
+  }
+  return st;
+}
+
+void DmrgEngine::contract_two_site(int i, const BlockTable& psi, double* x) {
+  const Site& A = site_[i];
+  const Site& B = site_[i + 1];
+  const QIndex& Pi = cfg_.mpo[i].phys;
+  const QIndex& Pj = cfg_.mpo[i + 1].phys;
+  const QIndex& BL = bonds_[i];
+  const QIndex& BM = bonds_[i + 1];
+  const QIndex& BR = bonds_[i + 2];
+  ContractList cl;
+  cl.flags = 0;
+  for (int64_t b = 0; b < psi.n_blocks(); ++b) {
+    const int si = psi.s(b, 0), sj = psi.s(b, 1), ls = psi.s(b, 2), rs = psi.s(b, 3);
+    const int64_t dL = BL.dim[ls], dR = BR.dim[rs], dsj = Pj.dim[sj];
+    const int ms = BM.find(fuse(BL.q[ls], Pi.q[si]));
+    int64_t ab = -1, bb = -1;
+    if (ms >= 0) {
+      int32_t as[3] = {si, ls, ms}, bs[3] = {sj, ms, rs};
+      ab = A.t.find(as);
+      bb = B.t.find(bs);
+    }
+    for (int64_t xi = 0; xi < Pi.dim[si]; ++xi)
+      for (int64_t xj = 0; xj < dsj; ++xj) {
+        cl.begin_out(psi.offset[b] + (xi * dsj + xj) * dL * dR, int(dL), int(dR), int(dR));
+        if (ab < 0 || bb < 0) continue;
+        const int64_t dm = BM.dim[ms];
+        cl.begin_term(B.t.offset[bb] + xj * dm * dR, int(dR), int(dm));
+        cl.add_src(A.t.offset[ab] + xi * dL * dm, int(dm), 1.0);
+      }
+  }
+  run_list(ctx_, cl, A.t.total, B.t.total, psi.total, A.d.get(), B.d.get(), x);
+}
+
+void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const double* x, int64_t max_bond,
+                                double cutoff, double lambda_floor, StepStats* st) {
+  const QIndex& Pi = cfg_.mpo[i].phys;
+  const QIndex& Pj = cfg_.mpo[i + 1].phys;
+  const QIndex& BL = bonds_[i];
+  const QIndex& BR = bonds_[i + 2];
+  // Middle charges of the (s_i, a_L) x (s_j, a_R) matricisation.  Row parts in
+  // (s_i sector, x_i) order, column parts in (s_j sector, x_j) order: the
+  // offsets of fuse_legs (block_sparse.rs:474-518) for a fixed fused charge.
+  struct Part {
+    int ssec, x, bsec;
+    int64_t off, d;
+  };
+  struct Mid {
+    QN q;
+    std::vector<Part> rows, cols;
+    int64_t nrow = 0, ncol = 0, n = 0, g_off = 0, w_off = 0;
+    int64_t keep = 0;
+  };
+  std::map<QN, Mid> mm;
+  for (int s = 0; s < Pi.n(); ++s)
+    for (int x = 0; x < Pi.dim[s]; ++x)
+      for (int l = 0; l < BL.n(); ++l) {
+        Mid& m = mm[fuse(Pi.q[s], BL.q[l])];
+        m.rows.push_back({s, x, l, m.nrow, BL.dim[l]});
+        m.nrow += BL.dim[l];
+      }
+  for (int s = 0; s < Pj.n(); ++s)
+    for (int x = 0; x < Pj.dim[s]; ++x)
+      for (int r = 0; r < BR.n(); ++r) {
+        Mid& m = mm[fuse(BR.q[r], dual(Pj.q[s]))];
+        m.cols.push_back({s, x, r, m.ncol, BR.dim[r]});
+        m.ncol += BR.dim[r];
+      }
+  std::vector<Mid> mids;
+  for (auto& [q, m] : mm)
+    if (m.nrow > 0 && m.ncol > 0) {
+      m.q = q;
+      mids.push_back(std::move(m));
+    }
+  int64_t gtot = 0, wtot = 0;
+  for (Mid& m : mids) {
+    m.n = lr ? m.nrow : m.ncol;
+    if (m.n > eig_max_n()) throw ArgError("SVD block larger than the eigensolver limit");
+    m.g_off = gtot;
+    m.w_off = wtot;
+    gtot += m.n * m.n;
+    wtot += m.n;
+  }
+  auto slice = [&](int ssi, int xi, int ls, int ssj, int xj, int rs) {
+    int32_t secs[4] = {ssi, ssj, ls, rs};
+    const int64_t b = psi.find(secs);
+    if (b < 0) throw std::logic_error("split: missing psi block");
+    return psi.offset[b] + (int64_t(xi) * Pj.dim[ssj] + xj) * BL.dim[ls] * BR.dim[rs];
+  };
+  // Gram matrices
+  DBuf<double> G(ctx_, size_t(std::max<int64_t>(1, gtot)));
+  ContractList gl;
+  gl.flags = lr ? kTransB : kTransA;
+  for (Mid& m : mids) {
+    if (lr) {
+      for (const Part& a : m.rows)
+        for (const Part& b : m.rows) {
+          gl.begin_out(m.g_off + a.off * m.n + b.off, int(a.d), int(b.d), int(m.n));
+          for (const Part& c : m.cols) {
+            gl.begin_term(slice(b.ssec, b.x, b.bsec, c.ssec, c.x, c.bsec), int(c.d), int(c.d));
+            gl.add_src(slice(a.ssec, a.x, a.bsec, c.ssec, c.x, c.bsec), int(c.d), 1.0);
+          }
+        }
+    } else {
+      for (const Part& a : m.cols)
+        for (const Part& b : m.cols) {
+          gl.begin_out(m.g_off + a.off * m.n + b.off, int(a.d), int(b.d), int(m.n));
+          for (const Part& r : m.rows) {
+            gl.begin_term(slice(r.ssec, r.x, r.bsec, b.ssec, b.x, b.bsec), int(b.d), int(r.d));
+            gl.add_src(slice(r.ssec, r.x, r.bsec, a.ssec, a.x, a.bsec), int(a.d), 1.0);
+          }
+        }
+    }
+  }
+  run_list(ctx_, gl, psi.total, psi.total, gtot, x, x, G.get());
+  // eigen-decomposition of every Gram block (descending)
+  std::vector<EigJob> jobs;
+  for (Mid& m : mids) {
+    if (m.g_off + m.n * m.n > gtot || m.w_off + m.n > wtot) throw std::out_of_range("eig job outside its buffers");
+    jobs.push_back({m.g_off, m.w_off, m.g_off, int32_t(m.n), 0});
+  }
+  DBuf<EigJob> dj(ctx_, jobs.size());
+  dj.upload(jobs.data(), jobs.size());
+  DBuf<double> Vws(ctx_, size_t(std::max<int64_t>(1, gtot))), Wd(ctx_, size_t(std::max<int64_t>(1, wtot))),
+      E(ctx_, size_t(std::max<int64_t>(1, gtot)));
+  launch_eig(ctx_, dj.get(), int(jobs.size()), G.get(), Vws.get(), Wd.get(), E.get(), true);
+  std::vector<double> lam(static_cast<size_t>(wtot));
+  Wd.download(lam.data(), lam.size());
+  ctx_->sync();
+  // global selection: sigma descending, keep while count < max_bond and
+  // sigma >= cutoff * sigma_max (truncation/mod.rs:72-82), numerical-zero floor
+  // on sigma^2, no cut inside a degenerate group, min_bond_dim floor
+  struct Cand {
+    double l;
+    int mid;
+    int idx;
+  };
+  std::vector<Cand> all;
+  for (int k = 0; k < int(mids.size()); ++k) {
+    const int64_t rank = std::min(mids[k].nrow, mids[k].ncol);
+    for (int64_t j = 0; j < rank; ++j) all.push_back({std::max(0.0, lam[mids[k].w_off + j]), k, int(j)});
+  }
+  std::stable_sort(all.begin(), all.end(), [](const Cand& a, const Cand& b) {
+    if (a.l != b.l) return a.l > b.l;
+    if (a.mid != b.mid) return a.mid < b.mid;
+    return a.idx < b.idx;
+  });
+  const double lmax = all.empty() ? 0.0 : all[0].l;
+  const double thr = std::max(cutoff * cutoff, lambda_floor) * lmax;
+  size_t keep = 0;
+  while (keep < all.size() && int64_t(keep) < max_bond && all[keep].l >= thr && all[keep].l > 0) ++keep;
+  if (keep < all.size() && keep > 0) {
+    size_t k2 = keep;
+    while (k2 > 0 && all[k2 - 1].l - all[k2].l <= cfg_.degeneracy_tol * lmax) --k2;
+    if (k2 > 0) keep = k2;
+  }
+  keep = std::max<size_t>(keep, size_t(std::min<int64_t>(cfg_.min_bond_dim, int64_t(all.size()))));
+  double sum_all = 0, sum_kept = 0;
+  for (size_t k = 0; k < all.size(); ++k) {
+    sum_all += all[k].l;
+    if (k < keep) {
+      sum_kept += all[k].l;
+      ++mids[all[k].mid].keep;
+    }
+  }
+  st->truncation_error = sum_all > 0 ? (sum_all - sum_kept) / sum_all : 0.0;
+  const double renorm = sum_kept > 0 ? 1.0 / std::sqrt(sum_kept) : 1.0;
+  // new bond
+  std::vector<std::pair<QN, int64_t>> nb;
+  std::map<QN, int> mid_of;
+  for (int k = 0; k < int(mids.size()); ++k)
+    if (mids[k].keep > 0) {
+      nb.push_back({mids[k].q, mids[k].keep});
+      mid_of[mids[k].q] = k;
+    }
+  QIndex newB(nb);
+  st->bond_dim = newB.total;
+  Site Ai, Aj;
+  Ai.t = make_block_table(cfg_.sym, {Pi, BL, newB}, {Dir::Out, Dir::Out, Dir::In}, QN{});
+  Aj.t = make_block_table(cfg_.sym, {Pj, newB, BR}, {Dir::Out, Dir::Out, Dir::In}, QN{});
+  Ai.d.reset(ctx_, size_t(std::max<int64_t>(1, Ai.t.total)));
+  Aj.d.reset(ctx_, size_t(std::max<int64_t>(1, Aj.t.total)));
+  // locate the row/column part of a site block
+  auto row_part = [&](const Mid& m, int s, int x) -> const Part& {
+    for (const Part& p : m.rows)
+      if (p.ssec == s && p.x == x) return p;
+    throw std::logic_error("split: row part not found");
+  };
+  auto col_part = [&](const Mid& m, int s, int x) -> const Part& {
+    for (const Part& p : m.cols)
+      if (p.ssec == s && p.x == x) return p;
+    throw std::logic_error("split: column part not found");
+  };
+  std::vector<CopyJob> copies;
+  ContractList wl;
+  if (lr) {
+    // A_i = U (left-canonical): copy the kept eigenvector columns
+    for (int64_t b = 0; b < Ai.t.n_blocks(); ++b) {
+      const int s = Ai.t.s(b, 0), l = Ai.t.s(b, 1), ms = Ai.t.s(b, 2);
+      const Mid& m = mids[mid_of.at(newB.q[ms])];
+      const int64_t dL = BL.dim[l], k = m.keep;
+      for (int x = 0; x < Pi.dim[s]; ++x) {
+        const Part& p = row_part(m, s, x);
+        copies.push_back({m.g_off + p.off * m.n, Ai.t.offset[b] + x * dL * k, int32_t(dL), int32_t(k), int32_t(m.n),
+                          int32_t(k), 0, 0});
+      }
+    }
+    // A_j = U^T theta / |kept| (the new centre)
+    wl.flags = kTransA;
+    for (int64_t b = 0; b < Aj.t.n_blocks(); ++b) {
+      const int s = Aj.t.s(b, 0), ms = Aj.t.s(b, 1), r = Aj.t.s(b, 2);
+      const Mid& m = mids[mid_of.at(newB.q[ms])];
+      const int64_t dR = BR.dim[r], k = m.keep;
+      for (int x = 0; x < Pj.dim[s]; ++x) {
+        wl.begin_out(Aj.t.offset[b] + x * k * dR, int(k), int(dR), int(dR));
+        for (const Part& rp : m.rows) {
+          wl.begin_term(slice(rp.ssec, rp.x, rp.bsec, s, x, r), int(dR), int(rp.d));
+          wl.add_src(m.g_off + rp.off * m.n, int(m.n), renorm);
+        }
+      }
+    }
+    launch_copies(ctx_, copies, E.get(), gtot, Ai.d.get(), Ai.t.total);
+    run_list(ctx_, wl, gtot, psi.total, Aj.t.total, E.get(), x, Aj.d.get());
+  } else {
+    // A_j = V^T (right-canonical)
+    for (int64_t b = 0; b < Aj.t.n_blocks(); ++b) {
+      const int s = Aj.t.s(b, 0), ms = Aj.t.s(b, 1), r = Aj.t.s(b, 2);
+      const Mid& m = mids[mid_of.at(newB.q[ms])];
+      const int64_t dR = BR.dim[r], k = m.keep;
+      for (int x = 0; x < Pj.dim[s]; ++x) {
+        const Part& p = col_part(m, s, x);
+        copies.push_back({m.g_off + p.off * m.n, Aj.t.offset[b] + x * k * dR, int32_t(k), int32_t(dR), int32_t(m.n),
+                          int32_t(dR), 1, 0});
+      }
+    }
+    // A_i = theta V / |kept| (the new centre)
+    wl.flags = 0;
+    for (int64_t b = 0; b < Ai.t.n_blocks(); ++b) {
+      const int s = Ai.t.s(b, 0), l = Ai.t.s(b, 1), ms = Ai.t.s(b, 2);
+      const Mid& m = mids[mid_of.at(newB.q[ms])];
+      const int64_t dL = BL.dim[l], k = m.keep;
+      for (int x = 0; x < Pi.dim[s]; ++x) {
+        wl.begin_out(Ai.t.offset[b] + x * dL * k, int(dL), int(k), int(k));
+        for (const Part& cp : m.cols) {
+          wl.begin_term(m.g_off + cp.off * m.n, int(m.n), int(cp.d));
+          wl.add_src(slice(s, x, l, cp.ssec, cp.x, cp.bsec), int(cp.d), renorm);
+        }
+      }
+    }
+    launch_copies(ctx_, copies, E.get(), gtot, Aj.d.get(), Aj.t.total);
+    run_list(ctx_, wl, psi.total, gtot, Ai.t.total, x, E.get(), Ai.d.get());
+  }
+  bonds_[i + 1] = newB;
+  site_[i] = std::move(Ai);
+  site_[i + 1] = std::move(Aj);
+}
+
+void DmrgEngine::grow_left(int i) {
+  // L'[a'_R, w_r, a_R] = sum A[s'](a'_L, a'_R) L[a'_L, w_l, a_L] W[s, s', w_l, w_r] A[s](a_L, a_R)
+  // (environments/mod.rs:225-355 in block-sparse form)
+  const Env& Lo = L_[i];
+  const Site& A = site_[i];
+  const MpoSite& W = cfg_.mpo[i];
+  const QIndex& P = W.phys;
+  const QIndex& BL = bonds_[i];
+  const QIndex& BR = bonds_[i + 1];
+  Env out;
+  out.t = make_block_table(cfg_.sym, {BR, W.wr, BR}, {Dir::In, Dir::In, Dir::Out}, QN{});
+  out.d.reset(ctx_, size_t(std::max<int64_t>(1, out.t.total)));
+  IndexMap PM(P), WLm(W.wl), WRm(W.wr);
+  // W entries grouped by (s_out, w_r)
+  std::vector<std::vector<std::tuple<int, int, double>>> src(size_t(P.total * W.wr.total));
+  for (int64_t s = 0; s < P.total; ++s)
+    for (int64_t so = 0; so < P.total; ++so)
+      for (int64_t a = 0; a < W.wl.total; ++a)
+        for (int64_t b = 0; b < W.wr.total; ++b) {
+          const double v = W.at(s, so, a, b);
+          if (v != 0.0) src[size_t(so * W.wr.total + b)].push_back({int(s), int(a), v});
+        }
+  // T1 registry keyed by (site block, x_s, w_l)
+  std::map<std::tuple<int64_t, int, int>, int64_t> t1_off;
+  struct T1 { int64_t off, b_off, a_off; int m, n, k, lda; };
+  std::vector<T1> t1s;
+  int64_t t1_total = 0;
+  ContractList s2;
+  s2.flags = kTransA | kTransC;
+  for (int64_t ob = 0; ob < out.t.n_blocks(); ++ob) {
+    const int rps = out.t.s(ob, 0), swr = out.t.s(ob, 1), rs = out.t.s(ob, 2);
+    const int64_t dRp = BR.dim[rps], dR = BR.dim[rs], dwr = W.wr.dim[swr];
+    for (int64_t iwr = 0; iwr < dwr; ++iwr) {
+      const int64_t wr = W.wr.off[swr] + iwr;
+      s2.begin_out(out.t.offset[ob] + iwr * dR, int(dR), int(dRp), int(dwr * dR));
+      for (int sps = 0; sps < P.n(); ++sps)
+        for (int xps = 0; xps < P.dim[sps]; ++xps) {
+          const QN qlp = fuse(BR.q[rps], dual(P.q[sps]));
+          const int lps = BL.find(qlp);
+          if (lps < 0) continue;
+          int32_t bsec[3] = {sps, lps, rps};
+          const int64_t bb = A.t.find(bsec);
+          if (bb < 0) continue;
+          const int64_t dLp = BL.dim[lps];
+          s2.begin_term(A.t.offset[bb] + xps * dLp * dRp, int(dRp), int(dLp));
+          const int64_t so = P.off[sps] + xps;
+          for (auto& [s, wlv, coef] : src[size_t(so * W.wr.total + wr)]) {
+            const int ssec = PM.sector[s], xs = PM.within[s];
+            const int swl = WLm.sector[wlv], iwl = WLm.within[wlv];
+            const int ls = BL.find(fuse(qlp, W.wl.q[swl]));
+            if (ls < 0) continue;
+            int32_t ksec[3] = {ssec, ls, rs};
+            const int64_t kb = A.t.find(ksec);
+            if (kb < 0) continue;
+            int32_t lsec[3] = {lps, swl, ls};
+            const int64_t lb = Lo.t.find(lsec);
+            if (lb < 0) continue;
+            auto key = std::make_tuple(kb, xs, int(wlv));
+            auto it = t1_off.find(key);
+            if (it == t1_off.end()) {
+              const int64_t dL = BL.dim[ls];
+              T1 t{t1_total, A.t.offset[kb] + xs * dL * dR, Lo.t.offset[lb] + iwl * dL, int(dLp), int(dR), int(dL),
+                   int(W.wl.dim[swl] * dL)};
+              t1s.push_back(t);
+              it = t1_off.emplace(key, t1_total).first;
+              t1_total += dLp * dR;
+            }
+            s2.add_src(it->second, int(dR), coef);
+          }
+          s2.drop_empty_term();
+        }
+    }
+  }
+  ContractList s1;
+  s1.flags = 0;
+  for (const T1& t : t1s) {
+    s1.begin_out(t.off, t.m, t.n, t.n);
+    s1.begin_term(t.b_off, t.n, t.k);
+    s1.add_src(t.a_off, t.lda, 1.0);
+  }
+  DBuf<double> T(ctx_, size_t(std::max<int64_t>(1, t1_total)));
+  run_list(ctx_, s1, Lo.t.total, A.t.total, t1_total, Lo.d.get(), A.d.get(), T.get());
+  run_list(ctx_, s2, t1_total, A.t.total, out.t.total, T.get(), A.d.get(), out.d.get());
+  L_[i + 1] = std::move(out);
+}
+
+void DmrgEngine::grow_right(int i) {
+  // R'[a'_L, w_l, a_L] = sum A[s'](a'_L, a'_R) R[a'_R, w_r, a_R] W[s, s', w_l, w_r] A[s](a_L, a_R)
+  // (environments/mod.rs:361-483 in block-sparse form)
+  const Env& Ro = R_[i + 1];
+  const Site& A = site_[i];
+  const MpoSite& W = cfg_.mpo[i];
+  const QIndex& P = W.phys;
+  const QIndex& BL = bonds_[i];
+  const QIndex& BR = bonds_[i + 1];
+  Env out;
+  out.t = make_block_table(cfg_.sym, {BL, W.wl, BL}, {Dir::In, Dir::In, Dir::Out}, QN{});
+  out.d.reset(ctx_, size_t(std::max<int64_t>(1, out.t.total)));
+  IndexMap PM(P), WLm(W.wl), WRm(W.wr);
+  // W entries grouped by (s_out, w_l)
+  std::vector<std::vector<std::tuple<int, int, double>>> src(size_t(P.total * W.wl.total));
+  for (int64_t s = 0; s < P.total; ++s)
+    for (int64_t so = 0; so < P.total; ++so)
+      for (int64_t a = 0; a < W.wl.total; ++a)
+        for (int64_t b = 0; b < W.wr.total; ++b) {
+          const double v = W.at(s, so, a, b);
+          if (v != 0.0) src[size_t(so * W.wl.total + a)].push_back({int(s), int(b), v});
+        }
+  std::map<std::tuple<int64_t, int, int>, int64_t> t1_off;
+  struct T1 { int64_t off, b_off, a_off; int m, n, k, lda, ldb; };
+  std::vector<T1> t1s;
+  int64_t t1_total = 0;
+  ContractList s2;
+  s2.flags = kTransA | kTransB | kTransC;
+  for (int64_t ob = 0; ob < out.t.n_blocks(); ++ob) {
+    const int lps = out.t.s(ob, 0), swl = out.t.s(ob, 1), ls = out.t.s(ob, 2);
+    const int64_t dLp = BL.dim[lps], dL = BL.dim[ls], dwl = W.wl.dim[swl];
+    for (int64_t iwl = 0; iwl < dwl; ++iwl) {
+      const int64_t wlv = W.wl.off[swl] + iwl;
+      s2.begin_out(out.t.offset[ob] + iwl * dL, int(dL), int(dLp), int(dwl * dL));
+      for (int sps = 0; sps < P.n(); ++sps)
+        for (int xps = 0; xps < P.dim[sps]; ++xps) {
+          const QN qrp = fuse(BL.q[lps], P.q[sps]);
+          const int rps = BR.find(qrp);
+          if (rps < 0) continue;
+          int32_t bsec[3] = {sps, lps, rps};
+          const int64_t bb = A.t.find(bsec);
+          if (bb < 0) continue;
+          const int64_t dRp = BR.dim[rps];
+          s2.begin_term(A.t.offset[bb] + xps * dLp * dRp, int(dRp), int(dRp));
+          const int64_t so = P.off[sps] + xps;
+          for (auto& [s, wrv, coef] : src[size_t(so * W.wl.total + wlv)]) {
+            const int ssec = PM.sector[s], xs = PM.within[s];
+            const int swr = WRm.sector[wrv], iwr = WRm.within[wrv];
+            const int rs = BR.find(fuse(qrp, W.wr.q[swr]));
+            if (rs < 0) continue;
+            int32_t ksec[3] = {ssec, ls, rs};
+            const int64_t kb = A.t.find(ksec);
+            if (kb < 0) continue;
+            int32_t rsec[3] = {rps, swr, rs};
+            const int64_t rb = Ro.t.find(rsec);
+            if (rb < 0) continue;
+            auto key = std::make_tuple(kb, xs, int(wrv));
+            auto it = t1_off.find(key);
+            if (it == t1_off.end()) {
+              const int64_t dR = BR.dim[rs];
+              T1 t{t1_total, A.t.offset[kb] + xs * dL * dR, Ro.t.offset[rb] + iwr * dR, int(dRp), int(dL), int(dR),
+                   int(W.wr.dim[swr] * dR), int(dR)};
+              t1s.push_back(t);
+              it = t1_off.emplace(key, t1_total).first;
+              t1_total += dRp * dL;
+            }
+            s2.add_src(it->second, int(dL), coef);
+          }
+          s2.drop_empty_term();
+        }
+    }
+  }
+  ContractList s1;
+  s1.flags = kTransB;
+  for (const T1& t : t1s) {
+    s1.begin_out(t.off, t.m, t.n, t.n);
+    s1.begin_term(t.b_off, t.ldb, t.k);
+    s1.add_src(t.a_off, t.lda, 1.0);
+  }
+  DBuf<double> T(ctx_, size_t(std::max<int64_t>(1, t1_total)));
+  run_list(ctx_, s1, Ro.t.total, A.t.total, t1_total, Ro.d.get(), A.d.get(), T.get());
+  run_list(ctx_, s2, t1_total, A.t.total, out.t.total, T.get(), A.d.get(), out.d.get());
+  R_[i] = std::move(out);
+}
+
+}  // namespace tk

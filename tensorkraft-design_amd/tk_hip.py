@@ -48,6 +48,27 @@ class tk_env(C.Structure):
     _fields_ = [("bond", tk_qindex), ("w", tk_qindex), ("data_dev", C.c_void_p)]
 
 
+class tk_dmrg_config(C.Structure):
+    _fields_ = [("prec", C.c_int32), ("sym", C.c_int32), ("n_sites", C.c_int32), ("n_schedule", C.c_int32),
+                ("bond_schedule", C.POINTER(C.c_int64)), ("svd_cutoff", C.c_double), ("min_bond_dim", C.c_int64),
+                ("degeneracy_tol", C.c_double), ("davidson_tol", C.c_double), ("max_subspace", C.c_int32),
+                ("max_iter", C.c_int32), ("precondition", C.c_int32), ("switch_rel", C.c_double)]
+
+
+class tk_sweep_stats(C.Structure):
+    _fields_ = [("energy", C.c_double), ("max_truncation_error", C.c_double), ("seconds", C.c_double),
+                ("t_plan", C.c_double), ("t_eig", C.c_double), ("t_svd", C.c_double), ("t_env", C.c_double),
+                ("max_bond_dim", C.c_int64), ("matvecs", C.c_int32), ("matvecs_fp64", C.c_int32),
+                ("steps", C.c_int32), ("unconverged", C.c_int32)]
+
+
+class tk_step_stats(C.Structure):
+    _fields_ = [("site", C.c_int32), ("left_to_right", C.c_int32), ("iterations", C.c_int32),
+                ("matvecs", C.c_int32), ("matvecs_fp64", C.c_int32), ("converged", C.c_int32),
+                ("energy", C.c_double), ("truncation_error", C.c_double), ("residual", C.c_double),
+                ("bond_dim", C.c_int64)]
+
+
 _lib = None
 
 
@@ -85,6 +106,14 @@ def lib():
         "tk_heff_stats": (I32, [P, pD, pD, pI64]),
         "tk_heff_stage_stats": (I32, [P, pD]),
         "tk_heff_time": (I32, [P, P, P, I32, pD, pD, pD]),
+        "tk_dmrg_create": (I32, [P, C.POINTER(tk_dmrg_config), C.POINTER(tk_mpo_site), C.POINTER(P)]),
+        "tk_dmrg_destroy": (I32, [P]),
+        "tk_dmrg_set_state": (I32, [P, C.POINTER(tk_qindex), C.POINTER(C.POINTER(D))]),
+        "tk_dmrg_prepare": (I32, [P]),
+        "tk_dmrg_sweep": (I32, [P, C.POINTER(tk_sweep_stats)]),
+        "tk_dmrg_steps": (I32, [P, I32, pI32, C.POINTER(tk_step_stats)]),
+        "tk_dmrg_bond": (I32, [P, I32, I32, pI32, pI32, pI64]),
+        "tk_dmrg_site": (I32, [P, I32, I64, pI64, pD]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -97,7 +126,9 @@ def lib():
 EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_last_error", "tk_device_alloc",
             "tk_device_free", "tk_memcpy_h2d", "tk_memcpy_d2h", "tk_pack_key", "tk_qindex_layout", "tk_block_table", "tk_fuse_map",
             "tk_heff_plan_create", "tk_heff_plan_destroy", "tk_heff_psi_table", "tk_heff_apply",
-            "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time"]
+            "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time",
+            "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
+            "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site"]
 
 
 def _check(st, ctx=None):
@@ -479,3 +510,161 @@ def c2_problem(seed=2024, total=1024):
     R = rng.standard_normal(r_tot)
     x = rng.standard_normal(p_tot)
     return {"bond_l": bl, "bond_r": br, "W": W, "L": L, "R": R, "x": x, "sym": SYM_U1U1}
+
+
+# ---------------------------------------------------------------- chains and states
+def chain_mpo(kind, n, **params):
+    """Site-by-site MPO of an open chain: [left, bulk, ..., bulk, right]."""
+    build = {"heisenberg": heisenberg_w, "hubbard": hubbard_w}[kind]
+    return [build(position="left" if i == 0 else "right" if i == n - 1 else "bulk", **params) for i in range(n)]
+
+
+def _qkey(q):
+    return (q, 0) if np.isscalar(q) else tuple(q)
+
+
+def initial_mps(sym, phys, total, d0, seed=0):
+    """Seeded random MPS with N(0,1) entries in every allowed block.
+
+    Bond i carries the charges reachable from the left with i sites and from
+    which `total` is reachable with the remaining sites; sector dimensions are
+    proportional to min(left count, right count) of basis configurations,
+    capped by that count, at least 1, rounded to d0 in total by largest
+    remainder.  Entries come from numpy's PCG64 seeded with `seed`, site by
+    site in block-table order.  Returns (bonds, data).
+    """
+    n = len(phys)
+    total = _qkey(total)
+    sq = [[(_qkey(q), int(d)) for q, d in p] for p in phys]
+
+    def conv(acc, site):
+        out = {}
+        for q, c in acc.items():
+            for s, d in site:
+                k = (q[0] + s[0], q[1] + s[1])
+                out[k] = out.get(k, 0) + c * d
+        return out
+
+    left = [{(0, 0): 1}]
+    for i in range(n):
+        left.append(conv(left[-1], sq[i]))
+    right = [None] * (n + 1)
+    right[n] = {(0, 0): 1}
+    for i in range(n - 1, -1, -1):
+        right[i] = conv(right[i + 1], sq[i])
+    bonds = []
+    for i in range(n + 1):
+        exact = {}
+        for q, c in left[i].items():
+            rest = (total[0] - q[0], total[1] - q[1])
+            if rest in right[i]:
+                exact[q] = min(c, right[i][rest])
+        qs = sorted(exact)
+        if not qs:
+            raise ValueError(f"no charge sector reachable on bond {i}")
+        cap = [exact[q] for q in qs]
+        if sum(cap) <= d0:
+            dims = cap
+        else:
+            tot = float(sum(cap))
+            raw = [d0 * c / tot for c in cap]
+            dims = [max(1, min(c, int(math.floor(r)))) for c, r in zip(cap, raw)]
+            order = sorted(range(len(qs)), key=lambda j: (-(raw[j] - math.floor(raw[j])), qs[j]))
+            while sum(dims) < d0:
+                moved = False
+                for j in order:
+                    if sum(dims) >= d0:
+                        break
+                    if dims[j] < cap[j]:
+                        dims[j] += 1
+                        moved = True
+                if not moved:
+                    break
+            while sum(dims) > d0:
+                j = max(range(len(qs)), key=lambda j: (dims[j], qs[j]))
+                if dims[j] <= 1:
+                    break
+                dims[j] -= 1
+        bonds.append([(q if sym == SYM_U1U1 else q, int(d)) for q, d in zip(qs, dims)])
+    rng = np.random.default_rng(seed)
+    data = []
+    for i in range(n):
+        _, _, _, tot = block_table(sym, [phys[i], bonds[i], bonds[i + 1]], [DIR_OUT, DIR_OUT, DIR_IN])
+        data.append(rng.standard_normal(tot))
+    return bonds, data
+
+
+class Dmrg:
+    """tk_dmrg: two-site DMRG with every contraction on the GPU."""
+
+    def __init__(self, ctx, mpo_sites, sym, prec="fp64", schedule=(64,), svd_cutoff=1e-10, davidson_tol=1e-10,
+                 max_subspace=60, max_iter=1000, precondition=True, switch_rel=1e-6, degeneracy_tol=1e-12,
+                 min_bond_dim=1):
+        self.ctx = ctx
+        self.sym = sym
+        self.n = len(mpo_sites)
+        self._keep = []
+        sites = []
+        for m in mpo_sites:
+            p, a, b = QIndexArg(m.phys), QIndexArg(m.wl), QIndexArg(m.wr)
+            w = np.ascontiguousarray(m.w, dtype=np.float64)
+            self._keep += [p, a, b, w]
+            sites.append(tk_mpo_site(p.c, a.c, b.c, w.ctypes.data_as(C.POINTER(C.c_double))))
+        arr = (tk_mpo_site * self.n)(*sites)
+        sched = np.array(schedule, dtype=np.int64)
+        self._keep.append(sched)
+        cfg = tk_dmrg_config(PREC[prec], sym, self.n, len(sched), sched.ctypes.data_as(C.POINTER(C.c_int64)),
+                             svd_cutoff, min_bond_dim, degeneracy_tol, davidson_tol, max_subspace, max_iter,
+                             1 if precondition else 0, switch_rel)
+        h = C.c_void_p()
+        _check(ctx.L.tk_dmrg_create(ctx.h, C.byref(cfg), arr, C.byref(h)), ctx.h)
+        self.h = h
+        ctx.adopt(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.ctx.L.tk_dmrg_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, bonds, data):
+        qs = [QIndexArg(b) for b in bonds]
+        arr = (tk_qindex * len(qs))(*[q.c for q in qs])
+        ds = [np.ascontiguousarray(d, dtype=np.float64) for d in data]
+        ptrs = (C.POINTER(C.c_double) * len(ds))(*[d.ctypes.data_as(C.POINTER(C.c_double)) for d in ds])
+        _check(self.ctx.L.tk_dmrg_set_state(self.h, arr, ptrs), self.ctx.h)
+
+    def prepare(self):
+        _check(self.ctx.L.tk_dmrg_prepare(self.h), self.ctx.h)
+
+    def sweep(self):
+        s = tk_sweep_stats()
+        _check(self.ctx.L.tk_dmrg_sweep(self.h, C.byref(s)), self.ctx.h)
+        return {f: getattr(s, f) for f, _ in tk_sweep_stats._fields_}
+
+    def steps(self):
+        n = C.c_int32(0)
+        buf = (tk_step_stats * (2 * self.n + 2))()
+        _check(self.ctx.L.tk_dmrg_steps(self.h, len(buf), C.byref(n), buf), self.ctx.h)
+        return [{f: getattr(buf[k], f) for f, _ in tk_step_stats._fields_} for k in range(n.value)]
+
+    def bond(self, i):
+        n = C.c_int32(0)
+        qn = np.zeros(2 * 4096, dtype=np.int32)
+        dims = np.zeros(4096, dtype=np.int64)
+        _check(self.ctx.L.tk_dmrg_bond(self.h, i, 4096, C.byref(n), qn.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       dims.ctypes.data_as(C.POINTER(C.c_int64))), self.ctx.h)
+        return [((int(qn[2 * k]), int(qn[2 * k + 1])), int(dims[k])) for k in range(n.value)]
+
+    def site(self, i):
+        n = C.c_int64(0)
+        self.ctx.L.tk_dmrg_site(self.h, i, 0, C.byref(n), None)
+        out = np.zeros(n.value)
+        _check(self.ctx.L.tk_dmrg_site(self.h, i, n.value, C.byref(n), out.ctypes.data_as(C.POINTER(C.c_double))),
+               self.ctx.h)
+        return out
