@@ -116,6 +116,58 @@ tk_status tk_fuse_map(int32_t sym, int32_t n_legs, const tk_qindex* legs, const 
                       int64_t capacity, int32_t* n_fused, int32_t* fused_qn, int64_t* fused_dims,
                       int64_t* n_combos, int32_t* combo_fused_sector, int64_t* combo_offset);
 
+/* ---- LinAlgBackend / SparseLinAlgBackend (FP64, device data) ------------ */
+/* LinAlgBackend::gemm (tk-linalg/src/traits.rs:114-121; faer.rs:591-639):
+ * C = alpha A B + beta C with MatRef strides (tk-core/src/matview.rs:12-33);
+ * each operand needs a unit stride on one axis, else TK_ERR_INVALID_ARG
+ * (NonContiguousForBlas, tk-linalg/src/error.rs:24-27).  Synchronous. */
+tk_status tk_gemm(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha, const double* a_dev, int64_t rs_a,
+                  int64_t cs_a, const double* b_dev, int64_t rs_b, int64_t cs_b, double beta, double* c_dev,
+                  int64_t rs_c, int64_t cs_c);
+
+/* Rank-2 block-sparse matrix, legs [Incoming, Outgoing] (block_gemm_realistic.rs:26). */
+typedef struct {
+  tk_qindex rows;
+  tk_qindex cols;
+  int32_t flux[2];
+  const double* data_dev; /* FlatBlockStorage order */
+} tk_block_matrix;
+
+/* SparseLinAlgBackend::block_gemm (traits.rs:190-205; faer.rs:390-575):
+ * the output's block table (blocks receiving at least one sector task, keys
+ * ascending, flux = a.flux + b.flux) ... */
+tk_status tk_block_gemm_table(int32_t sym, const tk_block_matrix* a, const tk_block_matrix* b, int64_t capacity,
+                              int64_t* n_blocks, uint64_t* keys, int64_t* offsets, int64_t* total);
+/* ... and the product into c_dev (that table's layout).  Equal output keys
+ * accumulate in the fixed order (flops descending, key_a, key_b).  Synchronous. */
+tk_status tk_block_gemm(tk_ctx* ctx, int32_t sym, const tk_block_matrix* a, const tk_block_matrix* b,
+                        double* c_dev);
+/* SparseLinAlgBackend::spmv (faer.rs:336-388): y = A x, y zeroed first; x and
+ * y over the full sorted index ranges of cols / rows.  Synchronous. */
+tk_status tk_spmv(tk_ctx* ctx, int32_t sym, const tk_block_matrix* a, const double* x_dev, double* y_dev);
+/* LinAlgBackend::svd_truncated (traits.rs:38-88; faer.rs:127-240): row-major
+ * m x n A; keeps at most max_rank singular values with sigma >= cutoff *
+ * sigma_max (numerical zeros, sigma^2 <= 1e-15 sigma_max^2, are dropped);
+ * U (m x rank), Vt (rank x n) row-major on the device, s (host) descending,
+ * truncation_error = sum of discarded sigma^2 (results.rs:18-23).  min(m, n)
+ * must not exceed 1024. */
+tk_status tk_svd_truncated(tk_ctx* ctx, int64_t m, int64_t n, const double* a_dev, int64_t max_rank, double cutoff,
+                           double* u_dev, double* s_host, double* vt_dev, int64_t* rank, double* truncation_error);
+/* LinAlgBackend::eigh_lowest (traits.rs:134-138), all pairs: eigenvalues
+ * ascending (host), eigenvectors as columns of a row-major n x n (device). */
+tk_status tk_eigh(tk_ctx* ctx, int64_t n, const double* a_dev, double* evals_host, double* evecs_dev);
+
+/* IterativeEigensolver::lowest_eigenpair (tk-dmrg/src/eigensolver/mod.rs:38-55)
+ * with DavidsonSolver semantics (eigensolver/mod.rs:300-516) on a caller
+ * matvec over device vectors: matvec(x_dev, y_dev, user) must compute y = A x
+ * on the context's stream (or synchronise before returning).  x_dev holds the
+ * start vector and receives the normalised eigenvector; diag_dev (nullable)
+ * is the preconditioner diagonal. */
+typedef void (*tk_matvec_fn)(const double* x_dev, double* y_dev, void* user);
+tk_status tk_davidson(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, const double* diag_dev, double tol,
+                      int32_t max_subspace, int32_t max_iter, double* x_dev, double* eigenvalue, double* residual,
+                      int32_t* matvecs);
+
 /* ---- two-site effective Hamiltonian ------------------------------------- */
 /* build_heff_two_site (tk-dmrg/src/environments/mod.rs:491-602;
  * techspec/6_tech-spec_tk-dmrg.md:561-572).  psi layout (s_i, s_j, a_L, a_R)

@@ -5,6 +5,7 @@
 #include <memory>
 #include <string>
 
+#include "backend.hpp"
 #include "device.hpp"
 #include "dmrg.hpp"
 #include "heff.hpp"
@@ -258,6 +259,101 @@ tk_status tk_fuse_map(int32_t sym, int32_t n_legs, const tk_qindex* legs, const 
       if (combo_fused_sector) combo_fused_sector[c] = m.combo_fused_sector[c];
       if (combo_offset) combo_offset[c] = m.combo_offset[c];
     }
+  });
+}
+
+tk_status tk_gemm(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t rs_a,
+                  int64_t cs_a, const double* b, int64_t rs_b, int64_t cs_b, double beta, double* c, int64_t rs_c,
+                  int64_t cs_c) {
+  return guard(ctx, [&] {
+    if (!ctx) throw tk::ArgError("null context");
+    if ((m > 0 && n > 0 && !c) || (m > 0 && n > 0 && k > 0 && (!a || !b))) throw tk::ArgError("null operand");
+    tk::dense_gemm(ctx->c.get(), m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, beta, c, rs_c, cs_c);
+  });
+}
+
+tk_status tk_block_gemm_table(int32_t sym, const tk_block_matrix* a, const tk_block_matrix* b, int64_t capacity,
+                              int64_t* n_blocks, uint64_t* keys, int64_t* offsets, int64_t* total) {
+  return guard(nullptr, [&] {
+    if (!a || !b || !n_blocks) throw tk::ArgError("null argument");
+    tk::Sym s = to_sym(sym);
+    tk::BlockGemmPlan p(s, to_qindex(s, a->rows), to_qindex(s, a->cols), tk::QN{a->flux[0], a->flux[1]},
+                        to_qindex(s, b->rows), to_qindex(s, b->cols), tk::QN{b->flux[0], b->flux[1]});
+    *n_blocks = int64_t(p.keys().size());
+    if (total) *total = p.total();
+    if (int64_t(p.keys().size()) > capacity) throw tk::ArgError("capacity");
+    for (size_t i = 0; i < p.keys().size(); ++i) {
+      if (keys) keys[i] = p.keys()[i];
+      if (offsets) offsets[i] = p.offsets()[i];
+    }
+  });
+}
+
+tk_status tk_block_gemm(tk_ctx* ctx, int32_t sym, const tk_block_matrix* a, const tk_block_matrix* b, double* c) {
+  return guard(ctx, [&] {
+    if (!ctx || !a || !b) throw tk::ArgError("null argument");
+    tk::Sym s = to_sym(sym);
+    tk::BlockGemmPlan p(s, to_qindex(s, a->rows), to_qindex(s, a->cols), tk::QN{a->flux[0], a->flux[1]},
+                        to_qindex(s, b->rows), to_qindex(s, b->cols), tk::QN{b->flux[0], b->flux[1]});
+    if (p.total() > 0 && (!c || !a->data_dev || !b->data_dev)) throw tk::ArgError("null data");
+    p.run(ctx->c.get(), a->data_dev, b->data_dev, c);
+  });
+}
+
+tk_status tk_spmv(tk_ctx* ctx, int32_t sym, const tk_block_matrix* a, const double* x, double* y) {
+  return guard(ctx, [&] {
+    if (!ctx || !a || !x || !y) throw tk::ArgError("null argument");
+    tk::Sym s = to_sym(sym);
+    tk::block_spmv(ctx->c.get(), s, to_qindex(s, a->rows), to_qindex(s, a->cols), tk::QN{a->flux[0], a->flux[1]},
+                   a->data_dev, x, y);
+  });
+}
+
+tk_status tk_svd_truncated(tk_ctx* ctx, int64_t m, int64_t n, const double* a, int64_t max_rank, double cutoff,
+                           double* u, double* s, double* vt, int64_t* rank, double* truncation_error) {
+  return guard(ctx, [&] {
+    if (!ctx || !a || !u || !vt || !rank) throw tk::ArgError("null argument");
+    tk::SvdOut o = tk::svd_truncated(ctx->c.get(), m, n, a, max_rank, cutoff, u, vt);
+    *rank = o.rank;
+    if (s) std::copy(o.s.begin(), o.s.end(), s);
+    if (truncation_error) *truncation_error = o.truncation_error;
+  });
+}
+
+tk_status tk_eigh(tk_ctx* ctx, int64_t n, const double* a, double* evals, double* evecs) {
+  return guard(ctx, [&] {
+    if (!ctx || !a || !evecs) throw tk::ArgError("null argument");
+    std::vector<double> w = tk::eigh(ctx->c.get(), n, a, evecs);
+    if (evals) std::copy(w.begin(), w.end(), evals);
+  });
+}
+
+namespace {
+struct CallbackUser {
+  tk_matvec_fn fn;
+  void* user;
+};
+void callback_matvec(const double* x, double* y, int, void* u) {
+  auto* cb = static_cast<CallbackUser*>(u);
+  cb->fn(x, y, cb->user);
+}
+}  // namespace
+
+tk_status tk_davidson(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, const double* diag, double tol,
+                      int32_t max_subspace, int32_t max_iter, double* x, double* eigenvalue, double* residual,
+                      int32_t* matvecs) {
+  return guard(ctx, [&] {
+    if (!ctx || !matvec || !x || n <= 0) throw tk::ArgError("bad Davidson arguments");
+    tk::DavidsonOpts o;
+    o.tol = tol;
+    o.max_subspace = max_subspace > 0 ? max_subspace : 60;
+    o.max_iter = max_iter > 0 ? max_iter : 1000;
+    o.precondition = diag != nullptr;
+    CallbackUser cb{matvec, user};
+    tk::DavidsonResult r = tk::davidson_generic(ctx->c.get(), n, callback_matvec, &cb, diag, false, x, o);
+    if (eigenvalue) *eigenvalue = r.eigenvalue;
+    if (residual) *residual = r.residual;
+    if (matvecs) *matvecs = r.matvecs;
   });
 }
 

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
                                                        const GTerm* __restrict__ terms,
                                                        const GSrc* __restrict__ srcs,
                                                        const TI* __restrict__ A, const TI* __restrict__ B,
-                                                       TO* __restrict__ C) {
+                                                       TO* __restrict__ C, double beta) {
   using M = Mfma<TI>;
   using acc_t = typename M::acc_t;
   const int lane = threadIdx.x & 63;
@@ -144,8 +144,8 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
     TO* __restrict__ Cp = C + o.c_off;
     auto put = [&](int row, int col, TI v) {
       if (row < o.m && col < o.n) {
-        if (TC) Cp[int64_t(col) * o.ldc + row] = TO(v);
-        else Cp[int64_t(row) * o.ldc + col] = TO(v);
+        TO* dst = TC ? &Cp[int64_t(col) * o.ldc + row] : &Cp[int64_t(row) * o.ldc + col];
+        *dst = beta == 0.0 ? TO(v) : TO(v) + TO(beta) * *dst;
       }
     };
 #pragma unroll
@@ -170,7 +170,7 @@ void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const 
 #define TK_CASE(F)                                                                                     \
   case F:                                                                                              \
     hipLaunchKernelGGL((contract_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
-                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C); \
+                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C, L.beta); \
     break;
   switch (L.flags & 7) {
     TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
@@ -203,6 +203,7 @@ void DeviceContractList::upload(Context* ctx, const ContractList& h) {
   n_items = int(h.items.size());
   flags = h.flags;
   flops = h.flops;
+  beta = h.beta;
 }
 
 void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C) {

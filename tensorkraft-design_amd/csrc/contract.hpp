@@ -49,6 +49,7 @@ struct ContractList {
   std::vector<GSrc> srcs;
   std::vector<GItem> items;
   int flags = 0;
+  double beta = 0;   // C = sum + beta * C (0: C is not read)
   double flops = 0;  // algorithmic flops of the list (2*m*n*k per term + 2*m*k per extra source)
 
   int begin_out(int64_t c_off, int m, int n, int ldc) {

@@ -42,7 +42,8 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __res
                                                               const GTerm* __restrict__ terms,
                                                               const GSrc* __restrict__ srcs,
                                                               const float* __restrict__ A,
-                                                              const float* __restrict__ B, TO* __restrict__ C) {
+                                                              const float* __restrict__ B, TO* __restrict__ C,
+                                                              double beta) {
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int nwaves = gridDim.x * (blockDim.x >> 6);
@@ -100,8 +101,8 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __res
     TO* __restrict__ Cp = C + o.c_off;
     auto put = [&](int row, int col, float v) {
       if (row < o.m && col < o.n) {
-        if (TC) Cp[int64_t(col) * o.ldc + row] = TO(v);
-        else Cp[int64_t(row) * o.ldc + col] = TO(v);
+        TO* dst = TC ? &Cp[int64_t(col) * o.ldc + row] : &Cp[int64_t(row) * o.ldc + col];
+        *dst = beta == 0.0 ? TO(v) : TO(v) + TO(beta) * *dst;
       }
     };
 #pragma unroll
@@ -124,7 +125,7 @@ void launch_bf16(Context* ctx, const DeviceContractList& L, const float* A, cons
 #define TK_CASE(F)                                                                                         \
   case F:                                                                                                  \
     hipLaunchKernelGGL((contract_bf16x3_kernel<TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
-                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C);   \
+                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C, L.beta); \
     break;
   switch (L.flags & 7) {
     TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)

@@ -19,6 +19,7 @@ struct DeviceContractList {
   int n_items = 0;
   int flags = 0;
   double flops = 0;
+  double beta = 0;  // C = sum + beta * C
   void upload(Context* ctx, const ContractList& h);
 };
 

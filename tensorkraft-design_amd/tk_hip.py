@@ -48,6 +48,13 @@ class tk_env(C.Structure):
     _fields_ = [("bond", tk_qindex), ("w", tk_qindex), ("data_dev", C.c_void_p)]
 
 
+class tk_block_matrix(C.Structure):
+    _fields_ = [("rows", tk_qindex), ("cols", tk_qindex), ("flux", C.c_int32 * 2), ("data_dev", C.c_void_p)]
+
+
+MATVEC_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
+
+
 class tk_dmrg_config(C.Structure):
     _fields_ = [("prec", C.c_int32), ("sym", C.c_int32), ("n_sites", C.c_int32), ("n_schedule", C.c_int32),
                 ("bond_schedule", C.POINTER(C.c_int64)), ("svd_cutoff", C.c_double), ("min_bond_dim", C.c_int64),
@@ -106,6 +113,14 @@ def lib():
         "tk_heff_stats": (I32, [P, pD, pD, pI64]),
         "tk_heff_stage_stats": (I32, [P, pD]),
         "tk_heff_time": (I32, [P, P, P, I32, pD, pD, pD]),
+        "tk_gemm": (I32, [P, I64, I64, I64, D, P, I64, I64, P, I64, I64, D, P, I64, I64]),
+        "tk_block_gemm_table": (I32, [I32, C.POINTER(tk_block_matrix), C.POINTER(tk_block_matrix), I64, pI64, pU64,
+                                      pI64, pI64]),
+        "tk_block_gemm": (I32, [P, I32, C.POINTER(tk_block_matrix), C.POINTER(tk_block_matrix), P]),
+        "tk_spmv": (I32, [P, I32, C.POINTER(tk_block_matrix), P, P]),
+        "tk_svd_truncated": (I32, [P, I64, I64, P, I64, D, P, pD, P, pI64, pD]),
+        "tk_eigh": (I32, [P, I64, P, pD, P]),
+        "tk_davidson": (I32, [P, I64, MATVEC_FN, P, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_dmrg_create": (I32, [P, C.POINTER(tk_dmrg_config), C.POINTER(tk_mpo_site), C.POINTER(P)]),
         "tk_dmrg_destroy": (I32, [P]),
         "tk_dmrg_set_state": (I32, [P, C.POINTER(tk_qindex), C.POINTER(C.POINTER(D))]),
@@ -128,7 +143,8 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_heff_plan_create", "tk_heff_plan_destroy", "tk_heff_psi_table", "tk_heff_apply",
             "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time",
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
-            "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site"]
+            "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
+            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson"]
 
 
 def _check(st, ctx=None):
@@ -293,6 +309,137 @@ class DeviceArray:
         if self.ptr:
             self.ctx.free(self.ptr)
             self.ptr = None
+
+
+# ---------------------------------------------------------------- backend traits
+class HipBackend:
+    """LinAlgBackend / SparseLinAlgBackend on the device (tk-linalg/src/traits.rs:25-205).
+
+    Methods take and return numpy arrays and move them through the context;
+    the device work is the library's.
+    """
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def gemm(self, alpha, a, b, beta, c):
+        """C = alpha A B + beta C (row-major numpy operands; returns the new C)."""
+        a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+        c = np.array(c, dtype=np.float64)
+        m, k = a.shape
+        k2, n = b.shape
+        if k2 != k or c.shape != (m, n):
+            raise ValueError("gemm: dimension mismatch")
+        da, db, dc = self.ctx.to_device(a), self.ctx.to_device(b), self.ctx.to_device(c)
+        _check(self.ctx.L.tk_gemm(self.ctx.h, m, n, k, alpha, C.c_void_p(da.ptr), k, 1, C.c_void_p(db.ptr), n, 1,
+                                  beta, C.c_void_p(dc.ptr), n, 1), self.ctx.h)
+        out = dc.get()
+        for x in (da, db, dc):
+            x.free()
+        return out
+
+    def gemm_strided(self, m, n, k, alpha, a_dev, rs_a, cs_a, b_dev, rs_b, cs_b, beta, c_dev, rs_c, cs_c):
+        _check(self.ctx.L.tk_gemm(self.ctx.h, m, n, k, alpha, C.c_void_p(a_dev.ptr), rs_a, cs_a,
+                                  C.c_void_p(b_dev.ptr), rs_b, cs_b, beta, C.c_void_p(c_dev.ptr), rs_c, cs_c),
+               self.ctx.h)
+
+    @staticmethod
+    def _bm(sym, rows, cols, flux, dev, keep):
+        r, c = QIndexArg(rows), QIndexArg(cols)
+        keep += [r, c]
+        f = (C.c_int32 * 2)(*((flux, 0) if np.isscalar(flux) else flux))
+        return tk_block_matrix(r.c, c.c, f, C.c_void_p(dev.ptr if dev is not None else None))
+
+    def block_gemm_table(self, sym, a_rows, a_cols, a_flux, b_rows, b_cols, b_flux):
+        keep = []
+        am = self._bm(sym, a_rows, a_cols, a_flux, None, keep)
+        bm = self._bm(sym, b_rows, b_cols, b_flux, None, keep)
+        n = C.c_int64(0)
+        tot = C.c_int64(0)
+        lib().tk_block_gemm_table(sym, C.byref(am), C.byref(bm), 0, C.byref(n), None, None, C.byref(tot))
+        keys = np.zeros(n.value, dtype=np.uint64)
+        offs = np.zeros(n.value, dtype=np.int64)
+        _check(lib().tk_block_gemm_table(sym, C.byref(am), C.byref(bm), n.value, C.byref(n),
+                                         keys.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         offs.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(tot)))
+        return keys, offs, tot.value
+
+    def block_gemm(self, sym, a_rows, a_cols, a_flux, a_data, b_rows, b_cols, b_flux, b_data):
+        """Returns (keys, offsets, data) of C = A B in FlatBlockStorage order."""
+        keys, offs, tot = self.block_gemm_table(sym, a_rows, a_cols, a_flux, b_rows, b_cols, b_flux)
+        keep = []
+        da, db = self.ctx.to_device(np.asarray(a_data, dtype=np.float64)), self.ctx.to_device(
+            np.asarray(b_data, dtype=np.float64))
+        dc = self.ctx.empty(max(tot, 1))
+        am = self._bm(sym, a_rows, a_cols, a_flux, da, keep)
+        bm = self._bm(sym, b_rows, b_cols, b_flux, db, keep)
+        _check(self.ctx.L.tk_block_gemm(self.ctx.h, sym, C.byref(am), C.byref(bm), C.c_void_p(dc.ptr)), self.ctx.h)
+        out = dc.get()[:tot]
+        for x in (da, db, dc):
+            x.free()
+        return keys, offs, out
+
+    def spmv(self, sym, rows, cols, flux, a_data, x):
+        keep = []
+        da, dx = self.ctx.to_device(np.asarray(a_data, dtype=np.float64)), self.ctx.to_device(
+            np.asarray(x, dtype=np.float64))
+        n_rows = sum(d for _, d in rows)
+        dy = self.ctx.empty(n_rows)
+        am = self._bm(sym, rows, cols, flux, da, keep)
+        _check(self.ctx.L.tk_spmv(self.ctx.h, sym, C.byref(am), C.c_void_p(dx.ptr), C.c_void_p(dy.ptr)), self.ctx.h)
+        out = dy.get()
+        for v in (da, dx, dy):
+            v.free()
+        return out
+
+    def svd_truncated(self, a, max_rank, cutoff):
+        """(U, s, Vt, truncation_error) of a row-major matrix (SvdResult, results.rs:10-24)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        m, n = a.shape
+        r = min(m, n)
+        da = self.ctx.to_device(a)
+        du, dv = self.ctx.empty(m * r), self.ctx.empty(r * n)
+        s = np.zeros(r)
+        rank = C.c_int64(0)
+        te = C.c_double(0)
+        _check(self.ctx.L.tk_svd_truncated(self.ctx.h, m, n, C.c_void_p(da.ptr), int(max_rank), cutoff,
+                                           C.c_void_p(du.ptr), s.ctypes.data_as(C.POINTER(C.c_double)),
+                                           C.c_void_p(dv.ptr), C.byref(rank), C.byref(te)), self.ctx.h)
+        k = rank.value
+        u = du.get()[:m * k].reshape(m, k)
+        vt = dv.get()[:k * n].reshape(k, n)
+        for x in (da, du, dv):
+            x.free()
+        return u, s[:k], vt, te.value
+
+    def eigh(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = a.shape[0]
+        da = self.ctx.to_device(a)
+        dv = self.ctx.empty(n * n)
+        w = np.zeros(n)
+        _check(self.ctx.L.tk_eigh(self.ctx.h, n, C.c_void_p(da.ptr), w.ctypes.data_as(C.POINTER(C.c_double)),
+                                  C.c_void_p(dv.ptr)), self.ctx.h)
+        v = dv.get().reshape(n, n)
+        da.free()
+        dv.free()
+        return w, v
+
+    def davidson(self, matvec_dev, n, x0, diag=None, tol=1e-10, max_subspace=60, max_iter=1000):
+        """Lowest eigenpair of a device matvec(x_ptr, y_ptr) (IterativeEigensolver)."""
+        cb = MATVEC_FN(lambda x, y, u: matvec_dev(x, y))
+        dx = self.ctx.to_device(np.asarray(x0, dtype=np.float64))
+        dd = self.ctx.to_device(np.asarray(diag, dtype=np.float64)) if diag is not None else None
+        ev, rn = C.c_double(0), C.c_double(0)
+        nmv = C.c_int32(0)
+        _check(self.ctx.L.tk_davidson(self.ctx.h, n, cb, None, C.c_void_p(dd.ptr) if dd else None, tol, max_subspace,
+                                      max_iter, C.c_void_p(dx.ptr), C.byref(ev), C.byref(rn), C.byref(nmv)),
+               self.ctx.h)
+        x = dx.get()
+        dx.free()
+        if dd:
+            dd.free()
+        return ev.value, x, rn.value, nmv.value
 
 
 # ---------------------------------------------------------------- H_eff plan
