@@ -230,6 +230,9 @@ typedef struct {
   int32_t site, left_to_right, iterations, matvecs, matvecs_fp64, converged;
   double energy, truncation_error, residual;
   int64_t bond_dim;
+  int64_t dim;       /* two-site vector length */
+  int64_t svd_nmax;  /* largest Gram block of the SVD */
+  double svd_n3;     /* sum of n^3 over the Gram blocks */
 } tk_step_stats;
 
 /* DMRGEngine::new (sweep/mod.rs:234-265) with the MPO given site by site

@@ -583,6 +583,9 @@ tk_status tk_dmrg_steps(tk_dmrg* d, int32_t capacity, int32_t* n, tk_step_stats*
       out[k].truncation_error = st[k].truncation_error;
       out[k].residual = st[k].residual;
       out[k].bond_dim = st[k].bond_dim;
+      out[k].dim = st[k].dim;
+      out[k].svd_nmax = st[k].svd_nmax;
+      out[k].svd_n3 = st[k].svd_n3;
     }
   });
 }

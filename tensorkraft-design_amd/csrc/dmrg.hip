@@ -403,6 +403,11 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
     }
   QIndex newB(nb);
   st->bond_dim = newB.total;
+  st->dim = psi.total;
+  for (const Mid& m : mids) {
+    st->svd_nmax = std::max(st->svd_nmax, m.n);
+    st->svd_n3 += double(m.n) * double(m.n) * double(m.n);
+  }
   Site Ai, Aj;
   Ai.t = make_block_table(cfg_.sym, {Pi, BL, newB}, {Dir::Out, Dir::Out, Dir::In}, QN{});
   Aj.t = make_block_table(cfg_.sym, {Pj, newB, BR}, {Dir::Out, Dir::Out, Dir::In}, QN{});

@@ -45,6 +45,9 @@ struct StepStats {
   int iterations = 0, matvecs = 0, matvecs_fp64 = 0;
   bool converged = true;
   double residual = 0;
+  int64_t dim = 0;        // two-site vector length
+  int64_t svd_nmax = 0;   // largest Gram block of the split
+  double svd_n3 = 0;      // sum of n^3 over the Gram blocks
 };
 
 struct SweepStats {

@@ -73,7 +73,7 @@ class tk_step_stats(C.Structure):
     _fields_ = [("site", C.c_int32), ("left_to_right", C.c_int32), ("iterations", C.c_int32),
                 ("matvecs", C.c_int32), ("matvecs_fp64", C.c_int32), ("converged", C.c_int32),
                 ("energy", C.c_double), ("truncation_error", C.c_double), ("residual", C.c_double),
-                ("bond_dim", C.c_int64)]
+                ("bond_dim", C.c_int64), ("dim", C.c_int64), ("svd_nmax", C.c_int64), ("svd_n3", C.c_double)]
 
 
 _lib = None
