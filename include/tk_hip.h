@@ -150,11 +150,12 @@ tk_status tk_spmv(tk_ctx* ctx, int32_t sym, const tk_block_matrix* a, const doub
  * sigma_max (numerical zeros, sigma^2 <= 1e-15 sigma_max^2, are dropped);
  * U (m x rank), Vt (rank x n) row-major on the device, s (host) descending,
  * truncation_error = sum of discarded sigma^2 (results.rs:18-23).  min(m, n)
- * must not exceed 1024. */
+ * must not exceed 2048. */
 tk_status tk_svd_truncated(tk_ctx* ctx, int64_t m, int64_t n, const double* a_dev, int64_t max_rank, double cutoff,
                            double* u_dev, double* s_host, double* vt_dev, int64_t* rank, double* truncation_error);
 /* LinAlgBackend::eigh_lowest (traits.rs:134-138), all pairs: eigenvalues
- * ascending (host), eigenvectors as columns of a row-major n x n (device). */
+ * ascending (host), eigenvectors as columns of a row-major n x n (device);
+ * n <= 2048. */
 tk_status tk_eigh(tk_ctx* ctx, int64_t n, const double* a_dev, double* evals_host, double* evecs_dev);
 
 /* IterativeEigensolver::lowest_eigenpair (tk-dmrg/src/eigensolver/mod.rs:38-55)

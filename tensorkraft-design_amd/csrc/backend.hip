@@ -7,7 +7,7 @@
 #include <cmath>
 #include <map>
 
-#include "eig.hpp"
+#include "tdeig.hpp"
 #include "kernels.hpp"
 
 namespace tk {
@@ -159,16 +159,14 @@ SvdOut svd_truncated(Context* ctx, int64_t m, int64_t n, const double* A, int64_
   if (m <= 0 || n <= 0) throw ArgError("svd: empty matrix");
   const bool rows_side = m < n;  // Gram of the smaller side
   const int64_t g = rows_side ? m : n;
-  if (g > eig_max_n()) throw ArgError("svd: the smaller dimension exceeds the eigensolver limit");
-  DBuf<double> G(ctx, size_t(g * g)), Vws(ctx, size_t(g * g)), E(ctx, size_t(g * g)), W(ctx, size_t(g));
+  if (g > kSymEigMaxN) throw ArgError("svd: the smaller dimension exceeds the eigensolver limit");
+  DBuf<double> G(ctx, size_t(g * g)), E(ctx, size_t(g * g)), W(ctx, size_t(g));
   if (rows_side)  // G = A A^T
     dense_gemm(ctx, m, m, n, 1.0, A, n, 1, A, 1, n, 0.0, G.get(), m, 1);
   else  // G = A^T A
     dense_gemm(ctx, n, n, m, 1.0, A, 1, n, A, n, 1, 0.0, G.get(), n, 1);
-  EigJob job{0, 0, 0, int32_t(g), 0};
-  DBuf<EigJob> dj(ctx, 1);
-  dj.upload(&job, 1);
-  launch_eig(ctx, dj.get(), 1, G.get(), Vws.get(), W.get(), E.get(), true);
+  SymEigBatch eig(ctx);
+  eig.values({SymJob{0, 0, int32_t(g)}}, G.get(), g * g, W.get(), g, true);
   std::vector<double> lam(static_cast<size_t>(g));
   W.download(lam.data(), lam.size());
   ctx->sync();
@@ -188,6 +186,7 @@ SvdOut svd_truncated(Context* ctx, int64_t m, int64_t n, const double* A, int64_
   out.rank = r;
   out.truncation_error = std::max(0.0, total - kept);  // SvdResult: sum of discarded sigma^2
   if (r == 0) return out;
+  eig.vectors({r}, E.get(), g * g);
   // scale columns of the eigenvectors by 1/sigma to form the other factor
   std::vector<double> inv(static_cast<size_t>(r));
   for (int64_t j = 0; j < r; ++j) inv[j] = 1.0 / out.s[j];
@@ -228,13 +227,12 @@ SvdOut svd_truncated(Context* ctx, int64_t m, int64_t n, const double* A, int64_
 }
 
 std::vector<double> eigh(Context* ctx, int64_t n, const double* A, double* evecs) {
-  if (n <= 0 || n > eig_max_n()) throw ArgError("eigh: dimension outside 1..1024");
-  DBuf<double> G(ctx, size_t(n * n)), Vws(ctx, size_t(n * n)), W(ctx, size_t(n));
+  if (n <= 0 || n > kSymEigMaxN) throw ArgError("eigh: dimension outside 1..2048");
+  DBuf<double> G(ctx, size_t(n * n)), W(ctx, size_t(n));
   TK_HIP(hipMemcpyAsync(G.get(), A, size_t(n * n) * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-  EigJob job{0, 0, 0, int32_t(n), 0};
-  DBuf<EigJob> dj(ctx, 1);
-  dj.upload(&job, 1);
-  launch_eig(ctx, dj.get(), 1, G.get(), Vws.get(), W.get(), evecs, false);
+  SymEigBatch eig(ctx);
+  eig.values({SymJob{0, 0, int32_t(n)}}, G.get(), n * n, W.get(), n, false);
+  eig.vectors({n}, evecs, n * n);
   std::vector<double> w(static_cast<size_t>(n));
   W.download(w.data(), w.size());
   ctx->sync();

@@ -5,25 +5,12 @@
 #include <cmath>
 #include <vector>
 
-#include "eig.hpp"
 #include "kernels.hpp"
+#include "tdeig.hpp"
 
 namespace tk {
 namespace {
 
-// P (n x n) <- symmetric matrix whose lower triangle is stored row-wise in H (ld).
-__global__ void pack_sym_kernel(const double* __restrict__ H, int ld, int n, double* __restrict__ P) {
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    const int i = e / n, j = e - i * n;
-    P[e] = j <= i ? H[i * ld + j] : H[j * ld + i];
-  }
-}
-// lowest Ritz pair of an ascending eigen-decomposition: c = column 0, theta = W[0]
-__global__ void ritz_kernel(const double* __restrict__ W, const double* __restrict__ E, int n,
-                            double* __restrict__ c, double* __restrict__ theta) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) c[i] = E[i * n];
-  if (threadIdx.x == 0) *theta = W[0];
-}
 __global__ void residual_dev_kernel(const double* __restrict__ ay, const double* __restrict__ y,
                                     const double* __restrict__ theta, double* __restrict__ r, int64_t n) {
   const double th = *theta;
@@ -48,8 +35,7 @@ int grid1(Context* ctx, int64_t n) {
 }
 
 struct Workspace {
-  DBuf<double> V, AV, H, P, Wv, E, Vws, c, h, scal, y, ay, r, diag;
-  DBuf<EigJob> job;
+  DBuf<double> V, AV, H, c, h, scal, y, ay, r;
 };
 
 }  // namespace
@@ -64,17 +50,12 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
   w.V.reset(ctx, size_t(m) * n);
   w.AV.reset(ctx, size_t(m) * n);
   w.H.reset(ctx, size_t(m) * m);
-  w.P.reset(ctx, size_t(m) * m);
-  w.Wv.reset(ctx, m);
-  w.E.reset(ctx, size_t(m) * m);
-  w.Vws.reset(ctx, size_t(m) * m);
   w.c.reset(ctx, m);
   w.h.reset(ctx, m);
   w.scal.reset(ctx, 8);
   w.y.reset(ctx, n);
   w.ay.reset(ctx, n);
   w.r.reset(ctx, n);
-  w.job.reset(ctx, 1);
   double* V = w.V.get();
   double* AV = w.AV.get();
   double* H = w.H.get();
@@ -102,12 +83,8 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
   int stall = 0;
   double host[3] = {0, 0, 0};
   for (int it = 0; it < opts.max_iter; ++it) {
-    // Ritz pair of the projected matrix (Jacobi on the device, ascending)
-    hipLaunchKernelGGL(pack_sym_kernel, dim3(1), dim3(256), 0, s, H, m, nsub, w.P.get());
-    EigJob jb{0, 0, 0, nsub, 0};
-    w.job.upload(&jb, 1);
-    launch_eig(ctx, w.job.get(), 1, w.P.get(), w.Vws.get(), w.Wv.get(), w.E.get(), false);
-    hipLaunchKernelGGL(ritz_kernel, dim3(1), dim3(64), 0, s, w.Wv.get(), w.E.get(), nsub, w.c.get(), theta);
+    // lowest Ritz pair of the projected matrix (one wavefront on the device)
+    lowest_eigpair(ctx, H, m, nsub, w.c.get(), theta);
     combine(ctx, V, n, nsub, w.c.get(), 0.0, w.y.get(), n);
     combine(ctx, AV, n, nsub, w.c.get(), 0.0, w.ay.get(), n);
     hipLaunchKernelGGL(residual_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), w.y.get(), theta,

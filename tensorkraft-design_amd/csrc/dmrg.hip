@@ -8,7 +8,7 @@
 #include <numeric>
 #include <tuple>
 
-#include "eig.hpp"
+#include "tdeig.hpp"
 #include "kernels.hpp"
 
 namespace tk {
@@ -303,7 +303,7 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
   int64_t gtot = 0, wtot = 0;
   for (Mid& m : mids) {
     m.n = lr ? m.nrow : m.ncol;
-    if (m.n > eig_max_n()) throw ArgError("SVD block larger than the eigensolver limit");
+    if (m.n > kSymEigMaxN) throw ArgError("SVD block larger than the eigensolver limit");
     m.g_off = gtot;
     m.w_off = wtot;
     gtot += m.n * m.n;
@@ -341,17 +341,12 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
     }
   }
   run_list(ctx_, gl, psi.total, psi.total, gtot, x, x, G.get());
-  // eigen-decomposition of every Gram block (descending)
-  std::vector<EigJob> jobs;
-  for (Mid& m : mids) {
-    if (m.g_off + m.n * m.n > gtot || m.w_off + m.n > wtot) throw std::out_of_range("eig job outside its buffers");
-    jobs.push_back({m.g_off, m.w_off, m.g_off, int32_t(m.n), 0});
-  }
-  DBuf<EigJob> dj(ctx_, jobs.size());
-  dj.upload(jobs.data(), jobs.size());
-  DBuf<double> Vws(ctx_, size_t(std::max<int64_t>(1, gtot))), Wd(ctx_, size_t(std::max<int64_t>(1, wtot))),
-      E(ctx_, size_t(std::max<int64_t>(1, gtot)));
-  launch_eig(ctx_, dj.get(), int(jobs.size()), G.get(), Vws.get(), Wd.get(), E.get(), true);
+  // eigenvalues of every Gram block (descending); vectors only for the kept ones
+  std::vector<SymJob> jobs;
+  for (Mid& m : mids) jobs.push_back({m.g_off, m.w_off, int32_t(m.n)});
+  DBuf<double> Wd(ctx_, size_t(std::max<int64_t>(1, wtot))), E(ctx_, size_t(std::max<int64_t>(1, gtot)));
+  SymEigBatch eig(ctx_);
+  eig.values(jobs, G.get(), gtot, Wd.get(), wtot, true);
   std::vector<double> lam(static_cast<size_t>(wtot));
   Wd.download(lam.data(), lam.size());
   ctx_->sync();
@@ -393,6 +388,11 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
   }
   st->truncation_error = sum_all > 0 ? (sum_all - sum_kept) / sum_all : 0.0;
   const double renorm = sum_kept > 0 ? 1.0 / std::sqrt(sum_kept) : 1.0;
+  {
+    std::vector<int64_t> keep_of;
+    for (const Mid& m : mids) keep_of.push_back(m.keep);
+    eig.vectors(keep_of, E.get(), gtot);
+  }
   // new bond
   std::vector<std::pair<QN, int64_t>> nb;
   std::map<QN, int> mid_of;

@@ -1,0 +1,990 @@
+// Batched FP64 symmetric eigensolver: Householder tridiagonalisation, Sturm
+// bisection, inverse iteration, blocked Gram-Schmidt and back-transformation.
+// Reference call sites and the method are summarised in tdeig.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+#include "tdeig.hpp"
+
+namespace tk {
+namespace {
+
+constexpr double kEps = DBL_EPSILON;
+constexpr int kLuFields = 5;
+
+struct DevJob {
+  int64_t a_off;   // matrix / reflectors (and output columns in Z)
+  int64_t w_off;   // eigenvalues, d, e, tau
+  int64_t z_off;   // inverse-iteration vectors: n x k row-major (ld k)
+  int64_t lu_off;  // LU factors: kLuFields doubles per (row, vector), interleaved by vector
+  int32_t n, k;
+  int32_t job;     // index of the job in the caller's list
+  int32_t pad;
+};
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// Sum over the workgroup; `red` holds 2 * (NT / 64) doubles used alternately,
+// so consecutive calls need no second barrier.
+template <int NT>
+__device__ double block_sum(double x, double* red, int& parity) {
+  constexpr int NW = NT / 64;
+  x = wave_sum(x);
+  double* r = red + parity * NW;
+  parity ^= 1;
+  if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = x;
+  __syncthreads();
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) s += r[i];
+  return s;
+}
+
+// Number of eigenvalues of the tridiagonal (d, e^2) smaller than x.
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+  double q = d[0] - x;
+  if (fabs(q) <= pivmin) q = -pivmin;
+  int cnt = q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+// Deterministic start vectors in [-1, 1).
+__device__ __forceinline__ double hash_unit(uint32_t a, uint32_t b) {
+  uint32_t h = a * 0x9E3779B1u + (b + 0x632BE5ABu) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return double(h >> 8) * (2.0 / 16777216.0) - 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// 1. Tridiagonalisation.  Step k: x = A[k][k+1:] (row k of the updated upper
+// triangle = column k below the diagonal), reflector H_k = I - tau v v^T with
+// v[k+1] = 1, T[k+1][k] = beta; p = A22 v, w = tau p - tau^2/2 (p.v) v,
+// A22 -= v w^T + w v^T.  The update of step k-1 is applied to row k first,
+// then to rows k+1.. in the same pass that forms A22 v for step k.
+// T = max number of 64-column tiles of a row (n <= 64 T).
+template <int NT, int T>
+__global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ jobs, double* __restrict__ A,
+                                                     double* __restrict__ Dg, double* __restrict__ Eg,
+                                                     double* __restrict__ TAUg) {
+  constexpr int NW = NT / 64;
+  constexpr int R = (64 * T + NT - 1) / NT;
+  constexpr bool kCache = T <= 16;  // keep v, w, vn of the lane's columns in registers
+  const DevJob jb = jobs[blockIdx.x];
+  const int n = jb.n;
+  double* a = A + jb.a_off;
+  double* d = Dg + jb.w_off;
+  double* e = Eg + jb.w_off;
+  double* tau = TAUg + jb.w_off;
+  extern __shared__ double sm[];
+  double* v = sm;
+  double* w = v + n;
+  double* vn = w + n;
+  double* prow = vn + n;
+  double* pw = prow + n;      // NW x n column partials, summed in wave order
+  double* red = pw + NW * n;  // 2 NW
+  double* scal = red + 2 * NW;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int parity = 0;
+  if (n <= 2) {
+    if (tid == 0) {
+      d[0] = a[0];
+      e[0] = 0;
+      tau[0] = 0;
+      if (n == 2) {
+        e[0] = a[1];
+        d[1] = a[3];
+        e[1] = 0;
+        tau[1] = 0;
+      }
+    }
+    return;
+  }
+  for (int j = tid; j < n; j += NT) {
+    v[j] = 0;
+    w[j] = 0;
+  }
+  __syncthreads();
+  for (int k = 0; k + 2 < n; ++k) {
+    // (a) row k with the pending update; sum of squares of x[1:]
+    const double vk = v[k], wk = w[k];
+    double* rowk = a + int64_t(k) * n;
+    double xr[R];
+    double ss = 0;
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int j = k + tid + u * NT;
+      xr[u] = 0;
+      if (j < n) {
+        const double x = rowk[j] - (vk * w[j] + wk * v[j]);
+        xr[u] = x;
+        if (j == k) d[k] = x;
+        if (j == k + 1) scal[0] = x;
+        if (j >= k + 2) ss += x * x;
+      }
+    }
+    const double sigma = block_sum<NT>(ss, red, parity);
+    const double alpha = scal[0];
+    double tk = 0, beta = alpha, scale = 0;
+    if (sigma > 0) {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0 ? -nrm : nrm;
+      tk = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int j = k + tid + u * NT;
+      if (j < n && j > k) {
+        const double vj = j == k + 1 ? 1.0 : xr[u] * scale;
+        vn[j] = vj;
+        if (j >= k + 2) rowk[j] = vj;
+      }
+    }
+    if (tid == 0) {
+      e[k] = beta;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    // (c) rows k+1..n-1: pending update, then row and column parts of A22 vn
+    double cacc[T];
+    double vj[kCache ? T : 1], wj[kCache ? T : 1], vnj[kCache ? T : 1];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      cacc[t] = 0;
+      if constexpr (kCache) {
+        const int j = 64 * t + lane;
+        const bool in = j > k && j < n;
+        vj[t] = in ? v[j] : 0.0;
+        wj[t] = in ? w[j] : 0.0;
+        vnj[t] = in ? vn[j] : 0.0;
+      }
+    }
+    const int first = k + 1;
+    const int i0 = first + ((wv - first % NW) + NW) % NW;
+    for (int i = i0; i < n; i += NW) {
+      const double vi = v[i], wi = w[i], vni = vn[i];
+      double* row = a + int64_t(i) * n;
+      double xv[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int j = 64 * t + lane;
+        xv[t] = (j >= i && j < n) ? row[j] : 0.0;
+      }
+      double racc = 0;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const int j = 64 * t + lane;
+        if (j >= i && j < n) {
+          double vjt, wjt, vnjt;
+          if constexpr (kCache) {
+            vjt = vj[t];
+            wjt = wj[t];
+            vnjt = vnj[t];
+          } else {
+            vjt = v[j];
+            wjt = w[j];
+            vnjt = vn[j];
+          }
+          const double x = xv[t] - (vi * wjt + wi * vjt);
+          row[j] = x;
+          racc += x * vnjt;
+          if (j > i) cacc[t] += x * vni;
+        }
+      }
+      racc = wave_sum(racc);
+      if (lane == 0) prow[i] = racc;
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int j = 64 * t + lane;
+      if (j > k && j < n) pw[wv * n + j] = cacc[t];
+    }
+    __syncthreads();
+    // (d) p = row part + column parts (fixed order); w = tau p - tau^2/2 (p.v) v; v <- vn
+    double pv = 0;
+    for (int j = k + 1 + tid; j < n; j += NT) {
+      double p = prow[j];
+#pragma unroll
+      for (int q = 0; q < NW; ++q) p += pw[q * n + j];
+      prow[j] = p;
+      pv += p * vn[j];
+    }
+    pv = block_sum<NT>(pv, red, parity);
+    const double alw = -0.5 * tk * tk * pv;
+    for (int j = k + 1 + tid; j < n; j += NT) {
+      w[j] = tk * prow[j] + alw * vn[j];
+      v[j] = vn[j];
+    }
+    if (tid == 0) {
+      v[k] = 0;
+      w[k] = 0;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int p = n - 2, q = n - 1;
+    const double app = a[int64_t(p) * n + p] - 2.0 * v[p] * w[p];
+    const double apq = a[int64_t(p) * n + q] - (v[p] * w[q] + w[p] * v[q]);
+    const double aqq = a[int64_t(q) * n + q] - 2.0 * v[q] * w[q];
+    d[p] = app;
+    e[p] = apq;
+    d[q] = aqq;
+    e[q] = 0;
+    tau[p] = 0;
+    tau[q] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Bisection: thread j of item (job, j0) finds eigenvalue j0 + j (ascending
+// index) to max(2 eps |lambda|, 2 eps |T|) absolute width.
+struct BisItem {
+  int32_t job, j0;
+};
+
+template <int NT>
+__global__ __launch_bounds__(NT) void bisect_kernel(const BisItem* __restrict__ items, const DevJob* __restrict__ jobs,
+                                                    const double* __restrict__ Dg, const double* __restrict__ Eg,
+                                                    double* __restrict__ W, double* __restrict__ TN, int descending) {
+  constexpr int NW = NT / 64;
+  const BisItem it = items[blockIdx.x];
+  const DevJob jb = jobs[it.job];
+  const int n = jb.n;
+  extern __shared__ double sm[];
+  double* d = sm;
+  double* e2 = d + n;
+  __shared__ double rmin[NW], rmax[NW], re2[NW];
+  const double* dg = Dg + jb.w_off;
+  const double* eg = Eg + jb.w_off;
+  const int tid = threadIdx.x;
+  double lo_l = INFINITY, hi_l = -INFINITY, emax = 0;
+  for (int i = tid; i < n; i += NT) {
+    const double di = dg[i];
+    const double el = i > 0 ? fabs(eg[i - 1]) : 0.0;
+    const double er = i + 1 < n ? fabs(eg[i]) : 0.0;
+    d[i] = di;
+    if (i + 1 < n) {
+      e2[i] = eg[i] * eg[i];
+      emax = fmax(emax, e2[i]);
+    }
+    lo_l = fmin(lo_l, di - el - er);
+    hi_l = fmax(hi_l, di + el + er);
+  }
+  lo_l = wave_min(lo_l);
+  hi_l = wave_max(hi_l);
+  emax = wave_max(emax);
+  if ((tid & 63) == 0) {
+    rmin[tid >> 6] = lo_l;
+    rmax[tid >> 6] = hi_l;
+    re2[tid >> 6] = emax;
+  }
+  __syncthreads();
+  double gl = rmin[0], gu = rmax[0], e2max = re2[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) {
+    gl = fmin(gl, rmin[i]);
+    gu = fmax(gu, rmax[i]);
+    e2max = fmax(e2max, re2[i]);
+  }
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  const double pivmin = DBL_MIN * fmax(1.0, e2max);
+  gl -= 4.0 * kEps * tnorm + 2.0 * pivmin;
+  gu += 4.0 * kEps * tnorm + 2.0 * pivmin;
+  if (it.j0 == 0 && tid == 0) TN[it.job] = tnorm;
+  const int j = it.j0 + tid;
+  if (j >= n) return;
+  const double atol = 2.0 * kEps * tnorm;
+  double lo = gl, hi = gu;
+  for (int iter = 0; iter < 256; ++iter) {
+    if (hi - lo <= fmax(atol, 2.0 * kEps * fmax(fabs(lo), fabs(hi)))) break;
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(d, e2, n, mid, pivmin) > j) hi = mid;
+    else lo = mid;
+  }
+  W[jb.w_off + (descending ? n - 1 - j : j)] = 0.5 * (lo + hi);
+}
+
+// ---------------------------------------------------------------------------
+// 3a. Inverse iteration, one thread per wanted vector: T - lambda I = P L U
+// with partial pivoting (U has two superdiagonals), three solves from a
+// hashed start, each normalised by its largest entry; finally unit 2-norm.
+struct VecItem {
+  int32_t job, c;
+};
+
+__global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ items, int n_items,
+                                                   const DevJob* __restrict__ jobs, const double* __restrict__ Dg,
+                                                   const double* __restrict__ Eg, const double* __restrict__ W,
+                                                   const double* __restrict__ TN, double* __restrict__ LU,
+                                                   double* __restrict__ Z) {
+  const int g = blockIdx.x * 64 + threadIdx.x;
+  if (g >= n_items) return;
+  const VecItem it = items[g];
+  const DevJob jb = jobs[it.job];
+  const int n = jb.n, k = jb.k, c = it.c;
+  const double* d = Dg + jb.w_off;
+  const double* e = Eg + jb.w_off;
+  const double lam = W[jb.w_off + c];
+  double* lu = LU + jb.lu_off;
+  double* z = Z + jb.z_off;
+  // row i of the factors: U0 (pivot), U1, U2 (superdiagonals), multiplier, swap flag
+  auto F = [&](int i, int f) -> double& { return lu[(int64_t(i) * kLuFields + f) * k + c]; };
+  auto Y = [&](int i) -> double& { return z[int64_t(i) * k + c]; };
+  if (n == 1) {
+    Y(0) = 1.0;
+    return;
+  }
+  const double piv = kEps * fmax(TN[jb.job], DBL_MIN);
+  auto guard = [&](double x) { return fabs(x) < piv ? (x < 0 ? -piv : piv) : x; };
+  double c0 = d[0] - lam, c1 = e[0];
+  for (int i = 0; i + 1 < n; ++i) {
+    const double ei = e[i];
+    const double dn = d[i + 1] - lam;
+    const double en = i + 2 < n ? e[i + 1] : 0.0;
+    if (fabs(c0) >= fabs(ei)) {
+      c0 = guard(c0);
+      const double m = ei / c0;
+      F(i, 0) = c0;
+      F(i, 1) = c1;
+      F(i, 2) = 0.0;
+      F(i, 3) = m;
+      F(i, 4) = 0.0;
+      c0 = dn - m * c1;
+      c1 = en;
+    } else {
+      const double m = c0 / ei;
+      F(i, 0) = ei;
+      F(i, 1) = dn;
+      F(i, 2) = en;
+      F(i, 3) = m;
+      F(i, 4) = 1.0;
+      c0 = c1 - m * dn;
+      c1 = -m * en;
+    }
+  }
+  F(n - 1, 0) = guard(c0);
+  F(n - 1, 1) = 0.0;
+  F(n - 1, 2) = 0.0;
+  for (int i = 0; i < n; ++i) Y(i) = hash_unit(uint32_t(c) * 2654435761u + uint32_t(jb.job), uint32_t(i));
+  double s = 1.0;
+  for (int itn = 0; itn < 3; ++itn) {
+    // forward: row interchanges and multipliers applied to s * y
+    double cur = s * Y(0);
+    for (int i = 0; i + 1 < n; ++i) {
+      double nxt = s * Y(i + 1);
+      if (F(i, 4) != 0.0) {
+        const double t = cur;
+        cur = nxt;
+        nxt = t;
+      }
+      nxt -= F(i, 3) * cur;
+      Y(i) = cur;
+      cur = nxt;
+    }
+    Y(n - 1) = cur;
+    // back substitution with the two superdiagonals
+    double z1 = 0, z2 = 0, zmax = 0;
+    for (int i = n - 1; i >= 0; --i) {
+      const double zi = (Y(i) - F(i, 1) * z1 - F(i, 2) * z2) / F(i, 0);
+      Y(i) = zi;
+      zmax = fmax(zmax, fabs(zi));
+      z2 = z1;
+      z1 = zi;
+    }
+    s = zmax > 0 ? 1.0 / zmax : 1.0;
+  }
+  double nrm = 0;
+  for (int i = 0; i < n; ++i) {
+    const double y = Y(i) * s;
+    nrm += y * y;
+  }
+  const double f = nrm > 0 ? s / sqrt(nrm) : 1.0;
+  for (int i = 0; i < n; ++i) Y(i) *= f;
+}
+
+// ---------------------------------------------------------------------------
+// 3b. Orthonormalisation of the k vectors of a job (columns of the n x k
+// row-major Z, ld k) in column order: each block of NB columns is projected
+// against all earlier columns (two passes, 64 earlier columns at a time), then
+// orthonormalised within the block (two passes per column).  A column that
+// keeps less than 1e-3 of its norm was numerically dependent on the earlier
+// ones; it restarts from a hashed vector projected against every earlier
+// column.
+template <int NT, int NB>
+__global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ jobs, double* __restrict__ Z) {
+  constexpr int NW = NT / 64;
+  const DevJob jb = jobs[blockIdx.x];
+  const int n = jb.n, k = jb.k;
+  double* z = Z + jb.z_off;
+  extern __shared__ double sm[];
+  double* Zb = sm;                      // n x NB: the current block
+  double* Hp = Zb + int64_t(n) * NB;    // NW x 64 x NB partial projections
+  double* H = Hp + NW * 64 * NB;        // 64 x NB projections
+  double* red = H + 64 * NB;            // NW x NB
+  double* hs = red + NW * NB;           // NB workgroup sums
+  double* nrm0 = hs + NB;               // NB norms on entry
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+  // hs[c] = sum over the workgroup of vals[c], c < nv
+  auto block_sums = [&](double(&vals)[NB], int nv) {
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c < nv) {
+        const double s = wave_sum(vals[c]);
+        if (lane == 0) red[wv * NB + c] = s;
+      }
+    __syncthreads();
+    if (tid < nv) {
+      double s = 0;
+      for (int q = 0; q < NW; ++q) s += red[q * NB + tid];
+      hs[tid] = s;
+    }
+    __syncthreads();
+  };
+  // one classical pass: columns [c_lo, c_hi) of Zb minus their projection on
+  // z[:, 0:p_end), 64 earlier columns at a time
+  auto project = [&](int p_end, int c_lo, int c_hi) {
+    for (int p0 = 0; p0 < p_end; p0 += 64) {
+      const int pc = min(64, p_end - p0);
+      double acc[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[c] = 0;
+      if (lane < pc)
+        for (int i = wv; i < n; i += NW) {
+          const double zp = z[int64_t(i) * k + p0 + lane];
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c >= c_lo && c < c_hi) acc[c] += zp * Zb[i * NB + c];
+        }
+#pragma unroll
+      for (int c = 0; c < NB; ++c) Hp[(wv * 64 + lane) * NB + c] = acc[c];
+      __syncthreads();
+      for (int e = tid; e < 64 * NB; e += NT) {
+        double s = 0;
+        for (int q = 0; q < NW; ++q) s += Hp[q * 64 * NB + e];
+        H[e] = s;
+      }
+      __syncthreads();
+      for (int i = tid; i < n; i += NT) {
+        double upd[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) upd[c] = 0;
+        for (int p = 0; p < pc; ++p) {
+          const double zp = z[int64_t(i) * k + p0 + p];
+#pragma unroll
+          for (int c = 0; c < NB; ++c) upd[c] += zp * H[p * NB + c];
+        }
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c >= c_lo && c < c_hi) Zb[i * NB + c] -= upd[c];
+      }
+      __syncthreads();
+    }
+  };
+
+  for (int b0 = 0; b0 < k; b0 += NB) {
+    const int nb = min(NB, k - b0);
+    for (int e = tid; e < n * NB; e += NT) {
+      const int i = e / NB, c = e - i * NB;
+      Zb[e] = c < nb ? z[int64_t(i) * k + b0 + c] : 0.0;
+    }
+    __syncthreads();
+    {
+      double part[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) part[c] = 0;
+      for (int i = tid; i < n; i += NT)
+#pragma unroll
+        for (int c = 0; c < NB; ++c) part[c] += Zb[i * NB + c] * Zb[i * NB + c];
+      block_sums(part, nb);
+      if (tid < nb) nrm0[tid] = sqrt(hs[tid]);
+      __syncthreads();
+    }
+    if (b0 > 0) {
+      project(b0, 0, nb);
+      project(b0, 0, nb);
+    }
+    for (int c = 0; c < nb; ++c) {
+      for (int attempt = 0;; ++attempt) {
+        for (int pass = 0; pass < 2 && c > 0; ++pass) {
+          double part[NB];
+#pragma unroll
+          for (int q = 0; q < NB; ++q) part[q] = 0;
+          for (int i = tid; i < n; i += NT) {
+            const double zc = Zb[i * NB + c];
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+              if (q < c) part[q] += Zb[i * NB + q] * zc;
+          }
+          block_sums(part, c);
+          for (int i = tid; i < n; i += NT) {
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+              if (q < c) s += hs[q] * Zb[i * NB + q];
+            Zb[i * NB + c] -= s;
+          }
+          __syncthreads();
+        }
+        double part[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) part[q] = 0;
+        for (int i = tid; i < n; i += NT) part[0] += Zb[i * NB + c] * Zb[i * NB + c];
+        block_sums(part, 1);
+        const double nu = sqrt(hs[0]);
+        if (nu > 1e-3 * nrm0[c] || attempt == 2) {
+          const double f = nu > 0 ? 1.0 / nu : 0.0;
+          for (int i = tid; i < n; i += NT) Zb[i * NB + c] *= f;
+          __syncthreads();
+          break;
+        }
+        // numerically dependent: restart from a hashed vector
+        for (int i = tid; i < n; i += NT)
+          Zb[i * NB + c] =
+              hash_unit(uint32_t(b0 + c) * 40503u + uint32_t(attempt + 1) * 977u + uint32_t(jb.job), uint32_t(i));
+        __syncthreads();
+        double p2[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) p2[q] = 0;
+        for (int i = tid; i < n; i += NT) p2[0] += Zb[i * NB + c] * Zb[i * NB + c];
+        block_sums(p2, 1);
+        if (tid == 0) nrm0[c] = sqrt(hs[0]);
+        __syncthreads();
+        if (b0 > 0) {
+          project(b0, c, c + 1);
+          project(b0, c, c + 1);
+        }
+      }
+    }
+    for (int e = tid; e < n * NB; e += NT) {
+      const int i = e / NB, c = e - i * NB;
+      if (c < nb) z[int64_t(i) * k + b0 + c] = Zb[e];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3c. Back-transformation y = H_0 H_1 ... H_{n-3} z for NB columns per
+// workgroup; the result is written as columns c0.. of the n x n row-major
+// output at a_off.
+struct BtItem {
+  int32_t job, c0;
+};
+
+template <int NT, int NB>
+__global__ __launch_bounds__(NT) void backtransform_kernel(const BtItem* __restrict__ items,
+                                                           const DevJob* __restrict__ jobs,
+                                                           const double* __restrict__ A,
+                                                           const double* __restrict__ TAUg,
+                                                           const double* __restrict__ Zi, double* __restrict__ Zo) {
+  constexpr int NW = NT / 64;
+  const BtItem it = items[blockIdx.x];
+  const DevJob jb = jobs[it.job];
+  const int n = jb.n, k = jb.k, c0 = it.c0;
+  const int nb = min(NB, k - c0);
+  const double* a = A + jb.a_off;
+  const double* tau = TAUg + jb.w_off;
+  extern __shared__ double sm[];
+  double* Zb = sm;                    // n x NB
+  double* red = Zb + int64_t(n) * NB;  // 2 x NW x NB
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int e2 = tid; e2 < n * NB; e2 += NT) {
+    const int i = e2 / NB, c = e2 - i * NB;
+    Zb[e2] = c < nb ? Zi[jb.z_off + int64_t(i) * k + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  int parity = 0;
+  for (int kk = n - 3; kk >= 0; --kk) {
+    const double t = tau[kk];
+    if (t == 0.0) continue;
+    const double* vr = a + int64_t(kk) * n;  // v[kk+1] = 1, v[j] = vr[j] for j >= kk+2
+    double s[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) s[c] = 0;
+    for (int i = kk + 1 + tid; i < n; i += NT) {
+      const double vi = i == kk + 1 ? 1.0 : vr[i];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) s[c] += vi * Zb[i * NB + c];
+    }
+    double* r = red + parity * NW * NB;
+    parity ^= 1;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const double x = wave_sum(s[c]);
+      if (lane == 0) r[wv * NB + c] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      double x = 0;
+      for (int q = 0; q < NW; ++q) x += r[q * NB + c];
+      s[c] = t * x;
+    }
+    for (int i = kk + 1 + tid; i < n; i += NT) {
+      const double vi = i == kk + 1 ? 1.0 : vr[i];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) Zb[i * NB + c] -= s[c] * vi;
+    }
+    __syncthreads();
+  }
+  for (int e2 = tid; e2 < n * NB; e2 += NT) {
+    const int i = e2 / NB, c = e2 - i * NB;
+    if (c < nb) Zo[jb.a_off + int64_t(i) * n + c0 + c] = Zb[e2];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lowest eigenpair of a small symmetric matrix, one wavefront.  Lane i owns
+// row i.  Tridiagonalisation in LDS, multisection on the Sturm count for the
+// smallest eigenvalue (64 points per round), inverse iteration on T by lane 0,
+// back-transformation by the stored reflectors.
+__global__ __launch_bounds__(64) void lowest_eig_kernel(const double* __restrict__ H, int ld, int m,
+                                                        double* __restrict__ c_out, double* __restrict__ theta) {
+  __shared__ double a[64][65];  // full symmetric matrix; column k below the diagonal then holds reflector k
+  __shared__ double tau[64], d[64], e[64], e2[64], vv[64], ww[64], z[64];
+  __shared__ double u0[64], u1[64], u2[64], ml[64];
+  __shared__ int sw[64];
+  const int l = threadIdx.x;
+  for (int i = 0; i < m; ++i)
+    if (l <= i) {
+      const double x = H[int64_t(i) * ld + l];
+      a[i][l] = x;
+      a[l][i] = x;
+    }
+  __syncthreads();
+  for (int k = 0; k + 2 < m; ++k) {
+    const double x = (l > k && l < m) ? a[l][k] : 0.0;
+    const double alpha = a[k + 1][k];
+    const double dk = a[k][k];
+    const double sigma = wave_sum(l >= k + 2 && l < m ? x * x : 0.0);
+    double tk = 0, beta = alpha, scale = 0;
+    if (sigma > 0) {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0 ? -nrm : nrm;
+      tk = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    const double vi = l == k + 1 ? 1.0 : (l >= k + 2 && l < m ? x * scale : 0.0);
+    __syncthreads();
+    if (l > k && l < m) a[l][k] = vi;
+    vv[l] = vi;
+    if (l == 0) {
+      d[k] = dk;
+      e[k] = beta;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    double p = 0;
+    if (l > k && l < m)
+      for (int j = k + 1; j < m; ++j) p += a[l][j] * vv[j];
+    const double pv = wave_sum(p * vi);
+    const double wi = tk * p - 0.5 * tk * tk * pv * vi;
+    ww[l] = wi;
+    __syncthreads();
+    if (l > k && l < m)
+      for (int j = k + 1; j < m; ++j) a[l][j] -= vi * ww[j] + wi * vv[j];
+    __syncthreads();
+  }
+  if (l == 0) {
+    if (m >= 2) {
+      d[m - 2] = a[m - 2][m - 2];
+      e[m - 2] = a[m - 1][m - 2];
+      tau[m - 2] = 0;
+    }
+    d[m - 1] = a[m - 1][m - 1];
+    e[m - 1] = 0;
+    tau[m - 1] = 0;
+  }
+  __syncthreads();
+  // Gershgorin interval, then multisection for the smallest eigenvalue
+  double gl = INFINITY, gu = -INFINITY, em = 0;
+  if (l < m) {
+    const double el = l > 0 ? fabs(e[l - 1]) : 0.0, er = l + 1 < m ? fabs(e[l]) : 0.0;
+    gl = d[l] - el - er;
+    gu = d[l] + el + er;
+    if (l + 1 < m) {
+      e2[l] = e[l] * e[l];
+      em = e2[l];
+    }
+  }
+  gl = wave_min(gl);
+  gu = wave_max(gu);
+  em = wave_max(em);
+  __syncthreads();
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  const double pivmin = DBL_MIN * fmax(1.0, em);
+  double lo = gl - 4.0 * kEps * tnorm - 2.0 * pivmin, hi = gu + 4.0 * kEps * tnorm + 2.0 * pivmin;
+  const double atol = 2.0 * kEps * tnorm;
+  for (int round = 0; round < 40; ++round) {
+    if (hi - lo <= fmax(atol, 2.0 * kEps * fmax(fabs(lo), fabs(hi)))) break;
+    const double x = lo + (hi - lo) * double(l + 1) / 65.0;
+    const int cnt = (x > lo && x < hi) ? sturm_count(d, e2, m, x, pivmin) : (x >= hi ? 1 : 0);
+    const unsigned long long hit = __ballot(cnt >= 1);
+    double nlo = lo, nhi = hi;
+    if (hit == 0ull) {
+      nlo = __shfl(x, 63, 64);
+    } else {
+      const int f = __ffsll((long long)hit) - 1;
+      nhi = __shfl(x, f, 64);
+      if (f > 0) nlo = __shfl(x, f - 1, 64);
+    }
+    if (nlo == lo && nhi == hi) break;
+    lo = nlo;
+    hi = nhi;
+  }
+  const double lam = 0.5 * (lo + hi);
+  // inverse iteration on T (lane 0)
+  if (l == 0) {
+    if (m == 1) {
+      z[0] = 1.0;
+    } else {
+      const double piv = kEps * fmax(tnorm, DBL_MIN);
+      auto guard = [&](double x) { return fabs(x) < piv ? (x < 0 ? -piv : piv) : x; };
+      double c0 = d[0] - lam, c1 = e[0];
+      for (int i = 0; i + 1 < m; ++i) {
+        const double ei = e[i], dn = d[i + 1] - lam, en = i + 2 < m ? e[i + 1] : 0.0;
+        if (fabs(c0) >= fabs(ei)) {
+          c0 = guard(c0);
+          const double mu = ei / c0;
+          u0[i] = c0;
+          u1[i] = c1;
+          u2[i] = 0.0;
+          ml[i] = mu;
+          sw[i] = 0;
+          c0 = dn - mu * c1;
+          c1 = en;
+        } else {
+          const double mu = c0 / ei;
+          u0[i] = ei;
+          u1[i] = dn;
+          u2[i] = en;
+          ml[i] = mu;
+          sw[i] = 1;
+          c0 = c1 - mu * dn;
+          c1 = -mu * en;
+        }
+      }
+      u0[m - 1] = guard(c0);
+      u1[m - 1] = 0.0;
+      u2[m - 1] = 0.0;
+      for (int i = 0; i < m; ++i) z[i] = 1.0 + 0.01 * double((i * 37) % 11);
+      for (int itn = 0; itn < 3; ++itn) {
+        double cur = z[0];
+        for (int i = 0; i + 1 < m; ++i) {
+          double nxt = z[i + 1];
+          if (sw[i]) {
+            const double t = cur;
+            cur = nxt;
+            nxt = t;
+          }
+          nxt -= ml[i] * cur;
+          z[i] = cur;
+          cur = nxt;
+        }
+        z[m - 1] = cur;
+        double z1 = 0, z2 = 0, zmax = 0;
+        for (int i = m - 1; i >= 0; --i) {
+          const double zi = (z[i] - u1[i] * z1 - u2[i] * z2) / u0[i];
+          z[i] = zi;
+          zmax = fmax(zmax, fabs(zi));
+          z2 = z1;
+          z1 = zi;
+        }
+        const double s = zmax > 0 ? 1.0 / zmax : 1.0;
+        for (int i = 0; i < m; ++i) z[i] *= s;
+      }
+    }
+  }
+  __syncthreads();
+  // back-transformation y = H_0 ... H_{m-3} z
+  double y = l < m ? z[l] : 0.0;
+  for (int k = m - 3; k >= 0; --k) {
+    const double vi = (l > k && l < m) ? a[l][k] : 0.0;
+    const double s = wave_sum(vi * y);
+    y -= tau[k] * s * vi;
+  }
+  const double nrm = sqrt(wave_sum(y * y));
+  if (l < m) c_out[l] = y / nrm;
+  if (l == 0) *theta = lam;
+}
+
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    TK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(bytes)));
+}
+
+template <int NT, int T>
+void launch_tridiag_t(Context* ctx, const DevJob* jobs, int n_jobs, int nmax, double* A, double* D, double* E,
+                      double* TAU) {
+  constexpr int NW = NT / 64;
+  const size_t lds = ((4 + NW) * size_t(nmax) + 2 * NW + 8) * sizeof(double);
+  allow_lds(tridiag_kernel<NT, T>, lds);
+  hipLaunchKernelGGL((tridiag_kernel<NT, T>), dim3(n_jobs), dim3(NT), lds, ctx->stream, jobs, A, D, E, TAU);
+}
+
+template <int NB>
+void launch_vectors_t(Context* ctx, int nmax, int n_jobs, int n_bt, const DevJob* djd, const BtItem* bid,
+                      const double* A, const double* TAU, double* zi, double* Z) {
+  constexpr int kNT = 1024, NW = kNT / 64;
+  const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
+  allow_lds(orth_kernel<kNT, NB>, lds);
+  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(n_jobs), dim3(kNT), lds, ctx->stream, djd, zi);
+  TK_HIP(hipGetLastError());
+  constexpr int kBtNT = 256;
+  const size_t lds2 = (size_t(nmax) * NB + 2 * (kBtNT / 64) * NB) * sizeof(double);
+  allow_lds(backtransform_kernel<kBtNT, NB>, lds2);
+  hipLaunchKernelGGL((backtransform_kernel<kBtNT, NB>), dim3(n_bt), dim3(kBtNT), lds2, ctx->stream, bid, djd, A, TAU,
+                     zi, Z);
+  TK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n, double* W, int64_t w_n,
+                         bool descending) {
+  jobs_ = jobs;
+  A_ = A;
+  W_ = W;
+  nmax_ = 0;
+  for (const SymJob& j : jobs) {
+    if (j.n < 1 || j.n > kSymEigMaxN) throw ArgError("eigensolver: matrix dimension outside 1..2048");
+    if (j.a_off < 0 || j.a_off + int64_t(j.n) * j.n > a_n || j.w_off < 0 || j.w_off + j.n > w_n)
+      throw std::out_of_range("eigensolver: job outside its buffers");
+    nmax_ = std::max(nmax_, j.n);
+  }
+  if (jobs.empty()) return;
+  // d, e, tau share the eigenvalue offsets
+  de_.reset(ctx_, size_t(3 * std::max<int64_t>(w_n, 1)));
+  tn_.reset(ctx_, jobs.size());
+  double* D = de_.get();
+  double* E = D + w_n;
+  double* TAU = E + w_n;
+  // tridiagonalisation, largest matrices first
+  std::vector<int> order(jobs.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].n > jobs[y].n; });
+  std::vector<DevJob> dj;
+  dj.reserve(jobs.size() * 2);
+  for (int o : order) dj.push_back({jobs[o].a_off, jobs[o].w_off, 0, 0, jobs[o].n, 0, o, 0});
+  // second copy in caller order for the bisection items
+  for (size_t o = 0; o < jobs.size(); ++o) dj.push_back({jobs[o].a_off, jobs[o].w_off, 0, 0, jobs[o].n, 0, int(o), 0});
+  std::vector<BisItem> items;
+  constexpr int kBisNT = 256;
+  for (size_t o = 0; o < jobs.size(); ++o)
+    for (int j0 = 0; j0 < jobs[o].n; j0 += kBisNT) items.push_back({int32_t(jobs.size() + o), j0});
+  const size_t jb_bytes = dj.size() * sizeof(DevJob);
+  DBuf<char> meta(ctx_, jb_bytes + items.size() * sizeof(BisItem));
+  TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb_bytes, hipMemcpyHostToDevice, ctx_->stream));
+  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes, items.data(), items.size() * sizeof(BisItem), hipMemcpyHostToDevice,
+                        ctx_->stream));
+  const DevJob* djd = reinterpret_cast<const DevJob*>(meta.get());
+  const BisItem* itd = reinterpret_cast<const BisItem*>(meta.get() + jb_bytes);
+  const int nj = int(jobs.size());
+  if (nmax_ <= 128) launch_tridiag_t<512, 2>(ctx_, djd, nj, nmax_, A, D, E, TAU);
+  else if (nmax_ <= 256) launch_tridiag_t<512, 4>(ctx_, djd, nj, nmax_, A, D, E, TAU);
+  else if (nmax_ <= 512) launch_tridiag_t<512, 8>(ctx_, djd, nj, nmax_, A, D, E, TAU);
+  else if (nmax_ <= 1024) launch_tridiag_t<512, 16>(ctx_, djd, nj, nmax_, A, D, E, TAU);
+  else launch_tridiag_t<256, 32>(ctx_, djd, nj, nmax_, A, D, E, TAU);
+  TK_HIP(hipGetLastError());
+  // the TN slot of a job is indexed by its position in the device job array
+  DBuf<double> tn_all(ctx_, dj.size());
+  hipLaunchKernelGGL((bisect_kernel<kBisNT>), dim3(items.size()), dim3(kBisNT),
+                     2 * size_t(nmax_) * sizeof(double), ctx_->stream, itd, djd, D, E, W, tn_all.get(),
+                     descending ? 1 : 0);
+  TK_HIP(hipGetLastError());
+  TK_HIP(hipMemcpyAsync(tn_.get(), tn_all.get() + jobs.size(), jobs.size() * sizeof(double),
+                        hipMemcpyDeviceToDevice, ctx_->stream));
+  // keep the metadata alive until the kernels have consumed it
+  ctx_->sync();
+}
+
+void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z_n) {
+  if (keep.size() != jobs_.size()) throw ArgError("eigensolver: one kept count per job");
+  if (Z == A_) throw ArgError("eigensolver: the vector output must not alias the reflectors");
+  std::vector<DevJob> dj;
+  std::vector<VecItem> vitems;
+  int64_t z_tot = 0, lu_tot = 0;
+  for (size_t o = 0; o < jobs_.size(); ++o) {
+    const SymJob& j = jobs_[o];
+    const int64_t k = keep[o];
+    if (k < 0 || k > j.n) throw ArgError("eigensolver: kept count outside 0..n");
+    if (j.a_off + int64_t(j.n) * j.n > z_n) throw std::out_of_range("eigensolver: vector output too small");
+    if (k == 0) continue;
+    const int idx = int(dj.size());
+    dj.push_back({j.a_off, j.w_off, z_tot, lu_tot, j.n, int32_t(k), int32_t(o), 0});
+    for (int64_t c = 0; c < k; ++c) vitems.push_back({int32_t(idx), int32_t(c)});
+    z_tot += int64_t(j.n) * k;
+    lu_tot += kLuFields * int64_t(j.n) * k;
+  }
+  if (dj.empty()) return;
+  const int NBv = nmax_ <= 1024 ? 8 : 4;
+  std::vector<BtItem> bitems;
+  for (size_t q = 0; q < dj.size(); ++q)
+    for (int c0 = 0; c0 < dj[q].k; c0 += NBv) bitems.push_back({int32_t(q), c0});
+  const size_t jb_bytes = dj.size() * sizeof(DevJob);
+  const size_t vi_bytes = vitems.size() * sizeof(VecItem);
+  DBuf<char> meta(ctx_, jb_bytes + vi_bytes + bitems.size() * sizeof(BtItem));
+  TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb_bytes, hipMemcpyHostToDevice, ctx_->stream));
+  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes, vitems.data(), vi_bytes, hipMemcpyHostToDevice, ctx_->stream));
+  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes + vi_bytes, bitems.data(), bitems.size() * sizeof(BtItem),
+                        hipMemcpyHostToDevice, ctx_->stream));
+  const DevJob* djd = reinterpret_cast<const DevJob*>(meta.get());
+  const VecItem* vid = reinterpret_cast<const VecItem*>(meta.get() + jb_bytes);
+  const BtItem* bid = reinterpret_cast<const BtItem*>(meta.get() + jb_bytes + vi_bytes);
+  // per-job Gershgorin bounds indexed by the caller's job number
+  DBuf<double> zi(ctx_, size_t(z_tot)), lu(ctx_, size_t(lu_tot));
+  const double* D = de_.get();
+  const int64_t w_n = int64_t(de_.size() / 3);
+  const double* E = D + w_n;
+  const double* TAU = E + w_n;
+  const int nv = int(vitems.size());
+  hipLaunchKernelGGL(invit_kernel, dim3((nv + 63) / 64), dim3(64), 0, ctx_->stream, vid, nv, djd, D, E, W_,
+                     tn_.get(), lu.get(), zi.get());
+  TK_HIP(hipGetLastError());
+  if (NBv == 8)
+    launch_vectors_t<8>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), Z);
+  else
+    launch_vectors_t<4>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), Z);
+  ctx_->sync();
+}
+
+void lowest_eigpair(Context* ctx, const double* H, int ld, int m, double* c, double* theta) {
+  if (m < 1 || m > 64 || ld < m) throw ArgError("lowest_eigpair: dimension outside 1..64");
+  hipLaunchKernelGGL(lowest_eig_kernel, dim3(1), dim3(64), 0, ctx->stream, H, ld, m, c, theta);
+  TK_HIP(hipGetLastError());
+}
+
+}  // namespace tk

@@ -1,0 +1,62 @@
+// Batched FP64 symmetric eigensolver for the truncated SVD and the Krylov
+// subspace problems (tdeig.hip).
+//
+// Method (one workgroup per matrix where a matrix must be walked in order):
+//   1. Householder tridiagonalisation A = Q T Q^T, the rank-2 update of step
+//      k fused with the symmetric product of step k+1, so the trailing matrix
+//      is read and written once per column; reflectors are kept in A.
+//   2. All eigenvalues of T by Sturm-count bisection, one thread per
+//      eigenvalue, sorted by construction.
+//   3. For the k wanted eigenvalues only: inverse iteration on T (LU with
+//      partial pivoting, one thread per vector), blocked classical
+//      Gram-Schmidt applied twice in eigenvalue order (orthonormal to working
+//      precision even inside clusters), then back-transformation by Q.
+//
+// Reference call sites: the Gram eigen-decomposition behind the block-diagonal
+// truncated SVD (tk-dmrg/src/truncation/mod.rs:56-127 -> faer thin_svd,
+// tk-linalg/src/device/faer.rs:191-240) and the projected matrix of the
+// Lanczos/Davidson solvers (symmetric_eigen_small,
+// tk-dmrg/src/eigensolver/mod.rs:532-601).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "device.hpp"
+
+namespace tk {
+
+constexpr int kSymEigMaxN = 2048;
+
+// One symmetric n x n matrix, row-major at a_off of A (upper triangle read,
+// overwritten by the reflectors); its eigenvalues at w_off of W.
+struct SymJob {
+  int64_t a_off;
+  int64_t w_off;
+  int32_t n;
+};
+
+class SymEigBatch {
+ public:
+  explicit SymEigBatch(Context* ctx) : ctx_(ctx) {}
+  // Eigenvalues of every job into W (descending or ascending).
+  void values(const std::vector<SymJob>& jobs, double* A, int64_t a_n, double* W, int64_t w_n, bool descending);
+  // The first keep[j] eigenvectors (in W order) of job j as the columns
+  // 0..keep[j]-1 of the n x n row-major matrix at a_off of Z (Z != A).
+  void vectors(const std::vector<int64_t>& keep, double* Z, int64_t z_n);
+
+ private:
+  Context* ctx_;
+  std::vector<SymJob> jobs_;
+  double* A_ = nullptr;
+  double* W_ = nullptr;
+  int nmax_ = 0;
+  DBuf<double> de_;   // per job: d (n), e (n), tau (n) at 3 * w_off
+  DBuf<double> tn_;   // per job: Gershgorin bound of |T|
+};
+
+// Lowest eigenpair of the m x m symmetric matrix whose lower triangle is
+// stored row-wise in H (H[i*ld + j], j <= i), m <= 64: one wavefront,
+// everything in LDS.  c (m) and *theta are device outputs.
+void lowest_eigpair(Context* ctx, const double* H, int ld, int m, double* c, double* theta);
+
+}  // namespace tk
