@@ -6,8 +6,9 @@
 // eigensolver/mod.rs:532-601), full SVD of the (s_i a_L) x (s_j a_R)
 // matricisation per middle charge by one-sided Jacobi (Hestenes), the
 // truncation rule of truncation/mod.rs:56-127 (relative cutoff, max bond,
-// min bond) plus a numerical-zero floor on sigma^2 and the no-cut-inside-a-
-// degenerate-group rule, write-back into the MPS, and block-sparse
+// min bond) plus a numerical-zero floor (sigma^2 <= 1e-26 sigma_max^2) and
+// the rule of never cutting inside a group |sigma_k - sigma_k+1| <= tol *
+// sigma_max, write-back into the MPS, and block-sparse
 // environment updates (environments/mod.rs:225-483).
 #include <omp.h>
 
@@ -27,6 +28,10 @@
 #include "tk_oracle_internal.hpp"
 
 namespace orc {
+
+// sigma^2 <= kLambdaZero sigma_max^2 is a numerical zero (sigma <= 1e-13
+// sigma_max, below the reference's default cutoff of 1e-12).
+constexpr double kLambdaZero = 1e-26;
 
 namespace {
 
@@ -365,7 +370,8 @@ struct Dmrg {
     while (keep < all.size() && long(keep) < max_bond_dim && all[keep].l >= thr && all[keep].l > 0) ++keep;
     if (keep < all.size() && keep > 0) {
       size_t k2 = keep;
-      while (k2 > 0 && all[k2 - 1].l - all[k2].l <= cfg.deg_tol * lmax) --k2;
+      const double smax = std::sqrt(lmax);
+      while (k2 > 0 && std::sqrt(all[k2 - 1].l) - std::sqrt(all[k2].l) <= cfg.deg_tol * smax) --k2;
       if (k2 > 0) keep = k2;
     }
     keep = std::max<size_t>(keep, size_t(std::min<long>(cfg.min_bond, long(all.size()))));
@@ -537,7 +543,7 @@ struct Dmrg {
       Tensor th = theta(i);
       double te;
       long bd;
-      split(i, false, th, 1L << 40, 0.0, 1e-15, &te, &bd);
+      split(i, false, th, 1L << 40, 0.0, kLambdaZero, &te, &bd);
     }
     L.assign(n + 1, Tensor());
     R.assign(n + 1, Tensor());
@@ -551,7 +557,7 @@ struct Dmrg {
     std::vector<double> x = flat(th);
     const double e = davidson(i, x, nmv);
     th.load(x.data());
-    split(i, lr, th, D, cfg.cutoff, std::max(cfg.cutoff * cfg.cutoff, 1e-15), terr, bdim);
+    split(i, lr, th, D, cfg.cutoff, std::max(cfg.cutoff * cfg.cutoff, kLambdaZero), terr, bdim);
     if (lr) grow_left(i);
     else grow_right(i + 1);
     return e;

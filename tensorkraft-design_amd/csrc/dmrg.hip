@@ -8,6 +8,7 @@
 #include <numeric>
 #include <tuple>
 
+#include "gramsvd.hpp"
 #include "tdeig.hpp"
 #include "kernels.hpp"
 
@@ -144,7 +145,7 @@ void DmrgEngine::prepare() {
     DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total)));
     contract_two_site(i, psi, x.get());
     StepStats st;
-    split_two_site(i, false, psi, x.get(), INT64_MAX, 0.0, 1e-15, &st);
+    split_two_site(i, false, psi, x.get(), INT64_MAX, 0.0, kLambdaZero, &st);
   }
   L_[0] = boundary_env(bonds_[0], cfg_.mpo[0].wl);
   R_[n_] = boundary_env(bonds_[n_], cfg_.mpo[n_ - 1].wr);
@@ -209,7 +210,7 @@ StepStats DmrgEngine::step(int i, bool lr) {
   st.matvecs_fp64 = dr.matvecs_fp64;
   st.converged = dr.converged;
   st.residual = dr.residual;
-  const double floor = std::max(cfg_.svd_cutoff * cfg_.svd_cutoff, 1e-15);
+  const double floor = std::max(cfg_.svd_cutoff * cfg_.svd_cutoff, kLambdaZero);
   split_two_site(i, lr, psi, x.get(), max_bond, cfg_.svd_cutoff, floor, &st);
   double t3 = now();
   if (lr) {
@@ -300,14 +301,15 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
       m.q = q;
       mids.push_back(std::move(m));
     }
-  int64_t gtot = 0, wtot = 0;
+  // dense block of every middle charge: X = M (left-to-right) or M^T
+  // (right-to-left), so the kept vectors are always the left ones of X
+  int64_t xtot = 0;
+  std::vector<int64_t> x_off;
   for (Mid& m : mids) {
     m.n = lr ? m.nrow : m.ncol;
     if (m.n > kSymEigMaxN) throw ArgError("SVD block larger than the eigensolver limit");
-    m.g_off = gtot;
-    m.w_off = wtot;
-    gtot += m.n * m.n;
-    wtot += m.n;
+    x_off.push_back(xtot);
+    xtot += m.nrow * m.ncol;
   }
   auto slice = [&](int ssi, int xi, int ls, int ssj, int xj, int rs) {
     int32_t secs[4] = {ssi, ssj, ls, rs};
@@ -315,66 +317,60 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
     if (b < 0) throw std::logic_error("split: missing psi block");
     return psi.offset[b] + (int64_t(xi) * Pj.dim[ssj] + xj) * BL.dim[ls] * BR.dim[rs];
   };
-  // Gram matrices
-  DBuf<double> G(ctx_, size_t(std::max<int64_t>(1, gtot)));
-  ContractList gl;
-  gl.flags = lr ? kTransB : kTransA;
-  for (Mid& m : mids) {
-    if (lr) {
-      for (const Part& a : m.rows)
-        for (const Part& b : m.rows) {
-          gl.begin_out(m.g_off + a.off * m.n + b.off, int(a.d), int(b.d), int(m.n));
-          for (const Part& c : m.cols) {
-            gl.begin_term(slice(b.ssec, b.x, b.bsec, c.ssec, c.x, c.bsec), int(c.d), int(c.d));
-            gl.add_src(slice(a.ssec, a.x, a.bsec, c.ssec, c.x, c.bsec), int(c.d), 1.0);
-          }
-        }
-    } else {
-      for (const Part& a : m.cols)
-        for (const Part& b : m.cols) {
-          gl.begin_out(m.g_off + a.off * m.n + b.off, int(a.d), int(b.d), int(m.n));
-          for (const Part& r : m.rows) {
-            gl.begin_term(slice(r.ssec, r.x, r.bsec, b.ssec, b.x, b.bsec), int(b.d), int(r.d));
-            gl.add_src(slice(r.ssec, r.x, r.bsec, a.ssec, a.x, a.bsec), int(a.d), 1.0);
-          }
+  DBuf<double> X(ctx_, size_t(std::max<int64_t>(1, xtot)));
+  {
+    std::vector<CopyJob> xc;
+    for (size_t k = 0; k < mids.size(); ++k) {
+      const Mid& m = mids[k];
+      for (const Part& rp : m.rows)
+        for (const Part& cp : m.cols) {
+          const int64_t src = slice(rp.ssec, rp.x, rp.bsec, cp.ssec, cp.x, cp.bsec);
+          if (lr)
+            xc.push_back({src, x_off[k] + rp.off * m.ncol + cp.off, int32_t(rp.d), int32_t(cp.d), int32_t(cp.d),
+                          int32_t(m.ncol), 0, 0});
+          else
+            xc.push_back({src, x_off[k] + cp.off * m.nrow + rp.off, int32_t(cp.d), int32_t(rp.d), int32_t(cp.d),
+                          int32_t(m.nrow), 1, 0});
         }
     }
+    launch_copies(ctx_, xc, x, psi.total, X.get(), xtot);
   }
-  run_list(ctx_, gl, psi.total, psi.total, gtot, x, x, G.get());
-  // eigenvalues of every Gram block (descending); vectors only for the kept ones
-  std::vector<SymJob> jobs;
-  for (Mid& m : mids) jobs.push_back({m.g_off, m.w_off, int32_t(m.n)});
-  DBuf<double> Wd(ctx_, size_t(std::max<int64_t>(1, wtot))), E(ctx_, size_t(std::max<int64_t>(1, gtot)));
-  SymEigBatch eig(ctx_);
-  eig.values(jobs, G.get(), gtot, Wd.get(), wtot, true);
-  std::vector<double> lam(static_cast<size_t>(wtot));
-  Wd.download(lam.data(), lam.size());
-  ctx_->sync();
+  // squared singular values of every block (descending), two-level Gram
+  std::vector<GramBlock> gb;
+  for (size_t k = 0; k < mids.size(); ++k)
+    gb.push_back({x_off[k], int32_t(mids[k].n), int32_t(lr ? mids[k].ncol : mids[k].nrow)});
+  GramSvd gs(ctx_);
+  const double lam_min_rel = std::max(cutoff * cutoff, lambda_floor);
+  const std::vector<std::vector<double>> lam = gs.values(gb, X.get(), xtot, lam_min_rel, max_bond);
+  for (size_t k = 0; k < mids.size(); ++k) mids[k].g_off = gs.u_offset(k);
+  const int64_t gtot = gs.u_size();
   // global selection: sigma descending, keep while count < max_bond and
   // sigma >= cutoff * sigma_max (truncation/mod.rs:72-82), numerical-zero floor
-  // on sigma^2, no cut inside a degenerate group, min_bond_dim floor
+  // on sigma^2, no cut inside a group |sigma_k - sigma_k+1| <= tol * sigma_max,
+  // min_bond_dim floor
   struct Cand {
     double l;
     int mid;
     int idx;
   };
   std::vector<Cand> all;
-  for (int k = 0; k < int(mids.size()); ++k) {
-    const int64_t rank = std::min(mids[k].nrow, mids[k].ncol);
-    for (int64_t j = 0; j < rank; ++j) all.push_back({std::max(0.0, lam[mids[k].w_off + j]), k, int(j)});
-  }
+  for (int k = 0; k < int(mids.size()); ++k)
+    for (size_t j = 0; j < lam[k].size(); ++j) all.push_back({std::max(0.0, lam[k][j]), k, int(j)});
   std::stable_sort(all.begin(), all.end(), [](const Cand& a, const Cand& b) {
     if (a.l != b.l) return a.l > b.l;
     if (a.mid != b.mid) return a.mid < b.mid;
     return a.idx < b.idx;
   });
   const double lmax = all.empty() ? 0.0 : all[0].l;
-  const double thr = std::max(cutoff * cutoff, lambda_floor) * lmax;
+  if (!(lmax > 0) || !std::isfinite(lmax))
+    throw ArgError("split: the two-site tensor is zero or not finite (zero-norm MPS?)");
+  const double thr = lam_min_rel * lmax;
   size_t keep = 0;
   while (keep < all.size() && int64_t(keep) < max_bond && all[keep].l >= thr && all[keep].l > 0) ++keep;
   if (keep < all.size() && keep > 0) {
+    const double smax = std::sqrt(lmax);
     size_t k2 = keep;
-    while (k2 > 0 && all[k2 - 1].l - all[k2].l <= cfg_.degeneracy_tol * lmax) --k2;
+    while (k2 > 0 && std::sqrt(all[k2 - 1].l) - std::sqrt(all[k2].l) <= cfg_.degeneracy_tol * smax) --k2;
     if (k2 > 0) keep = k2;
   }
   keep = std::max<size_t>(keep, size_t(std::min<int64_t>(cfg_.min_bond_dim, int64_t(all.size()))));
@@ -391,8 +387,9 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
   {
     std::vector<int64_t> keep_of;
     for (const Mid& m : mids) keep_of.push_back(m.keep);
-    eig.vectors(keep_of, E.get(), gtot);
+    gs.vectors(keep_of);
   }
+  const double* E = gs.u();
   // new bond
   std::vector<std::pair<QN, int64_t>> nb;
   std::map<QN, int> mid_of;
@@ -452,8 +449,8 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
         }
       }
     }
-    launch_copies(ctx_, copies, E.get(), gtot, Ai.d.get(), Ai.t.total);
-    run_list(ctx_, wl, gtot, psi.total, Aj.t.total, E.get(), x, Aj.d.get());
+    launch_copies(ctx_, copies, E, gtot, Ai.d.get(), Ai.t.total);
+    run_list(ctx_, wl, gtot, psi.total, Aj.t.total, E, x, Aj.d.get());
   } else {
     // A_j = V^T (right-canonical)
     for (int64_t b = 0; b < Aj.t.n_blocks(); ++b) {
@@ -480,8 +477,8 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
         }
       }
     }
-    launch_copies(ctx_, copies, E.get(), gtot, Aj.d.get(), Aj.t.total);
-    run_list(ctx_, wl, psi.total, gtot, Ai.t.total, x, E.get(), Ai.d.get());
+    launch_copies(ctx_, copies, E, gtot, Aj.d.get(), Aj.t.total);
+    run_list(ctx_, wl, psi.total, gtot, Ai.t.total, x, E, Ai.d.get());
   }
   bonds_[i + 1] = newB;
   site_[i] = std::move(Ai);

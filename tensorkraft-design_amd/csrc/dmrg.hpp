@@ -24,6 +24,10 @@
 
 namespace tk {
 
+// sigma^2 <= kLambdaZero sigma_max^2 counts as a numerical zero in every
+// split (sigma <= 1e-13 sigma_max, below the reference's default cutoff).
+constexpr double kLambdaZero = 1e-26;
+
 struct DmrgConfig {
   Sym sym = Sym::U1;
   std::vector<MpoSite> mpo;              // one per site; mpo[i].wr == mpo[i+1].wl
@@ -109,15 +113,5 @@ class DmrgEngine {
   std::vector<StepStats> steps_;
   SweepStats* cur_ = nullptr;
 };
-
-// 2-D strided copies (optionally transposed) in one launch.
-struct CopyJob {
-  int64_t src, dst;
-  int32_t m, n, src_ld, dst_ld;
-  int32_t trans, pad;  // trans: dst(i, j) = src(j, i)
-};
-void launch_copies(Context* ctx, const std::vector<CopyJob>& jobs, const double* src, int64_t src_n, double* dst,
-                   int64_t dst_n);
-void launch_fill(Context* ctx, double* p, double v, int64_t n);
 
 }  // namespace tk

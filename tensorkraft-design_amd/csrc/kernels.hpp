@@ -30,6 +30,18 @@ void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, 
 void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C);
 void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C);
 
+// 2-D strided copies (optionally transposed) in one launch (dmrg.hip).
+// dst is m x n with leading dimension dst_ld; src is m x n (trans = 0) or
+// n x m (trans = 1, dst(i, j) = src(j, i)) with leading dimension src_ld.
+struct CopyJob {
+  int64_t src, dst;
+  int32_t m, n, src_ld, dst_ld;
+  int32_t trans, pad;
+};
+void launch_copies(Context* ctx, const std::vector<CopyJob>& jobs, const double* src, int64_t src_n, double* dst,
+                   int64_t dst_n);
+void launch_fill(Context* ctx, double* p, double v, int64_t n);
+
 // ---- vector kernels (vecops.hip) ----
 void convert_f64_to_f32(Context* ctx, const double* x, float* y, int64_t n);
 void convert_f32_to_f64(Context* ctx, const float* x, double* y, int64_t n);

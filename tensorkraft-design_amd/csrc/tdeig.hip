@@ -17,6 +17,7 @@ namespace {
 
 constexpr double kEps = DBL_EPSILON;
 constexpr int kLuFields = 5;
+constexpr int kSmallTridiag = 128;  // largest batch dimension for the one-workgroup tridiagonalisation
 
 struct DevJob {
   int64_t a_off;   // matrix / reflectors (and output columns in Z)
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
                                                      double* __restrict__ TAUg) {
   constexpr int NW = NT / 64;
   constexpr int R = (64 * T + NT - 1) / NT;
-  constexpr bool kCache = T <= 16;  // keep v, w, vn of the lane's columns in registers
+  constexpr bool kCache = T <= 8;  // keep v, w, vn of the lane's columns in registers
   const DevJob jb = jobs[blockIdx.x];
   const int n = jb.n;
   double* a = A + jb.a_off;
@@ -190,16 +191,22 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
     }
     const int first = k + 1;
     const int i0 = first + ((wv - first % NW) + NW) % NW;
-    for (int i = i0; i < n; i += NW) {
+    // two rows per wave and iteration, so twice the loads are in flight
+    for (int i = i0; i < n; i += 2 * NW) {
+      const int i2 = i + NW;
+      const bool has2 = i2 < n;
       const double vi = v[i], wi = w[i], vni = vn[i];
+      const double vi2 = has2 ? v[i2] : 0.0, wi2 = has2 ? w[i2] : 0.0, vni2 = has2 ? vn[i2] : 0.0;
       double* row = a + int64_t(i) * n;
-      double xv[T];
+      double* row2 = a + int64_t(has2 ? i2 : i) * n;
+      double xv[T], xw[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
         const int j = 64 * t + lane;
         xv[t] = (j >= i && j < n) ? row[j] : 0.0;
+        xw[t] = (has2 && j >= i2 && j < n) ? row2[j] : 0.0;
       }
-      double racc = 0;
+      double racc = 0, racc2 = 0;
 #pragma unroll
       for (int t = 0; t < T; ++t) {
         const int j = 64 * t + lane;
@@ -218,10 +225,23 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
           row[j] = x;
           racc += x * vnjt;
           if (j > i) cacc[t] += x * vni;
+          if (has2 && j >= i2) {
+            const double x2 = xw[t] - (vi2 * wjt + wi2 * vjt);
+            row2[j] = x2;
+            racc2 += x2 * vnjt;
+            if (j > i2) cacc[t] += x2 * vni2;
+          }
         }
       }
-      racc = wave_sum(racc);
-      if (lane == 0) prow[i] = racc;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        racc += __shfl_xor(racc, o, 64);
+        racc2 += __shfl_xor(racc2, o, 64);
+      }
+      if (lane == 0) {
+        prow[i] = racc;
+        if (has2) prow[i2] = racc2;
+      }
     }
 #pragma unroll
     for (int t = 0; t < T; ++t) {
@@ -261,6 +281,165 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
     e[q] = 0;
     tau[p] = 0;
     tau[q] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1b. Tridiagonalisation for large matrices: one launch per column step k,
+// batched over the jobs, each workgroup owning kStepRows rows of one job
+// (one wavefront per row) of the full symmetric matrix.  Every workgroup
+// recomputes the O(n) quantities it needs: the pending rank-2 update of step
+// k-1 (from the previous step's A22 v, double-buffered) and the reflector of
+// step k (from row k, which no workgroup writes during the step).  Reflector k
+// is stored in column k below the subdiagonal by the owners of those rows and
+// moved into row k afterwards.  VP holds per job: v (two buffers) and A22 v
+// (two buffers), each indexed like the eigenvalues.
+constexpr int kStepRows = 4;
+constexpr int kStepThreads = 64 * kStepRows;
+
+__device__ __forceinline__ int step_count(int n, int k) {
+  if (n == 1) return k == 0 ? 1 : 0;
+  if (k + 2 < n) return (n - k - 1 + kStepRows - 1) / kStepRows;
+  return k + 2 == n ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kStepThreads) void tdstep_kernel(const DevJob* __restrict__ jobs, int n_jobs, int k,
+                                                              double* __restrict__ A, double* __restrict__ Dg,
+                                                              double* __restrict__ Eg, double* __restrict__ TAUg,
+                                                              double* __restrict__ VP, int64_t w_n) {
+  __shared__ double vprev[kSymEigMaxN], wprev[kSymEigMaxN], vcur[kSymEigMaxN];
+  __shared__ double red[2 * kStepRows];
+  int id = blockIdx.x, q = 0;
+  for (; q < n_jobs; ++q) {
+    const int cnt = step_count(jobs[q].n, k);
+    if (id < cnt) break;
+    id -= cnt;
+  }
+  if (q >= n_jobs) return;
+  const DevJob jb = jobs[q];
+  const int n = jb.n, chunk = id;
+  double* a = A + jb.a_off;
+  double* d = Dg + jb.w_off;
+  double* e = Eg + jb.w_off;
+  double* tau = TAUg + jb.w_off;
+  const double* vb_prev = VP + int64_t((k + 1) & 1) * w_n + jb.w_off;
+  double* vb_cur = VP + int64_t(k & 1) * w_n + jb.w_off;
+  const double* pb_prev = VP + int64_t(2 + ((k + 1) & 1)) * w_n + jb.w_off;
+  double* pb_cur = VP + int64_t(2 + (k & 1)) * w_n + jb.w_off;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int parity = 0;
+  if (n == 1) {
+    if (tid == 0) {
+      d[0] = a[0];
+      e[0] = 0;
+      tau[0] = 0;
+    }
+    return;
+  }
+  // pending update of step k-1: w = tau p - tau^2/2 (p.v) v over j >= k
+  const bool pend = k >= 1;
+  if (pend) {
+    const double tp = tau[k - 1];
+    double loc = 0;
+    for (int j = k + tid; j < n; j += kStepThreads) {
+      const double vj = j == k ? 1.0 : vb_prev[j];
+      vprev[j] = vj;
+      loc += pb_prev[j] * vj;
+    }
+    const double pv = block_sum<kStepThreads>(loc, red, parity);
+    const double alw = -0.5 * tp * tp * pv;
+    for (int j = k + tid; j < n; j += kStepThreads) wprev[j] = tp * pb_prev[j] + alw * vprev[j];
+    __syncthreads();
+  }
+  if (k + 2 == n) {  // last 2 x 2 block
+    if (tid == 0) {
+      const int p0 = k, p1 = k + 1;
+      double app = a[int64_t(p0) * n + p0], apq = a[int64_t(p0) * n + p1], aqq = a[int64_t(p1) * n + p1];
+      if (pend) {
+        app -= 2.0 * vprev[p0] * wprev[p0];
+        apq -= vprev[p0] * wprev[p1] + wprev[p0] * vprev[p1];
+        aqq -= 2.0 * vprev[p1] * wprev[p1];
+      }
+      d[p0] = app;
+      e[p0] = apq;
+      d[p1] = aqq;
+      e[p1] = 0;
+      tau[p0] = 0;
+      tau[p1] = 0;
+    }
+    return;
+  }
+  // row k with the pending update, then its reflector
+  const double* rowk = a + int64_t(k) * n;
+  double ss = 0;
+  for (int j = k + tid; j < n; j += kStepThreads) {
+    double x = rowk[j];
+    if (pend) x -= vprev[k] * wprev[j] + wprev[k] * vprev[j];
+    vcur[j] = x;
+    if (j >= k + 2) ss += x * x;
+  }
+  const double sigma = block_sum<kStepThreads>(ss, red, parity);
+  const double alpha = vcur[k + 1], dk = vcur[k];
+  double tk = 0, beta = alpha, scale = 0;
+  if (sigma > 0) {
+    const double nrm = sqrt(alpha * alpha + sigma);
+    beta = alpha >= 0 ? -nrm : nrm;
+    tk = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
+  }
+  __syncthreads();
+  for (int j = k + 1 + tid; j < n; j += kStepThreads) vcur[j] = j == k + 1 ? 1.0 : vcur[j] * scale;
+  __syncthreads();
+  if (chunk == 0 && tid == 0) {
+    d[k] = dk;
+    e[k] = beta;
+    tau[k] = tk;
+  }
+  // this workgroup's rows: pending update over columns >= k+1, then (A22 v)_i
+  const int i = k + 1 + chunk * kStepRows + wv;
+  if (i < n) {
+    double* row = a + int64_t(i) * n;
+    const double vi = pend ? vprev[i] : 0.0, wi = pend ? wprev[i] : 0.0;
+    double acc = 0;
+#pragma unroll 4
+    for (int j = k + 1 + lane; j < n; j += 64) {
+      double x = row[j];
+      if (pend) x -= vi * wprev[j] + wi * vprev[j];
+      row[j] = x;
+      acc += x * vcur[j];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      pb_cur[i] = acc;
+      vb_cur[i] = vcur[i];
+      if (i >= k + 2) row[k] = vcur[i];
+    }
+  }
+}
+
+// Reflector k from column k (rows >= k+2) into row k (columns >= k+2), for
+// the back-transformation; 32 x 32 tiles of the strictly lower part.
+struct TileItem {
+  int32_t job, ti, tj, pad;
+};
+__global__ __launch_bounds__(256) void reflector_rows_kernel(const TileItem* __restrict__ items,
+                                                             const DevJob* __restrict__ jobs, double* __restrict__ A) {
+  __shared__ double t[32][33];
+  const TileItem it = items[blockIdx.x];
+  const DevJob jb = jobs[it.job];
+  const int n = jb.n;
+  double* a = A + jb.a_off;
+  const int r0 = it.ti * 32, c0 = it.tj * 32;  // source tile rows r0.., columns c0.. (r0 >= c0)
+  for (int e2 = threadIdx.x; e2 < 32 * 32; e2 += 256) {
+    const int r = e2 >> 5, c = e2 & 31;
+    const int gi = r0 + r, gj = c0 + c;
+    t[r][c] = (gi < n && gj < n) ? a[int64_t(gi) * n + gj] : 0.0;
+  }
+  __syncthreads();
+  for (int e2 = threadIdx.x; e2 < 32 * 32; e2 += 256) {
+    const int r = e2 >> 5, c = e2 & 31;  // destination row c0 + r, column r0 + c
+    const int gk = c0 + r, gj = r0 + c;
+    if (gk < n && gj < n && gj >= gk + 2 && gk + 2 < n) a[int64_t(gk) * n + gj] = t[c][r];
   }
 }
 
@@ -341,6 +520,35 @@ __global__ __launch_bounds__(NT) void bisect_kernel(const BisItem* __restrict__ 
 struct VecItem {
   int32_t job, c;
 };
+
+// One solve with the factors of T - lambda I stored by invit_kernel for
+// vector c (row i, field f at lu[(i * kLuFields + f) * k + c]): y <- U^-1 L^-1 P y
+// for the vector y[i * ys], in place, by the calling thread; returns max |y|.
+__device__ double lu_solve_inplace(const double* __restrict__ lu, int k, int c, int n, double* y, int ys) {
+  auto F = [&](int i, int f) { return lu[(int64_t(i) * kLuFields + f) * k + c]; };
+  double cur = y[0];
+  for (int i = 0; i + 1 < n; ++i) {
+    double nxt = y[(i + 1) * ys];
+    if (F(i, 4) != 0.0) {
+      const double t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    nxt -= F(i, 3) * cur;
+    y[i * ys] = cur;
+    cur = nxt;
+  }
+  y[(n - 1) * ys] = cur;
+  double z1 = 0, z2 = 0, zmax = 0;
+  for (int i = n - 1; i >= 0; --i) {
+    const double zi = (y[i * ys] - F(i, 1) * z1 - F(i, 2) * z2) / F(i, 0);
+    y[i * ys] = zi;
+    zmax = fmax(zmax, fabs(zi));
+    z2 = z1;
+    z1 = zi;
+  }
+  return zmax;
+}
 
 __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ items, int n_items,
                                                    const DevJob* __restrict__ jobs, const double* __restrict__ Dg,
@@ -441,7 +649,8 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
 // ones; it restarts from a hashed vector projected against every earlier
 // column.
 template <int NT, int NB>
-__global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ jobs, double* __restrict__ Z) {
+__global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ jobs, double* __restrict__ Z,
+                                                  const double* __restrict__ LU) {
   constexpr int NW = NT / 64;
   const DevJob jb = jobs[blockIdx.x];
   const int n = jb.n, k = jb.k;
@@ -568,11 +777,23 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
           __syncthreads();
           break;
         }
-        // numerically dependent: restart from a hashed vector
+        // numerically dependent: restart from a hashed vector, pulled back into
+        // the eigenspace by one solve with this vector's factors when the
+        // columns are inverse-iteration vectors (LU != null)
         for (int i = tid; i < n; i += NT)
           Zb[i * NB + c] =
               hash_unit(uint32_t(b0 + c) * 40503u + uint32_t(attempt + 1) * 977u + uint32_t(jb.job), uint32_t(i));
         __syncthreads();
+        if (LU != nullptr) {
+          if (tid == 0) {
+            const double zmax = lu_solve_inplace(LU + jb.lu_off, k, b0 + c, n, Zb + c, NB);
+            nrm0[c] = zmax > 0 ? 1.0 / zmax : 1.0;
+          }
+          __syncthreads();
+          const double f = nrm0[c];
+          for (int i = tid; i < n; i += NT) Zb[i * NB + c] *= f;
+          __syncthreads();
+        }
         double p2[NB];
 #pragma unroll
         for (int q = 0; q < NB; ++q) p2[q] = 0;
@@ -857,11 +1078,11 @@ void launch_tridiag_t(Context* ctx, const DevJob* jobs, int n_jobs, int nmax, do
 
 template <int NB>
 void launch_vectors_t(Context* ctx, int nmax, int n_jobs, int n_bt, const DevJob* djd, const BtItem* bid,
-                      const double* A, const double* TAU, double* zi, double* Z) {
+                      const double* A, const double* TAU, double* zi, const double* lu, double* Z) {
   constexpr int kNT = 1024, NW = kNT / 64;
   const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
   allow_lds(orth_kernel<kNT, NB>, lds);
-  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(n_jobs), dim3(kNT), lds, ctx->stream, djd, zi);
+  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(n_jobs), dim3(kNT), lds, ctx->stream, djd, zi, lu);
   TK_HIP(hipGetLastError());
   constexpr int kBtNT = 256;
   const size_t lds2 = (size_t(nmax) * NB + 2 * (kBtNT / 64) * NB) * sizeof(double);
@@ -913,11 +1134,39 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   const DevJob* djd = reinterpret_cast<const DevJob*>(meta.get());
   const BisItem* itd = reinterpret_cast<const BisItem*>(meta.get() + jb_bytes);
   const int nj = int(jobs.size());
-  if (nmax_ <= 128) launch_tridiag_t<512, 2>(ctx_, djd, nj, nmax_, A, D, E, TAU);
-  else if (nmax_ <= 256) launch_tridiag_t<512, 4>(ctx_, djd, nj, nmax_, A, D, E, TAU);
-  else if (nmax_ <= 512) launch_tridiag_t<512, 8>(ctx_, djd, nj, nmax_, A, D, E, TAU);
-  else if (nmax_ <= 1024) launch_tridiag_t<512, 16>(ctx_, djd, nj, nmax_, A, D, E, TAU);
-  else launch_tridiag_t<256, 32>(ctx_, djd, nj, nmax_, A, D, E, TAU);
+  DBuf<double> vp;
+  DBuf<TileItem> tiles;
+  if (nmax_ <= kSmallTridiag) {
+    launch_tridiag_t<512, 2>(ctx_, djd, nj, nmax_, A, D, E, TAU);
+  } else {
+    // multi-workgroup column steps for every job, then reflectors into rows
+    vp.reset(ctx_, size_t(4 * std::max<int64_t>(w_n, 1)));
+    std::vector<int> ns(jobs.size());
+    for (size_t o = 0; o < jobs.size(); ++o) ns[o] = jobs[order[o]].n;
+    for (int k = 0; k + 1 < nmax_; ++k) {
+      int64_t grid = 0;
+      for (int n : ns) {
+        if (n == 1) grid += k == 0;
+        else if (k + 2 < n) grid += (n - k - 1 + kStepRows - 1) / kStepRows;
+        else grid += k + 2 == n;
+      }
+      if (grid == 0) continue;
+      hipLaunchKernelGGL(tdstep_kernel, dim3(unsigned(grid)), dim3(kStepThreads), 0, ctx_->stream, djd, nj, k, A, D,
+                         E, TAU, vp.get(), w_n);
+    }
+    std::vector<TileItem> ti;
+    for (size_t o = 0; o < jobs.size(); ++o) {
+      const int n = ns[o], nt = (n + 31) / 32;
+      if (n < 3) continue;
+      for (int a2 = 0; a2 < nt; ++a2)
+        for (int b2 = 0; b2 <= a2; ++b2) ti.push_back({int32_t(o), a2, b2, 0});
+    }
+    if (!ti.empty()) {
+      tiles.reset(ctx_, ti.size());
+      tiles.upload(ti.data(), ti.size());
+      hipLaunchKernelGGL(reflector_rows_kernel, dim3(ti.size()), dim3(256), 0, ctx_->stream, tiles.get(), djd, A);
+    }
+  }
   TK_HIP(hipGetLastError());
   // the TN slot of a job is indexed by its position in the device job array
   DBuf<double> tn_all(ctx_, dj.size());
@@ -975,10 +1224,39 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
                      tn_.get(), lu.get(), zi.get());
   TK_HIP(hipGetLastError());
   if (NBv == 8)
-    launch_vectors_t<8>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), Z);
+    launch_vectors_t<8>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z);
   else
-    launch_vectors_t<4>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), Z);
+    launch_vectors_t<4>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z);
   ctx_->sync();
+}
+
+void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, double* Z, int64_t z_n) {
+  std::vector<DevJob> dj;
+  int nmax = 0;
+  for (const OrthJob& j : jobs) {
+    if (j.n < 1 || j.n > kSymEigMaxN || j.k < 0 || j.k > j.n) throw ArgError("orthonormalize: bad shape");
+    if (j.off < 0 || j.off + int64_t(j.n) * j.k > z_n) throw std::out_of_range("orthonormalize: job outside Z");
+    if (j.k == 0) continue;
+    dj.push_back({0, 0, j.off, 0, j.n, j.k, int32_t(dj.size()), 0});
+    nmax = std::max(nmax, j.n);
+  }
+  if (dj.empty()) return;
+  DBuf<DevJob> d(ctx, dj.size());
+  d.upload(dj.data(), dj.size());
+  constexpr int kNT = 1024, NW = kNT / 64;
+  if (nmax <= 1024) {
+    const size_t lds = (size_t(nmax) * 8 + NW * 64 * 8 + 64 * 8 + NW * 8 + 2 * 8) * sizeof(double);
+    allow_lds(orth_kernel<kNT, 8>, lds);
+    hipLaunchKernelGGL((orth_kernel<kNT, 8>), dim3(dj.size()), dim3(kNT), lds, ctx->stream, d.get(), Z,
+                       static_cast<const double*>(nullptr));
+  } else {
+    const size_t lds = (size_t(nmax) * 4 + NW * 64 * 4 + 64 * 4 + NW * 4 + 2 * 4) * sizeof(double);
+    allow_lds(orth_kernel<kNT, 4>, lds);
+    hipLaunchKernelGGL((orth_kernel<kNT, 4>), dim3(dj.size()), dim3(kNT), lds, ctx->stream, d.get(), Z,
+                       static_cast<const double*>(nullptr));
+  }
+  TK_HIP(hipGetLastError());
+  ctx->sync();
 }
 
 void lowest_eigpair(Context* ctx, const double* H, int ld, int m, double* c, double* theta) {

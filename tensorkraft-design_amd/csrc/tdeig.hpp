@@ -54,6 +54,15 @@ class SymEigBatch {
   DBuf<double> tn_;   // per job: Gershgorin bound of |T|
 };
 
+// Orthonormalise the k columns of each n x k row-major matrix (leading
+// dimension k) at off of Z, in column order (classical Gram-Schmidt applied
+// twice, blocked), in place.
+struct OrthJob {
+  int64_t off;
+  int32_t n, k;
+};
+void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, double* Z, int64_t z_n);
+
 // Lowest eigenpair of the m x m symmetric matrix whose lower triangle is
 // stored row-wise in H (H[i*ld + j], j <= i), m <= 64: one wavefront,
 // everything in LDS.  c (m) and *theta are device outputs.

@@ -11,8 +11,6 @@ namespace tk {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPerThread = 8;
-constexpr int kChunk = kThreads * kPerThread;
 constexpr int kMaxVec = 128;
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -21,35 +19,50 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// partial[i * nchunks + c] = sum over chunk c of V_i . x for the kVecGroup
+// vectors of group blockIdx.y; chunks of kDotChunk elements, one barrier.
+constexpr int kDotPer = 4;
+constexpr int kDotChunk = kThreads * kDotPer;
+constexpr int kVecGroup = 8;
+
 __global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __restrict__ V, int64_t ldv, int nvec,
                                                                const double* __restrict__ x, int64_t n,
                                                                double* __restrict__ partial, int nchunks) {
-  __shared__ double red[kThreads / 64];
-  const int64_t base = int64_t(blockIdx.x) * kChunk + threadIdx.x;
-  double xr[kPerThread];
+  __shared__ double red[kThreads / 64][kVecGroup];
+  const int64_t base = int64_t(blockIdx.x) * kDotChunk + threadIdx.x;
+  const int i0 = blockIdx.y * kVecGroup;
+  const int nv = min(kVecGroup, nvec - i0);
+  double xr[kDotPer];
 #pragma unroll
-  for (int j = 0; j < kPerThread; ++j) {
-    int64_t idx = base + int64_t(j) * kThreads;
+  for (int j = 0; j < kDotPer; ++j) {
+    const int64_t idx = base + int64_t(j) * kThreads;
     xr[j] = idx < n ? x[idx] : 0.0;
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = 0; i < nvec; ++i) {
-    const double* __restrict__ v = V + int64_t(i) * ldv;
-    double s = 0.0;
+  double s[kVecGroup];
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-      int64_t idx = base + int64_t(j) * kThreads;
-      if (idx < n) s += v[idx] * xr[j];
+  for (int q = 0; q < kVecGroup; ++q) {
+    s[q] = 0.0;
+    if (q < nv) {
+      const double* __restrict__ v = V + int64_t(i0 + q) * ldv;
+#pragma unroll
+      for (int j = 0; j < kDotPer; ++j) {
+        const int64_t idx = base + int64_t(j) * kThreads;
+        if (idx < n) s[q] += v[idx] * xr[j];
+      }
     }
-    s = wave_sum(s);
-    if (lane == 0) red[wave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < kThreads / 64; ++w) t += red[w];
-      partial[int64_t(i) * nchunks + blockIdx.x] = t;
-    }
-    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < kVecGroup; ++q) {
+    const double t = wave_sum(s[q]);
+    if (lane == 0) red[wave][q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) t += red[w][threadIdx.x];
+    partial[int64_t(i0 + threadIdx.x) * nchunks + blockIdx.x] = t;
   }
 }
 
@@ -134,10 +147,10 @@ double* scratch(Context* ctx, size_t n) { return static_cast<double*>(ctx->scrat
 void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double* x, int64_t n, double* out) {
   if (nvec <= 0) return;
   if (nvec > kMaxVec) throw ArgError("dot_many: too many vectors");
-  int nchunks = int(std::max<int64_t>(1, (n + kChunk - 1) / kChunk));
+  const int nchunks = int(std::max<int64_t>(1, (n + kDotChunk - 1) / kDotChunk));
   double* part = scratch(ctx, size_t(nchunks) * nvec);
-  hipLaunchKernelGGL(dot_partial_kernel, dim3(nchunks), dim3(kThreads), 0, ctx->stream, V, ldv, nvec, x, n, part,
-                     nchunks);
+  const dim3 grid(nchunks, (nvec + kVecGroup - 1) / kVecGroup);
+  hipLaunchKernelGGL(dot_partial_kernel, grid, dim3(kThreads), 0, ctx->stream, V, ldv, nvec, x, n, part, nchunks);
   hipLaunchKernelGGL(reduce_rows_kernel, dim3(nvec), dim3(kThreads), 0, ctx->stream, part, nchunks, out);
   TK_HIP(hipGetLastError());
 }

@@ -670,15 +670,18 @@ def _qkey(q):
     return (q, 0) if np.isscalar(q) else tuple(q)
 
 
-def initial_mps(sym, phys, total, d0, seed=0):
+def initial_mps(sym, phys, total, d0, seed=0, min_sector_dim=1):
     """Seeded random MPS with N(0,1) entries in every allowed block.
 
     Bond i carries the charges reachable from the left with i sites and from
     which `total` is reachable with the remaining sites; sector dimensions are
     proportional to min(left count, right count) of basis configurations,
-    capped by that count, at least 1, rounded to d0 in total by largest
-    remainder.  Entries come from numpy's PCG64 seeded with `seed`, site by
-    site in block-table order.  Returns (bonds, data).
+    capped by that count, at least `min_sector_dim`, rounded to d0 in total by
+    largest remainder.  With min_sector_dim=0 the improbable charges (those
+    that round to zero) are left out, so the state starts with the
+    binomially concentrated charge distribution instead of every reachable
+    one.  Entries come from numpy's PCG64 seeded with `seed`, site by site in
+    block-table order.  Returns (bonds, data).
     """
     n = len(phys)
     total = _qkey(total)
@@ -715,7 +718,7 @@ def initial_mps(sym, phys, total, d0, seed=0):
         else:
             tot = float(sum(cap))
             raw = [d0 * c / tot for c in cap]
-            dims = [max(1, min(c, int(math.floor(r)))) for c, r in zip(cap, raw)]
+            dims = [max(min_sector_dim, min(c, int(math.floor(r)))) for c, r in zip(cap, raw)]
             order = sorted(range(len(qs)), key=lambda j: (-(raw[j] - math.floor(raw[j])), qs[j]))
             while sum(dims) < d0:
                 moved = False
@@ -729,15 +732,39 @@ def initial_mps(sym, phys, total, d0, seed=0):
                     break
             while sum(dims) > d0:
                 j = max(range(len(qs)), key=lambda j: (dims[j], qs[j]))
-                if dims[j] <= 1:
+                if dims[j] <= max(1, min_sector_dim):
                     break
                 dims[j] -= 1
-        bonds.append([(q if sym == SYM_U1U1 else q, int(d)) for q, d in zip(qs, dims)])
+        kept = [(q, int(d)) for q, d in zip(qs, dims) if d > 0]
+        bonds.append(kept)
     rng = np.random.default_rng(seed)
     data = []
     for i in range(n):
         _, _, _, tot = block_table(sym, [phys[i], bonds[i], bonds[i + 1]], [DIR_OUT, DIR_OUT, DIR_IN])
         data.append(rng.standard_normal(tot))
+    return bonds, data
+
+
+def product_mps(sym, phys, states):
+    """Product-state MPS: site i in the first basis state of physical sector
+    states[i] (an index into phys[i]).  Every bond has one charge of
+    dimension 1.  Returns (bonds, data) like initial_mps."""
+    n = len(phys)
+    q = (0, 0)
+    bonds = [[(q, 1)]]
+    for i in range(n):
+        s = _qkey(phys[i][states[i]][0])
+        q = (q[0] + s[0], q[1] + s[1])
+        bonds.append([(q, 1)])
+    data = []
+    for i in range(n):
+        keys, offs, secs, tot = block_table(sym, [phys[i], bonds[i], bonds[i + 1]], [DIR_OUT, DIR_OUT, DIR_IN])
+        # single-charge bonds admit exactly the block of the chosen physical charge
+        if len(keys) != 1:
+            raise ValueError(f"product_mps: site {i} admits {len(keys)} blocks")
+        x = np.zeros(tot)
+        x[offs[0]] = 1.0
+        data.append(x)
     return bonds, data
 
 

@@ -142,6 +142,43 @@ def test_eigh_lowest(be):
     assert abs(w[0] - (2 - math.sqrt(2))) < 1e-10 and w[0] <= w[1] <= w[2]
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 50, 128, 129, 300, 700])
+def test_eigh_matches_lapack(be, n):
+    """Tridiagonalisation (one workgroup up to 128, column steps above),
+    bisection, inverse iteration, CGS2 and back-transformation against
+    LAPACK's eigenvalues; eigenvectors by residual and orthonormality."""
+    rng = np.random.default_rng(1000 + n)
+    x = rng.standard_normal((n, n))
+    a = (x + x.T) / 2
+    w, v = be.eigh(a)
+    ref = np.linalg.eigvalsh(a)
+    scale = np.abs(ref).max()
+    assert np.abs(w - ref).max() < 1e-12 * scale * max(1.0, math.sqrt(n) / 4)
+    assert np.abs(a @ v - v * w).max() < 1e-11 * scale
+    assert np.abs(v.T @ v - np.eye(n)).max() < 1e-12 * max(1.0, n / 50)
+
+
+@pytest.mark.parametrize("n", [40, 300])
+def test_eigh_degenerate_spectrum(be, n):
+    """Exactly repeated eigenvalues (multiplets of 1, 2, 3 and 5): the
+    eigenvectors must stay orthonormal and span each eigenspace."""
+    rng = np.random.default_rng(7 * n)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    mult = [1, 2, 3, 5]
+    lam = []
+    level = 0
+    while len(lam) < n:
+        lam += [float(level)] * mult[level % 4]
+        level += 1
+    lam = np.array(lam[:n])
+    a = (q * lam) @ q.T
+    a = (a + a.T) / 2
+    w, v = be.eigh(a)
+    assert np.abs(w - np.sort(lam)).max() < 1e-11 * lam.max()
+    assert np.abs(a @ v - v * w).max() < 1e-10 * lam.max()
+    assert np.abs(v.T @ v - np.eye(n)).max() < 1e-11
+
+
 class _Dev:
     def __init__(self, p):
         self.ptr = p
