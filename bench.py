@@ -2,6 +2,10 @@
 """Benchmark of the block-sparse two-site DMRG hot path on one MI355X.
 
 Workloads
+  sweep  (default) two-site DMRG on the Hubbard chain L=64, t=1, U=4, half
+         filling, Sz=0, U(1)xU(1): bond dimension 256 -> 2048 over 8 sweeps
+         (the first 7 untimed), run in FP64 and in mixed precision from the
+         same seeded MPS; a step is one full sweep at D=2048.
   heff   one H_eff*psi on the U(1)xU(1) Hubbard microbenchmark bond
          (d=4, MPO bond 6, D=1024 on both sides, 145 sectors); a step is one
          matvec with inputs resident in HBM.
@@ -111,22 +115,97 @@ def warmup_schedule(d0, d_max, n):
     return out
 
 
-def run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, n_sweeps, prec, tol, log):
+def run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, warmup, steps, prec, tol, log):
+    """warmup + steps sweeps from the same seeded MPS; the last `steps` are
+    timed (device synchronised on both sides) with kernel timing enabled."""
     d = tk_hip.Dmrg(ctx, mpo, sym, prec=prec, schedule=sched, davidson_tol=tol, svd_cutoff=1e-12)
     d.set_state(bonds, data)
     t0 = time.perf_counter()
     d.prepare()
     log(f"[{prec}] prepared in {time.perf_counter() - t0:.2f}s")
-    out = []
-    for k in range(n_sweeps):
+    out, wall = [], 0.0
+    for k in range(warmup + steps):
+        timed = k >= warmup
+        if timed and k == warmup:
+            ctx.sync()
+            ctx.timing(True)
+        t0 = time.perf_counter()
         s = d.sweep()
+        ctx.sync()
+        el = time.perf_counter() - t0
+        if timed:
+            wall += el
         out.append(s)
-        log(f"[{prec}] sweep {k + 1}/{n_sweeps} D={sched[min(k, len(sched) - 1)]} E={s['energy']:.12f} "
-            f"t={s['seconds']:.3f}s (plan {s['t_plan']:.2f} eig {s['t_eig']:.2f} svd {s['t_svd']:.2f} "
-            f"env {s['t_env']:.2f}) matvecs={s['matvecs']} fp64={s['matvecs_fp64']} maxD={s['max_bond_dim']} "
-            f"trunc={s['max_truncation_error']:.2e}")
+        log(f"[{prec}] sweep {k + 1}/{warmup + steps}{' (timed)' if timed else ''} "
+            f"D={sched[min(k, len(sched) - 1)]} E={s['energy']:.13f} t={el:.3f}s (plan {s['t_plan']:.2f} "
+            f"eig {s['t_eig']:.2f} svd {s['t_svd']:.2f} env {s['t_env']:.2f}) matvecs={s['matvecs']} "
+            f"fp64={s['matvecs_fp64']} maxD={s['max_bond_dim']} trunc={s['max_truncation_error']:.2e}")
+    timing = ctx.timing_read()
+    ctx.timing(False)
+    n = d.n
+    centre = (d.bond(n // 2 - 1), d.bond(n // 2 + 1))
     d.close()
+    return out, wall, timing, centre
+
+
+def roofline_of(timing):
+    """Roofline of the kernel with the largest event time: algorithmic bytes
+    per launch over the mean launch duration, against the HBM peak."""
+    name = max(timing, key=lambda k: timing[k]["ms"])
+    t = timing[name]
+    sec = t["ms"] * 1e-3
+    achieved = t["bytes"] / sec / 1e9 if sec > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "launches": t["launches"], "mean_launch_us": round(1e3 * t["ms"] / max(1, t["launches"]), 3),
+            "bytes_per_launch": round(t["bytes"] / max(1, t["launches"]), 1),
+            "tflops": round(t["flops"] / sec / 1e12, 3) if sec > 0 else 0.0}
+
+
+def kernel_table(timing):
+    out = {}
+    for name, t in sorted(timing.items(), key=lambda kv: -kv[1]["ms"]):
+        sec = t["ms"] * 1e-3
+        out[name] = {"launches": t["launches"], "ms": round(t["ms"], 3),
+                     "GB/s": round(t["bytes"] / sec / 1e9, 1) if sec > 0 else 0.0,
+                     "TF/s": round(t["flops"] / sec / 1e12, 3) if sec > 0 else 0.0}
     return out
+
+
+def cpu_baseline_sweep(tk_hip, ctx, centre, heff_flops, budget_s, log):
+    """The oracle's FP64 H_eff*psi (OpenMP over the host cores) on a two-site
+    problem with the timed sweep's centre-bond sectors (random entries), timed
+    for about budget_s, scaled to the timed sweep by its summed matvec flops."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    W = tk_hip.hubbard_w(1.0, 4.0)
+    p = tk_hip.heff_problem(centre[0], centre[1], W, seed=1)
+    L, R = ctx.to_device(p["L"]), ctx.to_device(p["R"])
+    plan = tk_hip.HeffPlan(ctx, "fp64", p["sym"], p["bond_l"], W.wl, L, p["bond_r"], W.wr, R, W, W)
+    f_centre = plan.stats()["flops"]
+    plan.close()
+    L.free()
+    R.free()
+    oracle.heff_apply(p, p["x"])
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.heff_apply(p, p["x"])
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    per = el / n
+    rate = f_centre / per
+    cores = os.cpu_count()
+    log(f"[cpu] oracle H_eff*psi at the centre bonds ({sum(d for _, d in centre[0])}, "
+        f"{sum(d for _, d in centre[1])}): {per * 1e3:.1f} ms/matvec, {rate / 1e9:.1f} GFLOP/s on {cores} threads")
+    return {"value": round(heff_flops / rate, 3), "unit": "s/sweep (H_eff*psi matvecs only)", "cores": cores,
+            "kind": "port",
+            "sample": f"{n} oracle FP64 H_eff*psi on the timed sweep's centre bonds (D="
+                      f"{sum(d for _, d in centre[0])}, random entries), {per * 1e3:.1f} ms each = "
+                      f"{rate / 1e9:.1f} GFLOP/s, scaled by the sweep's {heff_flops / 1e12:.2f} TFLOP of matvecs; "
+                      f"SVD, environments and Krylov vector work excluded"}
 
 
 def bench_sweep(args):
@@ -137,19 +216,21 @@ def bench_sweep(args):
 
     n = args.sites
     sched = warmup_schedule(args.d0, args.dmax, args.ramp)
-    n_sweeps = args.warmup + args.steps
     mpo = tk_hip.chain_mpo("hubbard", n, t=1.0, U=4.0)
     sym = tk_hip.SYM_U1U1
-    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (n // 2, n // 2), sched[0], seed=0)
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (n // 2, n // 2), sched[0], seed=0,
+                                     min_sector_dim=0)
     ctx = tk_hip.Context(0)
     res = {}
     for prec in ("fp64", args.mixed):
-        res[prec] = run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, n_sweeps, prec, args.tol, log)
-    timed64 = res["fp64"][args.warmup:]
-    timedlo = res[args.mixed][args.warmup:]
-    t64 = sum(s["seconds"] for s in timed64) / len(timed64)
-    tlo = sum(s["seconds"] for s in timedlo) / len(timedlo)
-    de = abs(res[args.mixed][-1]["energy"] - res["fp64"][-1]["energy"])
+        res[prec] = run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, args.warmup, args.steps, prec, args.tol,
+                               log)
+    s64, w64, tim64, centre = res["fp64"]
+    slo, wlo, timlo, _ = res[args.mixed]
+    t64 = w64 / args.steps
+    tlo = wlo / args.steps
+    de = abs(slo[-1]["energy"] - s64[-1]["energy"])
+    timed64, timedlo = s64[args.warmup:], slo[args.warmup:]
     rec = {
         "metric": METRIC,
         "value": round(tlo, 4),
@@ -161,23 +242,36 @@ def bench_sweep(args):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32-mfma+f64-refine", "bf16x3": "bf16x3-f32acc+f64-refine"}[args.mixed],
+        "dtype": {"fp32": "f32 (f64 Krylov, SVD, refinement)", "bf16x3": "bf16x3-f32acc (f64 Krylov, SVD, "
+                                                                           "refinement)"}[args.mixed],
         "data": "synthetic",
-        "config": {"workload": "dmrg_sweep_hubbard", "sites": n, "t": 1.0, "U": 4.0, "filling": "half, Sz=0",
-                   "schedule": sched, "davidson_tol": args.tol, "timed_sweep_bond_dim": sched[-1],
-                   "mixed_precision": args.mixed},
+        "config": {"workload": "dmrg_sweep_hubbard_L64_D2048", "sites": n, "t": 1.0, "U": 4.0,
+                   "filling": "half, Sz=0", "symmetry": "U(1)xU(1)", "schedule": sched,
+                   "timed_sweep_bond_dim": sched[-1], "davidson_tol": args.tol, "svd_cutoff": 1e-12,
+                   "mixed_precision": args.mixed, "initial_mps": "PCG64 seed 0, binomially concentrated charges"},
+        "roofline": roofline_of(timlo),
         "fp64_seconds_per_sweep": round(t64, 4),
         "speedup_mixed_vs_fp64": round(t64 / tlo, 3),
-        "energy_fp64": res["fp64"][-1]["energy"],
-        "energy_mixed": res[args.mixed][-1]["energy"],
+        "energy_fp64": s64[-1]["energy"],
+        "energy_mixed": slo[-1]["energy"],
         "abs_delta_e": de,
         "delta_e_ok": de <= 1e-10,
-        "max_truncation_error": res[args.mixed][-1]["max_truncation_error"],
-        "phases_mixed": {k: round(sum(s[k] for s in timedlo) / len(timedlo), 4) for k in ("t_plan", "t_eig", "t_svd", "t_env")},
-        "phases_fp64": {k: round(sum(s[k] for s in timed64) / len(timed64), 4) for k in ("t_plan", "t_eig", "t_svd", "t_env")},
-        "matvecs_mixed": sum(s["matvecs"] for s in timedlo), "matvecs_mixed_fp64": sum(s["matvecs_fp64"] for s in timedlo),
+        "max_bond_dim": slo[-1]["max_bond_dim"],
+        "max_truncation_error": slo[-1]["max_truncation_error"],
+        "phases_mixed": {k: round(sum(s[k] for s in timedlo) / len(timedlo), 4)
+                         for k in ("t_plan", "t_eig", "t_svd", "t_env")},
+        "phases_fp64": {k: round(sum(s[k] for s in timed64) / len(timed64), 4)
+                        for k in ("t_plan", "t_eig", "t_svd", "t_env")},
+        "matvecs_mixed": sum(s["matvecs"] for s in timedlo),
+        "matvecs_mixed_fp64": sum(s["matvecs_fp64"] for s in timedlo),
         "matvecs_fp64": sum(s["matvecs"] for s in timed64),
+        "kernels_mixed": kernel_table(timlo),
+        "kernels_fp64": kernel_table(tim64),
     }
+    if not args.no_cpu_baseline:
+        heff_flops = sum(t["flops"] for k, t in tim64.items() if k.startswith("heff_"))
+        rec["cpu_baseline"] = cpu_baseline_sweep(tk_hip, ctx, centre, heff_flops / args.steps, args.cpu_seconds,
+                                                 log)
     ctx.close()
     return rec
 

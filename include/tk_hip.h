@@ -93,6 +93,22 @@ tk_status tk_device_free(tk_ctx* ctx, void* p_dev);
 tk_status tk_memcpy_h2d(tk_ctx* ctx, void* dst_dev, const void* src_host, int64_t bytes);
 tk_status tk_memcpy_d2h(tk_ctx* ctx, void* dst_host, const void* src_dev, int64_t bytes);
 
+/* Kernel timing (instrumentation; no counterpart in the reference).  When
+ * enabled, the H_eff stage kernels and the eigensolver kernels of the SVD are
+ * bracketed by HIP events on the context's stream; tk_ctx_timing_read waits
+ * for them and returns, per kernel name, the launches, the summed event time
+ * and the summed algorithmic bytes and flops, then clears the record.
+ * Enabling or disabling clears it too. */
+typedef struct {
+  char name[40];
+  int64_t launches;
+  double ms;
+  double bytes;
+  double flops;
+} tk_kernel_timing;
+tk_status tk_ctx_timing(tk_ctx* ctx, int32_t enable);
+tk_status tk_ctx_timing_read(tk_ctx* ctx, int32_t max, tk_kernel_timing* out, int32_t* n_out);
+
 /* ---- sector tables (host only) ------------------------------------------ */
 /* QIndex::new / iter_sectors / total_dim (tk-symmetry/src/sector_key.rs:108-158):
  * sectors sorted by quantum number, prefix offsets.  Outputs have n_sectors

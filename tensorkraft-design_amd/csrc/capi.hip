@@ -143,6 +143,30 @@ tk_status tk_ctx_sync(tk_ctx* ctx) {
   });
 }
 
+tk_status tk_ctx_timing(tk_ctx* ctx, int32_t enable) {
+  return guard(ctx, [&] {
+    if (!ctx) throw tk::ArgError("null context");
+    ctx->c->timer.enable(enable != 0);
+  });
+}
+
+tk_status tk_ctx_timing_read(tk_ctx* ctx, int32_t max, tk_kernel_timing* out, int32_t* n_out) {
+  return guard(ctx, [&] {
+    if (!ctx || !n_out || (max > 0 && !out)) throw tk::ArgError("null argument");
+    const std::vector<tk::KernelTimer::Total> t = ctx->c->timer.read();
+    if (int64_t(t.size()) > max) throw tk::ArgError("timing: output holds fewer entries than recorded kernels");
+    for (size_t i = 0; i < t.size(); ++i) {
+      std::memset(out[i].name, 0, sizeof(out[i].name));
+      std::strncpy(out[i].name, t[i].name.c_str(), sizeof(out[i].name) - 1);
+      out[i].launches = t[i].launches;
+      out[i].ms = t[i].ms;
+      out[i].bytes = t[i].bytes;
+      out[i].flops = t[i].flops;
+    }
+    *n_out = int32_t(t.size());
+  });
+}
+
 const char* tk_last_error(tk_ctx* ctx) {
   if (ctx && ctx->c) return ctx->c->last_error.c_str();
   return g_error.c_str();

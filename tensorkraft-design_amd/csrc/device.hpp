@@ -82,8 +82,99 @@ class DevicePool {
   size_t held_ = 0;
 };
 
+// Optional HIP-event timing of chosen launch sites on the context's stream
+// (benchmark instrumentation; off unless enabled).  Each mark brackets one or
+// more back-to-back launches of one kernel and carries their algorithmic
+// bytes and flops.
+class KernelTimer {
+ public:
+  struct Total {
+    std::string name;
+    int64_t launches = 0;
+    double ms = 0, bytes = 0, flops = 0;
+  };
+  bool on() const { return on_; }
+  void enable(bool on) {
+    on_ = on;
+    clear();
+  }
+  // -1 when timing is off
+  int begin(hipStream_t s) {
+    if (!on_) return -1;
+    Mark m;
+    m.a = take();
+    m.b = take();
+    TK_HIP(hipEventRecord(m.a, s));
+    marks_.push_back(m);
+    return int(marks_.size()) - 1;
+  }
+  void end(int tok, hipStream_t s, const char* name, double bytes, double flops, int64_t launches) {
+    if (tok < 0) return;
+    Mark& m = marks_[size_t(tok)];
+    TK_HIP(hipEventRecord(m.b, s));
+    m.name = name;
+    m.bytes = bytes;
+    m.flops = flops;
+    m.launches = launches;
+  }
+  // Waits for the recorded events and returns the totals per kernel name.
+  std::vector<Total> read() {
+    std::vector<Total> out;
+    for (Mark& m : marks_) {
+      TK_HIP(hipEventSynchronize(m.b));
+      float ms = 0;
+      TK_HIP(hipEventElapsedTime(&ms, m.a, m.b));
+      size_t i = 0;
+      while (i < out.size() && out[i].name != m.name) ++i;
+      if (i == out.size()) {
+        out.push_back(Total{});
+        out.back().name = m.name;
+      }
+      out[i].launches += m.launches;
+      out[i].ms += ms;
+      out[i].bytes += m.bytes;
+      out[i].flops += m.flops;
+    }
+    clear();
+    return out;
+  }
+  ~KernelTimer() {
+    clear();
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
+
+ private:
+  struct Mark {
+    hipEvent_t a = nullptr, b = nullptr;
+    std::string name;
+    double bytes = 0, flops = 0;
+    int64_t launches = 0;
+  };
+  hipEvent_t take() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    TK_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void clear() {
+    for (Mark& m : marks_) {
+      pool_.push_back(m.a);
+      pool_.push_back(m.b);
+    }
+    marks_.clear();
+  }
+  bool on_ = false;
+  std::vector<Mark> marks_;
+  std::vector<hipEvent_t> pool_;
+};
+
 struct Context {
   int device = 0;
+  KernelTimer timer;
   hipStream_t stream = nullptr;
   DevicePool pool;
   std::string last_error;

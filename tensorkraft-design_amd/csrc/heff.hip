@@ -238,6 +238,8 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
   stats_.s3_flops = h3_.flops;
   stats_.s1_bytes = es * (double(l_total_) + double(psi_.total) + double(t1_total));
   stats_.s3_bytes = es * (double(t1_total) + double(r_total_)) + 8.0 * double(psi_.total);
+  stats_.s1_bytes_fp64 = 8.0 * (double(l_total_) + double(psi_.total) + double(t1_total));
+  stats_.s3_bytes_fp64 = 8.0 * (double(t1_total) + double(r_total_) + double(psi_.total));
   stats_.bytes = stats_.s1_bytes + stats_.s3_bytes;
 }
 
@@ -262,38 +264,55 @@ void HeffPlan::rebind(const double* L, const double* R) {
 }
 
 void HeffPlan::apply_fp64(const double* x, double* y) {
+  int tk = ctx_->timer.begin(ctx_->stream);
   launch_contract(ctx_, d1_, L_, x, t1_.get());
+  ctx_->timer.end(tk, ctx_->stream, "heff_stage1_fp64", stats_.s1_bytes_fp64, stats_.s1_flops, 1);
+  tk = ctx_->timer.begin(ctx_->stream);
   launch_contract(ctx_, d3_, t1_.get(), R_, y);
+  ctx_->timer.end(tk, ctx_->stream, "heff_stage3_fp64", stats_.s3_bytes_fp64, stats_.s3_flops, 1);
 }
 
 void HeffPlan::stage1(const double* x) {
+  hipStream_t st = ctx_->stream;
+  int tk = -1;
   switch (prec_) {
     case Prec::FP64:
+      tk = ctx_->timer.begin(st);
       launch_contract(ctx_, d1_, L_, x, t1_.get());
+      ctx_->timer.end(tk, st, "heff_stage1_fp64", stats_.s1_bytes, stats_.s1_flops, 1);
       return;
     case Prec::FP32:
       ensure_low();
       convert_f64_to_f32(ctx_, x, xf_.get(), psi_.total);
+      tk = ctx_->timer.begin(st);
       launch_contract(ctx_, d1_, Lf_.get(), xf_.get(), t1f_.get());
+      ctx_->timer.end(tk, st, "heff_stage1_fp32", stats_.s1_bytes, stats_.s1_flops, 1);
       return;
     case Prec::BF16X3:
       ensure_low();
       convert_f64_to_f32(ctx_, x, xf_.get(), psi_.total);
+      tk = ctx_->timer.begin(st);
       launch_contract_bf16x3(ctx_, d1_, Lf_.get(), xf_.get(), t1f_.get());
+      ctx_->timer.end(tk, st, "heff_stage1_bf16x3", stats_.s1_bytes, stats_.s1_flops, 1);
       return;
   }
 }
 
 void HeffPlan::stage3(double* y) {
+  hipStream_t st = ctx_->stream;
+  const int tk = ctx_->timer.begin(st);
   switch (prec_) {
     case Prec::FP64:
       launch_contract(ctx_, d3_, t1_.get(), R_, y);
+      ctx_->timer.end(tk, st, "heff_stage3_fp64", stats_.s3_bytes, stats_.s3_flops, 1);
       return;
     case Prec::FP32:
       launch_contract(ctx_, d3_, t1f_.get(), Rf_.get(), y);
+      ctx_->timer.end(tk, st, "heff_stage3_fp32", stats_.s3_bytes, stats_.s3_flops, 1);
       return;
     case Prec::BF16X3:
       launch_contract_bf16x3(ctx_, d3_, t1f_.get(), Rf_.get(), y);
+      ctx_->timer.end(tk, st, "heff_stage3_bf16x3", stats_.s3_bytes, stats_.s3_flops, 1);
       return;
   }
 }

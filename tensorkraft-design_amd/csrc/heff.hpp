@@ -61,6 +61,7 @@ struct HeffStats {
   double flops = 0;      // algorithmic flops of one apply
   double bytes = 0;      // algorithmic HBM bytes of one apply (at the compute precision)
   double s1_flops = 0, s1_bytes = 0, s3_flops = 0, s3_bytes = 0;  // per launch
+  double s1_bytes_fp64 = 0, s3_bytes_fp64 = 0;                     // the FP64 refinement launches
   int64_t n_psi_blocks = 0, dim = 0, t1_elems = 0;
   int64_t stage1_items = 0, stage3_items = 0, stage1_tasks = 0, stage3_terms = 0;
 };

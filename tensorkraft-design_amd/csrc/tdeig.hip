@@ -17,7 +17,8 @@ namespace {
 
 constexpr double kEps = DBL_EPSILON;
 constexpr int kLuFields = 5;
-constexpr int kSmallTridiag = 128;  // largest batch dimension for the one-workgroup tridiagonalisation
+constexpr int kSmallTridiag = 128;
+constexpr int kBisLanes = 16;  // lanes per eigenvalue in the multisection  // largest batch dimension for the one-workgroup tridiagonalisation
 
 struct DevJob {
   int64_t a_off;   // matrix / reflectors (and output columns in Z)
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
 // is stored in column k below the subdiagonal by the owners of those rows and
 // moved into row k afterwards.  VP holds per job: v (two buffers) and A22 v
 // (two buffers), each indexed like the eigenvalues.
-constexpr int kStepRows = 4;
+constexpr int kStepRows = 16;
 constexpr int kStepThreads = 64 * kStepRows;
 
 __device__ __forceinline__ int step_count(int n, int k) {
@@ -306,8 +307,13 @@ __device__ __forceinline__ int step_count(int n, int k) {
 __global__ __launch_bounds__(kStepThreads) void tdstep_kernel(const DevJob* __restrict__ jobs, int n_jobs, int k,
                                                               double* __restrict__ A, double* __restrict__ Dg,
                                                               double* __restrict__ Eg, double* __restrict__ TAUg,
-                                                              double* __restrict__ VP, int64_t w_n) {
-  __shared__ double vprev[kSymEigMaxN], wprev[kSymEigMaxN], vcur[kSymEigMaxN];
+                                                              double* __restrict__ VP, int64_t w_n, int ldsn) {
+  constexpr int kPerThread = kSymEigMaxN / kStepThreads;  // row-k / vector entries per thread
+  constexpr int kPerLane = kSymEigMaxN / 64;              // own-row entries per lane
+  extern __shared__ double sm[];
+  double* vprev = sm;
+  double* wprev = sm + ldsn;
+  double* vcur = sm + 2 * ldsn;
   __shared__ double red[2 * kStepRows];
   int id = blockIdx.x, q = 0;
   for (; q < n_jobs; ++q) {
@@ -336,22 +342,48 @@ __global__ __launch_bounds__(kStepThreads) void tdstep_kernel(const DevJob* __re
     }
     return;
   }
-  // pending update of step k-1: w = tau p - tau^2/2 (p.v) v over j >= k
   const bool pend = k >= 1;
+  const bool last = k + 2 == n;
+  const int i = k + 1 + chunk * kStepRows + wv;  // this wavefront's row
+  const bool own = !last && i < n;
+  // every global operand of the step is requested up front
+  double pp[kPerThread], vv[kPerThread], xk[kPerThread], xr[kPerLane];
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    const int j = k + tid + u * kStepThreads;
+    const bool in = j < n;
+    pp[u] = (pend && in) ? pb_prev[j] : 0.0;
+    vv[u] = (pend && in) ? (j == k ? 1.0 : vb_prev[j]) : 0.0;
+    xk[u] = (!last && in) ? a[int64_t(k) * n + j] : 0.0;
+  }
+  const double* rowi = a + int64_t(own ? i : 0) * n;
+#pragma unroll
+  for (int u = 0; u < kPerLane; ++u) {
+    const int j = k + 1 + lane + 64 * u;
+    xr[u] = (own && j < n) ? rowi[j] : 0.0;
+  }
+  // pending update of step k-1: w = tau p - tau^2/2 (p.v) v over j >= k
   if (pend) {
     const double tp = tau[k - 1];
     double loc = 0;
-    for (int j = k + tid; j < n; j += kStepThreads) {
-      const double vj = j == k ? 1.0 : vb_prev[j];
-      vprev[j] = vj;
-      loc += pb_prev[j] * vj;
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      const int j = k + tid + u * kStepThreads;
+      if (j < n) {
+        vprev[j] = vv[u];
+        loc += pp[u] * vv[u];
+      }
     }
     const double pv = block_sum<kStepThreads>(loc, red, parity);
     const double alw = -0.5 * tp * tp * pv;
-    for (int j = k + tid; j < n; j += kStepThreads) wprev[j] = tp * pb_prev[j] + alw * vprev[j];
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      const int j = k + tid + u * kStepThreads;
+      if (j < n) wprev[j] = tp * pp[u] + alw * vv[u];
+    }
     __syncthreads();
   }
-  if (k + 2 == n) {  // last 2 x 2 block
+  if (last) {  // the final 2 x 2 block
     if (tid == 0) {
       const int p0 = k, p1 = k + 1;
       double app = a[int64_t(p0) * n + p0], apq = a[int64_t(p0) * n + p1], aqq = a[int64_t(p1) * n + p1];
@@ -370,13 +402,18 @@ __global__ __launch_bounds__(kStepThreads) void tdstep_kernel(const DevJob* __re
     return;
   }
   // row k with the pending update, then its reflector
-  const double* rowk = a + int64_t(k) * n;
+  const double vk = pend ? vprev[k] : 0.0, wk = pend ? wprev[k] : 0.0;
   double ss = 0;
-  for (int j = k + tid; j < n; j += kStepThreads) {
-    double x = rowk[j];
-    if (pend) x -= vprev[k] * wprev[j] + wprev[k] * vprev[j];
-    vcur[j] = x;
-    if (j >= k + 2) ss += x * x;
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    const int j = k + tid + u * kStepThreads;
+    if (j < n) {
+      double x = xk[u];
+      if (pend) x -= vk * wprev[j] + wk * vprev[j];
+      xk[u] = x;
+      vcur[j] = x;
+      if (j >= k + 2) ss += x * x;
+    }
   }
   const double sigma = block_sum<kStepThreads>(ss, red, parity);
   const double alpha = vcur[k + 1], dk = vcur[k];
@@ -388,25 +425,31 @@ __global__ __launch_bounds__(kStepThreads) void tdstep_kernel(const DevJob* __re
     scale = 1.0 / (alpha - beta);
   }
   __syncthreads();
-  for (int j = k + 1 + tid; j < n; j += kStepThreads) vcur[j] = j == k + 1 ? 1.0 : vcur[j] * scale;
+#pragma unroll
+  for (int u = 0; u < kPerThread; ++u) {
+    const int j = k + tid + u * kStepThreads;
+    if (j > k && j < n) vcur[j] = j == k + 1 ? 1.0 : xk[u] * scale;
+  }
   __syncthreads();
   if (chunk == 0 && tid == 0) {
     d[k] = dk;
     e[k] = beta;
     tau[k] = tk;
   }
-  // this workgroup's rows: pending update over columns >= k+1, then (A22 v)_i
-  const int i = k + 1 + chunk * kStepRows + wv;
-  if (i < n) {
+  // this wavefront's row: pending update over columns >= k+1, then (A22 v)_i
+  if (own) {
     double* row = a + int64_t(i) * n;
     const double vi = pend ? vprev[i] : 0.0, wi = pend ? wprev[i] : 0.0;
     double acc = 0;
-#pragma unroll 4
-    for (int j = k + 1 + lane; j < n; j += 64) {
-      double x = row[j];
-      if (pend) x -= vi * wprev[j] + wi * vprev[j];
-      row[j] = x;
-      acc += x * vcur[j];
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) {
+      const int j = k + 1 + lane + 64 * u;
+      if (j < n) {
+        double x = xr[u];
+        if (pend) x -= vi * wprev[j] + wi * vprev[j];
+        row[j] = x;
+        acc += x * vcur[j];
+      }
     }
     acc = wave_sum(acc);
     if (lane == 0) {
@@ -499,18 +542,39 @@ __global__ __launch_bounds__(NT) void bisect_kernel(const BisItem* __restrict__ 
   gl -= 4.0 * kEps * tnorm + 2.0 * pivmin;
   gu += 4.0 * kEps * tnorm + 2.0 * pivmin;
   if (it.j0 == 0 && tid == 0) TN[it.job] = tnorm;
-  const int j = it.j0 + tid;
-  if (j >= n) return;
+  // multisection: kBisLanes lanes per eigenvalue evaluate kBisLanes interior
+  // points of the bracket per round and keep the sub-interval of the target
+  const int g = (tid & 63) / kBisLanes, t = tid & (kBisLanes - 1);
+  const int j = it.j0 + (tid / kBisLanes);
+  const bool valid = j < n;
   const double atol = 2.0 * kEps * tnorm;
   double lo = gl, hi = gu;
-  for (int iter = 0; iter < 256; ++iter) {
-    if (hi - lo <= fmax(atol, 2.0 * kEps * fmax(fabs(lo), fabs(hi)))) break;
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (sturm_count(d, e2, n, mid, pivmin) > j) hi = mid;
-    else lo = mid;
+  for (int round = 0; round < 64; ++round) {
+    const bool done = !valid || hi - lo <= fmax(atol, 2.0 * kEps * fmax(fabs(lo), fabs(hi)));
+    const double x = lo + (hi - lo) * double(t + 1) / double(kBisLanes + 1);
+    const bool inside = !done && x > lo && x < hi;
+    const int cnt = inside ? sturm_count(d, e2, n, x, pivmin) : 0;
+    const unsigned long long above = __ballot(inside && cnt > j);
+    const unsigned long long any_in = __ballot(inside);
+    const unsigned gm = unsigned((above >> (g * kBisLanes)) & ((1ull << kBisLanes) - 1));
+    const unsigned gi = unsigned((any_in >> (g * kBisLanes)) & ((1ull << kBisLanes) - 1));
+    double nlo = lo, nhi = hi;
+    if (gm != 0u) {
+      const int f = __ffs(int(gm)) - 1;
+      nhi = __shfl(x, g * kBisLanes + f, 64);
+      if (f > 0) nlo = __shfl(x, g * kBisLanes + f - 1, 64);
+    } else {
+      const int top = 31 - __clz(int(gi | 1u));
+      const double xt = __shfl(x, g * kBisLanes + top, 64);
+      if (gi != 0u) nlo = xt;
+    }
+    if (gi != 0u) {
+      lo = nlo;
+      hi = nhi;
+    }
+    if (__ballot(gi != 0u) == 0ull) break;
   }
-  W[jb.w_off + (descending ? n - 1 - j : j)] = 0.5 * (lo + hi);
+  if (valid && t == 0) W[jb.w_off + (descending ? n - 1 - j : j)] = 0.5 * (lo + hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -555,6 +619,7 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
                                                    const double* __restrict__ Eg, const double* __restrict__ W,
                                                    const double* __restrict__ TN, double* __restrict__ LU,
                                                    double* __restrict__ Z) {
+  constexpr int C = 8;  // rows per prefetched chunk
   const int g = blockIdx.x * 64 + threadIdx.x;
   if (g >= n_items) return;
   const VecItem it = items[g];
@@ -575,29 +640,40 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
   const double piv = kEps * fmax(TN[jb.job], DBL_MIN);
   auto guard = [&](double x) { return fabs(x) < piv ? (x < 0 ? -piv : piv) : x; };
   double c0 = d[0] - lam, c1 = e[0];
-  for (int i = 0; i + 1 < n; ++i) {
-    const double ei = e[i];
-    const double dn = d[i + 1] - lam;
-    const double en = i + 2 < n ? e[i + 1] : 0.0;
-    if (fabs(c0) >= fabs(ei)) {
-      c0 = guard(c0);
-      const double m = ei / c0;
-      F(i, 0) = c0;
-      F(i, 1) = c1;
-      F(i, 2) = 0.0;
-      F(i, 3) = m;
-      F(i, 4) = 0.0;
-      c0 = dn - m * c1;
-      c1 = en;
-    } else {
-      const double m = c0 / ei;
-      F(i, 0) = ei;
-      F(i, 1) = dn;
-      F(i, 2) = en;
-      F(i, 3) = m;
-      F(i, 4) = 1.0;
-      c0 = c1 - m * dn;
-      c1 = -m * en;
+  for (int i0 = 0; i0 + 1 < n; i0 += C) {
+    double dn[C], ei[C], en[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      const int i = i0 + u;
+      dn[u] = i + 1 < n ? d[i + 1] : 0.0;
+      ei[u] = i + 1 < n ? e[i] : 0.0;
+      en[u] = i + 2 < n ? e[i + 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      const int i = i0 + u;
+      if (i + 1 >= n) break;
+      const double dd = dn[u] - lam;
+      if (fabs(c0) >= fabs(ei[u])) {
+        c0 = guard(c0);
+        const double m = ei[u] / c0;
+        F(i, 0) = c0;
+        F(i, 1) = c1;
+        F(i, 2) = 0.0;
+        F(i, 3) = m;
+        F(i, 4) = 0.0;
+        c0 = dd - m * c1;
+        c1 = en[u];
+      } else {
+        const double m = c0 / ei[u];
+        F(i, 0) = ei[u];
+        F(i, 1) = dd;
+        F(i, 2) = en[u];
+        F(i, 3) = m;
+        F(i, 4) = 1.0;
+        c0 = c1 - m * dd;
+        c1 = -m * en[u];
+      }
     }
   }
   F(n - 1, 0) = guard(c0);
@@ -608,36 +684,75 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
   for (int itn = 0; itn < 3; ++itn) {
     // forward: row interchanges and multipliers applied to s * y
     double cur = s * Y(0);
-    for (int i = 0; i + 1 < n; ++i) {
-      double nxt = s * Y(i + 1);
-      if (F(i, 4) != 0.0) {
-        const double t = cur;
-        cur = nxt;
-        nxt = t;
+    for (int i0 = 0; i0 + 1 < n; i0 += C) {
+      double yn[C], sw[C], ml[C];
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int i = i0 + u;
+        const bool in = i + 1 < n;
+        yn[u] = in ? Y(i + 1) : 0.0;
+        sw[u] = in ? F(i, 4) : 0.0;
+        ml[u] = in ? F(i, 3) : 0.0;
       }
-      nxt -= F(i, 3) * cur;
-      Y(i) = cur;
-      cur = nxt;
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int i = i0 + u;
+        if (i + 1 >= n) break;
+        double nxt = s * yn[u];
+        if (sw[u] != 0.0) {
+          const double t = cur;
+          cur = nxt;
+          nxt = t;
+        }
+        nxt -= ml[u] * cur;
+        Y(i) = cur;
+        cur = nxt;
+      }
     }
     Y(n - 1) = cur;
     // back substitution with the two superdiagonals
     double z1 = 0, z2 = 0, zmax = 0;
-    for (int i = n - 1; i >= 0; --i) {
-      const double zi = (Y(i) - F(i, 1) * z1 - F(i, 2) * z2) / F(i, 0);
-      Y(i) = zi;
-      zmax = fmax(zmax, fabs(zi));
-      z2 = z1;
-      z1 = zi;
+    for (int i1 = n - 1; i1 >= 0; i1 -= C) {
+      double yv[C], u0[C], u1[C], u2[C];
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int i = i1 - u;
+        const bool in = i >= 0;
+        yv[u] = in ? Y(i) : 0.0;
+        u0[u] = in ? F(i, 0) : 1.0;
+        u1[u] = in ? F(i, 1) : 0.0;
+        u2[u] = in ? F(i, 2) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const int i = i1 - u;
+        if (i < 0) break;
+        const double zi = (yv[u] - u1[u] * z1 - u2[u] * z2) / u0[u];
+        Y(i) = zi;
+        zmax = fmax(zmax, fabs(zi));
+        z2 = z1;
+        z1 = zi;
+      }
     }
     s = zmax > 0 ? 1.0 / zmax : 1.0;
   }
   double nrm = 0;
-  for (int i = 0; i < n; ++i) {
-    const double y = Y(i) * s;
-    nrm += y * y;
+  for (int i0 = 0; i0 < n; i0 += C) {
+    double yv[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u) yv[u] = i0 + u < n ? Y(i0 + u) * s : 0.0;
+#pragma unroll
+    for (int u = 0; u < C; ++u) nrm += yv[u] * yv[u];
   }
   const double f = nrm > 0 ? s / sqrt(nrm) : 1.0;
-  for (int i = 0; i < n; ++i) Y(i) *= f;
+  for (int i0 = 0; i0 < n; i0 += C) {
+    double yv[C];
+#pragma unroll
+    for (int u = 0; u < C; ++u) yv[u] = i0 + u < n ? Y(i0 + u) : 0.0;
+#pragma unroll
+    for (int u = 0; u < C; ++u)
+      if (i0 + u < n) Y(i0 + u) = yv[u] * f;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1076,19 +1191,32 @@ void launch_tridiag_t(Context* ctx, const DevJob* jobs, int n_jobs, int nmax, do
   hipLaunchKernelGGL((tridiag_kernel<NT, T>), dim3(n_jobs), dim3(NT), lds, ctx->stream, jobs, A, D, E, TAU);
 }
 
+// Algorithmic work of the vector phase: CGS2 reads the earlier columns twice
+// per block of NB columns (n k^2 / NB doubles) and does 4 n k^2 flops; the
+// back-transformation reads each reflector once per column block and
+// updates n k values per reflector (2 n^2 k flops).
+struct VecWork {
+  double orth_bytes = 0, orth_flops = 0, bt_bytes = 0, bt_flops = 0;
+};
+
 template <int NB>
 void launch_vectors_t(Context* ctx, int nmax, int n_jobs, int n_bt, const DevJob* djd, const BtItem* bid,
-                      const double* A, const double* TAU, double* zi, const double* lu, double* Z) {
+                      const double* A, const double* TAU, double* zi, const double* lu, double* Z,
+                      const VecWork& work) {
   constexpr int kNT = 1024, NW = kNT / 64;
   const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
   allow_lds(orth_kernel<kNT, NB>, lds);
+  int tk = ctx->timer.begin(ctx->stream);
   hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(n_jobs), dim3(kNT), lds, ctx->stream, djd, zi, lu);
+  ctx->timer.end(tk, ctx->stream, "orthonormalize", work.orth_bytes, work.orth_flops, 1);
   TK_HIP(hipGetLastError());
   constexpr int kBtNT = 256;
   const size_t lds2 = (size_t(nmax) * NB + 2 * (kBtNT / 64) * NB) * sizeof(double);
   allow_lds(backtransform_kernel<kBtNT, NB>, lds2);
+  tk = ctx->timer.begin(ctx->stream);
   hipLaunchKernelGGL((backtransform_kernel<kBtNT, NB>), dim3(n_bt), dim3(kBtNT), lds2, ctx->stream, bid, djd, A, TAU,
                      zi, Z);
+  ctx->timer.end(tk, ctx->stream, "backtransform", work.bt_bytes, work.bt_flops, 1);
   TK_HIP(hipGetLastError());
 }
 
@@ -1125,7 +1253,7 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   std::vector<BisItem> items;
   constexpr int kBisNT = 256;
   for (size_t o = 0; o < jobs.size(); ++o)
-    for (int j0 = 0; j0 < jobs[o].n; j0 += kBisNT) items.push_back({int32_t(jobs.size() + o), j0});
+    for (int j0 = 0; j0 < jobs[o].n; j0 += kBisNT / kBisLanes) items.push_back({int32_t(jobs.size() + o), j0});
   const size_t jb_bytes = dj.size() * sizeof(DevJob);
   DBuf<char> meta(ctx_, jb_bytes + items.size() * sizeof(BisItem));
   TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb_bytes, hipMemcpyHostToDevice, ctx_->stream));
@@ -1143,17 +1271,28 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
     vp.reset(ctx_, size_t(4 * std::max<int64_t>(w_n, 1)));
     std::vector<int> ns(jobs.size());
     for (size_t o = 0; o < jobs.size(); ++o) ns[o] = jobs[order[o]].n;
+    // algorithmic traffic: every step reads and writes the trailing block
+    double bytes = 0;
+    int64_t launches = 0;
+    const int tk = ctx_->timer.begin(ctx_->stream);
     for (int k = 0; k + 1 < nmax_; ++k) {
       int64_t grid = 0;
       for (int n : ns) {
-        if (n == 1) grid += k == 0;
-        else if (k + 2 < n) grid += (n - k - 1 + kStepRows - 1) / kStepRows;
-        else grid += k + 2 == n;
+        if (n == 1) {
+          grid += k == 0;
+        } else if (k + 2 < n) {
+          grid += (n - k - 1 + kStepRows - 1) / kStepRows;
+          bytes += 16.0 * double(n - k - 1) * double(n - k - 1);
+        } else {
+          grid += k + 2 == n;
+        }
       }
       if (grid == 0) continue;
-      hipLaunchKernelGGL(tdstep_kernel, dim3(unsigned(grid)), dim3(kStepThreads), 0, ctx_->stream, djd, nj, k, A, D,
-                         E, TAU, vp.get(), w_n);
+      hipLaunchKernelGGL(tdstep_kernel, dim3(unsigned(grid)), dim3(kStepThreads), 3 * size_t(nmax_) * sizeof(double),
+                         ctx_->stream, djd, nj, k, A, D, E, TAU, vp.get(), w_n, nmax_);
+      ++launches;
     }
+    ctx_->timer.end(tk, ctx_->stream, "tridiag_column_step", bytes, 0.0, launches);
     std::vector<TileItem> ti;
     for (size_t o = 0; o < jobs.size(); ++o) {
       const int n = ns[o], nt = (n + 31) / 32;
@@ -1170,9 +1309,11 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   TK_HIP(hipGetLastError());
   // the TN slot of a job is indexed by its position in the device job array
   DBuf<double> tn_all(ctx_, dj.size());
+  const int tb = ctx_->timer.begin(ctx_->stream);
   hipLaunchKernelGGL((bisect_kernel<kBisNT>), dim3(items.size()), dim3(kBisNT),
                      2 * size_t(nmax_) * sizeof(double), ctx_->stream, itd, djd, D, E, W, tn_all.get(),
                      descending ? 1 : 0);
+  ctx_->timer.end(tb, ctx_->stream, "bisection", 0.0, 0.0, 1);
   TK_HIP(hipGetLastError());
   TK_HIP(hipMemcpyAsync(tn_.get(), tn_all.get() + jobs.size(), jobs.size() * sizeof(double),
                         hipMemcpyDeviceToDevice, ctx_->stream));
@@ -1220,13 +1361,26 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   const double* E = D + w_n;
   const double* TAU = E + w_n;
   const int nv = int(vitems.size());
+  const int ti = ctx_->timer.begin(ctx_->stream);
   hipLaunchKernelGGL(invit_kernel, dim3((nv + 63) / 64), dim3(64), 0, ctx_->stream, vid, nv, djd, D, E, W_,
                      tn_.get(), lu.get(), zi.get());
+  // factors written once and read three times, the vector written and read 8 times
+  ctx_->timer.end(ti, ctx_->stream, "inverse_iteration", 8.0 * double(z_tot) * (4.0 * kLuFields + 8.0), 0.0, 1);
   TK_HIP(hipGetLastError());
+  VecWork work;
+  for (const DevJob& j : dj) {
+    const double n = j.n, k = j.k;
+    work.orth_bytes += 8.0 * (2.0 * n * k * k / NBv + 2.0 * n * k);
+    work.orth_flops += 4.0 * n * k * k;
+    work.bt_bytes += 8.0 * (n * n / 2.0 * std::ceil(k / NBv) + 2.0 * n * k);
+    work.bt_flops += 2.0 * n * n * k;
+  }
   if (NBv == 8)
-    launch_vectors_t<8>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z);
+    launch_vectors_t<8>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z,
+                        work);
   else
-    launch_vectors_t<4>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z);
+    launch_vectors_t<4>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z,
+                        work);
   ctx_->sync();
 }
 

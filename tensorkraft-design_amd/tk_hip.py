@@ -62,6 +62,11 @@ class tk_dmrg_config(C.Structure):
                 ("max_iter", C.c_int32), ("precondition", C.c_int32), ("switch_rel", C.c_double)]
 
 
+class tk_kernel_timing(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double),
+                ("flops", C.c_double)]
+
+
 class tk_sweep_stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("max_truncation_error", C.c_double), ("seconds", C.c_double),
                 ("t_plan", C.c_double), ("t_eig", C.c_double), ("t_svd", C.c_double), ("t_env", C.c_double),
@@ -129,6 +134,8 @@ def lib():
         "tk_dmrg_steps": (I32, [P, I32, pI32, C.POINTER(tk_step_stats)]),
         "tk_dmrg_bond": (I32, [P, I32, I32, pI32, pI32, pI64]),
         "tk_dmrg_site": (I32, [P, I32, I64, pI64, pD]),
+        "tk_ctx_timing": (I32, [P, I32]),
+        "tk_ctx_timing_read": (I32, [P, I32, C.POINTER(tk_kernel_timing), pI32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -144,7 +151,7 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time",
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
-            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson"]
+            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_ctx_timing", "tk_ctx_timing_read"]
 
 
 def _check(st, ctx=None):
@@ -268,6 +275,20 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def timing(self, enable=True):
+        """Enable (and clear) or disable the HIP-event timing of the H_eff and
+        eigensolver kernels on this context's stream."""
+        _check(self.L.tk_ctx_timing(self.h, 1 if enable else 0), self.h)
+
+    def timing_read(self):
+        """Per kernel: launches, event milliseconds, algorithmic bytes and flops
+        recorded since timing was enabled (clears the record)."""
+        buf = (tk_kernel_timing * 64)()
+        n = C.c_int32(0)
+        _check(self.L.tk_ctx_timing_read(self.h, len(buf), buf, C.byref(n)), self.h)
+        return {buf[i].name.decode(): {"launches": buf[i].launches, "ms": buf[i].ms, "bytes": buf[i].bytes,
+                                       "flops": buf[i].flops} for i in range(n.value)}
 
     def sync(self):
         _check(self.L.tk_ctx_sync(self.h), self.h)
@@ -637,6 +658,20 @@ def synthetic_uu_bond(total=1024, rmax=6, l1max=9, width=2.6):
 def env_table(sym, bond, wq):
     """Block table of an environment (bond, w, bond), q(bra) = q(ket) - q(w)."""
     return block_table(sym, [bond, wq, bond], [DIR_IN, DIR_IN, DIR_OUT])
+
+
+def heff_problem(bond_l, bond_r, W, seed=0, sym=SYM_U1U1):
+    """Two-site H_eff*psi inputs on given bonds (same MPO site on both sides):
+    L, R and psi block entries N(0,1) from PCG64(`seed`) in FlatBlockStorage
+    order (L, then R, then psi)."""
+    _, _, _, l_tot = env_table(sym, bond_l, W.wl)
+    _, _, _, r_tot = env_table(sym, bond_r, W.wr)
+    _, _, _, p_tot = block_table(sym, [W.phys, W.phys, bond_l, bond_r], [DIR_OUT, DIR_OUT, DIR_OUT, DIR_IN])
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal(l_tot)
+    R = rng.standard_normal(r_tot)
+    x = rng.standard_normal(p_tot)
+    return {"bond_l": bond_l, "bond_r": bond_r, "W": W, "L": L, "R": R, "x": x, "sym": sym}
 
 
 def c2_problem(seed=2024, total=1024):
