@@ -184,6 +184,15 @@ typedef void (*tk_matvec_fn)(const double* x_dev, double* y_dev, void* user);
 tk_status tk_davidson(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, const double* diag_dev, double tol,
                       int32_t max_subspace, int32_t max_iter, double* x_dev, double* eigenvalue, double* residual,
                       int32_t* matvecs);
+/* LanczosSolver::lowest_eigenpair (tk-dmrg/src/eigensolver/mod.rs:57-297) over
+ * the same matvec callback: full reorthogonalisation, at most max_krylov
+ * (<= 512) Krylov vectors per pass, restart from the Ritz vector up to
+ * max_restarts times; stops when the Ritz residual estimate is below tol.
+ * x_dev: start vector in, normalised eigenvector out; *residual is the
+ * explicit |A x - eigenvalue x|; *matvecs counts every product (including the
+ * final residual). */
+tk_status tk_lanczos(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, double tol, int32_t max_krylov,
+                     int32_t max_restarts, double* x_dev, double* eigenvalue, double* residual, int32_t* matvecs);
 
 /* ---- two-site effective Hamiltonian ------------------------------------- */
 /* build_heff_two_site (tk-dmrg/src/environments/mod.rs:491-602;
@@ -226,13 +235,16 @@ typedef struct {
   const int64_t* bond_schedule; /* max bond dimension per sweep (BondDimensionSchedule, truncation/mod.rs:131-172) */
   double svd_cutoff;            /* keep sigma >= cutoff * sigma_max */
   int64_t min_bond_dim;
-  double degeneracy_tol;        /* no cut inside |s_k^2 - s_{k+1}^2| <= tol * s_max^2 */
+  double degeneracy_tol;        /* no cut inside |s_k - s_{k+1}| <= tol * s_max */
   double davidson_tol;          /* absolute residual 2-norm */
   int32_t max_subspace;
   int32_t max_iter;
   int32_t precondition;         /* diagonal preconditioner on/off */
   double switch_rel;            /* low-precision -> FP64 refinement switch */
+  int32_t eigensolver;          /* TK_EIG_DAVIDSON (DMRGConfig's default) or TK_EIG_LANCZOS */
+  int32_t max_krylov;           /* Lanczos vectors per pass (<= 512; 0: 100) */
 } tk_dmrg_config;
+enum { TK_EIG_DAVIDSON = 0, TK_EIG_LANCZOS = 1 };
 
 typedef struct {
   double energy;                /* last step's eigenvalue (sweep/mod.rs:301) */

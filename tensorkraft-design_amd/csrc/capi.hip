@@ -381,6 +381,23 @@ tk_status tk_davidson(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, c
   });
 }
 
+tk_status tk_lanczos(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, double tol, int32_t max_krylov,
+                     int32_t max_restarts, double* x, double* eigenvalue, double* residual, int32_t* matvecs) {
+  return guard(ctx, [&] {
+    if (!ctx || !matvec || !x || n <= 0) throw tk::ArgError("bad Lanczos arguments");
+    if (max_krylov > 512) throw tk::ArgError("lanczos: max_krylov above 512");
+    tk::LanczosOpts o;
+    o.tol = tol;
+    o.max_krylov = max_krylov > 0 ? max_krylov : 100;
+    o.max_restarts = max_restarts >= 0 ? max_restarts : 1000;
+    CallbackUser cb{matvec, user};
+    tk::DavidsonResult r = tk::lanczos_generic(ctx->c.get(), n, callback_matvec, &cb, false, x, o);
+    if (eigenvalue) *eigenvalue = r.eigenvalue;
+    if (residual) *residual = r.residual;
+    if (matvecs) *matvecs = r.matvecs;
+  });
+}
+
 tk_status tk_heff_plan_create(tk_ctx* ctx, int32_t prec, int32_t sym, const tk_env* left, const tk_env* right,
                               const tk_mpo_site* wi, const tk_mpo_site* wj, tk_heff_plan** out) {
   return guard(ctx, [&] {
@@ -530,6 +547,14 @@ tk_status tk_dmrg_create(tk_ctx* ctx, const tk_dmrg_config* cfg, const tk_mpo_si
     c.davidson.max_iter = cfg->max_iter > 0 ? cfg->max_iter : 1000;
     c.davidson.precondition = cfg->precondition != 0;
     c.davidson.switch_rel = cfg->switch_rel > 0 ? cfg->switch_rel : 1e-6;
+    if (cfg->eigensolver != TK_EIG_DAVIDSON && cfg->eigensolver != TK_EIG_LANCZOS)
+      throw tk::ArgError("unknown eigensolver");
+    if (cfg->max_krylov > 512) throw tk::ArgError("max_krylov above 512");
+    c.eigensolver = cfg->eigensolver;
+    c.lanczos.tol = c.davidson.tol;
+    c.lanczos.max_krylov = cfg->max_krylov > 0 ? cfg->max_krylov : 100;
+    c.lanczos.max_restarts = c.davidson.max_iter;
+    c.lanczos.switch_rel = c.davidson.switch_rel;
     auto d = std::make_unique<tk_dmrg>();
     d->ctx = ctx;
     d->e = std::make_unique<tk::DmrgEngine>(ctx->c.get(), std::move(c));

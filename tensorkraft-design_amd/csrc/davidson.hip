@@ -170,6 +170,113 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
 }
 
 namespace {
+__global__ void sqrt_to_kernel(const double* __restrict__ nrm2, double* __restrict__ out) {
+  if (threadIdx.x == 0) *out = sqrt(fmax(0.0, *nrm2));
+}
+}  // namespace
+
+DavidsonResult lanczos_generic(Context* ctx, int64_t n, MatvecFn mv, void* user, bool low_precision, double* x,
+                               const LanczosOpts& opts) {
+  DavidsonResult res;
+  if (n <= 0) throw ArgError("lanczos: empty problem");
+  const int m = int(std::max<int64_t>(1, std::min<int64_t>({int64_t(opts.max_krylov), n, 512})));
+  hipStream_t s = ctx->stream;
+  DBuf<double> Vb(ctx, size_t(m) * n), wb(ctx, n), yb(ctx, n), ab(ctx, m), bb(ctx, m), hb(ctx, m), zb(ctx, m),
+      sc(ctx, 8);
+  double* V = Vb.get();
+  double* w = wb.get();
+  double* y = yb.get();
+  double* alpha = ab.get();
+  double* beta = bb.get();
+  double* theta = sc.get();
+  double* est = sc.get() + 1;
+  double* nrm2 = sc.get() + 2;
+  bool low = low_precision;
+  auto matvec = [&](const double* in, double* out) {
+    mv(in, out, low ? 0 : 1, user);
+    ++res.matvecs;
+    if (!low) ++res.matvecs_fp64;
+  };
+  auto normalise = [&](double* v) {
+    dot_many(ctx, v, n, 1, v, n, nrm2);
+    scale_by_inv_sqrt(ctx, v, nrm2, n);
+  };
+  TK_HIP(hipMemcpyAsync(y, x, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+  normalise(y);
+  double host[3] = {0, 0, 0};
+  for (int restart = 0; restart <= opts.max_restarts; ++restart) {
+    TK_HIP(hipMemcpyAsync(V, y, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    int kept = 0;
+    bool stop = false, to_fp64 = false;
+    for (int j = 0; j < m; ++j) {
+      double* vj = V + int64_t(j) * n;
+      matvec(vj, w);
+      // alpha_j and full reorthogonalisation (two classical passes)
+      dot_many(ctx, V, n, j + 1, w, n, hb.get());
+      TK_HIP(hipMemcpyAsync(alpha + j, hb.get() + j, sizeof(double), hipMemcpyDeviceToDevice, s));
+      subtract_projection(ctx, V, n, j + 1, hb.get(), w, n);
+      dot_many(ctx, V, n, j + 1, w, n, hb.get());
+      subtract_projection(ctx, V, n, j + 1, hb.get(), w, n);
+      dot_many(ctx, w, n, 1, w, n, nrm2);
+      hipLaunchKernelGGL(sqrt_to_kernel, dim3(1), dim3(64), 0, s, nrm2, beta + j);
+      tridiag_lowest(ctx, alpha, beta, j + 1, beta + j, zb.get(), theta, est);
+      TK_HIP(hipMemcpyAsync(host, sc.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      TK_HIP(hipMemcpyAsync(host + 2, beta + j, sizeof(double), hipMemcpyDeviceToHost, s));
+      TK_HIP(hipStreamSynchronize(s));
+      TK_HIP(hipGetLastError());
+      ++res.iterations;
+      kept = j + 1;
+      res.eigenvalue = host[0];
+      res.residual = host[1];
+      const double scale = std::max(1.0, std::fabs(host[0]));
+      if (low && (host[1] < opts.switch_rel * scale || host[1] < opts.tol)) {
+        to_fp64 = true;
+        break;
+      }
+      if (!low && host[1] < opts.tol) {
+        res.converged = true;
+        stop = true;
+        break;
+      }
+      if (host[2] < 1e-15 * scale) {  // invariant subspace: T holds exact eigenvalues
+        if (low) {
+          to_fp64 = true;
+        } else {
+          res.converged = true;
+          stop = true;
+        }
+        break;
+      }
+      if (j + 1 < m) {
+        double* vn = V + int64_t(j + 1) * n;
+        TK_HIP(hipMemcpyAsync(vn, w, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+        scale_by_inv_sqrt(ctx, vn, nrm2, n);
+      }
+    }
+    // Ritz vector y = V z
+    combine(ctx, V, n, kept, zb.get(), 0.0, y, n);
+    normalise(y);
+    if (stop) break;
+    if (to_fp64) low = false;
+  }
+  // explicit FP64 residual of the returned pair
+  low = false;
+  matvec(y, w);
+  dot_many(ctx, y, n, 1, w, n, theta);
+  hipLaunchKernelGGL(residual_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w, y, theta, V, n);
+  dot_many(ctx, V, n, 1, V, n, nrm2);
+  TK_HIP(hipMemcpyAsync(host, theta, sizeof(double), hipMemcpyDeviceToHost, s));
+  TK_HIP(hipMemcpyAsync(host + 1, nrm2, sizeof(double), hipMemcpyDeviceToHost, s));
+  TK_HIP(hipMemcpyAsync(x, y, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+  TK_HIP(hipStreamSynchronize(s));
+  res.eigenvalue = host[0];
+  res.residual = std::sqrt(std::max(0.0, host[1]));
+  res.converged = res.converged && res.residual < std::max(opts.tol * 10.0, 1e-12);
+  TK_HIP(hipGetLastError());
+  return res;
+}
+
+namespace {
 struct PlanUser {
   HeffPlan* plan;
 };
@@ -179,6 +286,11 @@ void plan_matvec(const double* x, double* y, int fp64, void* user) {
   else p->apply(x, y);
 }
 }  // namespace
+
+DavidsonResult lanczos(Context* ctx, HeffPlan& plan, double* x, const LanczosOpts& opts) {
+  PlanUser u{&plan};
+  return lanczos_generic(ctx, plan.psi().total, plan_matvec, &u, plan.prec() != Prec::FP64, x, opts);
+}
 
 DavidsonResult davidson(Context* ctx, HeffPlan& plan, double* x, const DavidsonOpts& opts) {
   const int64_t n = plan.psi().total;

@@ -7,8 +7,9 @@
 // vector collapses, restart to the current Ritz vector at max_subspace.
 // Differences (stated): the start vector is the caller's (warm start from the
 // current two-site tensor), orthogonalisation is classical Gram-Schmidt applied
-// twice, and the projected problem is solved by the batched Jacobi kernel on
-// the device.
+// twice, and the lowest pair of the projected problem comes from a one-
+// wavefront kernel (Householder tridiagonalisation, multisection, inverse
+// iteration) on the device.
 //
 // Mixed precision: while the plan's precision is below FP64, new basis vectors
 // are multiplied with the low-precision kernels; once the residual stalls or
@@ -49,5 +50,31 @@ DavidsonResult davidson(Context* ctx, HeffPlan& plan, double* x, const DavidsonO
 using MatvecFn = void (*)(const double* x, double* y, int fp64, void* user);
 DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user, const double* diag_dev,
                                 bool low_precision, double* x, const DavidsonOpts& opts);
+
+// Lanczos with full reorthogonalisation for the lowest eigenpair.
+//
+// Reference: LanczosSolver::lowest_eigenpair + solve_tridiagonal +
+// count_eigenvalues_below (tk-dmrg/src/eigensolver/mod.rs:57-297): three-term
+// recurrence alpha_j = v_j.(A v_j), full reorthogonalisation against every
+// Krylov vector, beta_j = |w|, stop at beta_j < 1e-15 or at max_krylov, lowest
+// eigenvalue of T by Sturm bisection.  Differences (stated): the eigenvector of
+// T comes from inverse iteration (the reference leaves it at all ones,
+// eigensolver/mod.rs:235-247), the solver stops when the Ritz residual
+// estimate |beta_j z_j| is below tol and restarts from the Ritz vector at
+// max_krylov (the reference returns after one pass), reorthogonalisation is
+// classical Gram-Schmidt applied twice, and the reported residual is the
+// explicit |A y - theta y| of the returned pair.  Mixed precision as in the
+// Davidson solver: low-precision matvecs until the estimate falls below
+// switch_rel * |theta|, then a restart with FP64 matvecs.
+struct LanczosOpts {
+  int max_krylov = 100;
+  int max_restarts = 1000;
+  double tol = 1e-10;
+  double switch_rel = 1e-6;
+};
+
+DavidsonResult lanczos_generic(Context* ctx, int64_t n, MatvecFn mv, void* user, bool low_precision, double* x,
+                               const LanczosOpts& opts);
+DavidsonResult lanczos(Context* ctx, HeffPlan& plan, double* x, const LanczosOpts& opts);
 
 }  // namespace tk

@@ -202,7 +202,8 @@ StepStats DmrgEngine::step(int i, bool lr) {
   contract_two_site(i, psi, x.get());
   ctx_->sync();
   double t1 = now();
-  DavidsonResult dr = davidson(ctx_, plan, x.get(), cfg_.davidson);
+  DavidsonResult dr = cfg_.eigensolver == 1 ? lanczos(ctx_, plan, x.get(), cfg_.lanczos)
+                                            : davidson(ctx_, plan, x.get(), cfg_.davidson);
   double t2 = now();
   st.energy = dr.eigenvalue;
   st.iterations = dr.iterations;

@@ -36,7 +36,9 @@ struct DmrgConfig {
   int64_t min_bond_dim = 1;
   double degeneracy_tol = 1e-12;         // never cut inside a group |l_k - l_{k+1}| <= tol * l_max (l = sigma^2)
   Prec prec = Prec::FP64;
+  int eigensolver = 0;  // 0 Davidson (DMRGConfig's default, sweep/mod.rs:169-177), 1 Lanczos
   DavidsonOpts davidson;
+  LanczosOpts lanczos;
   double energy_tol = 1e-10;             // sweep convergence |dE| / max(|E|, 1)
 };
 

@@ -1005,73 +1005,23 @@ __global__ __launch_bounds__(NT) void backtransform_kernel(const BtItem* __restr
 // row i.  Tridiagonalisation in LDS, multisection on the Sturm count for the
 // smallest eigenvalue (64 points per round), inverse iteration on T by lane 0,
 // back-transformation by the stored reflectors.
-__global__ __launch_bounds__(64) void lowest_eig_kernel(const double* __restrict__ H, int ld, int m,
-                                                        double* __restrict__ c_out, double* __restrict__ theta) {
-  __shared__ double a[64][65];  // full symmetric matrix; column k below the diagonal then holds reflector k
-  __shared__ double tau[64], d[64], e[64], e2[64], vv[64], ww[64], z[64];
-  __shared__ double u0[64], u1[64], u2[64], ml[64];
-  __shared__ int sw[64];
-  const int l = threadIdx.x;
-  for (int i = 0; i < m; ++i)
-    if (l <= i) {
-      const double x = H[int64_t(i) * ld + l];
-      a[i][l] = x;
-      a[l][i] = x;
-    }
-  __syncthreads();
-  for (int k = 0; k + 2 < m; ++k) {
-    const double x = (l > k && l < m) ? a[l][k] : 0.0;
-    const double alpha = a[k + 1][k];
-    const double dk = a[k][k];
-    const double sigma = wave_sum(l >= k + 2 && l < m ? x * x : 0.0);
-    double tk = 0, beta = alpha, scale = 0;
-    if (sigma > 0) {
-      const double nrm = sqrt(alpha * alpha + sigma);
-      beta = alpha >= 0 ? -nrm : nrm;
-      tk = (beta - alpha) / beta;
-      scale = 1.0 / (alpha - beta);
-    }
-    const double vi = l == k + 1 ? 1.0 : (l >= k + 2 && l < m ? x * scale : 0.0);
-    __syncthreads();
-    if (l > k && l < m) a[l][k] = vi;
-    vv[l] = vi;
-    if (l == 0) {
-      d[k] = dk;
-      e[k] = beta;
-      tau[k] = tk;
-    }
-    __syncthreads();
-    double p = 0;
-    if (l > k && l < m)
-      for (int j = k + 1; j < m; ++j) p += a[l][j] * vv[j];
-    const double pv = wave_sum(p * vi);
-    const double wi = tk * p - 0.5 * tk * tk * pv * vi;
-    ww[l] = wi;
-    __syncthreads();
-    if (l > k && l < m)
-      for (int j = k + 1; j < m; ++j) a[l][j] -= vi * ww[j] + wi * vv[j];
-    __syncthreads();
-  }
-  if (l == 0) {
-    if (m >= 2) {
-      d[m - 2] = a[m - 2][m - 2];
-      e[m - 2] = a[m - 1][m - 2];
-      tau[m - 2] = 0;
-    }
-    d[m - 1] = a[m - 1][m - 1];
-    e[m - 1] = 0;
-    tau[m - 1] = 0;
-  }
-  __syncthreads();
-  // Gershgorin interval, then multisection for the smallest eigenvalue
+// Smallest eigenpair of the symmetric tridiagonal (d, e) of order m held in
+// LDS, by one wavefront: Gershgorin bracket, multisection on the Sturm count
+// (64 points per round), then inverse iteration on T - lambda I (LU with
+// partial pivoting, three solves) by lane 0 into z (unit max-norm).  e2 and
+// the factor arrays (u0, u1, u2, ml: m each; sw: m) are LDS scratch.
+// Returns lambda on every lane.
+__device__ double wave_tridiag_lowest(const double* d, const double* e, double* e2, int m, double* z, double* u0,
+                                      double* u1, double* u2, double* ml, int* sw) {
+  const int l = threadIdx.x & 63;
   double gl = INFINITY, gu = -INFINITY, em = 0;
-  if (l < m) {
-    const double el = l > 0 ? fabs(e[l - 1]) : 0.0, er = l + 1 < m ? fabs(e[l]) : 0.0;
-    gl = d[l] - el - er;
-    gu = d[l] + el + er;
-    if (l + 1 < m) {
-      e2[l] = e[l] * e[l];
-      em = e2[l];
+  for (int i = l; i < m; i += 64) {
+    const double el = i > 0 ? fabs(e[i - 1]) : 0.0, er = i + 1 < m ? fabs(e[i]) : 0.0;
+    gl = fmin(gl, d[i] - el - er);
+    gu = fmax(gu, d[i] + el + er);
+    if (i + 1 < m) {
+      e2[i] = e[i] * e[i];
+      em = fmax(em, e2[i]);
     }
   }
   gl = wave_min(gl);
@@ -1100,7 +1050,6 @@ __global__ __launch_bounds__(64) void lowest_eig_kernel(const double* __restrict
     hi = nhi;
   }
   const double lam = 0.5 * (lo + hi);
-  // inverse iteration on T (lane 0)
   if (l == 0) {
     if (m == 1) {
       z[0] = 1.0;
@@ -1163,6 +1112,68 @@ __global__ __launch_bounds__(64) void lowest_eig_kernel(const double* __restrict
     }
   }
   __syncthreads();
+  return lam;
+}
+
+__global__ __launch_bounds__(64) void lowest_eig_kernel(const double* __restrict__ H, int ld, int m,
+                                                        double* __restrict__ c_out, double* __restrict__ theta) {
+  __shared__ double a[64][65];  // full symmetric matrix; column k below the diagonal then holds reflector k
+  __shared__ double tau[64], d[64], e[64], e2[64], vv[64], ww[64], z[64];
+  __shared__ double u0[64], u1[64], u2[64], ml[64];
+  __shared__ int sw[64];
+  const int l = threadIdx.x;
+  for (int i = 0; i < m; ++i)
+    if (l <= i) {
+      const double x = H[int64_t(i) * ld + l];
+      a[i][l] = x;
+      a[l][i] = x;
+    }
+  __syncthreads();
+  for (int k = 0; k + 2 < m; ++k) {
+    const double x = (l > k && l < m) ? a[l][k] : 0.0;
+    const double alpha = a[k + 1][k];
+    const double dk = a[k][k];
+    const double sigma = wave_sum(l >= k + 2 && l < m ? x * x : 0.0);
+    double tk = 0, beta = alpha, scale = 0;
+    if (sigma > 0) {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0 ? -nrm : nrm;
+      tk = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    const double vi = l == k + 1 ? 1.0 : (l >= k + 2 && l < m ? x * scale : 0.0);
+    __syncthreads();
+    if (l > k && l < m) a[l][k] = vi;
+    vv[l] = vi;
+    if (l == 0) {
+      d[k] = dk;
+      e[k] = beta;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    double p = 0;
+    if (l > k && l < m)
+      for (int j = k + 1; j < m; ++j) p += a[l][j] * vv[j];
+    const double pv = wave_sum(p * vi);
+    const double wi = tk * p - 0.5 * tk * tk * pv * vi;
+    ww[l] = wi;
+    __syncthreads();
+    if (l > k && l < m)
+      for (int j = k + 1; j < m; ++j) a[l][j] -= vi * ww[j] + wi * vv[j];
+    __syncthreads();
+  }
+  if (l == 0) {
+    if (m >= 2) {
+      d[m - 2] = a[m - 2][m - 2];
+      e[m - 2] = a[m - 1][m - 2];
+      tau[m - 2] = 0;
+    }
+    d[m - 1] = a[m - 1][m - 1];
+    e[m - 1] = 0;
+    tau[m - 1] = 0;
+  }
+  __syncthreads();
+  const double lam = wave_tridiag_lowest(d, e, e2, m, z, u0, u1, u2, ml, sw);
   // back-transformation y = H_0 ... H_{m-3} z
   double y = l < m ? z[l] : 0.0;
   for (int k = m - 3; k >= 0; --k) {
@@ -1173,6 +1184,35 @@ __global__ __launch_bounds__(64) void lowest_eig_kernel(const double* __restrict
   const double nrm = sqrt(wave_sum(y * y));
   if (l < m) c_out[l] = y / nrm;
   if (l == 0) *theta = lam;
+}
+
+// Lowest eigenpair of the Lanczos tridiagonal T_m (alpha, beta), m <= kTriMax:
+// z (m) normalised, *theta, and the residual estimate |beta_next z[m-1]| of
+// the Ritz pair (beta_next: the norm of the next Lanczos vector).
+constexpr int kTriMax = 512;
+__global__ __launch_bounds__(64) void tridiag_lowest_kernel(const double* __restrict__ alpha,
+                                                            const double* __restrict__ beta, int m,
+                                                            const double* __restrict__ beta_next,
+                                                            double* __restrict__ z_out, double* __restrict__ theta,
+                                                            double* __restrict__ est) {
+  __shared__ double d[kTriMax], e[kTriMax], e2[kTriMax], z[kTriMax];
+  __shared__ double u0[kTriMax], u1[kTriMax], u2[kTriMax], ml[kTriMax];
+  __shared__ int sw[kTriMax];
+  const int l = threadIdx.x;
+  for (int i = l; i < m; i += 64) {
+    d[i] = alpha[i];
+    e[i] = i + 1 < m ? beta[i] : 0.0;
+  }
+  __syncthreads();
+  const double lam = wave_tridiag_lowest(d, e, e2, m, z, u0, u1, u2, ml, sw);
+  double s = 0;
+  for (int i = l; i < m; i += 64) s += z[i] * z[i];
+  const double nrm = sqrt(wave_sum(s));
+  for (int i = l; i < m; i += 64) z_out[i] = z[i] / nrm;
+  if (l == 0) {
+    *theta = lam;
+    if (est) *est = fabs(*beta_next * z[m - 1] / nrm);
+  }
 }
 
 template <typename K>
@@ -1411,6 +1451,14 @@ void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, doub
   }
   TK_HIP(hipGetLastError());
   ctx->sync();
+}
+
+void tridiag_lowest(Context* ctx, const double* alpha, const double* beta, int m, const double* beta_next,
+                    double* z, double* theta, double* est) {
+  if (m < 1 || m > kTriMax) throw ArgError("tridiag_lowest: order outside 1..512");
+  hipLaunchKernelGGL(tridiag_lowest_kernel, dim3(1), dim3(64), 0, ctx->stream, alpha, beta, m, beta_next, z, theta,
+                     est);
+  TK_HIP(hipGetLastError());
 }
 
 void lowest_eigpair(Context* ctx, const double* H, int ld, int m, double* c, double* theta) {

@@ -68,4 +68,11 @@ void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, doub
 // everything in LDS.  c (m) and *theta are device outputs.
 void lowest_eigpair(Context* ctx, const double* H, int ld, int m, double* c, double* theta);
 
+// Lowest eigenpair of the symmetric tridiagonal with diagonal alpha (m) and
+// off-diagonal beta (m-1), m <= 512, all on the device: z (m, unit norm),
+// *theta, and when est is not null *est = |*beta_next * z[m-1]| (the Lanczos
+// residual estimate of the Ritz pair).  One wavefront.
+void tridiag_lowest(Context* ctx, const double* alpha, const double* beta, int m, const double* beta_next,
+                    double* z, double* theta, double* est);
+
 }  // namespace tk

@@ -59,7 +59,8 @@ class tk_dmrg_config(C.Structure):
     _fields_ = [("prec", C.c_int32), ("sym", C.c_int32), ("n_sites", C.c_int32), ("n_schedule", C.c_int32),
                 ("bond_schedule", C.POINTER(C.c_int64)), ("svd_cutoff", C.c_double), ("min_bond_dim", C.c_int64),
                 ("degeneracy_tol", C.c_double), ("davidson_tol", C.c_double), ("max_subspace", C.c_int32),
-                ("max_iter", C.c_int32), ("precondition", C.c_int32), ("switch_rel", C.c_double)]
+                ("max_iter", C.c_int32), ("precondition", C.c_int32), ("switch_rel", C.c_double),
+                ("eigensolver", C.c_int32), ("max_krylov", C.c_int32)]
 
 
 class tk_kernel_timing(C.Structure):
@@ -126,6 +127,7 @@ def lib():
         "tk_svd_truncated": (I32, [P, I64, I64, P, I64, D, P, pD, P, pI64, pD]),
         "tk_eigh": (I32, [P, I64, P, pD, P]),
         "tk_davidson": (I32, [P, I64, MATVEC_FN, P, P, D, I32, I32, P, pD, pD, pI32]),
+        "tk_lanczos": (I32, [P, I64, MATVEC_FN, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_dmrg_create": (I32, [P, C.POINTER(tk_dmrg_config), C.POINTER(tk_mpo_site), C.POINTER(P)]),
         "tk_dmrg_destroy": (I32, [P]),
         "tk_dmrg_set_state": (I32, [P, C.POINTER(tk_qindex), C.POINTER(C.POINTER(D))]),
@@ -151,7 +153,8 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time",
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
-            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_ctx_timing", "tk_ctx_timing_read"]
+            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_ctx_timing",
+            "tk_ctx_timing_read"]
 
 
 def _check(st, ctx=None):
@@ -460,6 +463,18 @@ class HipBackend:
         dx.free()
         if dd:
             dd.free()
+        return ev.value, x, rn.value, nmv.value
+
+    def lanczos(self, matvec_dev, n, x0, tol=1e-10, max_krylov=100, max_restarts=1000):
+        """Lowest eigenpair by Lanczos with full reorthogonalisation (LanczosSolver)."""
+        cb = MATVEC_FN(lambda x, y, u: matvec_dev(x, y))
+        dx = self.ctx.to_device(np.asarray(x0, dtype=np.float64))
+        ev, rn = C.c_double(0), C.c_double(0)
+        nmv = C.c_int32(0)
+        _check(self.ctx.L.tk_lanczos(self.ctx.h, n, cb, None, tol, max_krylov, max_restarts, C.c_void_p(dx.ptr),
+                                     C.byref(ev), C.byref(rn), C.byref(nmv)), self.ctx.h)
+        x = dx.get()
+        dx.free()
         return ev.value, x, rn.value, nmv.value
 
 
@@ -808,7 +823,7 @@ class Dmrg:
 
     def __init__(self, ctx, mpo_sites, sym, prec="fp64", schedule=(64,), svd_cutoff=1e-10, davidson_tol=1e-10,
                  max_subspace=60, max_iter=1000, precondition=True, switch_rel=1e-6, degeneracy_tol=1e-12,
-                 min_bond_dim=1):
+                 min_bond_dim=1, eigensolver="davidson", max_krylov=100):
         self.ctx = ctx
         self.sym = sym
         self.n = len(mpo_sites)
@@ -824,7 +839,8 @@ class Dmrg:
         self._keep.append(sched)
         cfg = tk_dmrg_config(PREC[prec], sym, self.n, len(sched), sched.ctypes.data_as(C.POINTER(C.c_int64)),
                              svd_cutoff, min_bond_dim, degeneracy_tol, davidson_tol, max_subspace, max_iter,
-                             1 if precondition else 0, switch_rel)
+                             1 if precondition else 0, switch_rel, {"davidson": 0, "lanczos": 1}[eigensolver],
+                             max_krylov)
         h = C.c_void_p()
         _check(ctx.L.tk_dmrg_create(ctx.h, C.byref(cfg), arr, C.byref(h)), ctx.h)
         self.h = h

@@ -215,3 +215,53 @@ def test_davidson_random_vs_lapack(be, ctx):
     e, x, rn, nmv = be.davidson(mv, 48, rng.standard_normal(48), diag=np.diag(h).copy(), tol=1e-11)
     assert abs(e - np.linalg.eigvalsh(h)[0]) < 1e-10
     assert np.linalg.norm(h @ x - e * x) < 1e-9
+
+
+def test_lanczos_known_answers(be, ctx):
+    """LanczosSolver tests (eigensolver/mod.rs:660-700): diag(3,1,4,1,5) and
+    [[2,-1],[-1,2]] (max_krylov 10) from the reference start vector; the
+    eigenvector is checked by its residual (the reference leaves it unset)."""
+    x0 = np.array([((i * 7 + 13) % 97) / 97.0 - 0.5 for i in range(5)])
+    h = np.diag([3.0, 1, 4, 1, 5])
+    mv, keep = _dense_matvec(be, ctx, h)
+    e, x, rn, nmv = be.lanczos(mv, 5, x0)
+    assert abs(e - 1.0) < 1e-10
+    assert np.linalg.norm(h @ x - e * x) < 1e-9 and abs(np.linalg.norm(x) - 1) < 1e-12
+    h = np.array([[2.0, -1], [-1, 2]])
+    mv, keep = _dense_matvec(be, ctx, h)
+    e, x, rn, nmv = be.lanczos(mv, 2, x0[:2], max_krylov=10)
+    assert abs(e - 1.0) < 1e-10
+    assert np.linalg.norm(h @ x - e * x) < 1e-9
+
+
+def _lcg_diag(dim, seed):
+    """Random diagonal of proptest_dmrg.rs:18-30 (LCG, state = seed + 42)."""
+    state = (seed + 42) % 2 ** 64
+    out = []
+    for _ in range(dim):
+        state = (state * 6364136223846793005 + 1) % 2 ** 64
+        v = (state >> 33) & 0xFFFFFFFF
+        v = v - 2 ** 32 if v >= 2 ** 31 else v
+        out.append(v / float(1 << 30))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("dim,seed", [(3, 0), (5, 17), (8, 123), (13, 999), (20, 4242), (20, 9999)])
+def test_lanczos_random_diagonal(be, ctx, dim, seed):
+    """proptest lanczos_eigenvalue_accuracy (proptest_dmrg.rs:15-49): the
+    minimum of an LCG diagonal, from the reference start vector."""
+    d = _lcg_diag(dim, seed)
+    x0 = np.array([((i * 7 + 13) % 97) / 97.0 - 0.5 for i in range(dim)])
+    mv, keep = _dense_matvec(be, ctx, np.diag(d))
+    e, x, rn, nmv = be.lanczos(mv, dim, x0)
+    assert np.isfinite(e) and abs(e - d.min()) < 1e-9
+
+
+def test_lanczos_random_vs_lapack(be, ctx):
+    rng = np.random.default_rng(12)
+    a = rng.standard_normal((200, 200))
+    h = a + a.T
+    mv, keep = _dense_matvec(be, ctx, h)
+    e, x, rn, nmv = be.lanczos(mv, 200, rng.standard_normal(200), tol=1e-11, max_krylov=60)
+    assert abs(e - np.linalg.eigvalsh(h)[0]) < 1e-10
+    assert np.linalg.norm(h @ x - e * x) < 1e-9 and rn < 1e-9

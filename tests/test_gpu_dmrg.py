@@ -22,8 +22,9 @@ def _setup(kind, n, total, d0=16, seed=0, **mp):
     return mpo, sym, bonds, data
 
 
-def _gpu(ctx, mpo, sym, bonds, data, sched, sweeps, prec="fp64", tol=1e-12, cutoff=1e-12):
-    d = tk_hip.Dmrg(ctx, mpo, sym, prec=prec, schedule=sched, davidson_tol=tol, svd_cutoff=cutoff)
+def _gpu(ctx, mpo, sym, bonds, data, sched, sweeps, prec="fp64", tol=1e-12, cutoff=1e-12, eigensolver="davidson"):
+    d = tk_hip.Dmrg(ctx, mpo, sym, prec=prec, schedule=sched, davidson_tol=tol, svd_cutoff=cutoff,
+                    eigensolver=eigensolver)
     d.set_state(bonds, data)
     d.prepare()
     st = [d.sweep() for _ in range(sweeps)]
@@ -54,6 +55,22 @@ def test_gpu_dmrg_hubbard_matches_ed(ctx):
     mpo, sym, bonds, data = _setup("hubbard", n, (3, 3), t=1.0, U=0.0)
     d, st = _gpu(ctx, mpo, sym, bonds, data, (64,), 4)
     assert abs(st[-1]["energy"] - free_fermion_energy(n, 3, 3)) < 1e-9
+
+
+def test_gpu_dmrg_lanczos_matches_ed_and_davidson(ctx):
+    """The sweep with the Lanczos solver (eigensolver/mod.rs:57-297) reaches the
+    same ground state as with Davidson and as exact diagonalisation."""
+    print("case dmrg lanczos heisenberg n=10, hubbard n=6", flush=True)
+    n = 10
+    mpo, sym, bonds, data = _setup("heisenberg", n, (0, 0))
+    d, st = _gpu(ctx, mpo, sym, bonds, data, (32,), 4, eigensolver="lanczos")
+    e_l = st[-1]["energy"]
+    assert abs(e_l - heisenberg_ground_energy(n)) < 1e-9
+    d, st = _gpu(ctx, mpo, sym, bonds, data, (32,), 4)
+    assert abs(e_l - st[-1]["energy"]) < 1e-10
+    mpo, sym, bonds, data = _setup("hubbard", 6, (3, 3), t=1.0, U=4.0)
+    d, st = _gpu(ctx, mpo, sym, bonds, data, (64,), 4, eigensolver="lanczos")
+    assert abs(st[-1]["energy"] - hubbard_ground_energy(6, 3, 3, 1.0, 4.0)) < 1e-9
 
 
 def _oracle(mpo, sym, bonds, data, sched, sweeps, tol=1e-12, cutoff=1e-12):
