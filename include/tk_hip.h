@@ -132,6 +132,28 @@ tk_status tk_fuse_map(int32_t sym, int32_t n_legs, const tk_qindex* legs, const 
                       int64_t capacity, int32_t* n_fused, int32_t* fused_qn, int64_t* fused_dims,
                       int64_t* n_combos, int32_t* combo_fused_sector, int64_t* combo_offset);
 
+/* ---- block-sparse data movement (FP64, device data) ---------------------- */
+/* Data are in the tk_block_table layout of the given legs (every flux-allowed
+ * block, keys ascending, row-major blocks).  Synchronous. */
+/* BlockSparseTensor::permute (tk-symmetry/src/block_sparse.rs:400-431):
+ * leg k of the result is leg perm[k] of the input (QIndex and direction);
+ * keys repacked and re-sorted; out_dev in the layout of the permuted legs. */
+tk_status tk_block_permute(tk_ctx* ctx, int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs,
+                           const int32_t* flux, const int32_t* perm, const double* in_dev, int64_t in_elems,
+                           double* out_dev, int64_t out_elems);
+/* BlockSparseTensor::fuse_legs(start..end) (block_sparse.rs:444-611): out_dev
+ * in the layout of legs [0,start), the fused leg (tk_fuse_map's QIndex of legs
+ * [start,end), direction Incoming), legs [end,rank). */
+tk_status tk_block_fuse(tk_ctx* ctx, int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs,
+                        const int32_t* flux, int32_t start, int32_t end, const double* in_dev, int64_t in_elems,
+                        double* out_dev, int64_t out_elems);
+/* BlockSparseTensor::split_leg (block_sparse.rs:622-793): the inverse of
+ * tk_block_fuse for the same leg description; in_dev in the fused layout,
+ * out_dev in the layout of the original legs. */
+tk_status tk_block_split(tk_ctx* ctx, int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs,
+                         const int32_t* flux, int32_t start, int32_t end, const double* in_dev, int64_t in_elems,
+                         double* out_dev, int64_t out_elems);
+
 /* ---- LinAlgBackend / SparseLinAlgBackend (FP64, device data) ------------ */
 /* LinAlgBackend::gemm (tk-linalg/src/traits.rs:114-121; faer.rs:591-639):
  * C = alpha A B + beta C with MatRef strides (tk-core/src/matview.rs:12-33);

@@ -231,6 +231,120 @@ def dense_to_psi(sym, phys_i, phys_j, bond_l, bond_r, dense):
     return flat
 
 
+def _sorted_leg(leg):
+    return sorted(Q(leg).sectors)
+
+
+def block_permute(sym, legs, dirs, flux, perm, x):
+    """BlockSparseTensor::permute (block_sparse.rs:400-431): unpack each key,
+    permute the quantum numbers and the block, repack, re-sort (the output
+    table is the permuted legs' table)."""
+    rank = len(legs)
+    keys, offs, tot = block_table(sym, legs, dirs, flux)
+    okeys, ooffs, otot = block_table(sym, [legs[i] for i in perm], [dirs[i] for i in perm], flux)
+    where = {int(k): int(o) for k, o in zip(okeys, ooffs)}
+    dims_of = [dict(_sorted_leg(l)) for l in legs]
+    x = np.asarray(x, dtype=np.float64)
+    out = np.zeros(otot)
+    for k, off in zip(keys, offs):
+        qs = unpack_key(sym, k, rank)
+        dims = [dims_of[i][qs[i]] for i in range(rank)]
+        n = int(np.prod(dims))
+        blk = x[off:off + n].reshape(dims)
+        o = where[pack_key(sym, [qs[i] for i in perm])]
+        out[o:o + n] = blk.transpose(perm).ravel()
+    return out
+
+
+def fused_legs(sym, legs, dirs, start, end):
+    """Legs and directions after fuse_legs(start..end): the fused QIndex of the
+    fuse map with direction Incoming (block_sparse.rs:522-537); a single leg
+    is unchanged (block_sparse.rs:457-468)."""
+    if end - start == 1:
+        return list(legs), list(dirs)
+    fused, _, _ = fuse_map(sym, legs[start:end], dirs[start:end])
+    return list(legs[:start]) + [fused] + list(legs[end:]), list(dirs[:start]) + [0] + list(dirs[end:])
+
+
+def _fuse_moves(sym, legs, dirs, flux, start, end):
+    """(original offset, fused offset, outer, chunk, fused stride) per block:
+    each original block is `outer` chunks of fuse_dim * trail elements that sit
+    at offset * trail inside rows of fused_total * trail (block_sparse.rs:583-600)."""
+    rank = len(legs)
+    fused, cs, co = fuse_map(sym, legs[start:end], dirs[start:end])
+    subs = [_sorted_leg(l) for l in legs[start:end]]
+    pos = [{q: i for i, (q, _) in enumerate(sl)} for sl in subs]
+    radix = [len(sl) for sl in subs]
+    nl, nd = fused_legs(sym, legs, dirs, start, end)
+    keys, offs, tot = block_table(sym, legs, dirs, flux)
+    okeys, ooffs, otot = block_table(sym, nl, nd, flux)
+    where = {int(k): int(o) for k, o in zip(okeys, ooffs)}
+    dims_of = [dict(_sorted_leg(l)) for l in legs]
+    moves = []
+    for k, off in zip(keys, offs):
+        qs = unpack_key(sym, k, rank)
+        dims = [dims_of[i][qs[i]] for i in range(rank)]
+        c = 0
+        for l in range(start, end):
+            c = c * radix[l - start] + pos[l - start][qs[l]]
+        fq, ftot = fused[int(cs[c])]
+        o = where[pack_key(sym, list(qs[:start]) + [fq] + list(qs[end:]))]
+        outer = int(np.prod(dims[:start])) if start > 0 else 1
+        fdim = int(np.prod(dims[start:end]))
+        trail = int(np.prod(dims[end:])) if end < rank else 1
+        moves.append((int(off), o + int(co[c]) * trail, outer, fdim * trail, ftot * trail))
+    return moves, tot, otot
+
+
+def block_fuse(sym, legs, dirs, flux, start, end, x):
+    """BlockSparseTensor::fuse_legs(start..end) (block_sparse.rs:444-611)."""
+    x = np.asarray(x, dtype=np.float64)
+    if end - start == 1:
+        return x.copy()
+    moves, tot, otot = _fuse_moves(sym, legs, dirs, flux, start, end)
+    out = np.zeros(otot)
+    for src, dst, outer, chunk, fstride in moves:
+        for o in range(outer):
+            out[dst + o * fstride:dst + o * fstride + chunk] = x[src + o * chunk:src + (o + 1) * chunk]
+    return out
+
+
+def block_split(sym, legs, dirs, flux, start, end, xf):
+    """BlockSparseTensor::split_leg (block_sparse.rs:622-793): the inverse of
+    block_fuse for the same original legs."""
+    xf = np.asarray(xf, dtype=np.float64)
+    if end - start == 1:
+        return xf.copy()
+    moves, tot, otot = _fuse_moves(sym, legs, dirs, flux, start, end)
+    out = np.zeros(tot)
+    for src, dst, outer, chunk, fstride in moves:
+        for o in range(outer):
+            out[src + o * chunk:src + (o + 1) * chunk] = xf[dst + o * fstride:dst + o * fstride + chunk]
+    return out
+
+
+def to_dense(sym, legs, dirs, flux, x):
+    """Full dense array of a FlatBlockStorage tensor: sectors in QIndex order
+    along every leg (test helper)."""
+    rank = len(legs)
+    offs = []
+    for l in legs:
+        o, acc = {}, 0
+        for q, d in _sorted_leg(l):
+            o[q] = (acc, d)
+            acc += d
+        offs.append((o, acc))
+    keys, boffs, tot = block_table(sym, legs, dirs, flux)
+    dense = np.zeros(tuple(t for _, t in offs))
+    for k, off in zip(keys, boffs):
+        qs = unpack_key(sym, k, rank)
+        sl = [offs[i][0][qs[i]] for i in range(rank)]
+        shape = tuple(d for _, d in sl)
+        n = int(np.prod(shape))
+        dense[tuple(slice(o, o + d) for o, d in sl)] = np.asarray(x[off:off + n]).reshape(shape)
+    return dense
+
+
 def qindex_layout(sectors):
     """QIndex::new + iter_sectors offsets (sector_key.rs:108-158)."""
     q = Q(sectors)

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "backend.hpp"
+#include "blockops.hpp"
 #include "device.hpp"
 #include "dmrg.hpp"
 #include "heff.hpp"
@@ -253,6 +254,58 @@ tk_status tk_block_table(int32_t sym, int32_t rank, const tk_qindex* legs, const
       if (sectors)
         for (int l = 0; l < rank; ++l) sectors[b * rank + l] = t.s(b, l);
     }
+  });
+}
+
+namespace {
+struct Legs {
+  tk::Sym sym;
+  std::vector<tk::QIndex> q;
+  std::vector<tk::Dir> d;
+  tk::QN flux;
+};
+Legs parse_legs(int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs, const int32_t* flux) {
+  if (rank <= 0 || rank > 8 || !legs || !dirs || !flux) throw tk::ArgError("bad leg description");
+  Legs L;
+  L.sym = to_sym(sym);
+  for (int i = 0; i < rank; ++i) {
+    L.q.push_back(to_qindex(L.sym, legs[i]));
+    L.d.push_back(to_dir(dirs[i]));
+  }
+  L.flux = tk::QN{flux[0], flux[1]};
+  (void)tk::pack_qn(L.sym, L.flux);
+  return L;
+}
+}  // namespace
+
+tk_status tk_block_permute(tk_ctx* ctx, int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs,
+                           const int32_t* flux, const int32_t* perm, const double* in_dev, int64_t in_elems,
+                           double* out_dev, int64_t out_elems) {
+  return guard(ctx, [&] {
+    if (!ctx || !perm || !in_dev || !out_dev) throw tk::ArgError("null argument");
+    Legs L = parse_legs(sym, rank, legs, dirs, flux);
+    tk::block_permute(ctx->c.get(), L.sym, L.q, L.d, L.flux, std::vector<int>(perm, perm + rank), in_dev, in_elems,
+                      out_dev, out_elems);
+  });
+}
+
+tk_status tk_block_fuse(tk_ctx* ctx, int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs,
+                        const int32_t* flux, int32_t start, int32_t end, const double* in_dev, int64_t in_elems,
+                        double* out_dev, int64_t out_elems) {
+  return guard(ctx, [&] {
+    if (!ctx || !in_dev || !out_dev) throw tk::ArgError("null argument");
+    Legs L = parse_legs(sym, rank, legs, dirs, flux);
+    tk::block_fuse(ctx->c.get(), L.sym, L.q, L.d, L.flux, start, end, in_dev, in_elems, out_dev, out_elems);
+  });
+}
+
+tk_status tk_block_split(tk_ctx* ctx, int32_t sym, int32_t rank, const tk_qindex* legs, const int32_t* dirs,
+                         const int32_t* flux, int32_t start, int32_t end, const double* in_dev, int64_t in_elems,
+                         double* out_dev, int64_t out_elems) {
+  return guard(ctx, [&] {
+    if (!ctx || !in_dev || !out_dev) throw tk::ArgError("null argument");
+    Legs L = parse_legs(sym, rank, legs, dirs, flux);
+    tk::block_split(ctx->c.get(), L.sym, L.q, L.d, L.flux, start, end, in_dev, in_elems, out_dev, out_elems);
   });
 }
 

@@ -137,6 +137,9 @@ def lib():
         "tk_dmrg_bond": (I32, [P, I32, I32, pI32, pI32, pI64]),
         "tk_dmrg_site": (I32, [P, I32, I64, pI64, pD]),
         "tk_ctx_timing": (I32, [P, I32]),
+        "tk_block_permute": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, pI32, P, I64, P, I64]),
+        "tk_block_fuse": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, I32, I32, P, I64, P, I64]),
+        "tk_block_split": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, I32, I32, P, I64, P, I64]),
         "tk_ctx_timing_read": (I32, [P, I32, C.POINTER(tk_kernel_timing), pI32]),
     }
     for name, (res, args) in sig.items():
@@ -154,7 +157,7 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
             "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_ctx_timing",
-            "tk_ctx_timing_read"]
+            "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split"]
 
 
 def _check(st, ctx=None):
@@ -668,6 +671,55 @@ def synthetic_uu_bond(total=1024, rmax=6, l1max=9, width=2.6):
     dims = base + 1
     assert int(dims.sum()) == total
     return [(q, int(d)) for q, d in zip(qs, dims)]
+
+
+def _leg_args(legs, dirs, flux):
+    qs = [QIndexArg(l) for l in legs]
+    arr = (tk_qindex * len(qs))(*[q.c for q in qs])
+    d = np.ascontiguousarray(dirs, dtype=np.int32)
+    f = np.ascontiguousarray(_qkey(flux), dtype=np.int32)
+    return qs, arr, d, f
+
+
+def block_permute(ctx, sym, legs, dirs, flux, perm, x):
+    """BlockSparseTensor::permute on the device; x and the result are host
+    arrays in the block-table layouts of legs and of the permuted legs."""
+    qs, arr, d, f = _leg_args(legs, dirs, flux)
+    p = np.ascontiguousarray(perm, dtype=np.int32)
+    _, _, _, tot = block_table(sym, [legs[i] for i in perm], [dirs[i] for i in perm], _qkey(flux))
+    dx = ctx.to_device(np.asarray(x, dtype=np.float64))
+    dy = ctx.empty(max(1, tot))
+    _check(ctx.L.tk_block_permute(ctx.h, sym, len(legs), arr, d.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  f.ctypes.data_as(C.POINTER(C.c_int32)), p.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  C.c_void_p(dx.ptr), len(x), C.c_void_p(dy.ptr), tot), ctx.h)
+    y = dy.get()[:tot]
+    dx.free()
+    dy.free()
+    return y
+
+
+def _fuse_call(fn, ctx, sym, legs, dirs, flux, start, end, x, n_out):
+    qs, arr, d, f = _leg_args(legs, dirs, flux)
+    dx = ctx.to_device(np.asarray(x, dtype=np.float64))
+    dy = ctx.empty(max(1, n_out))
+    _check(fn(ctx.h, sym, len(legs), arr, d.ctypes.data_as(C.POINTER(C.c_int32)),
+              f.ctypes.data_as(C.POINTER(C.c_int32)), start, end, C.c_void_p(dx.ptr), len(x), C.c_void_p(dy.ptr),
+              n_out), ctx.h)
+    y = dy.get()[:n_out]
+    dx.free()
+    dy.free()
+    return y
+
+
+def block_fuse(ctx, sym, legs, dirs, flux, start, end, x):
+    """BlockSparseTensor::fuse_legs(start..end) on the device (host arrays)."""
+    return _fuse_call(ctx.L.tk_block_fuse, ctx, sym, legs, dirs, flux, start, end, x, len(x))
+
+
+def block_split(ctx, sym, legs, dirs, flux, start, end, xf):
+    """BlockSparseTensor::split_leg on the device: xf in the fused layout of
+    (legs, start, end); the result in the layout of legs."""
+    return _fuse_call(ctx.L.tk_block_split, ctx, sym, legs, dirs, flux, start, end, xf, len(xf))
 
 
 def env_table(sym, bond, wq):
