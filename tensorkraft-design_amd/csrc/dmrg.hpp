@@ -34,7 +34,7 @@ struct DmrgConfig {
   std::vector<int64_t> bond_schedule{200};  // max bond dimension per sweep (last value repeats)
   double svd_cutoff = 1e-12;             // relative: keep sigma >= cutoff * sigma_max
   int64_t min_bond_dim = 1;
-  double degeneracy_tol = 1e-12;         // never cut inside a group |l_k - l_{k+1}| <= tol * l_max (l = sigma^2)
+  double degeneracy_tol = 1e-12;         // never cut inside a group |s_k - s_{k+1}| <= tol * s_max (s = sigma)
   Prec prec = Prec::FP64;
   int eigensolver = 0;  // 0 Davidson (DMRGConfig's default, sweep/mod.rs:169-177), 1 Lanczos
   DavidsonOpts davidson;
