@@ -3,7 +3,8 @@
 // Each wavefront owns one 32x32 output tile at a time (2x2 MFMA 16x16 tiles)
 // and walks the LPT-ordered item list with a grid stride, so one launch covers
 // every sector block of a contraction stage whatever its shape.  Operands are
-// read straight into MFMA operand registers; the A operand of a term is the
+// read straight into MFMA operand registers, a chunk of 16 (FP64) or 32 (FP32)
+// inner indices at a time; the A operand of a term is the
 // coefficient-weighted sum of its sources, formed in registers (this is where
 // the MPO W_i W_j application is fused).  FP64 uses v_mfma_f64_16x16x4f64,
 // FP32 v_mfma_f32_16x16x4f32; split-BF16 lives in contract_bf16.hip.
@@ -96,7 +97,14 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
-template <typename TI, typename TO, bool TA, bool TB, bool TC>
+// Inner-dimension chunks of 4*U: lane group lk supplies k = k0 + U*lk + u at
+// MFMA step u (a permutation of the k order inside the chunk, the same for A
+// and B), so each lane reads U consecutive k of a row-major A row.  The work
+// item index is made wave-uniform so every descriptor (item, output, term,
+// source) is a scalar load, and the sources of a term are taken two at a
+// time so that the operand loads of a chunk are all in flight before the
+// first FMA.
+template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
 __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__ items, int n_items,
                                                        const GOut* __restrict__ outs,
                                                        const GTerm* __restrict__ terms,
@@ -108,7 +116,8 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int nwaves = gridDim.x * (blockDim.x >> 6);
-  for (int it = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < n_items; it += nwaves) {
+  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  for (int it = wave0; it < n_items; it += nwaves) {
     const GItem item = items[it];
     const GOut o = outs[item.out];
     const bool two_m = item.m0 + 16 < o.m;  // wave-uniform: second row tile live
@@ -116,28 +125,59 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
     acc_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
     const int r0 = item.m0 + li, r1 = r0 + 16;
     const int q0 = item.n0 + li, q1 = q0 + 16;
+    const bool in_r0 = r0 < o.m, in_r1 = r1 < o.m, in_q0 = q0 < o.n, in_q1 = q1 < o.n;
+    // one operand element of source sp at inner index kk (0 outside the block)
+    auto lda_at = [&](const TI* __restrict__ Ap, int lda, int row, bool rin, int kk, bool kin) -> TI {
+      return (kin && rin) ? (TA ? Ap[int64_t(kk) * lda + row] : Ap[int64_t(row) * lda + kk]) : TI(0);
+    };
     for (int t = o.t_begin; t < o.t_end; ++t) {
       const GTerm tm = terms[t];
       const TI* __restrict__ Bp = B + tm.b_off;
-      for (int k0 = 0; k0 < tm.k; k0 += 4) {
-        const int kk = k0 + lk;
-        TI a0 = 0, a1 = 0, b0 = 0, b1 = 0;
-        if (kk < tm.k) {
-          for (int s = tm.s_begin; s < tm.s_end; ++s) {
-            const GSrc sr = srcs[s];
-            const TI* __restrict__ Ap = A + sr.a_off;
-            const TI cf = TI(sr.coef);
-            if (r0 < o.m) a0 += cf * (TA ? Ap[int64_t(kk) * sr.lda + r0] : Ap[int64_t(r0) * sr.lda + kk]);
-            if (r1 < o.m) a1 += cf * (TA ? Ap[int64_t(kk) * sr.lda + r1] : Ap[int64_t(r1) * sr.lda + kk]);
-          }
-          if (q0 < o.n) b0 = TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0];
-          if (q1 < o.n) b1 = TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1];
+      for (int k0 = 0; k0 < tm.k; k0 += 4 * U) {
+        const int kb = k0 + U * lk;
+        TI a0[U], a1[U], b0[U], b1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kk = kb + u;
+          const bool kin = kk < tm.k;
+          b0[u] = (kin && in_q0) ? (TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0]) : TI(0);
+          b1[u] = (kin && in_q1) ? (TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1]) : TI(0);
+          a0[u] = 0;
+          a1[u] = 0;
         }
-        c00 = M::run(a0, b0, c00);
-        if (two_n) c01 = M::run(a0, b1, c01);
-        if (two_m) {
-          c10 = M::run(a1, b0, c10);
-          if (two_n) c11 = M::run(a1, b1, c11);
+        for (int s = tm.s_begin; s < tm.s_end; s += 2) {
+          const GSrc sa = srcs[s];
+          const bool two = s + 1 < tm.s_end;
+          const GSrc sb = srcs[two ? s + 1 : s];
+          const TI* __restrict__ Apa = A + sa.a_off;
+          const TI* __restrict__ Apb = A + sb.a_off;
+          const TI ca = TI(sa.coef), cb = two ? TI(sb.coef) : TI(0);
+          TI xa0[U], xa1[U], xb0[U], xb1[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int kk = kb + u;
+            const bool kin = kk < tm.k;
+            xa0[u] = lda_at(Apa, sa.lda, r0, in_r0, kk, kin);
+            xa1[u] = lda_at(Apa, sa.lda, r1, in_r1, kk, kin);
+            xb0[u] = lda_at(Apb, sb.lda, r0, in_r0 && two, kk, kin);
+            xb1[u] = lda_at(Apb, sb.lda, r1, in_r1 && two, kk, kin);
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            a0[u] += ca * xa0[u];
+            a1[u] += ca * xa1[u];
+            a0[u] += cb * xb0[u];
+            a1[u] += cb * xb1[u];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          c00 = M::run(a0[u], b0[u], c00);
+          if (two_n) c01 = M::run(a0[u], b1[u], c01);
+          if (two_m) {
+            c10 = M::run(a1[u], b0[u], c10);
+            if (two_n) c11 = M::run(a1[u], b1[u], c11);
+          }
         }
       }
     }
@@ -160,6 +200,10 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
   }
 }
 
+template <typename T> struct KUnroll;
+template <> struct KUnroll<double> { static constexpr int value = 4; };
+template <> struct KUnroll<float> { static constexpr int value = 8; };
+
 template <typename TI, typename TO>
 void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
   if (L.n_items == 0) return;
@@ -169,7 +213,7 @@ void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const 
   dim3 grid(blocks), block(64 * waves_per_block);
 #define TK_CASE(F)                                                                                     \
   case F:                                                                                              \
-    hipLaunchKernelGGL((contract_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
+    hipLaunchKernelGGL((contract_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC), KUnroll<TI>::value>), \
                        grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C, L.beta); \
     break;
   switch (L.flags & 7) {

@@ -47,7 +47,9 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __res
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
   const int nwaves = gridDim.x * (blockDim.x >> 6);
-  for (int it = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); it < n_items; it += nwaves) {
+  // wave-uniform item index: every descriptor is a scalar load
+  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  for (int it = wave0; it < n_items; it += nwaves) {
     const GItem item = items[it];
     const GOut o = outs[item.out];
     const bool two_m = item.m0 + 16 < o.m;

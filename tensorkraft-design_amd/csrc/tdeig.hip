@@ -3,14 +3,20 @@
 // Reference call sites and the method are summarised in tdeig.hpp.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
 
 #include "tdeig.hpp"
+
+#include "contract.hpp"
+#include "kernels.hpp"
 
 namespace tk {
 namespace {
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // 1b. Tridiagonalisation for large matrices: one launch per column step k,
-// batched over the jobs, each workgroup owning kStepRows rows of one job
+// batched over the jobs, each workgroup owning R rows of one job
 // (one wavefront per row) of the full symmetric matrix.  Every workgroup
 // recomputes the O(n) quantities it needs: the pending rank-2 update of step
 // k-1 (from the previous step's A22 v, double-buffered) and the reflector of
@@ -295,19 +301,22 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
 // is stored in column k below the subdiagonal by the owners of those rows and
 // moved into row k afterwards.  VP holds per job: v (two buffers) and A22 v
 // (two buffers), each indexed like the eigenvalues.
-constexpr int kStepRows = 16;
-constexpr int kStepThreads = 64 * kStepRows;
-
+// R rows (wavefronts) per workgroup; the launch picks R per column step so
+// that the step's rows spread over enough workgroups to keep every CU busy.
+template <int R>
 __device__ __forceinline__ int step_count(int n, int k) {
   if (n == 1) return k == 0 ? 1 : 0;
-  if (k + 2 < n) return (n - k - 1 + kStepRows - 1) / kStepRows;
+  if (k + 2 < n) return (n - k - 1 + R - 1) / R;
   return k + 2 == n ? 1 : 0;
 }
 
-__global__ __launch_bounds__(kStepThreads) void tdstep_kernel(const DevJob* __restrict__ jobs, int n_jobs, int k,
-                                                              double* __restrict__ A, double* __restrict__ Dg,
-                                                              double* __restrict__ Eg, double* __restrict__ TAUg,
-                                                              double* __restrict__ VP, int64_t w_n, int ldsn) {
+template <int R>
+__global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict__ jobs, int n_jobs, int k,
+                                                        double* __restrict__ A, double* __restrict__ Dg,
+                                                        double* __restrict__ Eg, double* __restrict__ TAUg,
+                                                        double* __restrict__ VP, int64_t w_n, int ldsn) {
+  constexpr int kStepThreads = 64 * R;
+  constexpr int kStepRows = R;
   constexpr int kPerThread = kSymEigMaxN / kStepThreads;  // row-k / vector entries per thread
   constexpr int kPerLane = kSymEigMaxN / 64;              // own-row entries per lane
   extern __shared__ double sm[];
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(kStepThreads) void tdstep_kernel(const DevJob* __re
   __shared__ double red[2 * kStepRows];
   int id = blockIdx.x, q = 0;
   for (; q < n_jobs; ++q) {
-    const int cnt = step_count(jobs[q].n, k);
+    const int cnt = step_count<R>(jobs[q].n, k);
     if (id < cnt) break;
     id -= cnt;
   }
@@ -931,76 +940,6 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
 }
 
 // ---------------------------------------------------------------------------
-// 3c. Back-transformation y = H_0 H_1 ... H_{n-3} z for NB columns per
-// workgroup; the result is written as columns c0.. of the n x n row-major
-// output at a_off.
-struct BtItem {
-  int32_t job, c0;
-};
-
-template <int NT, int NB>
-__global__ __launch_bounds__(NT) void backtransform_kernel(const BtItem* __restrict__ items,
-                                                           const DevJob* __restrict__ jobs,
-                                                           const double* __restrict__ A,
-                                                           const double* __restrict__ TAUg,
-                                                           const double* __restrict__ Zi, double* __restrict__ Zo) {
-  constexpr int NW = NT / 64;
-  const BtItem it = items[blockIdx.x];
-  const DevJob jb = jobs[it.job];
-  const int n = jb.n, k = jb.k, c0 = it.c0;
-  const int nb = min(NB, k - c0);
-  const double* a = A + jb.a_off;
-  const double* tau = TAUg + jb.w_off;
-  extern __shared__ double sm[];
-  double* Zb = sm;                    // n x NB
-  double* red = Zb + int64_t(n) * NB;  // 2 x NW x NB
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int e2 = tid; e2 < n * NB; e2 += NT) {
-    const int i = e2 / NB, c = e2 - i * NB;
-    Zb[e2] = c < nb ? Zi[jb.z_off + int64_t(i) * k + c0 + c] : 0.0;
-  }
-  __syncthreads();
-  int parity = 0;
-  for (int kk = n - 3; kk >= 0; --kk) {
-    const double t = tau[kk];
-    if (t == 0.0) continue;
-    const double* vr = a + int64_t(kk) * n;  // v[kk+1] = 1, v[j] = vr[j] for j >= kk+2
-    double s[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) s[c] = 0;
-    for (int i = kk + 1 + tid; i < n; i += NT) {
-      const double vi = i == kk + 1 ? 1.0 : vr[i];
-#pragma unroll
-      for (int c = 0; c < NB; ++c) s[c] += vi * Zb[i * NB + c];
-    }
-    double* r = red + parity * NW * NB;
-    parity ^= 1;
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      const double x = wave_sum(s[c]);
-      if (lane == 0) r[wv * NB + c] = x;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      double x = 0;
-      for (int q = 0; q < NW; ++q) x += r[q * NB + c];
-      s[c] = t * x;
-    }
-    for (int i = kk + 1 + tid; i < n; i += NT) {
-      const double vi = i == kk + 1 ? 1.0 : vr[i];
-#pragma unroll
-      for (int c = 0; c < NB; ++c) Zb[i * NB + c] -= s[c] * vi;
-    }
-    __syncthreads();
-  }
-  for (int e2 = tid; e2 < n * NB; e2 += NT) {
-    const int i = e2 / NB, c = e2 - i * NB;
-    if (c < nb) Zo[jb.a_off + int64_t(i) * n + c0 + c] = Zb[e2];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Lowest eigenpair of a small symmetric matrix, one wavefront.  Lane i owns
 // row i.  Tridiagonalisation in LDS, multisection on the Sturm count for the
 // smallest eigenvalue (64 points per round), inverse iteration on T by lane 0,
@@ -1235,29 +1174,197 @@ void launch_tridiag_t(Context* ctx, const DevJob* jobs, int n_jobs, int nmax, do
 // per block of NB columns (n k^2 / NB doubles) and does 4 n k^2 flops; the
 // back-transformation reads each reflector once per column block and
 // updates n k values per reflector (2 n^2 k flops).
+// ---------------------------------------------------------------------------
+// 3d. Compact-WY back-transformation.  Reflectors kk = j0 .. j0+nb-1 form
+// H_j0 ... H_(j0+nb-1) = I - V T V^T (V: n x nb, v_kk[kk+1] = 1, zero above;
+// T upper triangular, T[l][l] = tau_l, T[l][i] = -tau_i sum_m T[l][m] S[m][i]
+// with S = V^T V).  Rows of A become the rows of V^T in place.
+constexpr int kWyNb = 128;   // reflectors per block
+constexpr int kWySeg = 256;  // rows per split-K segment of V^T Y
+
+// Row kk of every job: zero columns <= kk, 1 at kk+1 (rows n-2, n-1: zero).
+__global__ __launch_bounds__(256) void wy_rows_kernel(const DevJob* __restrict__ jobs, double* __restrict__ A) {
+  const DevJob jb = jobs[blockIdx.y];
+  const int n = jb.n;
+  double* a = A + jb.a_off;
+  for (int kk = blockIdx.x; kk < n; kk += gridDim.x) {
+    double* row = a + int64_t(kk) * n;
+    const int lim = kk + 2 < n ? kk + 1 : n;  // columns [0, lim) cleared
+    for (int j = threadIdx.x; j < lim; j += blockDim.x) row[j] = 0.0;
+    if (threadIdx.x == 0 && kk + 2 < n) row[kk + 1] = 1.0;
+    if (kk + 2 >= n)
+      for (int j = lim + threadIdx.x; j < n; j += blockDim.x) row[j] = 0.0;
+  }
+}
+
+struct WyBlock {
+  int64_t t_off;    // S and T (kWyNb x kWyNb, ld kWyNb) of this block
+  int64_t tau_off;  // tau of reflector j0
+  int32_t nb, pad;
+};
+
+// One thread per row l of T: row l depends only on its own earlier entries.
+__global__ __launch_bounds__(kWyNb) void wy_t_kernel(const WyBlock* __restrict__ blocks,
+                                                     const double* __restrict__ S, const double* __restrict__ TAUg,
+                                                     double* __restrict__ T) {
+  const WyBlock b = blocks[blockIdx.x];
+  const double* s = S + b.t_off;
+  double* t = T + b.t_off;
+  const double* tau = TAUg + b.tau_off;
+  const int l = threadIdx.x, nb = b.nb;
+  double* tl = t + int64_t(l) * kWyNb;
+  for (int m = 0; m < kWyNb; ++m)
+    if (l >= nb || m < l || m >= nb) tl[m] = 0.0;
+  if (l >= nb) return;
+  tl[l] = tau[l];
+  for (int i = l + 1; i < nb; ++i) {
+    double acc = 0;
+    for (int m = l; m < i; ++m) acc += tl[m] * s[int64_t(m) * kWyNb + i];
+    tl[i] = -tau[i] * acc;
+  }
+}
+
 struct VecWork {
   double orth_bytes = 0, orth_flops = 0, bt_bytes = 0, bt_flops = 0;
 };
 
 template <int NB>
-void launch_vectors_t(Context* ctx, int nmax, int n_jobs, int n_bt, const DevJob* djd, const BtItem* bid,
-                      const double* A, const double* TAU, double* zi, const double* lu, double* Z,
-                      const VecWork& work) {
+void launch_orth_t(Context* ctx, int nmax, int n_jobs, const DevJob* djd, double* zi, const double* lu,
+                   const VecWork& work) {
   constexpr int kNT = 1024, NW = kNT / 64;
   const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
   allow_lds(orth_kernel<kNT, NB>, lds);
-  int tk = ctx->timer.begin(ctx->stream);
+  const int tk = ctx->timer.begin(ctx->stream);
   hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(n_jobs), dim3(kNT), lds, ctx->stream, djd, zi, lu);
   ctx->timer.end(tk, ctx->stream, "orthonormalize", work.orth_bytes, work.orth_flops, 1);
   TK_HIP(hipGetLastError());
-  constexpr int kBtNT = 256;
-  const size_t lds2 = (size_t(nmax) * NB + 2 * (kBtNT / 64) * NB) * sizeof(double);
-  allow_lds(backtransform_kernel<kBtNT, NB>, lds2);
-  tk = ctx->timer.begin(ctx->stream);
-  hipLaunchKernelGGL((backtransform_kernel<kBtNT, NB>), dim3(n_bt), dim3(kBtNT), lds2, ctx->stream, bid, djd, A, TAU,
-                     zi, Z);
-  ctx->timer.end(tk, ctx->stream, "backtransform", work.bt_bytes, work.bt_flops, 1);
+}
+
+// Launches one list; the device and host copies of its tables are kept in
+// `keep` until the caller has synchronised.
+void run_list(Context* ctx, ContractList& cl, int64_t a_n, int64_t b_n, int64_t c_n, const double* A, const double* B,
+              double* C, std::vector<std::unique_ptr<DeviceContractList>>& keep) {
+  if (cl.outs.empty()) return;
+  cl.finalize();
+  cl.validate(a_n, b_n, c_n);
+  keep.push_back(std::make_unique<DeviceContractList>());
+  keep.back()->upload(ctx, cl);
+  launch_contract(ctx, *keep.back(), A, B, C);
+}
+
+// Z (output, ld n, the first k columns at a_off) = Q * zi, with Q the product
+// of the reflectors stored in the rows of A; blocks of kWyNb reflectors are
+// applied last block first, each as W = V^T Y (split over row segments) and
+// Y -= V T W (one contraction summing the segments).
+void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd, double* A, int64_t a_n,
+                      const double* TAU, const double* zi, int64_t z_tot, double* Z, int64_t z_n) {
+  const int nj = int(dj.size());
+  int nmax = 0;
+  for (const DevJob& j : dj) nmax = std::max(nmax, j.n);
+  // Y = zi (ld k) into Z (ld n)
+  std::vector<CopyJob> cp;
+  for (const DevJob& j : dj) cp.push_back({j.z_off, j.a_off, j.n, j.k, j.k, j.n, 0, 0});
+  launch_copies(ctx, cp, zi, z_tot, Z, z_n);
+  if (nmax < 3) return;
+  hipLaunchKernelGGL(wy_rows_kernel, dim3(unsigned(std::min(nmax, 1024)), unsigned(nj)), dim3(256), 0, ctx->stream,
+                     djd, A);
   TK_HIP(hipGetLastError());
+  // blocks of every job
+  struct HB {
+    int job, j0, nb;
+  };
+  std::vector<HB> hb;
+  std::vector<WyBlock> wb;
+  std::vector<std::vector<int>> blocks_of(nj);
+  for (int q = 0; q < nj; ++q) {
+    const int n = dj[q].n;
+    for (int j0 = 0; j0 + 2 < n; j0 += kWyNb) {
+      const int nb = std::min(kWyNb, n - 2 - j0);
+      blocks_of[q].push_back(int(hb.size()));
+      wb.push_back({int64_t(hb.size()) * kWyNb * kWyNb, dj[q].w_off + j0, nb, 0});
+      hb.push_back({q, j0, nb});
+    }
+  }
+  if (hb.empty()) return;
+  std::vector<std::unique_ptr<DeviceContractList>> keep;
+  const int64_t st_n = int64_t(hb.size()) * kWyNb * kWyNb;
+  DBuf<double> S(ctx, size_t(st_n)), T(ctx, size_t(st_n)), Ut(ctx, size_t(a_n));
+  DBuf<WyBlock> wbd(ctx, wb.size());
+  wbd.upload(wb.data(), wb.size());
+  {  // S = V^T V per block
+    ContractList cl;
+    cl.flags = kTransB;
+    for (size_t b = 0; b < hb.size(); ++b) {
+      const DevJob& j = dj[hb[b].job];
+      const int j0 = hb[b].j0, nb = hb[b].nb;
+      const int64_t v = j.a_off + int64_t(j0) * j.n + (j0 + 1);
+      cl.begin_out(wb[b].t_off, nb, nb, kWyNb);
+      cl.begin_term(v, j.n, j.n - j0 - 1);
+      cl.add_src(v, j.n, 1.0);
+    }
+    run_list(ctx, cl, a_n, a_n, st_n, A, A, S.get(), keep);
+  }
+  hipLaunchKernelGGL(wy_t_kernel, dim3(unsigned(hb.size())), dim3(kWyNb), 0, ctx->stream, wbd.get(), S.get(), TAU,
+                     T.get());
+  TK_HIP(hipGetLastError());
+  {  // U^T = T^T V^T per block, stored in the rows of Ut like V^T in A
+    ContractList cl;
+    cl.flags = kTransA;
+    for (size_t b = 0; b < hb.size(); ++b) {
+      const DevJob& j = dj[hb[b].job];
+      const int j0 = hb[b].j0, nb = hb[b].nb;
+      const int64_t v = j.a_off + int64_t(j0) * j.n + (j0 + 1);
+      cl.begin_out(v, nb, j.n - j0 - 1, j.n);
+      cl.begin_term(v, j.n, nb);
+      cl.add_src(wb[b].t_off, kWyNb, 1.0);
+    }
+    run_list(ctx, cl, st_n, a_n, a_n, T.get(), A, Ut.get(), keep);
+  }
+  // application, last block of every job first
+  size_t steps = 0;
+  int64_t w_max = 0;
+  for (int q = 0; q < nj; ++q) steps = std::max(steps, blocks_of[q].size());
+  for (size_t st = 0; st < steps; ++st) {
+    int64_t w = 0;
+    for (int q = 0; q < nj; ++q) {
+      const size_t nbq = blocks_of[q].size();
+      if (st >= nbq) continue;
+      const HB& h = hb[blocks_of[q][nbq - 1 - st]];
+      const int rows = dj[q].n - h.j0 - 1;
+      w += int64_t((rows + kWySeg - 1) / kWySeg) * h.nb * dj[q].k;
+    }
+    w_max = std::max(w_max, w);
+  }
+  DBuf<double> Wb(ctx, size_t(std::max<int64_t>(w_max, 1)));
+  for (size_t st = 0; st < steps; ++st) {
+    ContractList lw, lu;
+    lw.flags = 0;
+    lu.flags = kTransA | kTransC;
+    lu.beta = 1.0;
+    int64_t w = 0;
+    for (int q = 0; q < nj; ++q) {
+      const size_t nbq = blocks_of[q].size();
+      if (st >= nbq) continue;
+      const DevJob& j = dj[q];
+      const HB& h = hb[blocks_of[q][nbq - 1 - st]];
+      const int n = j.n, k = j.k, j0 = h.j0, nb = h.nb;
+      const int64_t vrow = j.a_off + int64_t(j0) * n;
+      // Y^T (rows j0+1..n-1 of Y) -= (sum_seg W_seg)^T U^T
+      lu.begin_out(j.a_off + int64_t(j0 + 1) * n, k, n - j0 - 1, n);
+      lu.begin_term(vrow + j0 + 1, n, nb);
+      for (int r0 = j0 + 1; r0 < n; r0 += kWySeg) {
+        const int r1 = std::min(n, r0 + kWySeg);
+        lw.begin_out(w, nb, k, k);
+        lw.begin_term(j.a_off + int64_t(r0) * n, n, r1 - r0);
+        lw.add_src(vrow + r0, n, 1.0);
+        lu.add_src(w, k, -1.0);
+        w += int64_t(nb) * k;
+      }
+    }
+    run_list(ctx, lw, a_n, z_n, w_max, A, Z, Wb.get(), keep);
+    run_list(ctx, lu, w_max, a_n, z_n, Wb.get(), Ut.get(), Z, keep);
+  }
+  ctx->sync();  // the lists' device tables are released on return
 }
 
 }  // namespace
@@ -1267,6 +1374,7 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   jobs_ = jobs;
   A_ = A;
   W_ = W;
+  a_n_ = a_n;
   nmax_ = 0;
   for (const SymJob& j : jobs) {
     if (j.n < 1 || j.n > kSymEigMaxN) throw ArgError("eigensolver: matrix dimension outside 1..2048");
@@ -1314,22 +1422,43 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
     // algorithmic traffic: every step reads and writes the trailing block
     double bytes = 0;
     int64_t launches = 0;
+    static const int rows_override = [] {
+      const char* e = std::getenv("TK_TRIDIAG_ROWS");
+      return e ? std::atoi(e) : 0;
+    }();
     const int tk = ctx_->timer.begin(ctx_->stream);
     for (int k = 0; k + 1 < nmax_; ++k) {
+      int64_t rows = 0;
+      for (int n : ns)
+        if (k + 2 < n) {
+          rows += n - k - 1;
+          bytes += 16.0 * double(n - k - 1) * double(n - k - 1);
+        }
+      // rows per workgroup: the largest of 16 / 8 / 4 that still gives >= 4
+      // workgroups per CU (TK_TRIDIAG_ROWS overrides, for tuning)
+      int r = rows >= int64_t(64) * ctx_->n_cu ? 16 : rows >= int64_t(32) * ctx_->n_cu ? 8 : 4;
+      if (rows_override == 4 || rows_override == 8 || rows_override == 16) r = rows_override;
       int64_t grid = 0;
       for (int n : ns) {
         if (n == 1) {
           grid += k == 0;
         } else if (k + 2 < n) {
-          grid += (n - k - 1 + kStepRows - 1) / kStepRows;
-          bytes += 16.0 * double(n - k - 1) * double(n - k - 1);
+          grid += (n - k - 1 + r - 1) / r;
         } else {
           grid += k + 2 == n;
         }
       }
       if (grid == 0) continue;
-      hipLaunchKernelGGL(tdstep_kernel, dim3(unsigned(grid)), dim3(kStepThreads), 3 * size_t(nmax_) * sizeof(double),
-                         ctx_->stream, djd, nj, k, A, D, E, TAU, vp.get(), w_n, nmax_);
+      const size_t lds = 3 * size_t(nmax_) * sizeof(double);
+      if (r == 16)
+        hipLaunchKernelGGL(tdstep_kernel<16>, dim3(unsigned(grid)), dim3(64 * 16), lds, ctx_->stream, djd, nj, k, A,
+                           D, E, TAU, vp.get(), w_n, nmax_);
+      else if (r == 8)
+        hipLaunchKernelGGL(tdstep_kernel<8>, dim3(unsigned(grid)), dim3(64 * 8), lds, ctx_->stream, djd, nj, k, A,
+                           D, E, TAU, vp.get(), w_n, nmax_);
+      else
+        hipLaunchKernelGGL(tdstep_kernel<4>, dim3(unsigned(grid)), dim3(64 * 4), lds, ctx_->stream, djd, nj, k, A,
+                           D, E, TAU, vp.get(), w_n, nmax_);
       ++launches;
     }
     ctx_->timer.end(tk, ctx_->stream, "tridiag_column_step", bytes, 0.0, launches);
@@ -1381,19 +1510,13 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   }
   if (dj.empty()) return;
   const int NBv = nmax_ <= 1024 ? 8 : 4;
-  std::vector<BtItem> bitems;
-  for (size_t q = 0; q < dj.size(); ++q)
-    for (int c0 = 0; c0 < dj[q].k; c0 += NBv) bitems.push_back({int32_t(q), c0});
   const size_t jb_bytes = dj.size() * sizeof(DevJob);
   const size_t vi_bytes = vitems.size() * sizeof(VecItem);
-  DBuf<char> meta(ctx_, jb_bytes + vi_bytes + bitems.size() * sizeof(BtItem));
+  DBuf<char> meta(ctx_, jb_bytes + vi_bytes);
   TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb_bytes, hipMemcpyHostToDevice, ctx_->stream));
   TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes, vitems.data(), vi_bytes, hipMemcpyHostToDevice, ctx_->stream));
-  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes + vi_bytes, bitems.data(), bitems.size() * sizeof(BtItem),
-                        hipMemcpyHostToDevice, ctx_->stream));
   const DevJob* djd = reinterpret_cast<const DevJob*>(meta.get());
   const VecItem* vid = reinterpret_cast<const VecItem*>(meta.get() + jb_bytes);
-  const BtItem* bid = reinterpret_cast<const BtItem*>(meta.get() + jb_bytes + vi_bytes);
   // per-job Gershgorin bounds indexed by the caller's job number
   DBuf<double> zi(ctx_, size_t(z_tot)), lu(ctx_, size_t(lu_tot));
   const double* D = de_.get();
@@ -1412,15 +1535,16 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
     const double n = j.n, k = j.k;
     work.orth_bytes += 8.0 * (2.0 * n * k * k / NBv + 2.0 * n * k);
     work.orth_flops += 4.0 * n * k * k;
-    work.bt_bytes += 8.0 * (n * n / 2.0 * std::ceil(k / NBv) + 2.0 * n * k);
+    work.bt_bytes += 8.0 * (n * n + 2.0 * k * n * n / kWyNb + 2.0 * n * k);
     work.bt_flops += 2.0 * n * n * k;
   }
   if (NBv == 8)
-    launch_vectors_t<8>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z,
-                        work);
+    launch_orth_t<8>(ctx_, nmax_, int(dj.size()), djd, zi.get(), lu.get(), work);
   else
-    launch_vectors_t<4>(ctx_, nmax_, int(dj.size()), int(bitems.size()), djd, bid, A_, TAU, zi.get(), lu.get(), Z,
-                        work);
+    launch_orth_t<4>(ctx_, nmax_, int(dj.size()), djd, zi.get(), lu.get(), work);
+  const int tb = ctx_->timer.begin(ctx_->stream);
+  wy_backtransform(ctx_, dj, djd, A_, a_n_, TAU, zi.get(), z_tot, Z, z_n);
+  ctx_->timer.end(tb, ctx_->stream, "backtransform", work.bt_bytes, work.bt_flops, 1);
   ctx_->sync();
 }
 

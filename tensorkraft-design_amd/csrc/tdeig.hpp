@@ -10,7 +10,9 @@
 //   3. For the k wanted eigenvalues only: inverse iteration on T (LU with
 //      partial pivoting, one thread per vector), blocked classical
 //      Gram-Schmidt applied twice in eigenvalue order (orthonormal to working
-//      precision even inside clusters), then back-transformation by Q.
+//      precision even inside clusters), then back-transformation by Q with
+//      the reflectors grouped 128 at a time in compact WY form
+//      (I - V T V^T), applied as grouped MFMA contractions.
 //
 // Reference call sites: the Gram eigen-decomposition behind the block-diagonal
 // truncated SVD (tk-dmrg/src/truncation/mod.rs:56-127 -> faer thin_svd,
@@ -49,6 +51,7 @@ class SymEigBatch {
   std::vector<SymJob> jobs_;
   double* A_ = nullptr;
   double* W_ = nullptr;
+  int64_t a_n_ = 0;
   int nmax_ = 0;
   DBuf<double> de_;   // per job: d (n), e (n), tau (n) at 3 * w_off
   DBuf<double> tn_;   // per job: Gershgorin bound of |T|
