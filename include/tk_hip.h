@@ -305,6 +305,65 @@ tk_status tk_dmrg_steps(tk_dmrg* d, int32_t capacity, int32_t* n, tk_step_stats*
 tk_status tk_dmrg_bond(tk_dmrg* d, int32_t bond, int32_t capacity, int32_t* n_sectors, int32_t* qn, int64_t* dims);
 tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, double* data);
 
+/* ---- DMFT impurity pieces (tk-dmft) ---------------------------------------
+ * BathParameters::hybridization_function (tk-dmft/src/impurity/bath.rs:61-83),
+ * batched over problems: out[(p * n_z + j) * 2 + {0,1}] = Re, Im Delta_p(z_j)
+ * with z_j = z_re[j] + i z_im[j] (the reference's omega + i*broadening, or
+ * i*omega_n on the Matsubara axis).  params (n_problems x 2 n_bath):
+ * eps[0..n_bath), v[0..n_bath).  geometry TK_BATH_STAR: sum_k v_k^2/(z - eps_k);
+ * TK_BATH_CHAIN: v_0^2/(z - eps_0 - v_1^2/(z - eps_1 - ...)), the impurity
+ * coupled to a tight-binding chain (v_k: hopping of chain sites k-1, k).
+ * Device pointers; synchronous. */
+#define TK_BATH_STAR 0
+#define TK_BATH_CHAIN 1
+tk_status tk_hybridization(tk_ctx* ctx, int32_t geometry, int32_t n_problems, int32_t n_bath,
+                           const double* params_dev, int32_t n_z, const double* z_re_dev, const double* z_im_dev,
+                           double* out_dev);
+/* No reference counterpart (the DMFT loop's least-squares bath update, used
+ * when no learned proposal is taken): per problem, star parameters minimising
+ * sum_j |Delta(i wn_j) - delta_j|^2 by Levenberg-Marquardt.  params_dev holds
+ * the start on entry and the fit on exit; cost_host (n_problems) the final
+ * cost.  n_z <= 256, n_bath <= 8.  Device pointers except cost_host. */
+tk_status tk_bath_fit(tk_ctx* ctx, int32_t n_problems, int32_t n_bath, int32_t n_z, const double* wn_dev,
+                      const double* delta_dev, double* params_dev, int32_t max_iter, double* cost_host);
+/* lanczos_tridiagonalize (tk-dmft/src/impurity/discretize.rs:54-274), the
+ * recursion on weighted points (x_j, w_j >= 0): eps[k] = alpha_k, v[0] =
+ * sqrt(sum w), v[k] = beta_k; the reference's grid form passes x = omega,
+ * w = -Im Delta(omega) d_omega / pi.  Host arrays. */
+tk_status tk_lanczos_tridiagonalize(int32_t n, const double* x, const double* w, int32_t n_bath, double* eps,
+                                    double* v);
+/* BroydenState (tk-dmft/src/loop/mixing.rs:40-150): update(x, f, alpha). */
+typedef struct tk_broyden tk_broyden;
+tk_status tk_broyden_create(int32_t history_depth, tk_broyden** out);
+tk_status tk_broyden_update(tk_broyden* b, int32_t dim, const double* x, const double* f, double alpha, double* out,
+                            int32_t* n_updates);
+tk_status tk_broyden_destroy(tk_broyden* b);
+/* Impurity Green's function as a continued fraction (no reference
+ * counterpart; the reference's G(t) route is TDVP): Lanczos with full
+ * reorthogonalisation on the n x n CSR matrix (rowptr n+1, col, val; device)
+ * from v0 (device).  beta[0] = |v0|, alpha[j], beta[j+1] (host, m each);
+ * *steps = number of alphas. */
+tk_status tk_krylov_tridiag(tk_ctx* ctx, int64_t n, const int64_t* rowptr_dev, const int32_t* col_dev,
+                            const double* val_dev, const double* v0_dev, int32_t m, double* alpha, double* beta,
+                            int32_t* steps);
+/* No reference counterpart (the learned bath proposal): FP32 multilayer
+ * perceptron dims[0] -> ... -> dims[n_layers], tanh hidden layers, linear
+ * output, Glorot-uniform weights from `seed`.  Parameters are [W_1 (row-major
+ * d_0 x d_1), b_1, W_2, b_2, ...]. */
+typedef struct tk_mlp tk_mlp;
+tk_status tk_mlp_create(tk_ctx* ctx, int32_t n_layers, const int32_t* dims, uint64_t seed, tk_mlp** out);
+tk_status tk_mlp_destroy(tk_mlp* m);
+tk_status tk_mlp_n_params(tk_mlp* m, int64_t* n);
+tk_status tk_mlp_get_params(tk_mlp* m, float* host);
+tk_status tk_mlp_set_params(tk_mlp* m, const float* host);
+/* y (batch x dims[last]) = net(x (batch x dims[0])); device pointers. */
+tk_status tk_mlp_forward(tk_mlp* m, int32_t batch, const float* x_dev, float* y_dev);
+/* Mean squared error against t (device) and its gradient wrt every
+ * parameter (grad_host may be null); the gradient is kept for tk_mlp_adam. */
+tk_status tk_mlp_loss_grad(tk_mlp* m, int32_t batch, const float* x_dev, const float* t_dev, float* loss,
+                           float* grad_host);
+tk_status tk_mlp_adam(tk_mlp* m, float lr, float beta1, float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 #include "backend.hpp"
 #include "blockops.hpp"
 #include "device.hpp"
+#include "dmft.hpp"
 #include "dmrg.hpp"
 #include "heff.hpp"
 #include "sym.hpp"
@@ -22,6 +23,13 @@ struct tk_heff_plan {
 struct tk_dmrg {
   tk_ctx* ctx;
   std::unique_ptr<tk::DmrgEngine> e;
+};
+struct tk_broyden {
+  tk::BroydenMixer b;
+};
+struct tk_mlp {
+  tk_ctx* ctx;
+  std::unique_ptr<tk::Mlp> m;
 };
 
 namespace {
@@ -54,6 +62,10 @@ tk_status guard(tk_ctx* ctx, F&& f) {
     set_error(ctx, e.what());
     return TK_ERR_INTERNAL;
   }
+}
+
+void need_ctx(tk_ctx* ctx) {
+  if (!ctx || !ctx->c) throw tk::ArgError("null context");
 }
 
 tk::Sym to_sym(int32_t s) {
@@ -717,6 +729,125 @@ tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, d
     *n = int64_t(h.size());
     if (int64_t(h.size()) > capacity) throw tk::ArgError("capacity");
     if (data) std::copy(h.begin(), h.end(), data);
+  });
+}
+
+
+// ---- DMFT impurity pieces ----
+tk_status tk_hybridization(tk_ctx* ctx, int32_t geometry, int32_t n_problems, int32_t n_bath,
+                           const double* params_dev, int32_t n_z, const double* z_re_dev, const double* z_im_dev,
+                           double* out_dev) {
+  return guard(ctx, [&] {
+    need_ctx(ctx);
+    if (geometry != TK_BATH_STAR && geometry != TK_BATH_CHAIN) throw tk::ArgError("unknown bath geometry");
+    if (int64_t(n_problems) * n_z > 0 && (!params_dev || !z_re_dev || !z_im_dev || !out_dev))
+      throw tk::ArgError("hybridization: null pointer");
+    tk::hybridization(ctx->c.get(), tk::BathGeometry(geometry), n_problems, n_bath, params_dev, n_z, z_re_dev,
+                      z_im_dev, out_dev);
+  });
+}
+
+tk_status tk_bath_fit(tk_ctx* ctx, int32_t n_problems, int32_t n_bath, int32_t n_z, const double* wn_dev,
+                      const double* delta_dev, double* params_dev, int32_t max_iter, double* cost_host) {
+  return guard(ctx, [&] {
+    need_ctx(ctx);
+    if (n_problems < 0 || max_iter < 0) throw tk::ArgError("bath fit: negative count");
+    if (n_problems > 0 && (!wn_dev || !delta_dev || !params_dev || !cost_host))
+      throw tk::ArgError("bath fit: null pointer");
+    tk::bath_fit(ctx->c.get(), n_problems, n_bath, n_z, wn_dev, delta_dev, params_dev, max_iter, cost_host);
+  });
+}
+
+tk_status tk_lanczos_tridiagonalize(int32_t n, const double* x, const double* w, int32_t n_bath, double* eps,
+                                    double* v) {
+  return guard(nullptr, [&] {
+    if (n < 0 || n_bath < 0) throw tk::ArgError("discretise: negative size");
+    if ((n > 0 && (!x || !w)) || (n_bath > 0 && (!eps || !v))) throw tk::ArgError("discretise: null pointer");
+    for (int j = 0; j < n; ++j)
+      if (w[j] < 0) throw tk::ArgError("discretise: negative spectral weight");
+    tk::lanczos_tridiagonalize(n, x, w, n_bath, eps, v);
+  });
+}
+
+tk_status tk_broyden_create(int32_t history_depth, tk_broyden** out) {
+  return guard(nullptr, [&] {
+    if (!out || history_depth < 0) throw tk::ArgError("broyden: bad arguments");
+    *out = new tk_broyden{tk::BroydenMixer(history_depth)};
+  });
+}
+tk_status tk_broyden_update(tk_broyden* b, int32_t dim, const double* x, const double* f, double alpha, double* out,
+                            int32_t* n_updates) {
+  return guard(nullptr, [&] {
+    if (!b || dim < 0 || (dim > 0 && (!x || !f || !out))) throw tk::ArgError("broyden: bad arguments");
+    const std::vector<double> r = b->b.update(std::vector<double>(x, x + dim), std::vector<double>(f, f + dim), alpha);
+    std::copy(r.begin(), r.end(), out);
+    if (n_updates) *n_updates = b->b.n_updates();
+  });
+}
+tk_status tk_broyden_destroy(tk_broyden* b) {
+  return guard(nullptr, [&] { delete b; });
+}
+
+tk_status tk_krylov_tridiag(tk_ctx* ctx, int64_t n, const int64_t* rowptr_dev, const int32_t* col_dev,
+                            const double* val_dev, const double* v0_dev, int32_t m, double* alpha, double* beta,
+                            int32_t* steps) {
+  return guard(ctx, [&] {
+    need_ctx(ctx);
+    if (!rowptr_dev || !col_dev || !val_dev || !v0_dev || !alpha || !beta || !steps)
+      throw tk::ArgError("krylov: null pointer");
+    for (int j = 0; j < m; ++j) alpha[j] = beta[j] = 0.0;
+    *steps = tk::krylov_tridiag(ctx->c.get(), n, rowptr_dev, col_dev, val_dev, v0_dev, m, alpha, beta);
+  });
+}
+
+tk_status tk_mlp_create(tk_ctx* ctx, int32_t n_layers, const int32_t* dims, uint64_t seed, tk_mlp** out) {
+  return guard(ctx, [&] {
+    need_ctx(ctx);
+    if (!out || !dims || n_layers < 1) throw tk::ArgError("mlp: bad arguments");
+    auto m = std::make_unique<tk_mlp>();
+    m->ctx = ctx;
+    m->m = std::make_unique<tk::Mlp>(ctx->c.get(), std::vector<int>(dims, dims + n_layers + 1), seed);
+    *out = m.release();
+  });
+}
+tk_status tk_mlp_destroy(tk_mlp* m) {
+  return guard(nullptr, [&] { delete m; });
+}
+tk_status tk_mlp_n_params(tk_mlp* m, int64_t* n) {
+  return guard(m ? m->ctx : nullptr, [&] {
+    if (!m || !n) throw tk::ArgError("mlp: null pointer");
+    *n = m->m->n_params();
+  });
+}
+tk_status tk_mlp_get_params(tk_mlp* m, float* host) {
+  return guard(m ? m->ctx : nullptr, [&] {
+    if (!m || !host) throw tk::ArgError("mlp: null pointer");
+    m->m->get_params(host);
+  });
+}
+tk_status tk_mlp_set_params(tk_mlp* m, const float* host) {
+  return guard(m ? m->ctx : nullptr, [&] {
+    if (!m || !host) throw tk::ArgError("mlp: null pointer");
+    m->m->set_params(host);
+  });
+}
+tk_status tk_mlp_forward(tk_mlp* m, int32_t batch, const float* x_dev, float* y_dev) {
+  return guard(m ? m->ctx : nullptr, [&] {
+    if (!m || !x_dev || !y_dev) throw tk::ArgError("mlp: null pointer");
+    m->m->forward(batch, x_dev, y_dev);
+  });
+}
+tk_status tk_mlp_loss_grad(tk_mlp* m, int32_t batch, const float* x_dev, const float* t_dev, float* loss,
+                           float* grad_host) {
+  return guard(m ? m->ctx : nullptr, [&] {
+    if (!m || !x_dev || !t_dev || !loss) throw tk::ArgError("mlp: null pointer");
+    *loss = m->m->loss_grad(batch, x_dev, t_dev, grad_host);
+  });
+}
+tk_status tk_mlp_adam(tk_mlp* m, float lr, float beta1, float beta2, float eps) {
+  return guard(m ? m->ctx : nullptr, [&] {
+    if (!m) throw tk::ArgError("mlp: null pointer");
+    m->m->adam_step(lr, beta1, beta2, eps);
   });
 }
 

@@ -141,6 +141,21 @@ def lib():
         "tk_block_fuse": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, I32, I32, P, I64, P, I64]),
         "tk_block_split": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, I32, I32, P, I64, P, I64]),
         "tk_ctx_timing_read": (I32, [P, I32, C.POINTER(tk_kernel_timing), pI32]),
+        "tk_hybridization": (I32, [P, I32, I32, I32, P, I32, P, P, P]),
+        "tk_bath_fit": (I32, [P, I32, I32, I32, P, P, P, I32, pD]),
+        "tk_lanczos_tridiagonalize": (I32, [I32, pD, pD, I32, pD, pD]),
+        "tk_broyden_create": (I32, [I32, C.POINTER(P)]),
+        "tk_broyden_update": (I32, [P, I32, pD, pD, D, pD, pI32]),
+        "tk_broyden_destroy": (I32, [P]),
+        "tk_krylov_tridiag": (I32, [P, I64, P, P, P, P, I32, pD, pD, pI32]),
+        "tk_mlp_create": (I32, [P, I32, pI32, U64, C.POINTER(P)]),
+        "tk_mlp_destroy": (I32, [P]),
+        "tk_mlp_n_params": (I32, [P, pI64]),
+        "tk_mlp_get_params": (I32, [P, C.POINTER(C.c_float)]),
+        "tk_mlp_set_params": (I32, [P, C.POINTER(C.c_float)]),
+        "tk_mlp_forward": (I32, [P, I32, P, P]),
+        "tk_mlp_loss_grad": (I32, [P, I32, P, P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "tk_mlp_adam": (I32, [P, C.c_float, C.c_float, C.c_float, C.c_float]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -157,7 +172,10 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
             "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_ctx_timing",
-            "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split"]
+            "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split", "tk_hybridization",
+            "tk_bath_fit", "tk_lanczos_tridiagonalize", "tk_broyden_create", "tk_broyden_update",
+            "tk_broyden_destroy", "tk_krylov_tridiag", "tk_mlp_create", "tk_mlp_destroy", "tk_mlp_n_params",
+            "tk_mlp_get_params", "tk_mlp_set_params", "tk_mlp_forward", "tk_mlp_loss_grad", "tk_mlp_adam"]
 
 
 def _check(st, ctx=None):
