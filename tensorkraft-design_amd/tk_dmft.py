@@ -145,6 +145,19 @@ def lanczos_tridiagonalize(delta_target, omega, n_bath):
     return BathParameters(eps, v)
 
 
+def chain_to_star(ctx, chain):
+    """The star bath of a chain: poles = eigenvalues of the bath-site
+    tridiagonal (eps'_k on the diagonal, v'_1.. off it), weights v'_0^2 times
+    the squared first eigenvector components (device eigensolver)."""
+    eps, v = (np.asarray(a, dtype=np.float64) for a in chain)
+    n = eps.size
+    h = np.diag(eps)
+    for k in range(1, n):
+        h[k - 1, k] = h[k, k - 1] = v[k]
+    w, vec = tk_hip.HipBackend(ctx).eigh(h)
+    return BathParameters(w, np.abs(v[0] * vec[0, :]))
+
+
 def star_to_chain(bath):
     """The chain (eps'_k, v'_k) with the same Delta(z) as the star bath: the
     Lanczos recursion on the poles eps_k with weights v_k^2."""
@@ -476,12 +489,15 @@ def features(delta):
 class DMFTLoop:
     """Bethe-lattice DMFT (loop/mod.rs:107-419) on the Matsubara axis.
 
-    One iteration: impurity solve (DMRG ground state, Krylov G) -> Delta_new =
-    t^2 G (Bethe, t = half bandwidth / 2) -> star fit of Delta_new (device LM)
-    -> chain mapping -> mixing of the chain parameters.  With `proposal`, the
-    network proposes the next chain bath from Delta_new; the proposal is
-    judged by one impurity solve and taken when its self-consistency
-    distance is below that of the current bath.
+    One iteration: impurity solve of the current chain bath (DMRG ground
+    state, Krylov G) -> Delta_new = t^2 G (Bethe, t = half bandwidth / 2) ->
+    star fit of Delta_new (device Levenberg-Marquardt) -> chain mapping F ->
+    mixing of the chain parameters.  Convergence: the fixed-point residual
+    ||Delta(F(b)) - Delta(b)||_inf / ||Delta(b)||_inf < tol on the Matsubara
+    axis (the reference compares consecutive baths, loop/mod.rs:195-205).
+    With `proposal`, a network proposes a chain bath from the first
+    Delta_new; one impurity solve judges it, and it replaces the current bath
+    when its fixed-point residual is smaller.
     """
 
     def __init__(self, ctx, n_bath=7, U=4.0, half_bandwidth=2.0, beta=50.0, n_matsubara=256, bond_dim=256,
@@ -491,26 +507,30 @@ class DMFTLoop:
         self.t = half_bandwidth / 2.0
         self.wn = matsubara(beta, n_matsubara)
         self.solver = ImpuritySolver(ctx, n_bath, U, bond_dim=bond_dim)
-        self.mixing, self.alpha, self.tol, self.max_iterations = mixing, alpha, tol, max_iterations
-        self.broyden = BroydenState(history) if mixing == "broyden" else None
+        self.mixing, self.alpha, self.history_depth = mixing, alpha, history
+        self.tol, self.max_iterations = tol, max_iterations
         self.history = []
 
     def delta_chain(self, chain):
         eps, v = chain
         return hybridization(self.ctx, CHAIN, np.concatenate([eps, v]), 1j * self.wn)[0]
 
-    def distance(self, a, b):
+    @staticmethod
+    def distance(a, b):
         return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), np.finfo(float).eps))
-
-    def self_consistency(self, chain):
-        """One impurity solve: (Delta_in, Delta_out = t^2 G)."""
-        _, g = self.solver.green(chain[0], chain[1], self.wn)
-        return self.delta_chain(chain), self.t ** 2 * g
 
     def fit(self, delta, start_star):
         p, cost = bath_fit(self.ctx, self.wn, delta, start_star)
-        star = BathParameters(p[0, :self.n_bath], p[0, self.n_bath:])
-        return star, float(cost[0])
+        return BathParameters(p[0, :self.n_bath], p[0, self.n_bath:]), float(cost[0])
+
+    def step(self, chain, star):
+        """One impurity solve: (Delta_new, fitted star, F(chain), residual)."""
+        _, g = self.solver.green(chain[0], chain[1], self.wn)
+        d_new = self.t ** 2 * g
+        star_new, _ = self.fit(d_new, star.params())
+        target = star_to_chain(star_new)
+        res = self.distance(self.delta_chain(target), self.delta_chain(chain))
+        return d_new, star_new, target, res
 
     def initial_chain(self):
         """Chain of the non-interacting Bethe bath: Delta_0 = t^2 G_0 with
@@ -521,40 +541,76 @@ class DMFTLoop:
         return star_to_chain(star), star
 
     def solve(self, chain=None, star=None, proposal=None):
-        """Iterate to ||Delta_out - Delta_in|| / ||Delta_in|| < tol; returns the
-        converged chain and the number of impurity solves."""
+        """Iterate to a fixed-point residual below tol.  Returns (chain,
+        number of impurity solves, converged).
+
+        With `proposal`, the bath after the first solve is the network's
+        proposal from that solve's Delta_new instead of the mixed update; the
+        next impurity solve judges it: when its residual is not below the
+        first one, the loop falls back to the mixed update of the first
+        bath (the judging solve is then the only extra cost)."""
         if chain is None:
             chain, star = self.initial_chain()
         if star is None:
-            star = BathParameters(chain[0], chain[1])
+            star = chain_to_star(self.ctx, chain)
+        mixer = BroydenState(self.history_depth) if self.mixing == "broyden" else None
         self.history = []
+        start = self.solver.n_solves
+        pending = None  # (residual, chain, target, star) of the bath the proposal replaced
+
+        def mixed(ch, tg):
+            x, f = np.concatenate(ch), np.concatenate(tg)
+            xn = mixer.update(x, f, self.alpha) if mixer is not None else (1 - self.alpha) * x + self.alpha * f
+            return (xn[:self.n_bath], xn[self.n_bath:])
+
         for it in range(self.max_iterations):
-            d_in, d_out = self.self_consistency(chain)
-            dist = self.distance(d_out, d_in)
-            self.history.append({"iteration": it, "distance": dist, "solves": self.solver.n_solves,
-                                 "delta_out": d_out})
-            if dist < self.tol:
-                return chain, self.solver.n_solves, True
-            if proposal is not None:
-                prop = proposal(d_out)
+            d_new, star_new, target, res = self.step(chain, star)
+            rec = {"iteration": it, "residual": res, "delta_new": d_new}
+            self.history.append(rec)
+            if pending is not None:
+                r0, c0, t0, s0 = pending
+                pending = None
+                rec["proposal_taken"] = bool(res < r0)
+                if not res < r0:
+                    if res < self.tol:
+                        return chain, self.solver.n_solves - start, True
+                    chain, star = mixed(c0, t0), s0
+                    continue
+            if res < self.tol:
+                return chain, self.solver.n_solves - start, True
+            if proposal is not None and it == 0:
+                prop = proposal(d_new)
                 if prop is not None:
-                    p_in, p_out = self.self_consistency(prop)
-                    pd = self.distance(p_out, p_in)
-                    self.history[-1]["proposal_distance"] = pd
-                    if pd < dist:
-                        chain = prop
-                        star = BathParameters(*chain)
-                        if pd < self.tol:
-                            return chain, self.solver.n_solves, True
-                        continue
-            star_new, _ = self.fit(d_out, star.params())
-            target = star_to_chain(star_new)
-            x = np.concatenate(chain)
-            f = np.concatenate(target)
-            if self.broyden is not None:
-                xn = self.broyden.update(x, f, self.alpha)
-            else:
-                xn = (1 - self.alpha) * x + self.alpha * f
-            chain = (xn[:self.n_bath], xn[self.n_bath:])
-            star = star_new
-        return chain, self.solver.n_solves, False
+                    pending = (res, chain, target, star_new)
+                    chain, star = prop, chain_to_star(self.ctx, prop)
+                    continue
+            chain, star = mixed(chain, target), star_new
+        return chain, self.solver.n_solves - start, False
+
+
+def train_bath_proposal(ctx, runs, dims=(512, 256, 256, 14), epochs=4000, lr=1e-3, seed=0):
+    """The learned bath proposal: an Mlp from Delta_new(i w_n) (every
+    iteration of the given converged DMFTLoop runs) to the chain parameters of
+    that run's self-consistent bath, inputs and targets standardised over the
+    training set.  runs: [(history, converged chain)].  Returns (network,
+    proposal function for DMFTLoop.solve, loss history)."""
+    xs, ts = [], []
+    for hist, chain in runs:
+        tgt = np.concatenate(chain)
+        for rec in hist:
+            xs.append(features(rec["delta_new"])[0])
+            ts.append(tgt)
+    x = np.array(xs, dtype=np.float64)
+    t = np.array(ts, dtype=np.float64)
+    xm, xsd = x.mean(0), x.std(0) + 1e-8
+    tm, tsd = t.mean(0), t.std(0) + 1e-8
+    net = Mlp(ctx, list(dims), seed=seed)
+    loss = net.train(((x - xm) / xsd).astype(np.float32), ((t - tm) / tsd).astype(np.float32), epochs=epochs, lr=lr)
+    nb = dims[-1] // 2
+
+    def proposal(delta_new):
+        z = ((features(delta_new)[0].astype(np.float64) - xm) / xsd).astype(np.float32)
+        y = net.forward(z)[0].astype(np.float64) * tsd + tm
+        return (y[:nb], np.abs(y[nb:]))
+
+    return net, proposal, loss

@@ -147,3 +147,29 @@ def test_mlp_forward_gradient_adam_vs_oracle(ctx):
     # training lowers the loss
     hist = net.train(x, t, epochs=60, lr=1e-3)
     assert hist[-1] < 0.2 * hist[0]
+
+
+def test_dmft_loop_and_learned_proposal(ctx):
+    """Config-5 impurity (7 bath sites, D = 256, U around 4, beta = 50, 256
+    Matsubara points): the self-consistency loop converges, and the network
+    trained on the solver's own (Delta, converged bath) pairs from 16 other
+    U values reaches the same bath in fewer impurity solves."""
+    print("case dmft loop + learned proposal", flush=True)
+    runs = []
+    for k in range(16):
+        U = 3.0625 + 0.125 * k
+        loop = D.DMFTLoop(ctx, n_bath=7, U=U, tol=1e-5, max_iterations=60)
+        chain, n, ok = loop.solve()
+        assert ok, f"U={U} did not converge"
+        runs.append((loop.history, chain))
+    net, proposal, loss = D.train_bath_proposal(ctx, runs, epochs=4000)
+    assert loss[-1] < 0.01 * loss[0]
+    for U in (4.0, 4.25):
+        plain = D.DMFTLoop(ctx, n_bath=7, U=U, tol=1e-5, max_iterations=60)
+        ca, na, oka = plain.solve()
+        learned = D.DMFTLoop(ctx, n_bath=7, U=U, tol=1e-5, max_iterations=60)
+        cb, nb, okb = learned.solve(proposal=proposal)
+        print(f"U={U}: plain {na} solves, learned {nb} solves", flush=True)
+        assert oka and okb
+        assert nb < na
+        assert np.max(np.abs(np.concatenate(ca) - np.concatenate(cb))) < 1e-3
