@@ -764,20 +764,39 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
   }
 }
 
+// Column 2-norms of the n x k row-major Z of every job into norms (at w_off).
+__global__ __launch_bounds__(256) void colnorm_kernel(const DevJob* __restrict__ jobs, const double* __restrict__ Z,
+                                                      double* __restrict__ norms) {
+  const DevJob jb = jobs[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= jb.k) return;
+  const double* z = Z + jb.z_off;
+  double s = 0;
+  for (int i = 0; i < jb.n; ++i) {
+    const double x = z[int64_t(i) * jb.k + c];
+    s += x * x;
+  }
+  norms[jb.w_off + c] = sqrt(s);
+}
+
 // ---------------------------------------------------------------------------
-// 3b. Orthonormalisation of the k vectors of a job (columns of the n x k
-// row-major Z, ld k) in column order: each block of NB columns is projected
-// against all earlier columns (two passes, 64 earlier columns at a time), then
-// orthonormalised within the block (two passes per column).  A column that
-// keeps less than 1e-3 of its norm was numerically dependent on the earlier
-// ones; it restarts from a hashed vector projected against every earlier
-// column.
+// 3b. Orthonormalisation of the columns [c_begin, c_begin + c_count) of the
+// n x k row-major Z (ld k) of each job, in column order, the columns before
+// c_begin being orthonormal and already projected out: each block of NB
+// columns is projected against the earlier columns of the range (two passes,
+// 64 columns at a time), then orthonormalised within the block (two passes
+// per column).  A column that keeps less than 1e-3 of its norm on entry
+// (norms, at w_off of the job) was numerically dependent on the earlier ones;
+// it restarts from a hashed vector projected against every earlier column.
 template <int NT, int NB>
 __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ jobs, double* __restrict__ Z,
-                                                  const double* __restrict__ LU) {
+                                                  const double* __restrict__ LU, int c_begin, int c_count,
+                                                  const double* __restrict__ norms) {
   constexpr int NW = NT / 64;
   const DevJob jb = jobs[blockIdx.x];
   const int n = jb.n, k = jb.k;
+  if (k <= c_begin) return;
+  const int c_end = min(k, c_begin + c_count);
   double* z = Z + jb.z_off;
   extern __shared__ double sm[];
   double* Zb = sm;                      // n x NB: the current block
@@ -805,9 +824,9 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
     __syncthreads();
   };
   // one classical pass: columns [c_lo, c_hi) of Zb minus their projection on
-  // z[:, 0:p_end), 64 earlier columns at a time
-  auto project = [&](int p_end, int c_lo, int c_hi) {
-    for (int p0 = 0; p0 < p_end; p0 += 64) {
+  // z[:, p_lo:p_end), 64 earlier columns at a time
+  auto project = [&](int p_lo, int p_end, int c_lo, int c_hi) {
+    for (int p0 = p_lo; p0 < p_end; p0 += 64) {
       const int pc = min(64, p_end - p0);
       double acc[NB];
 #pragma unroll
@@ -845,27 +864,19 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
     }
   };
 
-  for (int b0 = 0; b0 < k; b0 += NB) {
-    const int nb = min(NB, k - b0);
+  for (int b0 = c_begin; b0 < c_end; b0 += NB) {
+    const int nb = min(NB, c_end - b0);
     for (int e = tid; e < n * NB; e += NT) {
       const int i = e / NB, c = e - i * NB;
       Zb[e] = c < nb ? z[int64_t(i) * k + b0 + c] : 0.0;
     }
     __syncthreads();
-    {
-      double part[NB];
-#pragma unroll
-      for (int c = 0; c < NB; ++c) part[c] = 0;
-      for (int i = tid; i < n; i += NT)
-#pragma unroll
-        for (int c = 0; c < NB; ++c) part[c] += Zb[i * NB + c] * Zb[i * NB + c];
-      block_sums(part, nb);
-      if (tid < nb) nrm0[tid] = sqrt(hs[tid]);
-      __syncthreads();
-    }
-    if (b0 > 0) {
-      project(b0, 0, nb);
-      project(b0, 0, nb);
+    // norms on entry (before any projection) for the dependence test
+    if (tid < nb) nrm0[tid] = norms[jb.w_off + b0 + tid];
+    __syncthreads();
+    if (b0 > c_begin) {
+      project(c_begin, b0, 0, nb);
+      project(c_begin, b0, 0, nb);
     }
     for (int c = 0; c < nb; ++c) {
       for (int attempt = 0;; ++attempt) {
@@ -926,8 +937,8 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
         if (tid == 0) nrm0[c] = sqrt(hs[0]);
         __syncthreads();
         if (b0 > 0) {
-          project(b0, c, c + 1);
-          project(b0, c, c + 1);
+          project(0, b0, c, c + 1);
+          project(0, b0, c, c + 1);
         }
       }
     }
@@ -1228,18 +1239,6 @@ struct VecWork {
   double orth_bytes = 0, orth_flops = 0, bt_bytes = 0, bt_flops = 0;
 };
 
-template <int NB>
-void launch_orth_t(Context* ctx, int nmax, int n_jobs, const DevJob* djd, double* zi, const double* lu,
-                   const VecWork& work) {
-  constexpr int kNT = 1024, NW = kNT / 64;
-  const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
-  allow_lds(orth_kernel<kNT, NB>, lds);
-  const int tk = ctx->timer.begin(ctx->stream);
-  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(n_jobs), dim3(kNT), lds, ctx->stream, djd, zi, lu);
-  ctx->timer.end(tk, ctx->stream, "orthonormalize", work.orth_bytes, work.orth_flops, 1);
-  TK_HIP(hipGetLastError());
-}
-
 // Launches one list; the device and host copies of its tables are kept in
 // `keep` until the caller has synchronised.
 void run_list(Context* ctx, ContractList& cl, int64_t a_n, int64_t b_n, int64_t c_n, const double* A, const double* B,
@@ -1365,6 +1364,83 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
     run_list(ctx, lu, w_max, a_n, z_n, Wb.get(), Ut.get(), Z, keep);
   }
   ctx->sync();  // the lists' device tables are released on return
+}
+
+constexpr int kOrthBlock = 64;  // columns per block of the projection contractions
+constexpr int kOrthSeg = 256;   // rows per split-K segment of Z^T Z_block
+
+template <int NB>
+void launch_orth_range(Context* ctx, int nmax, int nj, const DevJob* djd, double* Z, const double* lu, int c_begin,
+                       const double* norms) {
+  constexpr int kNT = 1024, NW = kNT / 64;
+  const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
+  allow_lds(orth_kernel<kNT, NB>, lds);
+  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(nj), dim3(kNT), lds, ctx->stream, djd, Z, lu, c_begin, kOrthBlock,
+                     norms);
+  TK_HIP(hipGetLastError());
+}
+
+// Blocked classical Gram-Schmidt applied twice: block b of kOrthBlock columns
+// is projected against all earlier columns by two grouped contractions per
+// pass (H = Z^T Z_b split over row segments, Z_b -= Z H summing the
+// segments), then orthonormalised within itself by orth_kernel.  Norms on
+// entry come from one pass over Z; w_off of each job indexes its norms.
+void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd, double* Z, int64_t z_n,
+                  const double* lu, int64_t w_n) {
+  const int nj = int(dj.size());
+  int kmax = 0, nmax = 0;
+  for (const DevJob& j : dj) {
+    kmax = std::max(kmax, j.k);
+    nmax = std::max(nmax, j.n);
+  }
+  if (kmax == 0) return;
+  DBuf<double> norms(ctx, size_t(std::max<int64_t>(w_n, 1)));
+  hipLaunchKernelGGL(colnorm_kernel, dim3(unsigned((kmax + 255) / 256), unsigned(nj)), dim3(256), 0, ctx->stream,
+                     djd, Z, norms.get());
+  TK_HIP(hipGetLastError());
+  int64_t h_max = 1;
+  for (int c0 = kOrthBlock; c0 < kmax; c0 += kOrthBlock) {
+    int64_t h = 0;
+    for (const DevJob& j : dj)
+      if (j.k > c0) h += int64_t((j.n + kOrthSeg - 1) / kOrthSeg) * c0 * std::min(kOrthBlock, j.k - c0);
+    h_max = std::max(h_max, h);
+  }
+  DBuf<double> H(ctx, size_t(h_max));
+  std::vector<std::unique_ptr<DeviceContractList>> keep;
+  for (int c0 = 0; c0 < kmax; c0 += kOrthBlock) {
+    if (c0 > 0) {
+      for (int pass = 0; pass < 2; ++pass) {
+        ContractList lh, lu2;
+        lh.flags = kTransA;
+        lu2.flags = kTransA | kTransB | kTransC;
+        lu2.beta = 1.0;
+        int64_t h = 0;
+        for (const DevJob& j : dj) {
+          if (j.k <= c0) continue;
+          const int nb = std::min(kOrthBlock, j.k - c0);
+          // Z_b^T (nb x n) -= (sum_seg H_seg)^T Z[:, 0:c0]^T
+          lu2.begin_out(j.z_off + c0, nb, j.n, j.k);
+          lu2.begin_term(j.z_off, j.k, c0);
+          for (int r0 = 0; r0 < j.n; r0 += kOrthSeg) {
+            const int r1 = std::min(j.n, r0 + kOrthSeg);
+            // H_seg (c0 x nb) = Z[r0:r1, 0:c0]^T Z[r0:r1, c0:c0+nb]
+            lh.begin_out(h, c0, nb, nb);
+            lh.begin_term(j.z_off + int64_t(r0) * j.k + c0, j.k, r1 - r0);
+            lh.add_src(j.z_off + int64_t(r0) * j.k, j.k, 1.0);
+            lu2.add_src(h, nb, -1.0);
+            h += int64_t(c0) * nb;
+          }
+        }
+        run_list(ctx, lh, z_n, z_n, h_max, Z, Z, H.get(), keep);
+        run_list(ctx, lu2, h_max, z_n, z_n, H.get(), Z, Z, keep);
+      }
+    }
+    if (nmax <= 1024)
+      launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, c0, norms.get());
+    else
+      launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, c0, norms.get());
+  }
+  ctx->sync();  // the lists' tables and the scratch buffers are released on return
 }
 
 }  // namespace
@@ -1509,7 +1585,6 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
     lu_tot += kLuFields * int64_t(j.n) * k;
   }
   if (dj.empty()) return;
-  const int NBv = nmax_ <= 1024 ? 8 : 4;
   const size_t jb_bytes = dj.size() * sizeof(DevJob);
   const size_t vi_bytes = vitems.size() * sizeof(VecItem);
   DBuf<char> meta(ctx_, jb_bytes + vi_bytes);
@@ -1533,15 +1608,14 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   VecWork work;
   for (const DevJob& j : dj) {
     const double n = j.n, k = j.k;
-    work.orth_bytes += 8.0 * (2.0 * n * k * k / NBv + 2.0 * n * k);
+    work.orth_bytes += 8.0 * (2.0 * n * k * k / kOrthBlock + 4.0 * n * k);
     work.orth_flops += 4.0 * n * k * k;
     work.bt_bytes += 8.0 * (n * n + 2.0 * k * n * n / kWyNb + 2.0 * n * k);
     work.bt_flops += 2.0 * n * n * k;
   }
-  if (NBv == 8)
-    launch_orth_t<8>(ctx_, nmax_, int(dj.size()), djd, zi.get(), lu.get(), work);
-  else
-    launch_orth_t<4>(ctx_, nmax_, int(dj.size()), djd, zi.get(), lu.get(), work);
+  const int to = ctx_->timer.begin(ctx_->stream);
+  orth_blocked(ctx_, dj, djd, zi.get(), z_tot, lu.get(), w_n);
+  ctx_->timer.end(to, ctx_->stream, "orthonormalize", work.orth_bytes, work.orth_flops, 1);
   const int tb = ctx_->timer.begin(ctx_->stream);
   wy_backtransform(ctx_, dj, djd, A_, a_n_, TAU, zi.get(), z_tot, Z, z_n);
   ctx_->timer.end(tb, ctx_->stream, "backtransform", work.bt_bytes, work.bt_flops, 1);
@@ -1550,31 +1624,18 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
 
 void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, double* Z, int64_t z_n) {
   std::vector<DevJob> dj;
-  int nmax = 0;
+  int64_t w = 0;
   for (const OrthJob& j : jobs) {
     if (j.n < 1 || j.n > kSymEigMaxN || j.k < 0 || j.k > j.n) throw ArgError("orthonormalize: bad shape");
     if (j.off < 0 || j.off + int64_t(j.n) * j.k > z_n) throw std::out_of_range("orthonormalize: job outside Z");
     if (j.k == 0) continue;
-    dj.push_back({0, 0, j.off, 0, j.n, j.k, int32_t(dj.size()), 0});
-    nmax = std::max(nmax, j.n);
+    dj.push_back({0, w, j.off, 0, j.n, j.k, int32_t(dj.size()), 0});
+    w += j.k;
   }
   if (dj.empty()) return;
   DBuf<DevJob> d(ctx, dj.size());
   d.upload(dj.data(), dj.size());
-  constexpr int kNT = 1024, NW = kNT / 64;
-  if (nmax <= 1024) {
-    const size_t lds = (size_t(nmax) * 8 + NW * 64 * 8 + 64 * 8 + NW * 8 + 2 * 8) * sizeof(double);
-    allow_lds(orth_kernel<kNT, 8>, lds);
-    hipLaunchKernelGGL((orth_kernel<kNT, 8>), dim3(dj.size()), dim3(kNT), lds, ctx->stream, d.get(), Z,
-                       static_cast<const double*>(nullptr));
-  } else {
-    const size_t lds = (size_t(nmax) * 4 + NW * 64 * 4 + 64 * 4 + NW * 4 + 2 * 4) * sizeof(double);
-    allow_lds(orth_kernel<kNT, 4>, lds);
-    hipLaunchKernelGGL((orth_kernel<kNT, 4>), dim3(dj.size()), dim3(kNT), lds, ctx->stream, d.get(), Z,
-                       static_cast<const double*>(nullptr));
-  }
-  TK_HIP(hipGetLastError());
-  ctx->sync();
+  orth_blocked(ctx, dj, d.get(), Z, z_n, nullptr, w);
 }
 
 void tridiag_lowest(Context* ctx, const double* alpha, const double* beta, int m, const double* beta_next,
