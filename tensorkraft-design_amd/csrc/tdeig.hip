@@ -864,6 +864,84 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
     }
   };
 
+  // Cholesky QR of the nb columns in Zb: G = Zb^T Zb, G = R^T R, Zb <- Zb R^-1.
+  // With `test`, fails (Zb untouched) when some R_cc <= 1e-3 nrm0[c].
+  auto cholqr = [&](int nb, bool test) -> bool {
+    constexpr int NU = NB * (NB + 1) / 2;
+    double gp[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) gp[u] = 0;
+    for (int i = tid; i < n; i += NT) {
+      double zr[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) zr[c] = Zb[i * NB + c];
+      int u = 0;
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = a; b < NB; ++b) gp[u++] += zr[a] * zr[b];
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const double x = wave_sum(gp[u]);
+      if (lane == 0) Hp[wv * NU + u] = x;
+    }
+    __syncthreads();
+    if (tid < NU) {
+      double x = 0;
+      for (int q = 0; q < NW; ++q) x += Hp[q * NU + tid];
+      H[tid] = x;
+    }
+    __syncthreads();
+    double* R = H + 64;  // NB x NB, upper triangle
+    if (tid == 0) {
+      bool ok = true;
+      double g[NB][NB];
+      int u = 0;
+      for (int a = 0; a < NB; ++a)
+        for (int b = a; b < NB; ++b) {
+          g[a][b] = H[u];
+          g[b][a] = H[u];
+          ++u;
+        }
+      for (int a = 0; a < nb && ok; ++a) {
+        double d = g[a][a];
+        for (int c = 0; c < a; ++c) d -= R[c * NB + a] * R[c * NB + a];
+        if (!(d > 0) || (test && sqrt(d) <= 1e-3 * nrm0[a])) {
+          ok = false;
+          break;
+        }
+        const double r = sqrt(d);
+        R[a * NB + a] = r;
+        for (int b = a + 1; b < nb; ++b) {
+          double t = g[a][b];
+          for (int c = 0; c < a; ++c) t -= R[c * NB + a] * R[c * NB + b];
+          R[a * NB + b] = t / r;
+        }
+      }
+      hs[0] = ok ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (hs[0] == 0.0) return false;
+    for (int i = tid; i < n; i += NT) {
+      double x[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c < nb) {
+          double t = Zb[i * NB + c];
+          for (int q = 0; q < c; ++q) t -= x[q] * R[q * NB + c];
+          x[c] = t / R[c * NB + c];
+        } else {
+          x[c] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NB; ++c) Zb[i * NB + c] = x[c];
+    }
+    __syncthreads();
+    return true;
+  };
+
   for (int b0 = c_begin; b0 < c_end; b0 += NB) {
     const int nb = min(NB, c_end - b0);
     for (int e = tid; e < n * NB; e += NT) {
@@ -877,6 +955,16 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
     if (b0 > c_begin) {
       project(c_begin, b0, 0, nb);
       project(c_begin, b0, 0, nb);
+    }
+    // fast path: Cholesky QR twice on the sub-block; the column-by-column
+    // path below runs only when a column is numerically dependent
+    if (cholqr(nb, true) && cholqr(nb, false)) {
+      for (int e = tid; e < n * NB; e += NT) {
+        const int i = e / NB, c = e - i * NB;
+        if (c < nb) z[int64_t(i) * k + b0 + c] = Zb[e];
+      }
+      __syncthreads();
+      continue;
     }
     for (int c = 0; c < nb; ++c) {
       for (int attempt = 0;; ++attempt) {
@@ -1214,25 +1302,35 @@ struct WyBlock {
   int32_t nb, pad;
 };
 
-// One thread per row l of T: row l depends only on its own earlier entries.
+// T column by column in LDS: T[0:i, i] = -tau_i T[0:i, 0:i] S[0:i, i], one
+// thread per row l, the column S[:, i] staged in LDS.
+constexpr size_t kWyTLds = (size_t(kWyNb) * (kWyNb + 1) + kWyNb) * sizeof(double);
+
 __global__ __launch_bounds__(kWyNb) void wy_t_kernel(const WyBlock* __restrict__ blocks,
                                                      const double* __restrict__ S, const double* __restrict__ TAUg,
                                                      double* __restrict__ T) {
+  extern __shared__ double sm[];
+  double* tl = sm;                              // kWyNb x (kWyNb + 1)
+  double* sc = sm + kWyNb * (kWyNb + 1);        // column i of S
   const WyBlock b = blocks[blockIdx.x];
   const double* s = S + b.t_off;
-  double* t = T + b.t_off;
   const double* tau = TAUg + b.tau_off;
   const int l = threadIdx.x, nb = b.nb;
-  double* tl = t + int64_t(l) * kWyNb;
-  for (int m = 0; m < kWyNb; ++m)
-    if (l >= nb || m < l || m >= nb) tl[m] = 0.0;
-  if (l >= nb) return;
-  tl[l] = tau[l];
-  for (int i = l + 1; i < nb; ++i) {
-    double acc = 0;
-    for (int m = l; m < i; ++m) acc += tl[m] * s[int64_t(m) * kWyNb + i];
-    tl[i] = -tau[i] * acc;
+  for (int m = 0; m < kWyNb; ++m) tl[l * (kWyNb + 1) + m] = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    sc[l] = l < i ? s[int64_t(l) * kWyNb + i] : 0.0;
+    __syncthreads();
+    if (l < i) {
+      double acc = 0;
+      for (int m = l; m < i; ++m) acc += tl[l * (kWyNb + 1) + m] * sc[m];
+      tl[l * (kWyNb + 1) + i] = -tau[i] * acc;
+    } else if (l == i) {
+      tl[l * (kWyNb + 1) + i] = tau[i];
+    }
+    __syncthreads();
   }
+  double* t = T + b.t_off;
+  for (int m = 0; m < kWyNb; ++m) t[int64_t(l) * kWyNb + m] = tl[l * (kWyNb + 1) + m];
 }
 
 struct VecWork {
@@ -1303,8 +1401,9 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
     }
     run_list(ctx, cl, a_n, a_n, st_n, A, A, S.get(), keep);
   }
-  hipLaunchKernelGGL(wy_t_kernel, dim3(unsigned(hb.size())), dim3(kWyNb), 0, ctx->stream, wbd.get(), S.get(), TAU,
-                     T.get());
+  allow_lds(wy_t_kernel, kWyTLds);
+  hipLaunchKernelGGL(wy_t_kernel, dim3(unsigned(hb.size())), dim3(kWyNb), kWyTLds, ctx->stream, wbd.get(), S.get(),
+                     TAU, T.get());
   TK_HIP(hipGetLastError());
   {  // U^T = T^T V^T per block, stored in the rows of Ut like V^T in A
     ContractList cl;
@@ -1366,16 +1465,25 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
   ctx->sync();  // the lists' device tables are released on return
 }
 
-constexpr int kOrthBlock = 64;  // columns per block of the projection contractions
-constexpr int kOrthSeg = 256;   // rows per split-K segment of Z^T Z_block
+constexpr int kOrthSeg = 256;  // rows per split-K segment of Z^T Z_block
+
+// Columns per block of the projection contractions (TK_ORTH_BLOCK overrides, for tuning).
+int orth_block() {
+  static const int b = [] {
+    const char* e = std::getenv("TK_ORTH_BLOCK");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 8 && v <= 256 && v % 8 == 0) ? v : 64;
+  }();
+  return b;
+}
 
 template <int NB>
 void launch_orth_range(Context* ctx, int nmax, int nj, const DevJob* djd, double* Z, const double* lu, int c_begin,
-                       const double* norms) {
+                       int c_count, const double* norms) {
   constexpr int kNT = 1024, NW = kNT / 64;
   const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
   allow_lds(orth_kernel<kNT, NB>, lds);
-  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(nj), dim3(kNT), lds, ctx->stream, djd, Z, lu, c_begin, kOrthBlock,
+  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(nj), dim3(kNT), lds, ctx->stream, djd, Z, lu, c_begin, c_count,
                      norms);
   TK_HIP(hipGetLastError());
 }
@@ -1394,6 +1502,7 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
     nmax = std::max(nmax, j.n);
   }
   if (kmax == 0) return;
+  const int kOrthBlock = orth_block();
   DBuf<double> norms(ctx, size_t(std::max<int64_t>(w_n, 1)));
   hipLaunchKernelGGL(colnorm_kernel, dim3(unsigned((kmax + 255) / 256), unsigned(nj)), dim3(256), 0, ctx->stream,
                      djd, Z, norms.get());
@@ -1436,9 +1545,9 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
       }
     }
     if (nmax <= 1024)
-      launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, c0, norms.get());
+      launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, c0, kOrthBlock, norms.get());
     else
-      launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, c0, norms.get());
+      launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, c0, kOrthBlock, norms.get());
   }
   ctx->sync();  // the lists' tables and the scratch buffers are released on return
 }
@@ -1608,7 +1717,7 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   VecWork work;
   for (const DevJob& j : dj) {
     const double n = j.n, k = j.k;
-    work.orth_bytes += 8.0 * (2.0 * n * k * k / kOrthBlock + 4.0 * n * k);
+    work.orth_bytes += 8.0 * (2.0 * n * k * k / orth_block() + 4.0 * n * k);
     work.orth_flops += 4.0 * n * k * k;
     work.bt_bytes += 8.0 * (n * n + 2.0 * k * n * n / kWyNb + 2.0 * n * k);
     work.bt_flops += 2.0 * n * n * k;
