@@ -148,15 +148,36 @@ def run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, warmup, steps, prec, t
     return out, wall, timing, centre
 
 
+# Measured HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated
+# with a copy kernel of known size) for kernels of this workload, from the
+# summaries in profiles/: "<kernel>": "<file>".
+TRAFFIC_PROFILES = {"tridiag_column_step": "tridiag_column_step_traffic_dmrg_sweep_L64_D2048.md"}
+
+
+def measured_traffic(name):
+    f = TRAFFIC_PROFILES.get(name)
+    if f is None:
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", f)) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        return None
+    return None
+
+
 def roofline_of(timing):
     """Roofline of the kernel with the largest event time: algorithmic bytes
-    per launch over the mean launch duration, against the HBM peak."""
+    per launch over the mean launch duration, against the HBM peak; traffic is
+    the profiled HBM bytes per launch of that kernel when a summary exists."""
     name = max(timing, key=lambda k: timing[k]["ms"])
     t = timing[name]
     sec = t["ms"] * 1e-3
     achieved = t["bytes"] / sec / 1e9 if sec > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name), "kernel": name,
             "launches": t["launches"], "mean_launch_us": round(1e3 * t["ms"] / max(1, t["launches"]), 3),
             "bytes_per_launch": round(t["bytes"] / max(1, t["launches"]), 1),
             "tflops": round(t["flops"] / sec / 1e12, 3) if sec > 0 else 0.0}
