@@ -23,7 +23,6 @@ namespace {
 
 constexpr double kEps = DBL_EPSILON;
 constexpr int kLuFields = 5;
-constexpr int kSmallTridiag = 128;
 constexpr int kBisLanes = 16;  // lanes per eigenvalue in the multisection  // largest batch dimension for the one-workgroup tridiagonalisation
 
 struct DevJob {
@@ -99,21 +98,46 @@ __device__ __forceinline__ double hash_unit(uint32_t a, uint32_t b) {
 // A22 -= v w^T + w v^T.  The update of step k-1 is applied to row k first,
 // then to rows k+1.. in the same pass that forms A22 v for step k.
 // T = max number of 64-column tiles of a row (n <= 64 T).
-template <int NT, int T>
-__global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ jobs, double* __restrict__ A,
+// One matrix per workgroup: the trailing block of size m at a_off (leading
+// dimension ld) of a matrix whose column steps before it were done by the
+// multi-workgroup kernel; `pending` >= 0 names the VP buffer pair holding
+// that kernel's last reflector v and A22 v, whose rank-2 update is still to
+// be applied (-1: a whole matrix, nothing pending).
+struct TailJob {
+  int64_t a_off;   // top-left element of the trailing block
+  int64_t w_off;   // d, e, tau (and the VP vectors) of its first row
+  int32_t m, ld;
+  int32_t pending, pad;
+};
+
+// With kLds the block is copied into LDS (m <= 128) and only the reflector
+// rows are written back.
+template <int NT, int T, bool kLds>
+__global__ __launch_bounds__(NT) void tridiag_kernel(const TailJob* __restrict__ jobs, double* __restrict__ A,
                                                      double* __restrict__ Dg, double* __restrict__ Eg,
-                                                     double* __restrict__ TAUg) {
+                                                     double* __restrict__ TAUg, const double* __restrict__ VP,
+                                                     int64_t w_n) {
   constexpr int NW = NT / 64;
   constexpr int R = (64 * T + NT - 1) / NT;
   constexpr bool kCache = T <= 8;  // keep v, w, vn of the lane's columns in registers
-  const DevJob jb = jobs[blockIdx.x];
-  const int n = jb.n;
-  double* a = A + jb.a_off;
+  const TailJob jb = jobs[blockIdx.x];
+  const int n = jb.m;
+  double* ag = A + jb.a_off;
   double* d = Dg + jb.w_off;
   double* e = Eg + jb.w_off;
   double* tau = TAUg + jb.w_off;
   extern __shared__ double sm[];
-  double* v = sm;
+  double* a = ag;
+  int ld = jb.ld;
+  double* sm2 = sm;
+  if constexpr (kLds) {
+    a = sm;
+    ld = n;
+    sm2 = sm + n * n;
+    for (int q = threadIdx.x; q < n * n; q += NT) a[q] = ag[int64_t(q / n) * jb.ld + q % n];
+    __syncthreads();
+  }
+  double* v = sm2;
   double* w = v + n;
   double* vn = w + n;
   double* prow = vn + n;
@@ -122,29 +146,47 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
   double* scal = red + 2 * NW;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int parity = 0;
-  if (n <= 2) {
+  if (jb.pending < 0 && n <= 2) {
     if (tid == 0) {
       d[0] = a[0];
       e[0] = 0;
       tau[0] = 0;
       if (n == 2) {
         e[0] = a[1];
-        d[1] = a[3];
+        d[1] = a[ld + 1];
         e[1] = 0;
         tau[1] = 0;
       }
     }
     return;
   }
-  for (int j = tid; j < n; j += NT) {
-    v[j] = 0;
-    w[j] = 0;
+  if (jb.pending < 0) {
+    for (int j = tid; j < n; j += NT) {
+      v[j] = 0;
+      w[j] = 0;
+    }
+  } else {
+    // the multi-workgroup kernel's last reflector (v[0] = 1) and p = A22 v:
+    // w = tau p - tau^2/2 (p.v) v
+    const double* vb = VP + int64_t(jb.pending) * w_n + jb.w_off;
+    const double* pb = VP + int64_t(2 + jb.pending) * w_n + jb.w_off;
+    const double tp = tau[-1];
+    double loc = 0;
+    for (int j = tid; j < n; j += NT) {
+      const double vj = j == 0 ? 1.0 : vb[j];
+      v[j] = vj;
+      prow[j] = pb[j];
+      loc += pb[j] * vj;
+    }
+    const double pv = block_sum<NT>(loc, red, parity);
+    const double alw = -0.5 * tp * tp * pv;
+    for (int j = tid; j < n; j += NT) w[j] = tp * prow[j] + alw * v[j];
   }
   __syncthreads();
   for (int k = 0; k + 2 < n; ++k) {
     // (a) row k with the pending update; sum of squares of x[1:]
     const double vk = v[k], wk = w[k];
-    double* rowk = a + int64_t(k) * n;
+    double* rowk = a + int64_t(k) * ld;
     double xr[R];
     double ss = 0;
 #pragma unroll
@@ -204,8 +246,8 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
       const bool has2 = i2 < n;
       const double vi = v[i], wi = w[i], vni = vn[i];
       const double vi2 = has2 ? v[i2] : 0.0, wi2 = has2 ? w[i2] : 0.0, vni2 = has2 ? vn[i2] : 0.0;
-      double* row = a + int64_t(i) * n;
-      double* row2 = a + int64_t(has2 ? i2 : i) * n;
+      double* row = a + int64_t(i) * ld;
+      double* row2 = a + int64_t(has2 ? i2 : i) * ld;
       double xv[T], xw[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
@@ -279,15 +321,23 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
   }
   if (tid == 0) {
     const int p = n - 2, q = n - 1;
-    const double app = a[int64_t(p) * n + p] - 2.0 * v[p] * w[p];
-    const double apq = a[int64_t(p) * n + q] - (v[p] * w[q] + w[p] * v[q]);
-    const double aqq = a[int64_t(q) * n + q] - 2.0 * v[q] * w[q];
+    const double app = a[int64_t(p) * ld + p] - 2.0 * v[p] * w[p];
+    const double apq = a[int64_t(p) * ld + q] - (v[p] * w[q] + w[p] * v[q]);
+    const double aqq = a[int64_t(q) * ld + q] - 2.0 * v[q] * w[q];
     d[p] = app;
     e[p] = apq;
     d[q] = aqq;
     e[q] = 0;
     tau[p] = 0;
     tau[q] = 0;
+  }
+  if constexpr (kLds) {
+    // reflector k is row k, columns k+2.. (the back-transformation's input)
+    __syncthreads();
+    for (int q = threadIdx.x; q < n * n; q += NT) {
+      const int r = q / n, c = q % n;
+      if (c >= r + 2) ag[int64_t(r) * jb.ld + c] = a[q];
+    }
   }
 }
 
@@ -303,11 +353,11 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const DevJob* __restrict__ 
 // (two buffers), each indexed like the eigenvalues.
 // R rows (wavefronts) per workgroup; the launch picks R per column step so
 // that the step's rows spread over enough workgroups to keep every CU busy.
+// Workgroups of a job at column step k; the job's column steps stop at k0
+// (DevJob::pad), where the single-workgroup kernel takes over its tail.
 template <int R>
-__device__ __forceinline__ int step_count(int n, int k) {
-  if (n == 1) return k == 0 ? 1 : 0;
-  if (k + 2 < n) return (n - k - 1 + R - 1) / R;
-  return k + 2 == n ? 1 : 0;
+__device__ __forceinline__ int step_count(int n, int k, int k0) {
+  return k < k0 ? (n - k - 1 + R - 1) / R : 0;
 }
 
 template <int R>
@@ -326,7 +376,7 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
   __shared__ double red[2 * kStepRows];
   int id = blockIdx.x, q = 0;
   for (; q < n_jobs; ++q) {
-    const int cnt = step_count<R>(jobs[q].n, k);
+    const int cnt = step_count<R>(jobs[q].n, k, jobs[q].pad);
     if (id < cnt) break;
     id -= cnt;
   }
@@ -469,8 +519,9 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
   }
 }
 
-// Reflector k from column k (rows >= k+2) into row k (columns >= k+2), for
-// the back-transformation; 32 x 32 tiles of the strictly lower part.
+// Reflector k < k0 (DevJob::pad) from column k (rows >= k+2) into row k
+// (columns >= k+2), for the back-transformation; 32 x 32 tiles of the
+// strictly lower part.
 struct TileItem {
   int32_t job, ti, tj, pad;
 };
@@ -491,7 +542,7 @@ __global__ __launch_bounds__(256) void reflector_rows_kernel(const TileItem* __r
   for (int e2 = threadIdx.x; e2 < 32 * 32; e2 += 256) {
     const int r = e2 >> 5, c = e2 & 31;  // destination row c0 + r, column r0 + c
     const int gk = c0 + r, gj = r0 + c;
-    if (gk < n && gj < n && gj >= gk + 2 && gk + 2 < n) a[int64_t(gk) * n + gj] = t[c][r];
+    if (gk < jb.pad && gj < n && gj >= gk + 2 && gk + 2 < n) a[int64_t(gk) * n + gj] = t[c][r];
   }
 }
 
@@ -1260,13 +1311,27 @@ void allow_lds(K kernel, size_t bytes) {
                                int(bytes)));
 }
 
-template <int NT, int T>
-void launch_tridiag_t(Context* ctx, const DevJob* jobs, int n_jobs, int nmax, double* A, double* D, double* E,
-                      double* TAU) {
+template <int NT, int T, bool kLds>
+void launch_tridiag_t(Context* ctx, const TailJob* jobs, int n_jobs, int mmax, double* A, double* D, double* E,
+                      double* TAU, const double* VP, int64_t w_n) {
   constexpr int NW = NT / 64;
-  const size_t lds = ((4 + NW) * size_t(nmax) + 2 * NW + 8) * sizeof(double);
-  allow_lds(tridiag_kernel<NT, T>, lds);
-  hipLaunchKernelGGL((tridiag_kernel<NT, T>), dim3(n_jobs), dim3(NT), lds, ctx->stream, jobs, A, D, E, TAU);
+  const size_t lds =
+      ((kLds ? size_t(mmax) * mmax : 0) + (4 + NW) * size_t(mmax) + 2 * NW + 8) * sizeof(double);
+  allow_lds(tridiag_kernel<NT, T, kLds>, lds);
+  hipLaunchKernelGGL((tridiag_kernel<NT, T, kLds>), dim3(n_jobs), dim3(NT), lds, ctx->stream, jobs, A, D, E, TAU,
+                     VP, w_n);
+}
+
+// Trailing size below which a job's tridiagonalisation finishes in one
+// workgroup with the block in LDS (TK_TRIDIAG_TAIL overrides with 64, 96 or
+// 128, for tuning).
+int tridiag_tail() {
+  static const int t = [] {
+    const char* e = std::getenv("TK_TRIDIAG_TAIL");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 64 || v == 96 || v == 128) ? v : 128;
+  }();
+  return t;
 }
 
 // Algorithmic work of the vector phase: CGS2 reads the earlier columns twice
@@ -1578,81 +1643,102 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   std::vector<int> order(jobs.size());
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].n > jobs[y].n; });
+  // column steps of job q on the multi-workgroup kernel: k < k0_q; the
+  // trailing block of size n - k0_q is finished by one workgroup
+  const int tail = tridiag_tail();
+  auto stop_of = [&](int n) { return std::max(0, n - tail); };
   std::vector<DevJob> dj;
   dj.reserve(jobs.size() * 2);
-  for (int o : order) dj.push_back({jobs[o].a_off, jobs[o].w_off, 0, 0, jobs[o].n, 0, o, 0});
+  for (int o : order)
+    dj.push_back({jobs[o].a_off, jobs[o].w_off, 0, 0, jobs[o].n, 0, o, stop_of(jobs[o].n)});
   // second copy in caller order for the bisection items
-  for (size_t o = 0; o < jobs.size(); ++o) dj.push_back({jobs[o].a_off, jobs[o].w_off, 0, 0, jobs[o].n, 0, int(o), 0});
+  for (size_t o = 0; o < jobs.size(); ++o)
+    dj.push_back({jobs[o].a_off, jobs[o].w_off, 0, 0, jobs[o].n, 0, int(o), stop_of(jobs[o].n)});
+  std::vector<TailJob> tj;
+  int mmax = 0;
+  for (int o : order) {
+    const int n = jobs[o].n, k0 = stop_of(n);
+    tj.push_back({jobs[o].a_off + int64_t(k0) * n + k0, jobs[o].w_off + k0, n - k0, n, k0 > 0 ? (k0 - 1) & 1 : -1, 0});
+    mmax = std::max(mmax, n - k0);
+  }
   std::vector<BisItem> items;
   constexpr int kBisNT = 256;
   for (size_t o = 0; o < jobs.size(); ++o)
     for (int j0 = 0; j0 < jobs[o].n; j0 += kBisNT / kBisLanes) items.push_back({int32_t(jobs.size() + o), j0});
   const size_t jb_bytes = dj.size() * sizeof(DevJob);
-  DBuf<char> meta(ctx_, jb_bytes + items.size() * sizeof(BisItem));
+  const size_t tj_bytes = tj.size() * sizeof(TailJob);
+  DBuf<char> meta(ctx_, jb_bytes + tj_bytes + items.size() * sizeof(BisItem));
   TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb_bytes, hipMemcpyHostToDevice, ctx_->stream));
-  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes, items.data(), items.size() * sizeof(BisItem), hipMemcpyHostToDevice,
-                        ctx_->stream));
+  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes, tj.data(), tj_bytes, hipMemcpyHostToDevice, ctx_->stream));
+  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes + tj_bytes, items.data(), items.size() * sizeof(BisItem),
+                        hipMemcpyHostToDevice, ctx_->stream));
   const DevJob* djd = reinterpret_cast<const DevJob*>(meta.get());
-  const BisItem* itd = reinterpret_cast<const BisItem*>(meta.get() + jb_bytes);
+  const TailJob* tjd = reinterpret_cast<const TailJob*>(meta.get() + jb_bytes);
+  const BisItem* itd = reinterpret_cast<const BisItem*>(meta.get() + jb_bytes + tj_bytes);
   const int nj = int(jobs.size());
-  DBuf<double> vp;
+  DBuf<double> vp(ctx_, size_t(4 * std::max<int64_t>(w_n, 1)));
   DBuf<TileItem> tiles;
-  if (nmax_ <= kSmallTridiag) {
-    launch_tridiag_t<512, 2>(ctx_, djd, nj, nmax_, A, D, E, TAU);
-  } else {
-    // multi-workgroup column steps for every job, then reflectors into rows
-    vp.reset(ctx_, size_t(4 * std::max<int64_t>(w_n, 1)));
-    std::vector<int> ns(jobs.size());
-    for (size_t o = 0; o < jobs.size(); ++o) ns[o] = jobs[order[o]].n;
-    // algorithmic traffic: every step reads and writes the trailing block
-    double bytes = 0;
-    int64_t launches = 0;
+  std::vector<int> ns(jobs.size()), k0s(jobs.size());
+  int kmax0 = 0;
+  for (size_t o = 0; o < jobs.size(); ++o) {
+    ns[o] = jobs[order[o]].n;
+    k0s[o] = stop_of(ns[o]);
+    kmax0 = std::max(kmax0, k0s[o]);
+  }
+  const int tk = ctx_->timer.begin(ctx_->stream);
+  double bytes = 0;
+  int64_t launches = 0;
+  if (kmax0 > 0) {
     static const int rows_override = [] {
       const char* e = std::getenv("TK_TRIDIAG_ROWS");
       return e ? std::atoi(e) : 0;
     }();
-    const int tk = ctx_->timer.begin(ctx_->stream);
-    for (int k = 0; k + 1 < nmax_; ++k) {
-      int64_t rows = 0;
-      for (int n : ns)
-        if (k + 2 < n) {
-          rows += n - k - 1;
-          bytes += 16.0 * double(n - k - 1) * double(n - k - 1);
+    for (int k = 0; k < kmax0; ++k) {
+      // algorithmic traffic: every step reads and writes the trailing block
+      int64_t rows = 0, grid4 = 0, grid8 = 0, grid16 = 0;
+      for (size_t o = 0; o < ns.size(); ++o)
+        if (k < k0s[o]) {
+          const int64_t r = ns[o] - k - 1;
+          rows += r;
+          bytes += 16.0 * double(r) * double(r);
+          grid4 += (r + 3) / 4;
+          grid8 += (r + 7) / 8;
+          grid16 += (r + 15) / 16;
         }
+      if (rows == 0) continue;
       // rows per workgroup: the largest of 16 / 8 / 4 that still gives >= 4
       // workgroups per CU (TK_TRIDIAG_ROWS overrides, for tuning)
       int r = rows >= int64_t(64) * ctx_->n_cu ? 16 : rows >= int64_t(32) * ctx_->n_cu ? 8 : 4;
       if (rows_override == 4 || rows_override == 8 || rows_override == 16) r = rows_override;
-      int64_t grid = 0;
-      for (int n : ns) {
-        if (n == 1) {
-          grid += k == 0;
-        } else if (k + 2 < n) {
-          grid += (n - k - 1 + r - 1) / r;
-        } else {
-          grid += k + 2 == n;
-        }
-      }
-      if (grid == 0) continue;
       const size_t lds = 3 * size_t(nmax_) * sizeof(double);
       if (r == 16)
-        hipLaunchKernelGGL(tdstep_kernel<16>, dim3(unsigned(grid)), dim3(64 * 16), lds, ctx_->stream, djd, nj, k, A,
-                           D, E, TAU, vp.get(), w_n, nmax_);
+        hipLaunchKernelGGL(tdstep_kernel<16>, dim3(unsigned(grid16)), dim3(64 * 16), lds, ctx_->stream, djd, nj, k,
+                           A, D, E, TAU, vp.get(), w_n, nmax_);
       else if (r == 8)
-        hipLaunchKernelGGL(tdstep_kernel<8>, dim3(unsigned(grid)), dim3(64 * 8), lds, ctx_->stream, djd, nj, k, A,
+        hipLaunchKernelGGL(tdstep_kernel<8>, dim3(unsigned(grid8)), dim3(64 * 8), lds, ctx_->stream, djd, nj, k, A,
                            D, E, TAU, vp.get(), w_n, nmax_);
       else
-        hipLaunchKernelGGL(tdstep_kernel<4>, dim3(unsigned(grid)), dim3(64 * 4), lds, ctx_->stream, djd, nj, k, A,
+        hipLaunchKernelGGL(tdstep_kernel<4>, dim3(unsigned(grid4)), dim3(64 * 4), lds, ctx_->stream, djd, nj, k, A,
                            D, E, TAU, vp.get(), w_n, nmax_);
       ++launches;
     }
-    ctx_->timer.end(tk, ctx_->stream, "tridiag_column_step", bytes, 0.0, launches);
+  }
+  ctx_->timer.end(tk, ctx_->stream, "tridiag_column_step", bytes, 0.0, launches);
+  // the trailing blocks (whole matrices below the threshold), one workgroup each
+  const int tt = ctx_->timer.begin(ctx_->stream);
+  double tail_bytes = 0;
+  for (const TailJob& t : tj) tail_bytes += 8.0 * double(t.m) * double(t.m) * double(t.m) / 3.0;
+  launch_tridiag_t<512, 2, true>(ctx_, tjd, nj, mmax, A, D, E, TAU, vp.get(), w_n);
+  TK_HIP(hipGetLastError());
+  ctx_->timer.end(tt, ctx_->stream, "tridiag_tail", tail_bytes, 0.0, 1);
+  if (kmax0 > 0) {
+    // reflectors of the multi-workgroup steps from columns into rows
     std::vector<TileItem> ti;
     for (size_t o = 0; o < jobs.size(); ++o) {
-      const int n = ns[o], nt = (n + 31) / 32;
-      if (n < 3) continue;
+      const int n = ns[o], nt = (n + 31) / 32, k0 = k0s[o];
+      if (k0 == 0) continue;
       for (int a2 = 0; a2 < nt; ++a2)
-        for (int b2 = 0; b2 <= a2; ++b2) ti.push_back({int32_t(o), a2, b2, 0});
+        for (int b2 = 0; b2 <= a2 && 32 * b2 < k0; ++b2) ti.push_back({int32_t(o), a2, b2, 0});
     }
     if (!ti.empty()) {
       tiles.reset(ctx_, ti.size());
