@@ -172,7 +172,7 @@ def roofline_of(timing):
     """Roofline of the kernel with the largest event time: algorithmic bytes
     per launch over the mean launch duration, against the HBM peak; traffic is
     the profiled HBM bytes per launch of that kernel when a summary exists."""
-    name = max(timing, key=lambda k: timing[k]["ms"])
+    name = max((k for k in timing if not k.startswith("phase_")), key=lambda k: timing[k]["ms"])
     t = timing[name]
     sec = t["ms"] * 1e-3
     achieved = t["bytes"] / sec / 1e9 if sec > 0 else 0.0

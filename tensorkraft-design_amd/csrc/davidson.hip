@@ -90,7 +90,30 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     hipLaunchKernelGGL(residual_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), w.y.get(), theta,
                        w.r.get(), n);
     dot_many(ctx, w.r.get(), n, 1, w.r.get(), n, rn2);
-    TK_HIP(hipMemcpyAsync(host, w.scal.get(), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    // the next correction vector is formed before the one host sync of the
+    // iteration (discarded when the iteration converges or switches precision)
+    const bool can_grow = nsub < n;
+    if (can_grow && nsub == m) {
+      // restart from the Ritz vector; A y is carried over
+      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.y.get(), V, n);
+      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), AV, n);
+      dot_many(ctx, V, n, 1, AV, n, H);
+      nsub = 1;
+    }
+    double* slot = V + int64_t(nsub) * n;
+    if (can_grow) {
+      if (opts.precondition && diag_dev)
+        hipLaunchKernelGGL(precond_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), diag_dev, theta, slot,
+                           n);
+      else
+        hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), slot, n);
+      for (int pass = 0; pass < 2; ++pass) {
+        dot_many(ctx, V, n, nsub, slot, n, w.h.get());
+        subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n);
+      }
+      dot_many(ctx, slot, n, 1, slot, n, tn2);
+    }
+    TK_HIP(hipMemcpyAsync(host, w.scal.get(), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     TK_HIP(hipStreamSynchronize(s));
     TK_HIP(hipGetLastError());
     res.iterations = it + 1;
@@ -114,27 +137,7 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
       res.converged = true;
       break;
     }
-    if (nsub >= n) break;  // the basis spans the space: exact
-    // correction vector in the next basis slot (or slot 1 after a restart)
-    if (nsub == m) {
-      // restart from the Ritz vector; A y is carried over
-      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.y.get(), V, n);
-      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), AV, n);
-      dot_many(ctx, V, n, 1, AV, n, H);
-      nsub = 1;
-    }
-    double* slot = V + int64_t(nsub) * n;
-    if (opts.precondition && diag_dev)
-      hipLaunchKernelGGL(precond_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), diag_dev, theta, slot, n);
-    else
-      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), slot, n);
-    for (int pass = 0; pass < 2; ++pass) {
-      dot_many(ctx, V, n, nsub, slot, n, w.h.get());
-      subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n);
-    }
-    dot_many(ctx, slot, n, 1, slot, n, tn2);
-    TK_HIP(hipMemcpyAsync(host + 2, tn2, sizeof(double), hipMemcpyDeviceToHost, s));
-    TK_HIP(hipStreamSynchronize(s));
+    if (!can_grow) break;  // the basis spans the space: exact
     if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
       // the preconditioned correction collapsed into the basis: raw residual
       hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), slot, n);

@@ -342,7 +342,9 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
     gb.push_back({x_off[k], int32_t(mids[k].n), int32_t(lr ? mids[k].ncol : mids[k].nrow)});
   GramSvd gs(ctx_);
   const double lam_min_rel = std::max(cutoff * cutoff, lambda_floor);
+  const int ph_values = ctx_->timer.begin(ctx_->stream);
   const std::vector<std::vector<double>> lam = gs.values(gb, X.get(), xtot, lam_min_rel, max_bond);
+  ctx_->timer.end(ph_values, ctx_->stream, "phase_svd_values", 0.0, 0.0, 1);
   for (size_t k = 0; k < mids.size(); ++k) mids[k].g_off = gs.u_offset(k);
   const int64_t gtot = gs.u_size();
   // global selection: sigma descending, keep while count < max_bond and
@@ -388,8 +390,11 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
   {
     std::vector<int64_t> keep_of;
     for (const Mid& m : mids) keep_of.push_back(m.keep);
+    const int ph_vec = ctx_->timer.begin(ctx_->stream);
     gs.vectors(keep_of);
+    ctx_->timer.end(ph_vec, ctx_->stream, "phase_svd_vectors", 0.0, 0.0, 1);
   }
+  const int ph_write = ctx_->timer.begin(ctx_->stream);
   const double* E = gs.u();
   // new bond
   std::vector<std::pair<QN, int64_t>> nb;
@@ -481,6 +486,7 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
     launch_copies(ctx_, copies, E, gtot, Aj.d.get(), Aj.t.total);
     run_list(ctx_, wl, psi.total, gtot, Ai.t.total, x, E, Ai.d.get());
   }
+  ctx_->timer.end(ph_write, ctx_->stream, "phase_svd_writeback", 0.0, 0.0, 1);
   bonds_[i + 1] = newB;
   site_[i] = std::move(Ai);
   site_[i + 1] = std::move(Aj);

@@ -89,6 +89,7 @@ std::vector<std::vector<double>> GramSvd::values(const std::vector<GramBlock>& b
   if (!(lmax > 0) || lambda_min_rel >= kTau1 || k1_total >= max_keep || !remainder) return out;
   // level two: X2 = X - U1 (U1^T X), Gram of X2
   refined_ = true;
+  const int ph = ctx_->timer.begin(ctx_->stream);
   eig1_.vectors(k1_, E1_.get(), gtot_);
   std::vector<char> sel(nb, 0);
   std::vector<int64_t> c_off(nb, 0);
@@ -143,6 +144,7 @@ std::vector<std::vector<double>> GramSvd::values(const std::vector<GramBlock>& b
       out[b].push_back(l);
     }
   }
+  ctx_->timer.end(ph, ctx_->stream, "phase_svd_level2", 0.0, 0.0, 1);
   return out;
 }
 
@@ -184,7 +186,7 @@ void GramSvd::vectors(const std::vector<int64_t>& keep) {
     if (k1_[b] > 0) from1.push_back({g_off_[b], z_off[b], n, int32_t(k1_[b]), n, kk, 0, 0});
     from2.push_back({g_off_[b], z_off[b] + k1_[b], n, int32_t(k2), n, kk, 0, 0});
     back.push_back({z_off[b], g_off_[b], n, kk, kk, n, 0, 0});
-    oj.push_back({z_off[b], n, kk});
+    oj.push_back({z_off[b], n, kk, int32_t(k1_[b])});  // U1 columns are orthonormal already
   }
   launch_copies(ctx_, from1, E1_.get(), gtot_, Z.get(), ztot);
   launch_copies(ctx_, from2, E2_.get(), gtot_, Z.get(), ztot);

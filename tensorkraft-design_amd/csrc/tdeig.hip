@@ -832,7 +832,8 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const DevJob* __restrict__
 
 // ---------------------------------------------------------------------------
 // 3b. Orthonormalisation of the columns [c_begin, c_begin + c_count) of the
-// n x k row-major Z (ld k) of each job, in column order, the columns before
+// n x k row-major Z (ld k) of each job (c_begin = the job's first column,
+// DevJob::pad, + step * c_count), in column order, the columns before
 // c_begin being orthonormal and already projected out: each block of NB
 // columns is projected against the earlier columns of the range (two passes,
 // 64 columns at a time), then orthonormalised within the block (two passes
@@ -841,11 +842,12 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const DevJob* __restrict__
 // it restarts from a hashed vector projected against every earlier column.
 template <int NT, int NB>
 __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ jobs, double* __restrict__ Z,
-                                                  const double* __restrict__ LU, int c_begin, int c_count,
+                                                  const double* __restrict__ LU, int step, int c_count,
                                                   const double* __restrict__ norms) {
   constexpr int NW = NT / 64;
   const DevJob jb = jobs[blockIdx.x];
   const int n = jb.n, k = jb.k;
+  const int c_begin = jb.pad + step * c_count;  // DevJob::pad: the first column to process
   if (k <= c_begin) return;
   const int c_end = min(k, c_begin + c_count);
   double* z = Z + jb.z_off;
@@ -1543,12 +1545,12 @@ int orth_block() {
 }
 
 template <int NB>
-void launch_orth_range(Context* ctx, int nmax, int nj, const DevJob* djd, double* Z, const double* lu, int c_begin,
+void launch_orth_range(Context* ctx, int nmax, int nj, const DevJob* djd, double* Z, const double* lu, int step,
                        int c_count, const double* norms) {
   constexpr int kNT = 1024, NW = kNT / 64;
   const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
   allow_lds(orth_kernel<kNT, NB>, lds);
-  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(nj), dim3(kNT), lds, ctx->stream, djd, Z, lu, c_begin, c_count,
+  hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(nj), dim3(kNT), lds, ctx->stream, djd, Z, lu, step, c_count,
                      norms);
   TK_HIP(hipGetLastError());
 }
@@ -1572,47 +1574,51 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
   hipLaunchKernelGGL(colnorm_kernel, dim3(unsigned((kmax + 255) / 256), unsigned(nj)), dim3(256), 0, ctx->stream,
                      djd, Z, norms.get());
   TK_HIP(hipGetLastError());
+  // block `step` of job j covers columns [first_j + step * kOrthBlock, ...)
+  int steps = 0;
+  for (const DevJob& j : dj) steps = std::max(steps, (j.k - j.pad + kOrthBlock - 1) / kOrthBlock);
   int64_t h_max = 1;
-  for (int c0 = kOrthBlock; c0 < kmax; c0 += kOrthBlock) {
+  for (int st = 0; st < steps; ++st) {
     int64_t h = 0;
-    for (const DevJob& j : dj)
-      if (j.k > c0) h += int64_t((j.n + kOrthSeg - 1) / kOrthSeg) * c0 * std::min(kOrthBlock, j.k - c0);
+    for (const DevJob& j : dj) {
+      const int c0 = j.pad + st * kOrthBlock;
+      if (c0 > 0 && c0 < j.k) h += int64_t((j.n + kOrthSeg - 1) / kOrthSeg) * c0 * std::min(kOrthBlock, j.k - c0);
+    }
     h_max = std::max(h_max, h);
   }
   DBuf<double> H(ctx, size_t(h_max));
   std::vector<std::unique_ptr<DeviceContractList>> keep;
-  for (int c0 = 0; c0 < kmax; c0 += kOrthBlock) {
-    if (c0 > 0) {
-      for (int pass = 0; pass < 2; ++pass) {
-        ContractList lh, lu2;
-        lh.flags = kTransA;
-        lu2.flags = kTransA | kTransB | kTransC;
-        lu2.beta = 1.0;
-        int64_t h = 0;
-        for (const DevJob& j : dj) {
-          if (j.k <= c0) continue;
-          const int nb = std::min(kOrthBlock, j.k - c0);
-          // Z_b^T (nb x n) -= (sum_seg H_seg)^T Z[:, 0:c0]^T
-          lu2.begin_out(j.z_off + c0, nb, j.n, j.k);
-          lu2.begin_term(j.z_off, j.k, c0);
-          for (int r0 = 0; r0 < j.n; r0 += kOrthSeg) {
-            const int r1 = std::min(j.n, r0 + kOrthSeg);
-            // H_seg (c0 x nb) = Z[r0:r1, 0:c0]^T Z[r0:r1, c0:c0+nb]
-            lh.begin_out(h, c0, nb, nb);
-            lh.begin_term(j.z_off + int64_t(r0) * j.k + c0, j.k, r1 - r0);
-            lh.add_src(j.z_off + int64_t(r0) * j.k, j.k, 1.0);
-            lu2.add_src(h, nb, -1.0);
-            h += int64_t(c0) * nb;
-          }
+  for (int st = 0; st < steps; ++st) {
+    for (int pass = 0; pass < 2; ++pass) {
+      ContractList lh, lu2;
+      lh.flags = kTransA;
+      lu2.flags = kTransA | kTransB | kTransC;
+      lu2.beta = 1.0;
+      int64_t h = 0;
+      for (const DevJob& j : dj) {
+        const int c0 = j.pad + st * kOrthBlock;
+        if (c0 <= 0 || c0 >= j.k) continue;
+        const int nb = std::min(kOrthBlock, j.k - c0);
+        // Z_b^T (nb x n) -= (sum_seg H_seg)^T Z[:, 0:c0]^T
+        lu2.begin_out(j.z_off + c0, nb, j.n, j.k);
+        lu2.begin_term(j.z_off, j.k, c0);
+        for (int r0 = 0; r0 < j.n; r0 += kOrthSeg) {
+          const int r1 = std::min(j.n, r0 + kOrthSeg);
+          // H_seg (c0 x nb) = Z[r0:r1, 0:c0]^T Z[r0:r1, c0:c0+nb]
+          lh.begin_out(h, c0, nb, nb);
+          lh.begin_term(j.z_off + int64_t(r0) * j.k + c0, j.k, r1 - r0);
+          lh.add_src(j.z_off + int64_t(r0) * j.k, j.k, 1.0);
+          lu2.add_src(h, nb, -1.0);
+          h += int64_t(c0) * nb;
         }
-        run_list(ctx, lh, z_n, z_n, h_max, Z, Z, H.get(), keep);
-        run_list(ctx, lu2, h_max, z_n, z_n, H.get(), Z, Z, keep);
       }
+      run_list(ctx, lh, z_n, z_n, h_max, Z, Z, H.get(), keep);
+      run_list(ctx, lu2, h_max, z_n, z_n, H.get(), Z, Z, keep);
     }
     if (nmax <= 1024)
-      launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, c0, kOrthBlock, norms.get());
+      launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get());
     else
-      launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, c0, kOrthBlock, norms.get());
+      launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get());
   }
   ctx->sync();  // the lists' tables and the scratch buffers are released on return
 }
@@ -1821,10 +1827,11 @@ void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, doub
   std::vector<DevJob> dj;
   int64_t w = 0;
   for (const OrthJob& j : jobs) {
-    if (j.n < 1 || j.n > kSymEigMaxN || j.k < 0 || j.k > j.n) throw ArgError("orthonormalize: bad shape");
+    if (j.n < 1 || j.n > kSymEigMaxN || j.k < 0 || j.k > j.n || j.first < 0 || j.first > j.k)
+      throw ArgError("orthonormalize: bad shape");
     if (j.off < 0 || j.off + int64_t(j.n) * j.k > z_n) throw std::out_of_range("orthonormalize: job outside Z");
-    if (j.k == 0) continue;
-    dj.push_back({0, w, j.off, 0, j.n, j.k, int32_t(dj.size()), 0});
+    if (j.k == j.first) continue;
+    dj.push_back({0, w, j.off, 0, j.n, j.k, int32_t(dj.size()), j.first});
     w += j.k;
   }
   if (dj.empty()) return;

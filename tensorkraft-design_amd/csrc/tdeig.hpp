@@ -59,10 +59,12 @@ class SymEigBatch {
 
 // Orthonormalise the k columns of each n x k row-major matrix (leading
 // dimension k) at off of Z, in column order (classical Gram-Schmidt applied
-// twice, blocked), in place.
+// twice, blocked), in place.  Columns before `first` are orthonormal already
+// and stay as they are.
 struct OrthJob {
   int64_t off;
   int32_t n, k;
+  int32_t first = 0;
 };
 void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, double* Z, int64_t z_n);
 
