@@ -1712,9 +1712,9 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
           grid16 += (r + 15) / 16;
         }
       if (rows == 0) continue;
-      // rows per workgroup: the largest of 16 / 8 / 4 that still gives >= 4
-      // workgroups per CU (TK_TRIDIAG_ROWS overrides, for tuning)
-      int r = rows >= int64_t(64) * ctx_->n_cu ? 16 : rows >= int64_t(32) * ctx_->n_cu ? 8 : 4;
+      // rows per workgroup: 8 (measured best on the D=2048 sweep against 4
+      // and 16; TK_TRIDIAG_ROWS overrides, for tuning)
+      int r = 8;
       if (rows_override == 4 || rows_override == 8 || rows_override == 16) r = rows_override;
       const size_t lds = 3 * size_t(nmax_) * sizeof(double);
       if (r == 16)
