@@ -9,6 +9,10 @@ Workloads
   heff   one H_eff*psi on the U(1)xU(1) Hubbard microbenchmark bond
          (d=4, MPO bond 6, D=1024 on both sides, 145 sectors); a step is one
          matvec with inputs resident in HBM.
+  heff-c4  one H_eff*psi at the centre bond of the J1 Heisenberg 6 x 12
+         cylinder (snake order, MPO bond 20, U(1) D=4096 in 31 sectors);
+         FP64, FP32 and split-BF16 plans compared against the FP64 output.
+  copy   calibration copy of known size (for the HBM counters).
 
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 is refused: this build
 runs on one GPU.
@@ -31,10 +35,11 @@ def cpu_baseline_heff(p, budget_s=10.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    oracle.heff_apply(p, p["x"])  # warm
+    wi, wj = p["W"], p.get("Wj", p["W"])
+    oracle.heff_apply(p, p["x"], Wi=wi, Wj=wj)  # warm
     n, t0 = 0, time.perf_counter()
     while True:
-        oracle.heff_apply(p, p["x"])
+        oracle.heff_apply(p, p["x"], Wi=wi, Wj=wj)
         n += 1
         el = time.perf_counter() - t0
         if el >= budget_s or n >= 500:
@@ -44,17 +49,54 @@ def cpu_baseline_heff(p, budget_s=10.0):
             "sample": f"{n} FP64 H_eff*psi matvecs of the same workload on {cores} OpenMP threads"}
 
 
+HEFF_WORKLOADS = {
+    "heff": ("heff_c2", "H_eff*psi time, Hubbard U(1)xU(1) d=4 w=6 D=1024 (single matvec)"),
+    "heff-c4": ("heff_c4_cylinder_6x12_D4096",
+                "H_eff*psi time, J1 Heisenberg 6x12 cylinder U(1) d=2 w=20 D=4096 (single matvec)"),
+}
+
+
+def heff_problem_for(tk_hip, workload):
+    if workload == "heff-c4":
+        p = tk_hip.c4_problem(seed=4096)
+        desc = {"bond_dim": 4096, "sectors": len(p["bond_l"]), "mpo_bond_dim": int(p["W"].w.shape[3]),
+                "lattice": "cylinder 6 x 12, snake order", "bond": [p["site"], p["site"] + 1]}
+    else:
+        p = tk_hip.c2_problem(seed=2024)
+        desc = {"bond_dim": 1024, "sectors": len(p["bond_l"]), "mpo_bond_dim": 6}
+    return p, desc
+
+
 def bench_heff(args):
+    """One H_eff*psi per step at the chosen precision; every precision's plan
+    is also timed (HIP events) and compared with the FP64 plan's output."""
     import numpy as np
 
     import tk_hip
 
     ctx = tk_hip.Context(0)
-    p = tk_hip.c2_problem(seed=2024)
+    p, desc = heff_problem_for(tk_hip, args.workload)
     L, R, x = ctx.to_device(p["L"]), ctx.to_device(p["R"]), ctx.to_device(p["x"])
     y = ctx.empty(len(p["x"]))
-    W = p["W"]
-    plan = tk_hip.HeffPlan(ctx, args.prec, p["sym"], p["bond_l"], W.wl, L, p["bond_r"], W.wr, R, W, W)
+    Wi, Wj = p["W"], p.get("Wj", p["W"])
+
+    def make(prec):
+        return tk_hip.HeffPlan(ctx, prec, p["sym"], p["bond_l"], Wi.wl, L, p["bond_r"], Wj.wr, R, Wi, Wj)
+
+    compare, y64 = {}, None
+    for prec in ["fp64"] + [q for q in ("fp32", "bf16x3") if q != "fp64"]:
+        pl = make(prec)
+        pl.apply(x, y)
+        ctx.sync()
+        yy = y.get()
+        if prec == "fp64":
+            y64 = yy
+        us_a, _, _ = pl.time(x, y, 20)
+        compare[prec] = {"us": round(us_a, 3),
+                         "tflops": round(pl.stats()["flops"] / (us_a * 1e-6) / 1e12, 3),
+                         "rel_err_vs_fp64": float(np.linalg.norm(yy - y64) / np.linalg.norm(y64))}
+        pl.close()
+    plan = make(args.prec)
     st = plan.stats()
     ss = plan.stage_stats()
     for _ in range(args.warmup):
@@ -71,9 +113,22 @@ def bench_heff(args):
     b = ss["s1_bytes"] if dom == "stage1" else ss["s3_bytes"]
     f = ss["s1_flops"] if dom == "stage1" else ss["s3_flops"]
     t_dom = (us1 if dom == "stage1" else us3) * 1e-6
-    achieved = b / t_dom / 1e9
+    intensity = f / b if b else 0.0
+    ridge = MFMA_PEAK_TFLOPS[args.prec] * 1e12 / (HBM_PEAK_GBS * 1e9)
+    if intensity > ridge:
+        achieved = f / t_dom / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": MFMA_PEAK_TFLOPS[args.prec],
+                "unit": "TFLOP/s", "frac": round(achieved / MFMA_PEAK_TFLOPS[args.prec], 4)}
+    else:
+        achieved = b / t_dom / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    roof.update({"traffic": None, "kernel": dom, "kernel_us": round(t_dom * 1e6, 3), "kernel_bytes": b,
+                 "kernel_flops": f, "kernel_tflops": round(f / t_dom / 1e12, 3),
+                 "kernel_GBs": round(b / t_dom / 1e9, 1), "flops_per_byte": round(intensity, 2)})
+    workload, metric = HEFF_WORKLOADS[args.workload]
     rec = {
-        "metric": "H_eff*psi time, Hubbard U(1)xU(1) d=4 w=6 D=1024 (single matvec)",
+        "metric": metric,
         "value": round(1e3 * ms, 3),
         "unit": "us/matvec",
         "n_gpus": 1,
@@ -85,21 +140,47 @@ def bench_heff(args):
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3-f32acc"}[args.prec],
         "data": "synthetic",
-        "config": {"workload": "heff_c2", "bond_dim": 1024, "sectors": 145, "psi_blocks": st["psi_blocks"],
-                   "dim": st["dim"], "precision": args.prec},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                     "kernel_us": round(t_dom * 1e6, 3), "kernel_bytes": b,
-                     "kernel_tflops": round(f / t_dom / 1e12, 3)},
+        "config": dict({"workload": workload, "psi_blocks": st["psi_blocks"], "dim": st["dim"],
+                        "precision": args.prec}, **desc),
+        "roofline": roof,
         "event_us": {"apply": round(us_apply, 3), "stage1": round(us1, 3), "stage3": round(us3, 3)},
         "work": {"flops": st["flops"], "bytes": st["bytes"], "stage1_items": st["stage1_items"],
                  "stage3_items": st["stage3_items"]},
+        "precisions": compare,
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline_heff(p, args.cpu_seconds)
     plan.close()
     ctx.close()
     return rec
+
+
+def bench_copy(args):
+    """Calibration copy of known size through the library's strided copy
+    kernel: block_fuse of one 4096 x 4096 FP64 block (128 MiB read, 128 MiB
+    written per call).  Counter readings of this run scale the HBM counters of
+    the other kernels."""
+    import numpy as np
+
+    import tk_hip
+
+    ctx = tk_hip.Context(0)
+    legs = [[(0, 4096)], [(0, 4096)]]
+    dirs = [tk_hip.DIR_OUT, tk_hip.DIR_IN]
+    x = np.arange(4096 * 4096, dtype=np.float64)
+    t = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        yv = tk_hip.block_fuse(ctx, tk_hip.SYM_U1, legs, dirs, (0, 0), 0, 2, x)
+        t.append(time.perf_counter() - t0)
+    if not np.array_equal(yv, x):
+        raise SystemExit("bench.py: calibration copy returned wrong data")
+    ctx.close()
+    return {"metric": "calibration copy (block_fuse of one 4096 x 4096 FP64 block, host round trip)",
+            "value": round(1e3 * min(t), 3), "unit": "ms/call", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+            "ms_per_step": round(1e3 * min(t), 3), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "copy_4096x4096_f64", "bytes_read": 8 * x.size, "bytes_written": 8 * x.size}}
 
 
 def warmup_schedule(d0, d_max, n):
@@ -302,7 +383,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "heff"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "heff", "heff-c4", "copy"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--sites", type=int, default=64)
@@ -316,7 +397,10 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "heff":
+    if args.workload == "copy":
+        args.steps = 3 if args.steps is None else args.steps
+        rec = bench_copy(args)
+    elif args.workload in HEFF_WORKLOADS:
         args.steps = 200 if args.steps is None else args.steps
         args.warmup = 20 if args.warmup is None else args.warmup
         rec = bench_heff(args)

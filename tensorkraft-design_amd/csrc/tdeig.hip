@@ -360,25 +360,48 @@ __device__ __forceinline__ int step_count(int n, int k, int k0) {
   return k < k0 ? (n - k - 1 + R - 1) / R : 0;
 }
 
-template <int R>
+// MaxN bounds the trailing size n - k of every job in the launch (the
+// register arrays below hold one row and the step's vectors).
+template <int R, int MaxN>
 __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict__ jobs, int n_jobs, int k,
                                                         double* __restrict__ A, double* __restrict__ Dg,
                                                         double* __restrict__ Eg, double* __restrict__ TAUg,
                                                         double* __restrict__ VP, int64_t w_n, int ldsn) {
   constexpr int kStepThreads = 64 * R;
   constexpr int kStepRows = R;
-  constexpr int kPerThread = kSymEigMaxN / kStepThreads;  // row-k / vector entries per thread
-  constexpr int kPerLane = kSymEigMaxN / 64;              // own-row entries per lane
+  constexpr int kPerThread = (MaxN + kStepThreads - 1) / kStepThreads;  // row-k / vector entries per thread
+  constexpr int kPerLane = MaxN / 64;                                     // own-row entries per lane
   extern __shared__ double sm[];
   double* vprev = sm;
   double* wprev = sm + ldsn;
   double* vcur = sm + 2 * ldsn;
   __shared__ double red[2 * kStepRows];
-  int id = blockIdx.x, q = 0;
-  for (; q < n_jobs; ++q) {
-    const int cnt = step_count<R>(jobs[q].n, k, jobs[q].pad);
-    if (id < cnt) break;
-    id -= cnt;
+  // this workgroup's job: every wavefront scans the job list 64 entries at a
+  // time (one load per lane, an inclusive prefix sum of the per-job
+  // workgroup counts, a ballot), instead of one dependent load per job
+  int id = int(blockIdx.x), q = n_jobs;
+  {
+    const int ln = int(threadIdx.x & 63);
+    int base = 0;
+    for (int q0 = 0; q0 < n_jobs; q0 += 64) {
+      const int qq = q0 + ln;
+      const int cnt = qq < n_jobs ? step_count<R>(jobs[qq].n, k, jobs[qq].pad) : 0;
+      int incl = cnt;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (ln >= o) incl += t;
+      }
+      const int total = __shfl(incl, 63, 64);
+      if (id < base + total) {
+        const unsigned long long hit = __ballot(base + incl > id);
+        const int l = __ffsll(static_cast<long long>(hit)) - 1;
+        q = q0 + l;
+        id -= base + __shfl(incl - cnt, l, 64);
+        break;
+      }
+      base += total;
+    }
   }
   if (q >= n_jobs) return;
   const DevJob jb = jobs[q];
@@ -517,6 +540,35 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
       if (i >= k + 2) row[k] = vcur[i];
     }
   }
+}
+
+template <int R, int MaxN>
+void launch_tdstep_t(dim3 grid, size_t lds, hipStream_t st, const DevJob* jobs, int nj, int k, double* A, double* D,
+                     double* E, double* TAU, double* VP, int64_t w_n, int ldsn) {
+  hipLaunchKernelGGL((tdstep_kernel<R, MaxN>), grid, dim3(64 * R), lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+}
+
+template <int R>
+void launch_tdstep_r(int cap, dim3 grid, size_t lds, hipStream_t st, const DevJob* jobs, int nj, int k, double* A,
+                     double* D, double* E, double* TAU, double* VP, int64_t w_n, int ldsn) {
+  if (cap <= 512)
+    launch_tdstep_t<R, 512>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+  else if (cap <= 1024)
+    launch_tdstep_t<R, 1024>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+  else
+    launch_tdstep_t<R, kSymEigMaxN>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+}
+
+// one column step over every active job: R rows per workgroup, registers
+// sized for trailing blocks of at most `cap` (512, 1024 or 2048)
+void launch_tdstep(int r, int cap, dim3 grid, size_t lds, hipStream_t st, const DevJob* jobs, int nj, int k,
+                   double* A, double* D, double* E, double* TAU, double* VP, int64_t w_n, int ldsn) {
+  if (r == 16)
+    launch_tdstep_r<16>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+  else if (r == 4)
+    launch_tdstep_r<4>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+  else
+    launch_tdstep_r<8>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
 }
 
 // Reflector k < k0 (DevJob::pad) from column k (rows >= k+2) into row k
@@ -1702,6 +1754,7 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
     for (int k = 0; k < kmax0; ++k) {
       // algorithmic traffic: every step reads and writes the trailing block
       int64_t rows = 0, grid4 = 0, grid8 = 0, grid16 = 0;
+      int trail = 0;  // the largest n - k among the active jobs
       for (size_t o = 0; o < ns.size(); ++o)
         if (k < k0s[o]) {
           const int64_t r = ns[o] - k - 1;
@@ -1710,6 +1763,7 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
           grid4 += (r + 3) / 4;
           grid8 += (r + 7) / 8;
           grid16 += (r + 15) / 16;
+          trail = std::max(trail, int(r) + 1);
         }
       if (rows == 0) continue;
       // rows per workgroup: 8 (measured best on the D=2048 sweep against 4
@@ -1717,15 +1771,11 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
       int r = 8;
       if (rows_override == 4 || rows_override == 8 || rows_override == 16) r = rows_override;
       const size_t lds = 3 * size_t(nmax_) * sizeof(double);
-      if (r == 16)
-        hipLaunchKernelGGL(tdstep_kernel<16>, dim3(unsigned(grid16)), dim3(64 * 16), lds, ctx_->stream, djd, nj, k,
-                           A, D, E, TAU, vp.get(), w_n, nmax_);
-      else if (r == 8)
-        hipLaunchKernelGGL(tdstep_kernel<8>, dim3(unsigned(grid8)), dim3(64 * 8), lds, ctx_->stream, djd, nj, k, A,
-                           D, E, TAU, vp.get(), w_n, nmax_);
-      else
-        hipLaunchKernelGGL(tdstep_kernel<4>, dim3(unsigned(grid4)), dim3(64 * 4), lds, ctx_->stream, djd, nj, k, A,
-                           D, E, TAU, vp.get(), w_n, nmax_);
+      const int64_t grid = r == 16 ? grid16 : r == 8 ? grid8 : grid4;
+      // the smallest register capacity that holds the trailing size
+      const int cap = trail <= 512 ? 512 : trail <= 1024 ? 1024 : 2048;
+      launch_tdstep(r, cap, dim3(unsigned(grid)), lds, ctx_->stream, djd, nj, k, A, D, E, TAU, vp.get(), w_n,
+                    nmax_);
       ++launches;
     }
   }

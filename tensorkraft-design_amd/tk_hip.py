@@ -652,6 +652,136 @@ def heisenberg_w(J=1.0, position="bulk"):
     return _assemble(ops, 2, 5, SPIN_PHYS, wq, position, start=0, done=1)
 
 
+def snake_path(lx, ly):
+    """Boustrophedon order of an lx x ly grid (tk-dsl/src/lattice/square.rs:64-80):
+    row y = 0, 1, ... of lx sites each (site y * lx + x), even rows left to
+    right, odd rows right to left."""
+    out = []
+    for y in range(ly):
+        xs = range(lx) if y % 2 == 0 else range(lx - 1, -1, -1)
+        out += [y * lx + x for x in xs]
+    return out
+
+
+def cylinder_bonds(width, length):
+    """Nearest-neighbour bonds of a width x length cylinder, as chain positions
+    of the snake order: the open square lattice's bonds
+    (tk-dsl/src/lattice/square.rs:16-33, rows of `width` sites, `length` rows)
+    plus the wrap bond (0, width - 1) of every row when width > 2."""
+    pos = {site: k for k, site in enumerate(snake_path(width, length))}
+    bonds = []
+    for y in range(length):
+        for x in range(width):
+            s = y * width + x
+            if x + 1 < width:
+                bonds.append((s, s + 1))
+            if y + 1 < length:
+                bonds.append((s, s + width))
+        if width > 2:
+            bonds.append((y * width, y * width + width - 1))
+    return sorted(tuple(sorted((pos[a], pos[b]))) for a, b in bonds)
+
+
+def bond_heisenberg_mpo(n, bonds, J=1.0):
+    """Exact finite-state-automaton MPO of H = J sum_{(i,j) in bonds} S_i.S_j
+    on n chain positions, U(1) 2*Sz.  The states on the cut after site c are
+    start, done and, for every source i <= c with a partner j > c (ascending
+    i), "Sz_i / S+_i / S-_i placed": bond dim 2 + 3 * (open sources).
+    Caller order on each cut: start, done, Sz_i... (charge 0); S+_i... (-2);
+    S-_i... (+2).  Returns one MpoSite per site (first: left row `start`,
+    last: right column `done`)."""
+    partners = [set() for _ in range(n)]
+    for i, j in bonds:
+        if not (0 <= i < j < n):
+            raise ValueError(f"bond {(i, j)} is not an ordered pair of positions below {n}")
+        partners[i].add(j)
+    # sources open on the cut after site c
+    cuts = [[i for i in range(c + 1) if any(j > c for j in partners[i])] for c in range(n)]
+    cuts = [[]] + cuts  # the cut before site 0
+
+    def states(src):
+        k = len(src)
+        idx = {("start",): 0, ("done",): 1}
+        for t, i in enumerate(src):
+            idx[("z", i)] = 2 + t
+            idx[("p", i)] = 2 + k + t
+            idx[("m", i)] = 2 + 2 * k + t
+        q = [((0, 0), 2 + k)] + ([((-2, 0), k), ((2, 0), k)] if k else [])
+        return idx, 2 + 3 * k, q
+
+    out = []
+    for s in range(n):
+        li, lw, lq = states(cuts[s])
+        ri, rw, rq = states(cuts[s + 1])
+        ops = {(0, 0): S_ID, (1, 1): S_ID}
+        for i in cuts[s]:
+            if i in cuts[s + 1]:
+                for t in "zpm":
+                    ops[(li[(t, i)], ri[(t, i)])] = S_ID
+            if s in partners[i]:
+                ops[(li[("z", i)], 1)] = J * S_Z
+                ops[(li[("p", i)], 1)] = 0.5 * J * S_MINUS
+                ops[(li[("m", i)], 1)] = 0.5 * J * S_PLUS
+        if s in cuts[s + 1]:
+            ops[(0, ri[("z", s)])] = S_Z
+            ops[(0, ri[("p", s)])] = S_PLUS
+            ops[(0, ri[("m", s)])] = S_MINUS
+        full = np.zeros((2, 2, lw, rw))
+        for (a, b), O in ops.items():
+            full[:, :, a, b] = O.T
+        if n == 1:
+            out.append(MpoSite(SPIN_PHYS, [((0, 0), 1)], [((0, 0), 1)], full[:, :, 0:1, 1:2].copy()))
+        elif s == 0:
+            out.append(MpoSite(SPIN_PHYS, [((0, 0), 1)], rq, full[:, :, 0:1, :].copy()))
+        elif s == n - 1:
+            out.append(MpoSite(SPIN_PHYS, lq, [((0, 0), 1)], full[:, :, :, 1:2].copy()))
+        else:
+            out.append(MpoSite(SPIN_PHYS, lq, rq, full))
+    return out
+
+
+def cylinder_mpo(width=6, length=12, J=1.0):
+    """J1 Heisenberg on a width x length cylinder in snake order (72 sites and
+    MPO bond dims up to 23 for 6 x 12)."""
+    return bond_heisenberg_mpo(width * length, cylinder_bonds(width, length), J)
+
+
+def synthetic_u1_bond(total=4096, qmax=30, width=8.0):
+    """U(1) (2*Sz) bond of the cylinder microbenchmark: even charges
+    -qmax..qmax, dims proportional to exp(-q^2/(2 width^2)), every sector at
+    least 1, rounded to `total` by largest remainder."""
+    qs = list(range(-qmax, qmax + 1, 2))
+    wts = np.array([math.exp(-q * q / (2 * width * width)) for q in qs])
+    raw = wts / wts.sum() * (total - len(qs))
+    dims = np.floor(raw).astype(int) + 1
+    rem = total - dims.sum()
+    order = np.argsort(-(raw - np.floor(raw)), kind="stable")
+    for i in order[:rem]:
+        dims[i] += 1
+    return [((q, 0), int(dd)) for q, dd in zip(qs, dims)]
+
+
+def c4_problem(seed=4096, total=4096, width=6, length=12, site=None):
+    """Inputs of the cylinder H_eff*psi microbenchmark: J1 Heisenberg on the
+    width x length cylinder (snake order), the two-site bond (site, site + 1)
+    (default: the centre), U(1) bonds of `total` states (31 sectors) on both
+    sides, L, R and psi N(0,1) from PCG64(seed) in FlatBlockStorage order
+    (L, then R, then psi)."""
+    mpo = cylinder_mpo(width, length)
+    site = len(mpo) // 2 - 1 if site is None else site
+    Wi, Wj = mpo[site], mpo[site + 1]
+    bl = synthetic_u1_bond(total)
+    br = list(bl)
+    _, _, _, l_tot = env_table(SYM_U1, bl, Wi.wl)
+    _, _, _, r_tot = env_table(SYM_U1, br, Wj.wr)
+    _, _, _, p_tot = block_table(SYM_U1, [Wi.phys, Wj.phys, bl, br], [DIR_OUT, DIR_OUT, DIR_OUT, DIR_IN])
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal(l_tot)
+    R = rng.standard_normal(r_tot)
+    x = rng.standard_normal(p_tot)
+    return {"bond_l": bl, "bond_r": br, "W": Wi, "Wj": Wj, "L": L, "R": R, "x": x, "sym": SYM_U1, "site": site}
+
+
 def _assemble(ops, d, w, phys, wq, position, start, done):
     full = np.zeros((d, d, w, w))
     for (a, b), O in ops.items():

@@ -66,3 +66,27 @@ def free_fermion_energy(n, n_up, n_dn, t=1.0):
     """U = 0 open chain: single-particle levels -2t cos(k pi / (n+1)), k = 1..n."""
     lev = sorted(-2 * t * math.cos(k * math.pi / (n + 1)) for k in range(1, n + 1))
     return sum(lev[:n_up]) + sum(lev[:n_dn])
+
+
+def heisenberg_bonds_ground_energy(n, bonds, J=1.0, twice_sz=0):
+    """H = J sum_{(i,j) in bonds} S_i . S_j on n spins, sector 2*Sz = twice_sz."""
+    n_up = (n + twice_sz) // 2
+    states = [s for s in range(1 << n) if bin(s).count("1") == n_up]
+    index = {s: k for k, s in enumerate(states)}
+    rows, cols, vals = [], [], []
+    for k, s in enumerate(states):
+        diag = 0.0
+        for i, j in bonds:
+            a, b = (s >> i) & 1, (s >> j) & 1
+            diag += J * (0.25 if a == b else -0.25)
+            if a != b:
+                rows.append(index[s ^ (1 << i) ^ (1 << j)])
+                cols.append(k)
+                vals.append(0.5 * J)
+        rows.append(k)
+        cols.append(k)
+        vals.append(diag)
+    H = sp.csr_matrix((vals, (rows, cols)), shape=(len(states), len(states)))
+    if len(states) <= 400:
+        return float(np.linalg.eigvalsh(H.toarray())[0])
+    return float(spl.eigsh(H, k=1, which="SA", tol=1e-14)[0][0])

@@ -10,7 +10,7 @@ import pytest
 
 import oracle
 import tk_hip
-from ed import free_fermion_energy, heisenberg_ground_energy, hubbard_ground_energy
+from ed import free_fermion_energy, heisenberg_bonds_ground_energy, heisenberg_ground_energy, hubbard_ground_energy
 
 pytestmark = pytest.mark.gpu
 
@@ -106,3 +106,29 @@ def test_gpu_dmrg_mixed_precision_matches_fp64(ctx, prec):
     dlo, slo = _gpu(ctx, mpo, sym, bonds, data, (24, 48), 3, prec=prec, tol=1e-10)
     assert abs(slo[-1]["energy"] - s64[-1]["energy"]) <= 1e-10
     assert slo[-1]["matvecs_fp64"] < slo[-1]["matvecs"]
+
+
+def test_gpu_dmrg_cylinder_matches_ed(ctx):
+    """J1 Heisenberg on a 4 x 3 cylinder in snake order (MPO bond dims up to 14)."""
+    print("case dmrg cylinder 4x3", flush=True)
+    width, length = 4, 3
+    mpo = tk_hip.cylinder_mpo(width, length)
+    sym = tk_hip.SYM_U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (0, 0), 16, 0)
+    d, st = _gpu(ctx, mpo, sym, bonds, data, (16, 64), 5)
+    e_ed = heisenberg_bonds_ground_energy(width * length, tk_hip.cylinder_bonds(width, length))
+    assert abs(st[-1]["energy"] - e_ed) < 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_gpu_dmrg_cylinder_mixed_matches_fp64(ctx, prec):
+    """6 x 3 cylinder (MPO bond dim 20) truncated at D = 64: the mixed path
+    ends within 1e-10 of the FP64 path with the same truncation."""
+    print(f"case dmrg cylinder 6x3 mixed {prec}", flush=True)
+    mpo = tk_hip.cylinder_mpo(6, 3)
+    sym = tk_hip.SYM_U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (0, 0), 16, 0)
+    d64, s64 = _gpu(ctx, mpo, sym, bonds, data, (32, 64), 4, tol=1e-10)
+    dlo, slo = _gpu(ctx, mpo, sym, bonds, data, (32, 64), 4, prec=prec, tol=1e-10)
+    assert abs(slo[-1]["energy"] - s64[-1]["energy"]) <= 1e-10
+    assert abs(slo[-1]["max_truncation_error"] - s64[-1]["max_truncation_error"]) <= 1e-9

@@ -97,6 +97,35 @@ def test_c2_diag_matches_oracle(ctx, c2):
     assert _rel(d.get(), ref) <= 1e-13
 
 
+@pytest.fixture(scope="module")
+def c4():
+    p = tk_hip.c4_problem(seed=4096)
+    p["y_ref"] = oracle.heff_apply(p, p["x"], Wi=p["W"], Wj=p["Wj"])
+    return p
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_c4_cylinder_heff_matches_oracle(ctx, c4, prec):
+    """Width-6 cylinder centre bond: MPO bond 20, U(1) bonds of 4096 states in
+    31 skewed sectors (largest 406), 4.66 M psi entries."""
+    print(f"case c4 heff {prec}", flush=True)
+    plan, keep = _plan(ctx, c4, prec, c4["W"], c4["Wj"])
+    keys, offs, tot = plan.psi_table()
+    ok, oo, ot = oracle.block_table(c4["sym"], [c4["W"].phys, c4["Wj"].phys, c4["bond_l"], c4["bond_r"]],
+                                    [1, 1, 1, 0])
+    assert np.array_equal(keys, ok) and np.array_equal(offs, oo) and tot == ot == len(c4["x"])
+    x = ctx.to_device(c4["x"])
+    y = ctx.empty(len(c4["x"]))
+    plan.apply(x, y)
+    got = y.get()
+    ref = c4["y_ref"]
+    assert _rel(got, ref) <= TOL[prec], (prec, _rel(got, ref))
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= TOL[prec]
+    plan.close()
+    for b in keep + (x, y):
+        b.free()
+
+
 def test_heisenberg_two_site_singlet_gpu(ctx):
     print("case singlet", flush=True)
     Wl, Wr = tk_hip.heisenberg_w(1.0, "left"), tk_hip.heisenberg_w(1.0, "right")
