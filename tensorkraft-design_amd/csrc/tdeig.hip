@@ -361,21 +361,23 @@ __device__ __forceinline__ int step_count(int n, int k, int k0) {
 }
 
 // MaxN bounds the trailing size n - k of every job in the launch (the
-// register arrays below hold one row and the step's vectors).
-template <int R, int MaxN>
+// register arrays below hold the wave's rows and the step's vectors); each
+// wavefront owns RW rows (i, i + R, ...), so a workgroup covers R * RW rows
+// and shares its per-step preamble (job search, row k, reflector) among them.
+template <int R, int MaxN, int RW>
 __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict__ jobs, int n_jobs, int k,
                                                         double* __restrict__ A, double* __restrict__ Dg,
                                                         double* __restrict__ Eg, double* __restrict__ TAUg,
                                                         double* __restrict__ VP, int64_t w_n, int ldsn) {
   constexpr int kStepThreads = 64 * R;
-  constexpr int kStepRows = R;
+  constexpr int kStepRows = R * RW;
   constexpr int kPerThread = (MaxN + kStepThreads - 1) / kStepThreads;  // row-k / vector entries per thread
   constexpr int kPerLane = MaxN / 64;                                     // own-row entries per lane
   extern __shared__ double sm[];
   double* vprev = sm;
   double* wprev = sm + ldsn;
   double* vcur = sm + 2 * ldsn;
-  __shared__ double red[2 * kStepRows];
+  __shared__ double red[2 * R];
   // this workgroup's job: every wavefront scans the job list 64 entries at a
   // time (one load per lane, an inclusive prefix sum of the per-job
   // workgroup counts, a ballot), instead of one dependent load per job
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
     int base = 0;
     for (int q0 = 0; q0 < n_jobs; q0 += 64) {
       const int qq = q0 + ln;
-      const int cnt = qq < n_jobs ? step_count<R>(jobs[qq].n, k, jobs[qq].pad) : 0;
+      const int cnt = qq < n_jobs ? step_count<kStepRows>(jobs[qq].n, k, jobs[qq].pad) : 0;
       int incl = cnt;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
@@ -426,10 +428,9 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
   }
   const bool pend = k >= 1;
   const bool last = k + 2 == n;
-  const int i = k + 1 + chunk * kStepRows + wv;  // this wavefront's row
-  const bool own = !last && i < n;
+  const int i0 = k + 1 + chunk * kStepRows + wv;  // this wavefront's rows: i0 + R * rr
   // every global operand of the step is requested up front
-  double pp[kPerThread], vv[kPerThread], xk[kPerThread], xr[kPerLane];
+  double pp[kPerThread], vv[kPerThread], xk[kPerThread], xr[RW][kPerLane];
 #pragma unroll
   for (int u = 0; u < kPerThread; ++u) {
     const int j = k + tid + u * kStepThreads;
@@ -438,11 +439,16 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
     vv[u] = (pend && in) ? (j == k ? 1.0 : vb_prev[j]) : 0.0;
     xk[u] = (!last && in) ? a[int64_t(k) * n + j] : 0.0;
   }
-  const double* rowi = a + int64_t(own ? i : 0) * n;
 #pragma unroll
-  for (int u = 0; u < kPerLane; ++u) {
-    const int j = k + 1 + lane + 64 * u;
-    xr[u] = (own && j < n) ? rowi[j] : 0.0;
+  for (int rr = 0; rr < RW; ++rr) {
+    const int i = i0 + R * rr;
+    const bool own = !last && i < n;
+    const double* rowi = a + int64_t(own ? i : 0) * n;
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u) {
+      const int j = k + 1 + lane + 64 * u;
+      xr[rr][u] = (own && j < n) ? rowi[j] : 0.0;
+    }
   }
   // pending update of step k-1: w = tau p - tau^2/2 (p.v) v over j >= k
   if (pend) {
@@ -518,8 +524,11 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
     e[k] = beta;
     tau[k] = tk;
   }
-  // this wavefront's row: pending update over columns >= k+1, then (A22 v)_i
-  if (own) {
+  // this wavefront's rows: pending update over columns >= k+1, then (A22 v)_i
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const int i = i0 + R * rr;
+    if (i >= n) break;
     double* row = a + int64_t(i) * n;
     const double vi = pend ? vprev[i] : 0.0, wi = pend ? wprev[i] : 0.0;
     double acc = 0;
@@ -527,7 +536,7 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
     for (int u = 0; u < kPerLane; ++u) {
       const int j = k + 1 + lane + 64 * u;
       if (j < n) {
-        double x = xr[u];
+        double x = xr[rr][u];
         if (pend) x -= vi * wprev[j] + wi * vprev[j];
         row[j] = x;
         acc += x * vcur[j];
@@ -542,33 +551,36 @@ __global__ __launch_bounds__(64 * R) void tdstep_kernel(const DevJob* __restrict
   }
 }
 
-template <int R, int MaxN>
+template <int R, int MaxN, int RW>
 void launch_tdstep_t(dim3 grid, size_t lds, hipStream_t st, const DevJob* jobs, int nj, int k, double* A, double* D,
                      double* E, double* TAU, double* VP, int64_t w_n, int ldsn) {
-  hipLaunchKernelGGL((tdstep_kernel<R, MaxN>), grid, dim3(64 * R), lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+  hipLaunchKernelGGL((tdstep_kernel<R, MaxN, RW>), grid, dim3(64 * R), lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n,
+                     ldsn);
 }
 
-template <int R>
+template <int R, int RW>
 void launch_tdstep_r(int cap, dim3 grid, size_t lds, hipStream_t st, const DevJob* jobs, int nj, int k, double* A,
                      double* D, double* E, double* TAU, double* VP, int64_t w_n, int ldsn) {
   if (cap <= 512)
-    launch_tdstep_t<R, 512>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+    launch_tdstep_t<R, 512, RW>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
   else if (cap <= 1024)
-    launch_tdstep_t<R, 1024>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+    launch_tdstep_t<R, 1024, RW>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
   else
-    launch_tdstep_t<R, kSymEigMaxN>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+    launch_tdstep_t<R, kSymEigMaxN, RW>(grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
 }
 
-// one column step over every active job: R rows per workgroup, registers
-// sized for trailing blocks of at most `cap` (512, 1024 or 2048)
-void launch_tdstep(int r, int cap, dim3 grid, size_t lds, hipStream_t st, const DevJob* jobs, int nj, int k,
+// one column step over every active job: R waves per workgroup, rw rows per
+// wave, registers sized for trailing blocks of at most `cap` (512, 1024 or 2048)
+void launch_tdstep(int r, int rw, int cap, dim3 grid, size_t lds, hipStream_t st, const DevJob* jobs, int nj, int k,
                    double* A, double* D, double* E, double* TAU, double* VP, int64_t w_n, int ldsn) {
-  if (r == 16)
-    launch_tdstep_r<16>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+  if (rw == 2)
+    launch_tdstep_r<8, 2>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+  else if (r == 16)
+    launch_tdstep_r<16, 1>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
   else if (r == 4)
-    launch_tdstep_r<4>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+    launch_tdstep_r<4, 1>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
   else
-    launch_tdstep_r<8>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
+    launch_tdstep_r<8, 1>(cap, grid, lds, st, jobs, nj, k, A, D, E, TAU, VP, w_n, ldsn);
 }
 
 // Reflector k < k0 (DevJob::pad) from column k (rows >= k+2) into row k
@@ -1751,9 +1763,13 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
       const char* e = std::getenv("TK_TRIDIAG_ROWS");
       return e ? std::atoi(e) : 0;
     }();
+    static const int64_t rw2_rows = [] {
+      const char* e = std::getenv("TK_TRIDIAG_RW2_ROWS");
+      return e ? int64_t(std::atoll(e)) : int64_t(4096);
+    }();
     for (int k = 0; k < kmax0; ++k) {
       // algorithmic traffic: every step reads and writes the trailing block
-      int64_t rows = 0, grid4 = 0, grid8 = 0, grid16 = 0;
+      int64_t rows = 0, grid4 = 0, grid8 = 0, grid16 = 0, grid8x2 = 0;
       int trail = 0;  // the largest n - k among the active jobs
       for (size_t o = 0; o < ns.size(); ++o)
         if (k < k0s[o]) {
@@ -1763,6 +1779,7 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
           grid4 += (r + 3) / 4;
           grid8 += (r + 7) / 8;
           grid16 += (r + 15) / 16;
+          grid8x2 += (r + 15) / 16;
           trail = std::max(trail, int(r) + 1);
         }
       if (rows == 0) continue;
@@ -1771,10 +1788,14 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
       int r = 8;
       if (rows_override == 4 || rows_override == 8 || rows_override == 16) r = rows_override;
       const size_t lds = 3 * size_t(nmax_) * sizeof(double);
-      const int64_t grid = r == 16 ? grid16 : r == 8 ? grid8 : grid4;
+      // two rows per wave once the step has more rows than one round of
+      // single-row workgroups holds (TK_TRIDIAG_RW2_ROWS overrides the
+      // threshold, for tuning)
+      const int rw = (r == 8 && rows > rw2_rows) ? 2 : 1;
+      const int64_t grid = rw == 2 ? grid8x2 : r == 16 ? grid16 : r == 8 ? grid8 : grid4;
       // the smallest register capacity that holds the trailing size
       const int cap = trail <= 512 ? 512 : trail <= 1024 ? 1024 : 2048;
-      launch_tdstep(r, cap, dim3(unsigned(grid)), lds, ctx_->stream, djd, nj, k, A, D, E, TAU, vp.get(), w_n,
+      launch_tdstep(r, rw, cap, dim3(unsigned(grid)), lds, ctx_->stream, djd, nj, k, A, D, E, TAU, vp.get(), w_n,
                     nmax_);
       ++launches;
     }
