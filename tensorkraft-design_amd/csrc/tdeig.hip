@@ -948,13 +948,27 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
       double acc[NB];
 #pragma unroll
       for (int c = 0; c < NB; ++c) acc[c] = 0;
-      if (lane < pc)
-        for (int i = wv; i < n; i += NW) {
-          const double zp = z[int64_t(i) * k + p0 + lane];
+      if (lane < pc) {
+        // four rows in flight per wavefront
+        const double* zc = z + p0 + lane;
+        int i = wv;
+        for (; i + 3 * NW < n; i += 4 * NW) {
+          double zq[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) zq[q] = zc[int64_t(i + q * NW) * k];
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+              if (c >= c_lo && c < c_hi) acc[c] += zq[q] * Zb[(i + q * NW) * NB + c];
+        }
+        for (; i < n; i += NW) {
+          const double zp = zc[int64_t(i) * k];
 #pragma unroll
           for (int c = 0; c < NB; ++c)
             if (c >= c_lo && c < c_hi) acc[c] += zp * Zb[i * NB + c];
         }
+      }
 #pragma unroll
       for (int c = 0; c < NB; ++c) Hp[(wv * 64 + lane) * NB + c] = acc[c];
       __syncthreads();
@@ -968,8 +982,20 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
         double upd[NB];
 #pragma unroll
         for (int c = 0; c < NB; ++c) upd[c] = 0;
-        for (int p = 0; p < pc; ++p) {
-          const double zp = z[int64_t(i) * k + p0 + p];
+        // eight earlier columns of the row in flight
+        const double* zr = z + int64_t(i) * k + p0;
+        int p = 0;
+        for (; p + 8 <= pc; p += 8) {
+          double zq[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) zq[q] = zr[p + q];
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int c = 0; c < NB; ++c) upd[c] += zq[q] * H[(p + q) * NB + c];
+        }
+        for (; p < pc; ++p) {
+          const double zp = zr[p];
 #pragma unroll
           for (int c = 0; c < NB; ++c) upd[c] += zp * H[p * NB + c];
         }
@@ -1611,7 +1637,7 @@ int orth_block() {
 template <int NB>
 void launch_orth_range(Context* ctx, int nmax, int nj, const DevJob* djd, double* Z, const double* lu, int step,
                        int c_count, const double* norms) {
-  constexpr int kNT = 1024, NW = kNT / 64;
+  constexpr int kNT = 512, NW = kNT / 64;
   const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
   allow_lds(orth_kernel<kNT, NB>, lds);
   hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(nj), dim3(kNT), lds, ctx->stream, djd, Z, lu, step, c_count,
