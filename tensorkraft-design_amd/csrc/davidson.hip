@@ -36,6 +36,7 @@ int grid1(Context* ctx, int64_t n) {
 
 struct Workspace {
   DBuf<double> V, AV, H, c, h, scal, y, ay, r;
+  DBuf<int> flag;
 };
 
 }  // namespace
@@ -51,17 +52,20 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
   w.AV.reset(ctx, size_t(m) * n);
   w.H.reset(ctx, size_t(m) * m);
   w.c.reset(ctx, m);
-  w.h.reset(ctx, m);
+  w.h.reset(ctx, size_t(m) + 1);
   w.scal.reset(ctx, 8);
   w.y.reset(ctx, n);
   w.ay.reset(ctx, n);
   w.r.reset(ctx, n);
+  w.flag.reset(ctx, 1);
   double* V = w.V.get();
   double* AV = w.AV.get();
   double* H = w.H.get();
+  double* c = w.c.get();
   double* theta = w.scal.get();
   double* rn2 = w.scal.get() + 1;
   double* tn2 = w.scal.get() + 2;
+  const double* dprec = (opts.precondition && diag_dev) ? diag_dev : nullptr;
   bool low = low_precision;
 
   auto matvec = [&](const double* in, double* out) {
@@ -77,42 +81,46 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     matvec(V, AV);
     dot_many(ctx, V, n, 1, AV, n, H);
   };
+  // slot <- slot minus its projection on V[0..nsub): one classical pass, and
+  // a second one (flag on the device) when the first removed more than half
+  // of its squared norm; then *tn2 = |slot|^2
+  auto orthogonalise = [&](double* slot, int nsub) {
+    dot_many(ctx, V, n, nsub + 1, slot, n, w.h.get());  // V^T slot and slot.slot (slot = V[nsub])
+    subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n);
+    reorth_flag(ctx, w.h.get(), nsub, w.flag.get());
+    dot_many(ctx, V, n, nsub, slot, n, w.h.get(), w.flag.get());
+    subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n, w.flag.get());
+    dot_many(ctx, slot, n, 1, slot, n, tn2);
+  };
+  // Ritz vector y = V c of the current projected solution
+  auto ritz_vector = [&](int nsub) { combine(ctx, V, n, nsub, c, 0.0, w.y.get(), n); };
   start_from(x);
   int nsub = 1;
   double best = INFINITY;
   int stall = 0;
   double host[3] = {0, 0, 0};
+  bool c_current = false;
   for (int it = 0; it < opts.max_iter; ++it) {
     // lowest Ritz pair of the projected matrix (one wavefront on the device)
-    lowest_eigpair(ctx, H, m, nsub, w.c.get(), theta);
-    combine(ctx, V, n, nsub, w.c.get(), 0.0, w.y.get(), n);
-    combine(ctx, AV, n, nsub, w.c.get(), 0.0, w.ay.get(), n);
-    hipLaunchKernelGGL(residual_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), w.y.get(), theta,
-                       w.r.get(), n);
-    dot_many(ctx, w.r.get(), n, 1, w.r.get(), n, rn2);
-    // the next correction vector is formed before the one host sync of the
-    // iteration (discarded when the iteration converges or switches precision)
+    lowest_eigpair(ctx, H, m, nsub, c, theta);
+    c_current = true;
     const bool can_grow = nsub < n;
     if (can_grow && nsub == m) {
       // restart from the Ritz vector; A y is carried over
+      ritz_vector(nsub);
+      combine(ctx, AV, n, nsub, c, 0.0, w.ay.get(), n);
       hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.y.get(), V, n);
       hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.ay.get(), AV, n);
       dot_many(ctx, V, n, 1, AV, n, H);
+      fill_f64(ctx, c, 1.0, 1);
       nsub = 1;
     }
+    // the residual, its norm and the next correction vector in one pass over
+    // V and AV, formed before the one host sync of the iteration (discarded
+    // when the iteration converges or switches precision)
     double* slot = V + int64_t(nsub) * n;
-    if (can_grow) {
-      if (opts.precondition && diag_dev)
-        hipLaunchKernelGGL(precond_dev_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), diag_dev, theta, slot,
-                           n);
-      else
-        hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), slot, n);
-      for (int pass = 0; pass < 2; ++pass) {
-        dot_many(ctx, V, n, nsub, slot, n, w.h.get());
-        subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n);
-      }
-      dot_many(ctx, slot, n, 1, slot, n, tn2);
-    }
+    ritz_residual(ctx, V, AV, n, nsub, c, theta, dprec, can_grow ? slot : w.r.get(), n, rn2);
+    if (can_grow) orthogonalise(slot, nsub);
     TK_HIP(hipMemcpyAsync(host, w.scal.get(), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     TK_HIP(hipStreamSynchronize(s));
     TK_HIP(hipGetLastError());
@@ -129,7 +137,9 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
       if (res.residual < opts.switch_rel * std::max(1.0, std::fabs(res.eigenvalue)) || stall >= 3 ||
           res.residual < opts.tol) {
         low = false;  // FP64 residual refinement from the current Ritz vector
+        ritz_vector(nsub);
         start_from(w.y.get());
+        fill_f64(ctx, c, 1.0, 1);
         nsub = 1;
         continue;
       }
@@ -140,18 +150,16 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     if (!can_grow) break;  // the basis spans the space: exact
     if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
       // the preconditioned correction collapsed into the basis: raw residual
-      hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.r.get(), slot, n);
-      for (int pass = 0; pass < 2; ++pass) {
-        dot_many(ctx, V, n, nsub, slot, n, w.h.get());
-        subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n);
-      }
-      dot_many(ctx, slot, n, 1, slot, n, tn2);
+      ritz_residual(ctx, V, AV, n, nsub, c, theta, nullptr, slot, n, rn2);
+      orthogonalise(slot, nsub);
       TK_HIP(hipMemcpyAsync(host + 2, tn2, sizeof(double), hipMemcpyDeviceToHost, s));
       TK_HIP(hipStreamSynchronize(s));
       if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
         if (low) {  // stagnated in low precision: refine in FP64
           low = false;
+          ritz_vector(nsub);
           start_from(w.y.get());
+          fill_f64(ctx, c, 1.0, 1);
           nsub = 1;
           continue;
         }
@@ -162,8 +170,12 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     matvec(slot, AV + int64_t(nsub) * n);
     dot_many(ctx, V, n, nsub + 1, AV + int64_t(nsub) * n, n, H + int64_t(nsub) * m);
     ++nsub;
+    c_current = false;
   }
-  // eigenvector: normalised Ritz vector
+  // eigenvector: normalised Ritz vector (of the grown basis when the
+  // iteration budget ran out after a matvec)
+  if (!c_current) lowest_eigpair(ctx, H, m, nsub, c, theta);
+  ritz_vector(nsub);
   dot_many(ctx, w.y.get(), n, 1, w.y.get(), n, tn2);
   scale_by_inv_sqrt(ctx, w.y.get(), tn2, n);
   hipLaunchKernelGGL(copy_kernel, dim3(grid1(ctx, n)), dim3(256), 0, s, w.y.get(), x, n);

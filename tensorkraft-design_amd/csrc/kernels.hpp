@@ -46,12 +46,23 @@ void launch_fill(Context* ctx, double* p, double v, int64_t n);
 void convert_f64_to_f32(Context* ctx, const double* x, float* y, int64_t n);
 void convert_f32_to_f64(Context* ctx, const float* x, double* y, int64_t n);
 void fill_f64(Context* ctx, double* x, double v, int64_t n);
-// out[i] = sum_j V[i*ldv + j] * x[j] for i < nvec (deterministic two-pass reduction)
-void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double* x, int64_t n, double* out);
+// out[i] = sum_j V[i*ldv + j] * x[j] for i < nvec (deterministic two-pass
+// reduction); skipped when `active` (a device flag) reads 0
+void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double* x, int64_t n, double* out,
+              const int* active = nullptr);
 // y[j] = beta*y[j] + sum_i c[i] * V[i*ldv + j]; c on the device
 void combine(Context* ctx, const double* V, int64_t ldv, int nvec, const double* c, double beta, double* y, int64_t n);
-// y[j] -= sum_i h[i] * V[i*ldv + j]; h on the device
-void subtract_projection(Context* ctx, const double* V, int64_t ldv, int nvec, const double* h, double* y, int64_t n);
+// y[j] -= sum_i h[i] * V[i*ldv + j]; h on the device; skipped when `active` reads 0
+void subtract_projection(Context* ctx, const double* V, int64_t ldv, int nvec, const double* h, double* y, int64_t n,
+                         const int* active = nullptr);
+// t = r or r / (theta - diag) (|theta - diag| > 1e-14) for the Ritz residual
+// r = AV c - theta V c of nvec basis vectors, and *rn2 = |r|^2, in one pass
+// (c, theta, rn2 on the device; diag may be null)
+void ritz_residual(Context* ctx, const double* V, const double* AV, int64_t ldv, int nvec, const double* c,
+                   const double* theta, const double* diag, double* t, int64_t n, double* rn2);
+// *flag = 1 when a second Gram-Schmidt pass is needed: h[0..nvec) = V^T s and
+// h[nvec] = s.s from before the first pass, s.s - |h|^2 < s.s / 2
+void reorth_flag(Context* ctx, const double* h, int nvec, int* flag);
 // x *= 1/sqrt(*nrm2) read on the device (no-op guard when the norm is zero)
 void scale_by_inv_sqrt(Context* ctx, double* x, const double* nrm2, int64_t n);
 void scale(Context* ctx, double* x, double a, int64_t n);

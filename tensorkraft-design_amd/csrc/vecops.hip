@@ -25,9 +25,12 @@ constexpr int kDotPer = 4;
 constexpr int kDotChunk = kThreads * kDotPer;
 constexpr int kVecGroup = 8;
 
+// `active` (device flag, may be null): when it reads 0 the launch does nothing.
 __global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __restrict__ V, int64_t ldv, int nvec,
                                                                const double* __restrict__ x, int64_t n,
-                                                               double* __restrict__ partial, int nchunks) {
+                                                               double* __restrict__ partial, int nchunks,
+                                                               const int* __restrict__ active) {
+  if (active != nullptr && *active == 0) return;
   __shared__ double red[kThreads / 64][kVecGroup];
   const int64_t base = int64_t(blockIdx.x) * kDotChunk + threadIdx.x;
   const int i0 = blockIdx.y * kVecGroup;
@@ -67,7 +70,9 @@ __global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __r
 }
 
 __global__ __launch_bounds__(kThreads) void reduce_rows_kernel(const double* __restrict__ partial, int nchunks,
-                                                               double* __restrict__ out) {
+                                                               double* __restrict__ out,
+                                                               const int* __restrict__ active) {
+  if (active != nullptr && *active == 0) return;
   __shared__ double red[kThreads / 64];
   const double* row = partial + int64_t(blockIdx.x) * nchunks;
   double s = 0.0;
@@ -84,7 +89,9 @@ __global__ __launch_bounds__(kThreads) void reduce_rows_kernel(const double* __r
 
 __global__ __launch_bounds__(kThreads) void combine_kernel(const double* __restrict__ V, int64_t ldv, int nvec,
                                                            const double* __restrict__ c, double beta, double sign,
-                                                           double* __restrict__ y, int64_t n) {
+                                                           double* __restrict__ y, int64_t n,
+                                                           const int* __restrict__ active) {
+  if (active != nullptr && *active == 0) return;
   __shared__ double cs[kMaxVec];
   for (int i = threadIdx.x; i < nvec; i += kThreads) cs[i] = c[i];
   __syncthreads();
@@ -92,6 +99,60 @@ __global__ __launch_bounds__(kThreads) void combine_kernel(const double* __restr
     double s = 0.0;
     for (int i = 0; i < nvec; ++i) s += cs[i] * V[int64_t(i) * ldv + j];
     y[j] = (beta == 0.0 ? 0.0 : beta * y[j]) + sign * s;
+  }
+}
+
+// One pass over the basis V and its image AV (nvec vectors each): the Ritz
+// residual r = AV c - theta V c, written to t preconditioned as
+// r / (theta - d) where |theta - d| > 1e-14 (d null: r itself), with per-block
+// partial sums of r^2 (fixed grid-stride order) for its norm.
+__global__ __launch_bounds__(kThreads) void ritz_residual_kernel(const double* __restrict__ V,
+                                                                 const double* __restrict__ AV, int64_t ldv,
+                                                                 int nvec, const double* __restrict__ c,
+                                                                 const double* __restrict__ theta,
+                                                                 const double* __restrict__ d,
+                                                                 double* __restrict__ t, int64_t n,
+                                                                 double* __restrict__ partial) {
+  __shared__ double cs[kMaxVec];
+  __shared__ double red[kThreads / 64];
+  for (int i = threadIdx.x; i < nvec; i += kThreads) cs[i] = c[i];
+  __syncthreads();
+  const double th = *theta;
+  double rr = 0.0;
+  for (int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x; j < n; j += int64_t(gridDim.x) * kThreads) {
+    double y = 0.0, ay = 0.0;
+    for (int i = 0; i < nvec; ++i) {
+      y += cs[i] * V[int64_t(i) * ldv + j];
+      ay += cs[i] * AV[int64_t(i) * ldv + j];
+    }
+    const double r = ay - th * y;
+    rr += r * r;
+    if (d != nullptr) {
+      const double den = th - d[j];
+      t[j] = fabs(den) > 1e-14 ? r / den : r;
+    } else {
+      t[j] = r;
+    }
+  }
+  rr = wave_sum(rr);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+// Second Gram-Schmidt pass needed?  h[0..nvec) = V^T s and h[nvec] = s.s
+// before the first pass; |s - V h|^2 = s.s - |h|^2 for orthonormal V.  The
+// pass is repeated when the first one removed more than half of |s|^2.
+__global__ void reorth_flag_kernel(const double* __restrict__ h, int nvec, int* __restrict__ flag) {
+  if (threadIdx.x == 0) {
+    double hh = 0.0;
+    for (int i = 0; i < nvec; ++i) hh += h[i] * h[i];
+    const double ss = h[nvec];
+    flag[0] = (ss - hh < 0.5 * ss) ? 1 : 0;
   }
 }
 
@@ -144,14 +205,16 @@ double* scratch(Context* ctx, size_t n) { return static_cast<double*>(ctx->scrat
 
 }  // namespace
 
-void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double* x, int64_t n, double* out) {
+void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double* x, int64_t n, double* out,
+              const int* active) {
   if (nvec <= 0) return;
   if (nvec > kMaxVec) throw ArgError("dot_many: too many vectors");
   const int nchunks = int(std::max<int64_t>(1, (n + kDotChunk - 1) / kDotChunk));
   double* part = scratch(ctx, size_t(nchunks) * nvec);
   const dim3 grid(nchunks, (nvec + kVecGroup - 1) / kVecGroup);
-  hipLaunchKernelGGL(dot_partial_kernel, grid, dim3(kThreads), 0, ctx->stream, V, ldv, nvec, x, n, part, nchunks);
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(nvec), dim3(kThreads), 0, ctx->stream, part, nchunks, out);
+  hipLaunchKernelGGL(dot_partial_kernel, grid, dim3(kThreads), 0, ctx->stream, V, ldv, nvec, x, n, part, nchunks,
+                     active);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(nvec), dim3(kThreads), 0, ctx->stream, part, nchunks, out, active);
   TK_HIP(hipGetLastError());
 }
 
@@ -159,15 +222,32 @@ void combine(Context* ctx, const double* V, int64_t ldv, int nvec, const double*
              int64_t n) {
   if (nvec > kMaxVec) throw ArgError("combine: too many vectors");
   hipLaunchKernelGGL(combine_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, V, ldv, nvec, c, beta,
-                     1.0, y, n);
+                     1.0, y, n, static_cast<const int*>(nullptr));
   TK_HIP(hipGetLastError());
 }
 
 void subtract_projection(Context* ctx, const double* V, int64_t ldv, int nvec, const double* h, double* y,
-                         int64_t n) {
+                         int64_t n, const int* active) {
   if (nvec > kMaxVec) throw ArgError("subtract_projection: too many vectors");
   hipLaunchKernelGGL(combine_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, V, ldv, nvec, h, 1.0,
-                     -1.0, y, n);
+                     -1.0, y, n, active);
+  TK_HIP(hipGetLastError());
+}
+
+void ritz_residual(Context* ctx, const double* V, const double* AV, int64_t ldv, int nvec, const double* c,
+                   const double* theta, const double* diag, double* t, int64_t n, double* rn2) {
+  if (nvec <= 0 || nvec > kMaxVec) throw ArgError("ritz_residual: vector count outside 1..128");
+  const int grid = grid_for(ctx, n);
+  double* part = scratch(ctx, size_t(grid));
+  hipLaunchKernelGGL(ritz_residual_kernel, dim3(grid), dim3(kThreads), 0, ctx->stream, V, AV, ldv, nvec, c, theta,
+                     diag, t, n, part);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(kThreads), 0, ctx->stream, part, grid, rn2,
+                     static_cast<const int*>(nullptr));
+  TK_HIP(hipGetLastError());
+}
+
+void reorth_flag(Context* ctx, const double* h, int nvec, int* flag) {
+  hipLaunchKernelGGL(reorth_flag_kernel, dim3(1), dim3(64), 0, ctx->stream, h, nvec, flag);
   TK_HIP(hipGetLastError());
 }
 
