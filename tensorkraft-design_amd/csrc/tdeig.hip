@@ -342,6 +342,218 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const TailJob* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// 1a'. The same trailing-block tridiagonalisation with the block in registers
+// (m <= MT = 64 CG): wavefront wv holds rows wv + NW r (r < RPW) and lane l
+// columns l + 64 g (g < CG) of the full symmetric block.  Per column step the
+// owner of row k publishes it (pending update applied) to LDS; every thread
+// forms the reflector; each wavefront applies the pending rank-2 update to
+// its rows and forms their parts of A22 v, reduced across the lanes through
+// LDS (RPW partials per lane, summed by groups of lanes, no per-row
+// reduction chains).  Reflector k goes to row k, columns k+2.., of A.
+template <int NT, int MT>
+__global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restrict__ jobs, double* __restrict__ A,
+                                                         double* __restrict__ Dg, double* __restrict__ Eg,
+                                                         double* __restrict__ TAUg, const double* __restrict__ VP,
+                                                         int64_t w_n) {
+  constexpr int NW = NT / 64;
+  constexpr int RPW = MT / NW;  // rows per wavefront
+  constexpr int CG = MT / 64;   // column groups per lane
+  // lanes summing one row's partials: the largest power of two <= 64 / RPW
+  constexpr int LPR = 64 / RPW >= 8 ? 8 : 64 / RPW >= 4 ? 4 : 64 / RPW >= 2 ? 2 : 1;
+  static_assert(MT % NW == 0 && MT % 64 == 0 && RPW <= 64, "tile shape");
+  const TailJob jb = jobs[blockIdx.x];
+  const int m = jb.m;
+  double* ag = A + jb.a_off;
+  const int ld = jb.ld;
+  double* d = Dg + jb.w_off;
+  double* e = Eg + jb.w_off;
+  double* tau = TAUg + jb.w_off;
+  extern __shared__ double sm[];
+  double* v = sm;             // pending reflector (v[k] = 1 at its step)
+  double* w = v + MT;         // pending w
+  double* vn = w + MT;        // this step's reflector
+  double* p = vn + MT;        // A22 vn
+  double* rowk = p + MT;      // row k with the pending update
+  double* red = rowk + MT;    // 2 NW
+  double* fin = red + 2 * NW; // the last 2 x 2 block
+  double* part = fin + 8;     // NW x 64 x (RPW + 1) lane partials (padded against bank conflicts)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int parity = 0;
+  if (jb.pending < 0 && m <= 2) {
+    if (tid == 0) {
+      d[0] = ag[0];
+      e[0] = 0;
+      tau[0] = 0;
+      if (m == 2) {
+        e[0] = ag[1];
+        d[1] = ag[ld + 1];
+        e[1] = 0;
+        tau[1] = 0;
+      }
+    }
+    return;
+  }
+  double x[RPW][CG];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int i = wv + NW * r;
+#pragma unroll
+    for (int g = 0; g < CG; ++g) {
+      const int j = lane + 64 * g;
+      x[r][g] = (i < m && j < m) ? ag[int64_t(i) * ld + j] : 0.0;
+    }
+  }
+  if (jb.pending < 0) {
+    for (int j = tid; j < MT; j += NT) {
+      v[j] = 0;
+      w[j] = 0;
+    }
+  } else {
+    // the multi-workgroup kernel's last reflector (v[0] = 1) and p = A22 v:
+    // w = tau p - tau^2/2 (p.v) v
+    const double* vb = VP + int64_t(jb.pending) * w_n + jb.w_off;
+    const double* pb = VP + int64_t(2 + jb.pending) * w_n + jb.w_off;
+    const double tp = tau[-1];
+    double loc = 0;
+    for (int j = tid; j < MT; j += NT) {
+      const double vj = j < m ? (j == 0 ? 1.0 : vb[j]) : 0.0;
+      const double pj = j < m ? pb[j] : 0.0;
+      v[j] = vj;
+      p[j] = pj;
+      loc += pj * vj;
+    }
+    const double pv = block_sum<NT>(loc, red, parity);
+    const double alw = -0.5 * tp * tp * pv;
+    for (int j = tid; j < MT; j += NT) w[j] = tp * p[j] + alw * v[j];
+  }
+  __syncthreads();
+  for (int k = 0; k + 2 < m; ++k) {
+    // (a) the owner publishes row k with the pending update
+    if (wv == k % NW) {
+      const int r = k / NW;
+      const double vk = v[k], wk = w[k];
+#pragma unroll
+      for (int rr = 0; rr < RPW; ++rr)
+        if (rr == r) {
+#pragma unroll
+          for (int g = 0; g < CG; ++g) {
+            const int j = lane + 64 * g;
+            if (j >= k && j < m) rowk[j] = x[rr][g] - (vk * w[j] + wk * v[j]);
+          }
+        }
+    }
+    __syncthreads();
+    // (b) reflector of row k
+    double ss = 0;
+    for (int j = k + 2 + tid; j < m; j += NT) ss += rowk[j] * rowk[j];
+    const double sigma = block_sum<NT>(ss, red, parity);
+    const double alpha = rowk[k + 1];
+    double tk = 0, beta = alpha, scale = 0;
+    if (sigma > 0) {
+      const double nrm = sqrt(alpha * alpha + sigma);
+      beta = alpha >= 0 ? -nrm : nrm;
+      tk = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    for (int j = k + 1 + tid; j < m; j += NT) {
+      const double vj = j == k + 1 ? 1.0 : rowk[j] * scale;
+      vn[j] = vj;
+      if (j >= k + 2) ag[int64_t(k) * ld + j] = vj;
+    }
+    if (tid == 0) {
+      d[k] = rowk[k];
+      e[k] = beta;
+      tau[k] = tk;
+    }
+    __syncthreads();
+    // (c) rows k+1..m-1: pending update, then the lane partials of A22 vn
+    double vj[CG], wj[CG], vnj[CG];
+#pragma unroll
+    for (int g = 0; g < CG; ++g) {
+      const int j = lane + 64 * g;
+      const bool in = j > k && j < m;
+      vj[g] = in ? v[j] : 0.0;
+      wj[g] = in ? w[j] : 0.0;
+      vnj[g] = in ? vn[j] : 0.0;
+    }
+    double* pw = part + (wv * 64 + lane) * (RPW + 1);
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int i = wv + NW * r;
+      double acc = 0;
+      if (i > k && i < m) {
+        const double vi = v[i], wi = w[i];
+#pragma unroll
+        for (int g = 0; g < CG; ++g) {
+          const double xx = x[r][g] - (vi * wj[g] + wi * vj[g]);
+          x[r][g] = xx;
+          acc += xx * vnj[g];
+        }
+      }
+      pw[r] = acc;
+    }
+    __builtin_amdgcn_wave_barrier();
+    {
+      // lanes r * LPR .. r * LPR + LPR - 1 sum row r's 64 partials
+      const int r = lane / LPR, q = lane % LPR;
+      double s2 = 0;
+      if (r < RPW) {
+        const double* col = part + int64_t(wv * 64) * (RPW + 1) + r;
+#pragma unroll
+        for (int l = 0; l < 64 / LPR; ++l) s2 += col[(q * (64 / LPR) + l) * (RPW + 1)];
+      }
+#pragma unroll
+      for (int o = 1; o < LPR; o <<= 1) s2 += __shfl_xor(s2, o, 64);
+      const int i = wv + NW * r;
+      if (r < RPW && q == 0 && i > k && i < m) p[i] = s2;
+    }
+    __syncthreads();
+    // (d) w = tau p - tau^2/2 (p.vn) vn; v <- vn
+    double pv = 0;
+    for (int j = k + 1 + tid; j < m; j += NT) pv += p[j] * vn[j];
+    pv = block_sum<NT>(pv, red, parity);
+    const double alw = -0.5 * tk * tk * pv;
+    for (int j = k + 1 + tid; j < m; j += NT) {
+      w[j] = tk * p[j] + alw * vn[j];
+      v[j] = vn[j];
+    }
+    if (tid == 0) {
+      v[k] = 0;
+      w[k] = 0;
+    }
+    __syncthreads();
+  }
+  // the last 2 x 2 block with the pending update
+  const int pq[2] = {m - 2, m - 1};
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int i = pq[a];
+    if (wv == i % NW) {
+      const int r = i / NW;
+#pragma unroll
+      for (int rr = 0; rr < RPW; ++rr)
+        if (rr == r) {
+#pragma unroll
+          for (int g = 0; g < CG; ++g) {
+            const int j = lane + 64 * g;
+            if (j == m - 2 || j == m - 1) fin[a * 2 + (j - (m - 2))] = x[rr][g];
+          }
+        }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int p0 = m - 2, q0 = m - 1;
+    d[p0] = fin[0] - 2.0 * v[p0] * w[p0];
+    e[p0] = fin[1] - (v[p0] * w[q0] + w[p0] * v[q0]);
+    d[q0] = fin[3] - 2.0 * v[q0] * w[q0];
+    e[q0] = 0;
+    tau[p0] = 0;
+    tau[q0] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 1b. Tridiagonalisation for large matrices: one launch per column step k,
 // batched over the jobs, each workgroup owning R rows of one job
 // (one wavefront per row) of the full symmetric matrix.  Every workgroup
@@ -1414,14 +1626,25 @@ void launch_tridiag_t(Context* ctx, const TailJob* jobs, int n_jobs, int mmax, d
                      VP, w_n);
 }
 
+constexpr int kRegTailNT = 512, kRegTail = 192;
+
+void launch_tridiag_reg(Context* ctx, const TailJob* jobs, int n_jobs, double* A, double* D, double* E, double* TAU,
+                        const double* VP, int64_t w_n) {
+  constexpr int NW = kRegTailNT / 64, RPW = kRegTail / NW;
+  const size_t lds = (5 * size_t(kRegTail) + 2 * NW + 8 + size_t(NW) * 64 * (RPW + 1)) * sizeof(double);
+  allow_lds(tridiag_reg_kernel<kRegTailNT, kRegTail>, lds);
+  hipLaunchKernelGGL((tridiag_reg_kernel<kRegTailNT, kRegTail>), dim3(n_jobs), dim3(kRegTailNT), lds, ctx->stream,
+                     jobs, A, D, E, TAU, VP, w_n);
+}
+
 // Trailing size below which a job's tridiagonalisation finishes in one
-// workgroup with the block in LDS (TK_TRIDIAG_TAIL overrides with 64, 96 or
-// 128, for tuning).
+// workgroup: 192 with the block in registers (TK_TRIDIAG_TAIL = 64, 96 or 128
+// selects the LDS-resident kernel at that size, for comparisons).
 int tridiag_tail() {
   static const int t = [] {
     const char* e = std::getenv("TK_TRIDIAG_TAIL");
     const int v = e ? std::atoi(e) : 0;
-    return (v == 64 || v == 96 || v == 128) ? v : 128;
+    return (v == 64 || v == 96 || v == 128) ? v : kRegTail;
   }();
   return t;
 }
@@ -1831,7 +2054,10 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   const int tt = ctx_->timer.begin(ctx_->stream);
   double tail_bytes = 0;
   for (const TailJob& t : tj) tail_bytes += 8.0 * double(t.m) * double(t.m) * double(t.m) / 3.0;
-  launch_tridiag_t<512, 2, true>(ctx_, tjd, nj, mmax, A, D, E, TAU, vp.get(), w_n);
+  if (tail == kRegTail)
+    launch_tridiag_reg(ctx_, tjd, nj, A, D, E, TAU, vp.get(), w_n);
+  else
+    launch_tridiag_t<512, 2, true>(ctx_, tjd, nj, mmax, A, D, E, TAU, vp.get(), w_n);
   TK_HIP(hipGetLastError());
   ctx_->timer.end(tt, ctx_->stream, "tridiag_tail", tail_bytes, 0.0, 1);
   if (kmax0 > 0) {

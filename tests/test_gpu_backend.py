@@ -142,9 +142,9 @@ def test_eigh_lowest(be):
     assert abs(w[0] - (2 - math.sqrt(2))) < 1e-10 and w[0] <= w[1] <= w[2]
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 50, 128, 129, 300, 700, 1100, 2048])
+@pytest.mark.parametrize("n", [1, 2, 3, 50, 128, 129, 192, 193, 300, 700, 1100, 2048])
 def test_eigh_matches_lapack(be, n):
-    """Tridiagonalisation (one workgroup up to 128, column steps above),
+    """Tridiagonalisation (one workgroup up to 192, column steps above),
     bisection, inverse iteration, CGS2 and back-transformation against
     LAPACK's eigenvalues; eigenvectors by residual and orthonormality."""
     rng = np.random.default_rng(1000 + n)
