@@ -1119,8 +1119,9 @@ __global__ __launch_bounds__(256) void colnorm_kernel(const DevJob* __restrict__
 template <int NT, int NB>
 __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ jobs, double* __restrict__ Z,
                                                   const double* __restrict__ LU, int step, int c_count,
-                                                  const double* __restrict__ norms) {
+                                                  const double* __restrict__ norms, const int* __restrict__ done) {
   constexpr int NW = NT / 64;
+  if (done != nullptr && done[blockIdx.x] != 0) return;  // block finished by Cholesky QR
   const DevJob jb = jobs[blockIdx.x];
   const int n = jb.n, k = jb.k;
   const int c_begin = jb.pad + step * c_count;  // DevJob::pad: the first column to process
@@ -1845,7 +1846,73 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
   ctx->sync();  // the lists' device tables are released on return
 }
 
+// Cholesky factor and inverse of the nb x nb Gram block G = Z_b^T Z_b of each
+// job (one workgroup per job, goff[job] < 0: no block): G = R^T R and
+// Rinv = R^-1 (upper triangular, nb x nb at goff).  With `test`, column c is
+// numerically dependent when R_cc <= 1e-3 of its norm on entry (norms at
+// w_off + c0 + c); a job with a dependent column or a non-positive pivot, or
+// (without `test`) one whose ok flag is already 0, gets Rinv = I and ok = 0.
+template <int NB>
+__global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__ jobs, const int64_t* __restrict__ goff,
+                                                      const double* __restrict__ G, double* __restrict__ Rinv,
+                                                      const double* __restrict__ norms, int step, int c_count,
+                                                      int test, int* __restrict__ ok) {
+  extern __shared__ double sm[];
+  double* L = sm;                 // NB x (NB + 1): G, then R in the upper triangle
+  double* X = sm + NB * (NB + 1);  // NB x (NB + 1): R^-1
+  __shared__ int good;
+  const int64_t off = goff[blockIdx.x];
+  if (off < 0) return;
+  const DevJob jb = jobs[blockIdx.x];
+  const int c0 = jb.pad + step * c_count;
+  const int nb = min(c_count, jb.k - c0);
+  const int tid = threadIdx.x;
+  constexpr int LD = NB + 1;
+  for (int q = tid; q < nb * nb; q += 256) L[(q / nb) * LD + q % nb] = G[off + q];
+  if (tid == 0) good = test ? 1 : ok[blockIdx.x];
+  __syncthreads();
+  for (int a = 0; a < nb && good; ++a) {
+    if (tid == 0) {
+      const double dd = L[a * LD + a];
+      if (!(dd > 0) || (test && sqrt(dd) <= 1e-3 * norms[jb.w_off + c0 + a]))
+        good = 0;
+      else
+        L[a * LD + a] = sqrt(dd);
+    }
+    __syncthreads();
+    if (!good) break;
+    const double r = L[a * LD + a];
+    for (int b = a + 1 + tid; b < nb; b += 256) L[a * LD + b] /= r;
+    __syncthreads();
+    const int rem = nb - a - 1;
+    for (int q = tid; q < rem * rem; q += 256) {
+      const int b = a + 1 + q / rem, c = a + 1 + q % rem;
+      if (c >= b) L[b * LD + c] -= L[a * LD + b] * L[a * LD + c];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  const bool g = good != 0;
+  // column j of R^-1 by thread j: X_jj = 1 / R_jj, X_ij = -(sum_{l=i+1..j} R_il X_lj) / R_ii
+  if (g && tid < nb) {
+    const int j = tid;
+    X[j * LD + j] = 1.0 / L[j * LD + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double t = 0;
+      for (int l = i + 1; l <= j; ++l) t += L[i * LD + l] * X[l * LD + j];
+      X[i * LD + j] = -t / L[i * LD + i];
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < nb * nb; q += 256) {
+    const int i = q / nb, j = q % nb;
+    Rinv[off + q] = g ? (i <= j ? X[i * LD + j] : 0.0) : (i == j ? 1.0 : 0.0);
+  }
+  if (tid == 0) ok[blockIdx.x] = g ? 1 : 0;
+}
+
 constexpr int kOrthSeg = 256;  // rows per split-K segment of Z^T Z_block
+constexpr int kCholNB = 64;    // largest block finished by Cholesky QR
 
 // Columns per block of the projection contractions (TK_ORTH_BLOCK overrides, for tuning).
 int orth_block() {
@@ -1859,12 +1926,12 @@ int orth_block() {
 
 template <int NB>
 void launch_orth_range(Context* ctx, int nmax, int nj, const DevJob* djd, double* Z, const double* lu, int step,
-                       int c_count, const double* norms) {
+                       int c_count, const double* norms, const int* done) {
   constexpr int kNT = 512, NW = kNT / 64;
   const size_t lds = (size_t(nmax) * NB + NW * 64 * NB + 64 * NB + NW * NB + 2 * NB) * sizeof(double);
   allow_lds(orth_kernel<kNT, NB>, lds);
   hipLaunchKernelGGL((orth_kernel<kNT, NB>), dim3(nj), dim3(kNT), lds, ctx->stream, djd, Z, lu, step, c_count,
-                     norms);
+                     norms, done);
   TK_HIP(hipGetLastError());
 }
 
@@ -1901,6 +1968,29 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
   }
   DBuf<double> H(ctx, size_t(h_max));
   std::vector<std::unique_ptr<DeviceContractList>> keep;
+  // Cholesky QR scratch sized for the largest step: Gram blocks and R^-1
+  // (nb x nb per job), T (n x nb per job); per step and job the block offset
+  // (-1: none) and the success flag
+  int64_t cq_gt = 1, cq_tt = 1;
+  std::vector<int64_t> cq_offs(size_t(steps) * nj, -1);
+  for (int st = 0; st < steps; ++st) {
+    int64_t gt = 0, tt = 0;
+    for (int o = 0; o < nj; ++o) {
+      const DevJob& j = dj[size_t(o)];
+      const int c0 = j.pad + st * kOrthBlock;
+      if (c0 >= j.k) continue;
+      const int nb = std::min(kOrthBlock, j.k - c0);
+      cq_offs[size_t(st) * nj + o] = gt;
+      gt += int64_t(nb) * nb;
+      tt += int64_t(j.n) * nb;
+    }
+    cq_gt = std::max(cq_gt, gt);
+    cq_tt = std::max(cq_tt, tt);
+  }
+  DBuf<double> cq_g(ctx, size_t(cq_gt)), cq_r(ctx, size_t(cq_gt)), cq_t(ctx, size_t(cq_tt));
+  DBuf<int64_t> cq_off(ctx, std::max<size_t>(cq_offs.size(), 1));
+  if (!cq_offs.empty()) cq_off.upload(cq_offs.data(), cq_offs.size());
+  DBuf<int> cq_ok(ctx, std::max<size_t>(size_t(steps) * nj, 1));
   for (int st = 0; st < steps; ++st) {
     for (int pass = 0; pass < 2; ++pass) {
       ContractList lh, lu2;
@@ -1928,10 +2018,57 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
       run_list(ctx, lh, z_n, z_n, h_max, Z, Z, H.get(), keep);
       run_list(ctx, lu2, h_max, z_n, z_n, H.get(), Z, Z, keep);
     }
+    // within the block: Cholesky QR twice through grouped contractions,
+    // G = Z_b^T Z_b = R^T R, T = Z_b R^-1, then the same from T back into
+    // Z_b; a block with a numerically dependent column keeps Z_b (R^-1 = I)
+    // and is finished column by column by orth_kernel
+    const int* done = nullptr;
+    if (kOrthBlock <= kCholNB) {
+      int64_t gt = 0, tt = 0;
+      ContractList lg, lt, lg2, lz;
+      lg.flags = kTransA;
+      lg2.flags = kTransA;
+      for (int o = 0; o < nj; ++o) {
+        const DevJob& j = dj[size_t(o)];
+        const int c0 = j.pad + st * kOrthBlock;
+        if (c0 >= j.k) continue;
+        const int nb = std::min(kOrthBlock, j.k - c0);
+        lg.begin_out(gt, nb, nb, nb);
+        lg.begin_term(j.z_off + c0, j.k, j.n);
+        lg.add_src(j.z_off + c0, j.k, 1.0);
+        lt.begin_out(tt, j.n, nb, nb);
+        lt.begin_term(gt, nb, nb);
+        lt.add_src(j.z_off + c0, j.k, 1.0);
+        lg2.begin_out(gt, nb, nb, nb);
+        lg2.begin_term(tt, nb, j.n);
+        lg2.add_src(tt, nb, 1.0);
+        lz.begin_out(j.z_off + c0, j.n, nb, j.k);
+        lz.begin_term(gt, nb, nb);
+        lz.add_src(tt, nb, 1.0);
+        gt += int64_t(nb) * nb;
+        tt += int64_t(j.n) * nb;
+      }
+      if (gt > 0) {
+        const int64_t* offs = cq_off.get() + int64_t(st) * nj;
+        int* okf = cq_ok.get() + int64_t(st) * nj;
+        const size_t lds = 2 * size_t(kCholNB) * (kCholNB + 1) * sizeof(double);
+        allow_lds(cholinv_kernel<kCholNB>, lds);
+        run_list(ctx, lg, z_n, z_n, cq_gt, Z, Z, cq_g.get(), keep);
+        hipLaunchKernelGGL(cholinv_kernel<kCholNB>, dim3(unsigned(nj)), dim3(256), lds, ctx->stream, djd, offs,
+                           cq_g.get(), cq_r.get(), norms.get(), st, kOrthBlock, 1, okf);
+        run_list(ctx, lt, z_n, cq_gt, cq_tt, Z, cq_r.get(), cq_t.get(), keep);
+        run_list(ctx, lg2, cq_tt, cq_tt, cq_gt, cq_t.get(), cq_t.get(), cq_g.get(), keep);
+        hipLaunchKernelGGL(cholinv_kernel<kCholNB>, dim3(unsigned(nj)), dim3(256), lds, ctx->stream, djd, offs,
+                           cq_g.get(), cq_r.get(), norms.get(), st, kOrthBlock, 0, okf);
+        run_list(ctx, lz, cq_tt, cq_gt, z_n, cq_t.get(), cq_r.get(), Z, keep);
+        TK_HIP(hipGetLastError());
+        done = okf;
+      }
+    }
     if (nmax <= 1024)
-      launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get());
+      launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get(), done);
     else
-      launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get());
+      launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get(), done);
   }
   ctx->sync();  // the lists' tables and the scratch buffers are released on return
 }
