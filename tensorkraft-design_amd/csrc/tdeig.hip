@@ -1702,9 +1702,18 @@ __global__ __launch_bounds__(kWyNb) void wy_t_kernel(const WyBlock* __restrict__
     sc[l] = l < i ? s[int64_t(l) * kWyNb + i] : 0.0;
     __syncthreads();
     if (l < i) {
-      double acc = 0;
-      for (int m = l; m < i; ++m) acc += tl[l * (kWyNb + 1) + m] * sc[m];
-      tl[l * (kWyNb + 1) + i] = -tau[i] * acc;
+      // four partial sums, so the LDS reads of the row are issued ahead
+      const double* tr = tl + l * (kWyNb + 1);
+      double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+      int m = l;
+      for (; m + 3 < i; m += 4) {
+        a0 += tr[m] * sc[m];
+        a1 += tr[m + 1] * sc[m + 1];
+        a2 += tr[m + 2] * sc[m + 2];
+        a3 += tr[m + 3] * sc[m + 3];
+      }
+      for (; m < i; ++m) a0 += tr[m] * sc[m];
+      tl[l * (kWyNb + 1) + i] = -tau[i] * ((a0 + a1) + (a2 + a3));
     } else if (l == i) {
       tl[l * (kWyNb + 1) + i] = tau[i];
     }
@@ -1898,9 +1907,15 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
     const int j = tid;
     X[j * LD + j] = 1.0 / L[j * LD + j];
     for (int i = j - 1; i >= 0; --i) {
-      double t = 0;
-      for (int l = i + 1; l <= j; ++l) t += L[i * LD + l] * X[l * LD + j];
-      X[i * LD + j] = -t / L[i * LD + i];
+      const double* li = L + i * LD;
+      double t0 = 0, t1 = 0;
+      int l = i + 1;
+      for (; l + 1 <= j; l += 2) {
+        t0 += li[l] * X[l * LD + j];
+        t1 += li[l + 1] * X[(l + 1) * LD + j];
+      }
+      if (l <= j) t0 += li[l] * X[l * LD + j];
+      X[i * LD + j] = -(t0 + t1) / L[i * LD + i];
     }
   }
   __syncthreads();
