@@ -23,7 +23,15 @@ namespace {
 
 constexpr double kEps = DBL_EPSILON;
 constexpr int kLuFields = 5;
-constexpr int kBisLanes = 16;  // lanes per eigenvalue in the multisection  // largest batch dimension for the one-workgroup tridiagonalisation
+// lanes per eigenvalue in the multisection (TK_BIS_LANES = 4, 8 or 16 overrides, for tuning)
+int bis_lanes() {
+  static const int l = [] {
+    const char* e = std::getenv("TK_BIS_LANES");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 4 || v == 8 || v == 16) ? v : 16;
+  }();
+  return l;
+}
 
 struct DevJob {
   int64_t a_off;   // matrix / reflectors (and output columns in Z)
@@ -829,7 +837,7 @@ struct BisItem {
   int32_t job, j0;
 };
 
-template <int NT>
+template <int NT, int kBisLanes>
 __global__ __launch_bounds__(NT) void bisect_kernel(const BisItem* __restrict__ items, const DevJob* __restrict__ jobs,
                                                     const double* __restrict__ Dg, const double* __restrict__ Eg,
                                                     double* __restrict__ W, double* __restrict__ TN, int descending) {
@@ -2135,7 +2143,7 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   std::vector<BisItem> items;
   constexpr int kBisNT = 256;
   for (size_t o = 0; o < jobs.size(); ++o)
-    for (int j0 = 0; j0 < jobs[o].n; j0 += kBisNT / kBisLanes) items.push_back({int32_t(jobs.size() + o), j0});
+    for (int j0 = 0; j0 < jobs[o].n; j0 += kBisNT / bis_lanes()) items.push_back({int32_t(jobs.size() + o), j0});
   const size_t jb_bytes = dj.size() * sizeof(DevJob);
   const size_t tj_bytes = tj.size() * sizeof(TailJob);
   DBuf<char> meta(ctx_, jb_bytes + tj_bytes + items.size() * sizeof(BisItem));
@@ -2231,9 +2239,19 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   // the TN slot of a job is indexed by its position in the device job array
   DBuf<double> tn_all(ctx_, dj.size());
   const int tb = ctx_->timer.begin(ctx_->stream);
-  hipLaunchKernelGGL((bisect_kernel<kBisNT>), dim3(items.size()), dim3(kBisNT),
-                     2 * size_t(nmax_) * sizeof(double), ctx_->stream, itd, djd, D, E, W, tn_all.get(),
-                     descending ? 1 : 0);
+  {
+    const size_t lds = 2 * size_t(nmax_) * sizeof(double);
+    const int lanes = bis_lanes();
+    if (lanes == 4)
+      hipLaunchKernelGGL((bisect_kernel<kBisNT, 4>), dim3(items.size()), dim3(kBisNT), lds, ctx_->stream, itd, djd, D,
+                         E, W, tn_all.get(), descending ? 1 : 0);
+    else if (lanes == 8)
+      hipLaunchKernelGGL((bisect_kernel<kBisNT, 8>), dim3(items.size()), dim3(kBisNT), lds, ctx_->stream, itd, djd, D,
+                         E, W, tn_all.get(), descending ? 1 : 0);
+    else
+      hipLaunchKernelGGL((bisect_kernel<kBisNT, 16>), dim3(items.size()), dim3(kBisNT), lds, ctx_->stream, itd, djd,
+                         D, E, W, tn_all.get(), descending ? 1 : 0);
+  }
   ctx_->timer.end(tb, ctx_->stream, "bisection", 0.0, 0.0, 1);
   TK_HIP(hipGetLastError());
   TK_HIP(hipMemcpyAsync(tn_.get(), tn_all.get() + jobs.size(), jobs.size() * sizeof(double),
