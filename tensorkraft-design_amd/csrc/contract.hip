@@ -21,7 +21,9 @@ namespace tk {
 
 void ContractList::finalize() {
   items.clear();
+  witems.clear();
   flops = 0;
+  std::vector<GItem> expand;  // 32 x 32 tiling of the wide outputs
   for (size_t o = 0; o < outs.size(); ++o) {
     const GOut& g = outs[o];
     int64_t work = 0;
@@ -31,13 +33,25 @@ void ContractList::finalize() {
       work += int64_t(tm.k) * ns;
       flops += 2.0 * g.m * g.n * tm.k + 2.0 * g.m * tm.k * (ns - 1);
     }
+    const bool w = wide && g.n >= kWideMinN;
     for (int m0 = 0; m0 < g.m; m0 += 32)
       for (int n0 = 0; n0 < g.n; n0 += 32) {
         int64_t c = std::max<int64_t>(1, work * std::min(32, g.m - m0) * std::min(32, g.n - n0) / 64);
-        items.push_back({int32_t(o), m0, n0, int32_t(std::min<int64_t>(c, INT32_MAX))});
+        (w ? expand : items).push_back({int32_t(o), m0, n0, int32_t(std::min<int64_t>(c, INT32_MAX))});
       }
+    if (w)
+      for (int m0 = 0; m0 < g.m; m0 += 32)
+        for (int n0 = 0; n0 < g.n; n0 += 128) {
+          int64_t c = std::max<int64_t>(1, work * std::min(32, g.m - m0) * std::min(128, g.n - n0) / 64);
+          witems.push_back({int32_t(o), m0, n0, int32_t(std::min<int64_t>(c, INT32_MAX))});
+        }
   }
-  std::stable_sort(items.begin(), items.end(), [](const GItem& a, const GItem& b) { return a.cost > b.cost; });
+  auto lpt = [](const GItem& a, const GItem& b) { return a.cost > b.cost; };
+  std::stable_sort(items.begin(), items.end(), lpt);
+  std::stable_sort(expand.begin(), expand.end(), lpt);
+  std::stable_sort(witems.begin(), witems.end(), lpt);
+  n_narrow = int(items.size());
+  items.insert(items.end(), expand.begin(), expand.end());
 }
 
 void ContractList::validate(int64_t a_elems, int64_t b_elems, int64_t c_elems) const {
@@ -47,6 +61,9 @@ void ContractList::validate(int64_t a_elems, int64_t b_elems, int64_t c_elems) c
     return off + (rows - 1) * ld + cols;
   };
   for (const GItem& it : items) {
+    if (it.out < 0 || it.out >= int32_t(outs.size())) throw std::out_of_range("contract: item out of range");
+  }
+  for (const GItem& it : witems) {
     if (it.out < 0 || it.out >= int32_t(outs.size())) throw std::out_of_range("contract: item out of range");
   }
   for (const GOut& o : outs) {
@@ -200,26 +217,152 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
   }
 }
 
+// Workgroup items: four wavefronts own a 32 x 128 output tile (wavefront w:
+// columns n0 + 32 w ..).  Per inner chunk the workgroup forms the
+// source-summed A tile (32 rows x 4U inner indices, sources in list order)
+// once in LDS, double-buffered so one barrier per chunk suffices, and every
+// wavefront multiplies it by its own B columns held in registers: the A
+// operand (in H_eff stage 3 the MPO-weighted sum of T1 blocks) is read once
+// per four wavefront tiles.  Same k order and source order as the wavefront
+// kernel.
+template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
+__global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restrict__ items, int n_items,
+                                                            const GOut* __restrict__ outs,
+                                                            const GTerm* __restrict__ terms,
+                                                            const GSrc* __restrict__ srcs,
+                                                            const TI* __restrict__ A, const TI* __restrict__ B,
+                                                            TO* __restrict__ C, double beta) {
+  using M = Mfma<TI>;
+  using acc_t = typename M::acc_t;
+  constexpr int KC = 4 * U;              // inner indices per chunk
+  constexpr int PER = 32 * KC / 256;     // A-tile elements per thread
+  __shared__ TI As[2][32][KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  int buf = 0;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const GItem item = items[it];
+    const GOut o = outs[item.out];
+    const int nw0 = item.n0 + 32 * wv;  // this wavefront's first column
+    const bool live = nw0 < o.n;
+    const bool two_m = item.m0 + 16 < o.m;
+    const bool two_n = nw0 + 16 < o.n;
+    acc_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    const int q0 = nw0 + li, q1 = q0 + 16;
+    const bool in_q0 = live && q0 < o.n, in_q1 = live && q1 < o.n;
+    for (int t = o.t_begin; t < o.t_end; ++t) {
+      const GTerm tm = terms[t];
+      const TI* __restrict__ Bp = B + tm.b_off;
+      for (int k0 = 0; k0 < tm.k; k0 += KC) {
+        const int kb = k0 + U * lk;
+        TI b0[U], b1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kk = kb + u;
+          const bool kin = kk < tm.k;
+          b0[u] = (kin && in_q0) ? (TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0]) : TI(0);
+          b1[u] = (kin && in_q1) ? (TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1]) : TI(0);
+        }
+        TI av[PER];
+#pragma unroll
+        for (int h = 0; h < PER; ++h) av[h] = TI(0);
+        for (int s = tm.s_begin; s < tm.s_end; ++s) {
+          const GSrc sr = srcs[s];
+          const TI* __restrict__ Ap = A + sr.a_off;
+          const TI cf = TI(sr.coef);
+          TI x[PER];
+#pragma unroll
+          for (int h = 0; h < PER; ++h) {
+            const int e = tid + 256 * h;
+            const int row = TA ? (e & 31) : e / KC, kk = TA ? e / 32 : e % KC;
+            const int gr = item.m0 + row, gk = k0 + kk;
+            x[h] = (gr < o.m && gk < tm.k) ? (TA ? Ap[int64_t(gk) * sr.lda + gr] : Ap[int64_t(gr) * sr.lda + gk])
+                                           : TI(0);
+          }
+#pragma unroll
+          for (int h = 0; h < PER; ++h) av[h] += cf * x[h];
+        }
+#pragma unroll
+        for (int h = 0; h < PER; ++h) {
+          const int e = tid + 256 * h;
+          const int row = TA ? (e & 31) : e / KC, kk = TA ? e / 32 : e % KC;
+          As[buf][row][kk] = av[h];
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const TI a0 = As[buf][li][U * lk + u];
+            const TI a1 = As[buf][li + 16][U * lk + u];
+            c00 = M::run(a0, b0[u], c00);
+            if (two_n) c01 = M::run(a0, b1[u], c01);
+            if (two_m) {
+              c10 = M::run(a1, b0[u], c10);
+              if (two_n) c11 = M::run(a1, b1[u], c11);
+            }
+          }
+        }
+        buf ^= 1;
+      }
+    }
+    if (live) {
+      TO* __restrict__ Cp = C + o.c_off;
+      auto put = [&](int row, int col, TI v) {
+        if (row < o.m && col < o.n) {
+          TO* dst = TC ? &Cp[int64_t(col) * o.ldc + row] : &Cp[int64_t(row) * o.ldc + col];
+          *dst = beta == 0.0 ? TO(v) : TO(v) + TO(beta) * *dst;
+        }
+      };
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = item.m0 + M::row(lk, r);
+        const int col = nw0 + li;
+        put(row, col, c00[r]);
+        put(row, col + 16, c01[r]);
+        put(row + 16, col, c10[r]);
+        put(row + 16, col + 16, c11[r]);
+      }
+    }
+  }
+}
+
 template <typename T> struct KUnroll;
 template <> struct KUnroll<double> { static constexpr int value = 4; };
 template <> struct KUnroll<float> { static constexpr int value = 8; };
 
 template <typename TI, typename TO>
 void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
-  if (L.n_items == 0) return;
-  const int waves_per_block = 4;
-  int blocks = (L.n_items + waves_per_block - 1) / waves_per_block;
-  blocks = std::min(blocks, ctx->n_cu * 8);
-  dim3 grid(blocks), block(64 * waves_per_block);
+  // wavefront items: all of them, or only the narrow ones when the list has
+  // workgroup items for its wide outputs
+  const int n_wave = L.n_witems > 0 ? L.n_narrow : L.n_items;
+  if (n_wave > 0) {
+    const int waves_per_block = 4;
+    int blocks = (n_wave + waves_per_block - 1) / waves_per_block;
+    blocks = std::min(blocks, ctx->n_cu * 8);
+    dim3 grid(blocks), block(64 * waves_per_block);
 #define TK_CASE(F)                                                                                     \
   case F:                                                                                              \
     hipLaunchKernelGGL((contract_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC), KUnroll<TI>::value>), \
-                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C, L.beta); \
+                       grid, block, 0, ctx->stream, L.items, n_wave, L.outs, L.terms, L.srcs, A, B, C, L.beta); \
     break;
-  switch (L.flags & 7) {
-    TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
-  }
+    switch (L.flags & 7) {
+      TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+    }
 #undef TK_CASE
+  }
+  if (L.n_witems > 0) {
+    const int blocks = std::min(L.n_witems, ctx->n_cu * 4);
+#define TK_CASE(F)                                                                                     \
+  case F:                                                                                              \
+    hipLaunchKernelGGL((contract_wide_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC), KUnroll<TI>::value>), \
+                       dim3(blocks), dim3(256), 0, ctx->stream, L.witems, L.n_witems, L.outs, L.terms, L.srcs, A, B, C, \
+                       L.beta);                                                                        \
+    break;
+    switch (L.flags & 7) {
+      TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+    }
+#undef TK_CASE
+  }
   TK_HIP(hipGetLastError());
 }
 
@@ -228,12 +371,15 @@ void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const 
 void DeviceContractList::upload(Context* ctx, const ContractList& h) {
   auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
   size_t o_items = 0;
-  size_t o_outs = align(o_items + h.items.size() * sizeof(GItem));
+  size_t o_witems = align(o_items + h.items.size() * sizeof(GItem));
+  size_t o_outs = align(o_witems + h.witems.size() * sizeof(GItem));
   size_t o_terms = align(o_outs + h.outs.size() * sizeof(GOut));
   size_t o_srcs = align(o_terms + h.terms.size() * sizeof(GTerm));
   size_t total = align(o_srcs + h.srcs.size() * sizeof(GSrc));
   host.assign(total, 0);
   std::copy_n(reinterpret_cast<const char*>(h.items.data()), h.items.size() * sizeof(GItem), host.data() + o_items);
+  std::copy_n(reinterpret_cast<const char*>(h.witems.data()), h.witems.size() * sizeof(GItem),
+              host.data() + o_witems);
   std::copy_n(reinterpret_cast<const char*>(h.outs.data()), h.outs.size() * sizeof(GOut), host.data() + o_outs);
   std::copy_n(reinterpret_cast<const char*>(h.terms.data()), h.terms.size() * sizeof(GTerm), host.data() + o_terms);
   std::copy_n(reinterpret_cast<const char*>(h.srcs.data()), h.srcs.size() * sizeof(GSrc), host.data() + o_srcs);
@@ -241,10 +387,13 @@ void DeviceContractList::upload(Context* ctx, const ContractList& h) {
   buf.upload(host.data(), total);
   char* base = buf.get();
   items = reinterpret_cast<GItem*>(base + o_items);
+  witems = reinterpret_cast<GItem*>(base + o_witems);
   outs = reinterpret_cast<GOut*>(base + o_outs);
   terms = reinterpret_cast<GTerm*>(base + o_terms);
   srcs = reinterpret_cast<GSrc*>(base + o_srcs);
   n_items = int(h.items.size());
+  n_narrow = h.witems.empty() ? n_items : h.n_narrow;
+  n_witems = int(h.witems.size());
   flags = h.flags;
   flops = h.flops;
   beta = h.beta;

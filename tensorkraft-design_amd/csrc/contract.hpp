@@ -42,12 +42,22 @@ struct GItem {
 // Operand transposition flags of one launch.
 enum ContractFlags : int { kTransA = 1, kTransB = 2, kTransC = 4 };
 
+// Narrowest output (columns) tiled into workgroup items when a list is wide.
+constexpr int kWideMinN = 96;
+
 // Host-side list builder.
 struct ContractList {
   std::vector<GOut> outs;
   std::vector<GTerm> terms;
   std::vector<GSrc> srcs;
   std::vector<GItem> items;
+  // With `wide`, outputs of at least kWideMinN columns are also tiled into
+  // 32 x 128 workgroup items (`witems`); items[0, n_narrow) then tile only the
+  // other outputs, and the rest of `items` tiles the wide outputs in 32 x 32
+  // for kernels without a workgroup form.
+  bool wide = false;
+  std::vector<GItem> witems;
+  int n_narrow = 0;
   int flags = 0;
   double beta = 0;   // C = sum + beta * C (0: C is not read)
   double flops = 0;  // algorithmic flops of the list (2*m*n*k per term + 2*m*k per extra source)
@@ -78,7 +88,8 @@ struct ContractList {
   // buffers (throws std::out_of_range); run before a list is first launched.
   void validate(int64_t a_elems, int64_t b_elems, int64_t c_elems) const;
   void clear() {
-    outs.clear(); terms.clear(); srcs.clear(); items.clear();
+    outs.clear(); terms.clear(); srcs.clear(); items.clear(); witems.clear();
+    n_narrow = 0;
     flops = 0;
   }
 };

@@ -189,6 +189,7 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
         dslices_.push_back(ds);
       }
   }
+  h3_.wide = true;
   h3_.finalize();
 
   h1_.clear();
@@ -198,6 +199,7 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
     h1_.begin_term(r.b_off, r.n, r.k);
     h1_.add_src(r.a_off, r.lda, 1.0);
   }
+  h1_.wide = true;
   h1_.finalize();
   h1_.validate(l_total_, psi_.total, t1_total);
   h3_.validate(t1_total, r_total_, psi_.total);

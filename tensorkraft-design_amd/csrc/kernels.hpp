@@ -16,7 +16,10 @@ struct DeviceContractList {
   const GOut* outs = nullptr;
   const GTerm* terms = nullptr;
   const GSrc* srcs = nullptr;
+  const GItem* witems = nullptr;
   int n_items = 0;
+  int n_narrow = 0;
+  int n_witems = 0;
   int flags = 0;
   double flops = 0;
   double beta = 0;  // C = sum + beta * C
