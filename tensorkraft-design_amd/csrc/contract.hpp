@@ -54,7 +54,9 @@ struct ContractList {
   // With `wide`, outputs of at least kWideMinN columns are also tiled into
   // 32 x 128 workgroup items (`witems`); items[0, n_narrow) then tile only the
   // other outputs, and the rest of `items` tiles the wide outputs in 32 x 32
-  // for kernels without a workgroup form.
+  // for kernels without a workgroup form.  The H_eff stages set it; on the
+  // single-source Gram and projection lists of the eigensolver it measured no
+  // faster.
   bool wide = false;
   std::vector<GItem> witems;
   int n_narrow = 0;

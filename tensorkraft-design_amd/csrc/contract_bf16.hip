@@ -6,7 +6,8 @@
 // v_mfma_f32_16x16x32_bf16.  The dropped lo*lo term bounds the relative error
 // of each product near 2^-16; the Krylov solver wins FP64 accuracy back by
 // FP64 residual refinement.  Work-list format and tiling are those of
-// contract.hip (one wavefront per 32x32 output tile).
+// contract.hip (one wavefront per 32x32 output tile, or four per 32x128
+// workgroup item for the wide outputs).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -119,20 +120,144 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __res
   }
 }
 
+// Workgroup items (32 x 128, one wavefront per 32 columns): the
+// source-summed A chunk (32 rows x 32 inner indices) is formed once per
+// workgroup in LDS (double-buffered, one barrier per chunk), as in the FP64 /
+// FP32 workgroup kernel; each wavefront splits it and its own B columns.
+template <typename TO, bool TA, bool TB, bool TC>
+__global__ __launch_bounds__(256) void contract_bf16x3_wide_kernel(const GItem* __restrict__ items, int n_items,
+                                                                   const GOut* __restrict__ outs,
+                                                                   const GTerm* __restrict__ terms,
+                                                                   const GSrc* __restrict__ srcs,
+                                                                   const float* __restrict__ A,
+                                                                   const float* __restrict__ B, TO* __restrict__ C,
+                                                                   double beta) {
+  constexpr int KC = 32, PER = 32 * KC / 256;
+  __shared__ float As[2][32][KC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  int buf = 0;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const GItem item = items[it];
+    const GOut o = outs[item.out];
+    const int nw0 = item.n0 + 32 * wv;
+    const bool live = nw0 < o.n;
+    const bool two_m = item.m0 + 16 < o.m;
+    const bool two_n = nw0 + 16 < o.n;
+    v4f c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    const int q0 = nw0 + li, q1 = q0 + 16;
+    for (int t = o.t_begin; t < o.t_end; ++t) {
+      const GTerm tm = terms[t];
+      const float* __restrict__ Bp = B + tm.b_off;
+      for (int k0 = 0; k0 < tm.k; k0 += KC) {
+        float b0[8], b1[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int kk = k0 + 8 * lk + i;
+          const bool kin = live && kk < tm.k;
+          b0[i] = (kin && q0 < o.n) ? (TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0]) : 0.f;
+          b1[i] = (kin && q1 < o.n) ? (TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1]) : 0.f;
+        }
+        float av[PER];
+#pragma unroll
+        for (int h = 0; h < PER; ++h) av[h] = 0.f;
+        for (int s = tm.s_begin; s < tm.s_end; ++s) {
+          const GSrc sr = srcs[s];
+          const float* __restrict__ Ap = A + sr.a_off;
+          const float cf = float(sr.coef);
+          float x[PER];
+#pragma unroll
+          for (int h = 0; h < PER; ++h) {
+            const int e = tid + 256 * h;
+            const int row = TA ? (e & 31) : e / KC, kk = TA ? e / 32 : e % KC;
+            const int gr = item.m0 + row, gk = k0 + kk;
+            x[h] = (gr < o.m && gk < tm.k) ? (TA ? Ap[int64_t(gk) * sr.lda + gr] : Ap[int64_t(gr) * sr.lda + gk])
+                                           : 0.f;
+          }
+#pragma unroll
+          for (int h = 0; h < PER; ++h) av[h] += cf * x[h];
+        }
+#pragma unroll
+        for (int h = 0; h < PER; ++h) {
+          const int e = tid + 256 * h;
+          const int row = TA ? (e & 31) : e / KC, kk = TA ? e / 32 : e % KC;
+          As[buf][row][kk] = av[h];
+        }
+        __syncthreads();
+        if (live) {
+          float a0[8], a1[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            a0[i] = As[buf][li][8 * lk + i];
+            a1[i] = As[buf][li + 16][8 * lk + i];
+          }
+          bf16x8 a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l;
+          split8(a0, a0h, a0l);
+          split8(b0, b0h, b0l);
+          c00 = mfma3(a0h, a0l, b0h, b0l, c00);
+          if (two_n) {
+            split8(b1, b1h, b1l);
+            c01 = mfma3(a0h, a0l, b1h, b1l, c01);
+          }
+          if (two_m) {
+            split8(a1, a1h, a1l);
+            c10 = mfma3(a1h, a1l, b0h, b0l, c10);
+            if (two_n) c11 = mfma3(a1h, a1l, b1h, b1l, c11);
+          }
+        }
+        buf ^= 1;
+      }
+    }
+    if (live) {
+      TO* __restrict__ Cp = C + o.c_off;
+      auto put = [&](int row, int col, float v) {
+        if (row < o.m && col < o.n) {
+          TO* dst = TC ? &Cp[int64_t(col) * o.ldc + row] : &Cp[int64_t(row) * o.ldc + col];
+          *dst = beta == 0.0 ? TO(v) : TO(v) + TO(beta) * *dst;
+        }
+      };
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = item.m0 + 4 * lk + r;
+        const int col = nw0 + li;
+        put(row, col, c00[r]);
+        put(row, col + 16, c01[r]);
+        put(row + 16, col, c10[r]);
+        put(row + 16, col + 16, c11[r]);
+      }
+    }
+  }
+}
+
 template <typename TO>
 void launch_bf16(Context* ctx, const DeviceContractList& L, const float* A, const float* B, TO* C) {
-  if (L.n_items == 0) return;
-  int blocks = std::min((L.n_items + 3) / 4, ctx->n_cu * 8);
-  dim3 grid(blocks), block(256);
+  const int n_wave = L.n_witems > 0 ? L.n_narrow : L.n_items;
+  if (n_wave > 0) {
+    int blocks = std::min((n_wave + 3) / 4, ctx->n_cu * 8);
+    dim3 grid(blocks), block(256);
 #define TK_CASE(F)                                                                                         \
   case F:                                                                                                  \
     hipLaunchKernelGGL((contract_bf16x3_kernel<TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
-                       grid, block, 0, ctx->stream, L.items, L.n_items, L.outs, L.terms, L.srcs, A, B, C, L.beta); \
+                       grid, block, 0, ctx->stream, L.items, n_wave, L.outs, L.terms, L.srcs, A, B, C, L.beta); \
     break;
-  switch (L.flags & 7) {
-    TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
-  }
+    switch (L.flags & 7) {
+      TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+    }
 #undef TK_CASE
+  }
+  if (L.n_witems > 0) {
+    const int blocks = std::min(L.n_witems, ctx->n_cu * 4);
+#define TK_CASE(F)                                                                                         \
+  case F:                                                                                                  \
+    hipLaunchKernelGGL((contract_bf16x3_wide_kernel<TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
+                       dim3(blocks), dim3(256), 0, ctx->stream, L.witems, L.n_witems, L.outs, L.terms, L.srcs, A, B, \
+                       C, L.beta);                                                                         \
+    break;
+    switch (L.flags & 7) {
+      TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+    }
+#undef TK_CASE
+  }
   TK_HIP(hipGetLastError());
 }
 
