@@ -24,7 +24,10 @@ C ABI of libtk_hip.so:
   forward, backward and Adam kernels (tk_mlp_*).
 """
 import ctypes as C
+import json
 import math
+import os
+from dataclasses import asdict, dataclass, field
 
 import numpy as np
 
@@ -486,6 +489,39 @@ def features(delta):
 
 
 # ---------------------------------------------------------------- DMFT loop
+@dataclass
+class DMFTCheckpoint:
+    """Serializable DMFT checkpoint, the reference's DMFTCheckpoint
+    (tk-dmft/src/loop/mod.rs:449-488): JSON with the same field names,
+    written atomically ({path}.tmp, then rename).  The bath is the chain
+    bath of the last completed iteration; this loop works on the Matsubara
+    axis, so the spectral fields stay empty unless a caller fills them."""
+    iteration: int
+    converged: bool
+    config_json: str
+    bath_epsilon: list
+    bath_v: list
+    spectral_omega: list = field(default_factory=list)
+    spectral_values: list = field(default_factory=list)
+
+    def write_to_file(self, path):
+        path = os.fspath(path)
+        base, _ = os.path.splitext(path)
+        tmp = base + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump(asdict(self), fh, indent=2)
+        os.replace(tmp, path)
+
+    @staticmethod
+    def read_from_file(path):
+        with open(os.fspath(path)) as fh:
+            d = json.load(fh)
+        return DMFTCheckpoint(int(d["iteration"]), bool(d["converged"]), str(d["config_json"]),
+                              [float(x) for x in d["bath_epsilon"]], [float(x) for x in d["bath_v"]],
+                              [float(x) for x in d.get("spectral_omega", [])],
+                              [float(x) for x in d.get("spectral_values", [])])
+
+
 class DMFTLoop:
     """Bethe-lattice DMFT (loop/mod.rs:107-419) on the Matsubara axis.
 
@@ -501,8 +537,12 @@ class DMFTLoop:
     """
 
     def __init__(self, ctx, n_bath=7, U=4.0, half_bandwidth=2.0, beta=50.0, n_matsubara=256, bond_dim=256,
-                 mixing="broyden", alpha=0.5, history=5, tol=1e-4, max_iterations=40):
+                 mixing="broyden", alpha=0.5, history=5, tol=1e-4, max_iterations=40, checkpoint_path=None):
         self.ctx = ctx
+        self.checkpoint_path = checkpoint_path
+        self.config = {"n_bath": n_bath, "U": U, "half_bandwidth": half_bandwidth, "beta": beta,
+                       "n_matsubara": n_matsubara, "bond_dim": bond_dim, "mixing": mixing, "alpha": alpha,
+                       "history": history, "tol": tol, "max_iterations": max_iterations}
         self.n_bath, self.U = n_bath, U
         self.t = half_bandwidth / 2.0
         self.wn = matsubara(beta, n_matsubara)
@@ -510,6 +550,24 @@ class DMFTLoop:
         self.mixing, self.alpha, self.history_depth = mixing, alpha, history
         self.tol, self.max_iterations = tol, max_iterations
         self.history = []
+
+    def checkpoint(self, iteration, chain, converged):
+        """Write the DMFTCheckpoint of a completed iteration when a
+        checkpoint path is set (after every iteration, as loop/mod.rs:211-218
+        intends)."""
+        if self.checkpoint_path is None:
+            return
+        DMFTCheckpoint(int(iteration), bool(converged), json.dumps(self.config), [float(x) for x in chain[0]],
+                       [float(x) for x in chain[1]]).write_to_file(self.checkpoint_path)
+
+    def resume_from_checkpoint(self, cp):
+        """The chain bath to restart from (loop/mod.rs:271-300): checks the
+        bath size against this loop's."""
+        if len(cp.bath_epsilon) != len(cp.bath_v):
+            raise ValueError("checkpoint bath_epsilon and bath_v must have the same length")
+        if len(cp.bath_epsilon) != self.n_bath:
+            raise ValueError(f"checkpoint bath size ({len(cp.bath_epsilon)}) must match the loop's ({self.n_bath})")
+        return (np.array(cp.bath_epsilon, dtype=np.float64), np.array(cp.bath_v, dtype=np.float64))
 
     def delta_chain(self, chain):
         eps, v = chain
@@ -567,6 +625,7 @@ class DMFTLoop:
             d_new, star_new, target, res = self.step(chain, star)
             rec = {"iteration": it, "residual": res, "delta_new": d_new}
             self.history.append(rec)
+            self.checkpoint(it, chain, res < self.tol)
             if pending is not None:
                 r0, c0, t0, s0 = pending
                 pending = None

@@ -1093,3 +1093,64 @@ class Dmrg:
         _check(self.ctx.L.tk_dmrg_site(self.h, i, n.value, C.byref(n), out.ctypes.data_as(C.POINTER(C.c_double))),
                self.ctx.h)
         return out
+
+    def checkpoint(self, sweep_index, energy, stats=None):
+        """The engine's MPS (every bond QIndex and site tensor) with the
+        reference checkpoint's metadata (tk-dmrg/src/checkpoint.rs:16-24)."""
+        bonds = [self.bond(i) for i in range(self.n + 1)]
+        sites = [self.site(i) for i in range(self.n)]
+        return DMRGCheckpoint(int(sweep_index), float(energy), dict(stats or {}), bonds, sites)
+
+    def restore(self, cp, prepare=True):
+        """Resume from a DMRGCheckpoint: the saved MPS becomes the state
+        (then right-canonicalised and the environments rebuilt unless
+        prepare=False)."""
+        if len(cp.sites) != self.n or len(cp.bonds) != self.n + 1:
+            raise ValueError(f"checkpoint of {len(cp.sites)} sites for a chain of {self.n}")
+        self.set_state(cp.bonds, cp.sites)
+        if prepare:
+            self.prepare()
+
+
+@dataclass
+class DMRGCheckpoint:
+    """DMRG engine state on disk.  The reference's DMRGCheckpoint
+    (tk-dmrg/src/checkpoint.rs:16-40) holds sweep_index, energy and stats and
+    leaves the tensors to a later serde format; here the MPS is stored too:
+    every bond as (charges, dims) and every site tensor in FlatBlockStorage
+    order.  File format: numpy .npz without pickled objects (arrays
+    n, sweep_index, energy, stats_keys, stats_values, bond_<i>_q (k x 2
+    int32), bond_<i>_d (int64), site_<i> (float64)), written atomically to a
+    temporary file and renamed, as the reference's write_to_file."""
+    sweep_index: int
+    energy: float
+    stats: dict
+    bonds: list
+    sites: list
+
+    def write_to_file(self, path):
+        arrs = {"n": np.array(len(self.sites), dtype=np.int64),
+                "sweep_index": np.array(self.sweep_index, dtype=np.int64),
+                "energy": np.array(self.energy, dtype=np.float64),
+                "stats_keys": np.array(sorted(self.stats), dtype=np.str_),
+                "stats_values": np.array([float(self.stats[k]) for k in sorted(self.stats)], dtype=np.float64)}
+        for i, b in enumerate(self.bonds):
+            arrs[f"bond_{i}_q"] = np.array([list(_qkey(q)) for q, _ in b], dtype=np.int32).reshape(-1, 2)
+            arrs[f"bond_{i}_d"] = np.array([int(d) for _, d in b], dtype=np.int64)
+        for i, x in enumerate(self.sites):
+            arrs[f"site_{i}"] = np.ascontiguousarray(x, dtype=np.float64)
+        path = os.fspath(path)
+        tmp = path + ".tmp"
+        with open(tmp, "wb") as fh:
+            np.savez(fh, **arrs)
+        os.replace(tmp, path)
+
+    @staticmethod
+    def read_from_file(path):
+        with np.load(os.fspath(path), allow_pickle=False) as z:
+            n = int(z["n"])
+            bonds = [[((int(q[0]), int(q[1])), int(d)) for q, d in zip(z[f"bond_{i}_q"], z[f"bond_{i}_d"])]
+                     for i in range(n + 1)]
+            sites = [np.array(z[f"site_{i}"]) for i in range(n)]
+            stats = {str(k): float(v) for k, v in zip(z["stats_keys"], z["stats_values"])}
+            return DMRGCheckpoint(int(z["sweep_index"]), float(z["energy"]), stats, bonds, sites)

@@ -178,3 +178,21 @@ def test_dmft_loop_and_learned_proposal(ctx):
         tot_learned += nb
     print(f"total: plain {tot_plain}, learned {tot_learned}", flush=True)
     assert tot_learned < tot_plain
+
+
+def test_dmft_checkpoint_written_and_resumed(ctx, tmp_path):
+    """A loop with a checkpoint path writes the reference's JSON after every
+    iteration; resume_from_checkpoint returns that bath and rejects a bath
+    of another size (loop/mod.rs:282-293)."""
+    print("case dmft checkpoint", flush=True)
+    path = tmp_path / "dmft.json"
+    loop = D.DMFTLoop(ctx, n_bath=3, U=4.0, bond_dim=64, max_iterations=2, tol=1e-12, checkpoint_path=path)
+    chain, n, ok = loop.solve()
+    cp = D.DMFTCheckpoint.read_from_file(path)
+    assert cp.iteration == 1 and not cp.converged
+    assert len(cp.bath_epsilon) == 3 and len(cp.bath_v) == 3
+    eps, v = loop.resume_from_checkpoint(cp)
+    assert eps.shape == (3,) and np.all(np.isfinite(v))
+    bad = D.DMFTCheckpoint(0, False, "{}", [0.0] * 4, [0.5] * 4)
+    with pytest.raises(ValueError):
+        loop.resume_from_checkpoint(bad)

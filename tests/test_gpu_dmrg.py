@@ -132,3 +132,25 @@ def test_gpu_dmrg_cylinder_mixed_matches_fp64(ctx, prec):
     dlo, slo = _gpu(ctx, mpo, sym, bonds, data, (32, 64), 4, prec=prec, tol=1e-10)
     assert abs(slo[-1]["energy"] - s64[-1]["energy"]) <= 1e-10
     assert abs(slo[-1]["max_truncation_error"] - s64[-1]["max_truncation_error"]) <= 1e-9
+
+
+def test_gpu_dmrg_checkpoint_resume(ctx, tmp_path):
+    """Two sweeps, checkpoint to disk, a new engine resumes: the saved MPS
+    comes back bit-exact and two more sweeps reach the ED energy."""
+    print("case dmrg checkpoint", flush=True)
+    n = 10
+    mpo, sym, bonds, data = _setup("heisenberg", n, (0, 0))
+    d1, s1 = _gpu(ctx, mpo, sym, bonds, data, (16, 32), 2)
+    cp = d1.checkpoint(2, s1[-1]["energy"], {"max_bond_dim": s1[-1]["max_bond_dim"]})
+    path = tmp_path / "dmrg.npz"
+    cp.write_to_file(path)
+    back = tk_hip.DMRGCheckpoint.read_from_file(path)
+    d2 = tk_hip.Dmrg(ctx, mpo, sym, schedule=(32,), davidson_tol=1e-12, svd_cutoff=1e-12)
+    d2.restore(back, prepare=False)
+    for i in range(n):
+        assert np.array_equal(d2.site(i), d1.site(i))
+    for i in range(n + 1):
+        assert d2.bond(i) == d1.bond(i)
+    d2.prepare()
+    st = [d2.sweep() for _ in range(2)]
+    assert abs(st[-1]["energy"] - heisenberg_ground_energy(n)) < 1e-9
