@@ -9,6 +9,9 @@ Workloads
   heff   one H_eff*psi on the U(1)xU(1) Hubbard microbenchmark bond
          (d=4, MPO bond 6, D=1024 on both sides, 145 sectors); a step is one
          matvec with inputs resident in HBM.
+  sweep-c4 two-site DMRG on the J1 Heisenberg 6 x 12 cylinder (U(1), Sz=0),
+         D 256 -> 4096 over 5 sweeps, in FP64, FP32 and split-BF16 from the
+         same seeded MPS; a step is the sweep at D=4096.
   heff-c4  one H_eff*psi at the centre bond of the J1 Heisenberg 6 x 12
          cylinder (snake order, MPO bond 20, U(1) D=4096 in 31 sectors);
          FP64, FP32 and split-BF16 plans compared against the FP64 output.
@@ -250,18 +253,30 @@ def measured_traffic(name):
 
 
 def roofline_of(timing):
-    """Roofline of the kernel with the largest event time: algorithmic bytes
-    per launch over the mean launch duration, against the HBM peak; traffic is
-    the profiled HBM bytes per launch of that kernel when a summary exists."""
+    """Roofline of the kernel with the largest event time: against the HBM
+    peak (algorithmic bytes per launch over the mean launch duration), or,
+    for a contraction whose algorithmic flops per byte exceed the ridge of
+    its precision (the kernel name's suffix), against that MFMA peak (for
+    split-BF16 the dense BF16 peak, each product counted once although it
+    takes three BF16 MFMAs); traffic is the profiled HBM bytes per launch of
+    that kernel when a summary exists."""
     name = max((k for k in timing if not k.startswith("phase_")), key=lambda k: timing[k]["ms"])
     t = timing[name]
     sec = t["ms"] * 1e-3
+    common = {"traffic": measured_traffic(name), "kernel": name, "launches": t["launches"],
+              "mean_launch_us": round(1e3 * t["ms"] / max(1, t["launches"]), 3),
+              "bytes_per_launch": round(t["bytes"] / max(1, t["launches"]), 1),
+              "tflops": round(t["flops"] / sec / 1e12, 3) if sec > 0 else 0.0}
+    prec = next((p for p in MFMA_PEAK_TFLOPS if name.endswith("_" + p)), None)
+    if prec is not None and t["bytes"] > 0 and sec > 0:
+        peak = MFMA_PEAK_TFLOPS[prec]
+        if t["flops"] / t["bytes"] > peak * 1e12 / (HBM_PEAK_GBS * 1e9):
+            achieved = t["flops"] / sec / 1e12
+            return dict({"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                         "frac": round(achieved / peak, 4)}, **common)
     achieved = t["bytes"] / sec / 1e9 if sec > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name), "kernel": name,
-            "launches": t["launches"], "mean_launch_us": round(1e3 * t["ms"] / max(1, t["launches"]), 3),
-            "bytes_per_launch": round(t["bytes"] / max(1, t["launches"]), 1),
-            "tflops": round(t["flops"] / sec / 1e12, 3) if sec > 0 else 0.0}
+    return dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 4)}, **common)
 
 
 def kernel_table(timing):
@@ -378,18 +393,74 @@ def bench_sweep(args):
     return rec
 
 
+def bench_sweep_c4(args):
+    """Config-4 DMRG: J1 Heisenberg on the 6 x 12 cylinder (snake order, MPO
+    bond dims up to 20, U(1), Sz = 0), D ramped to 4096, the same seeded MPS
+    run in FP64, FP32-MFMA and split-BF16 (both with FP64 refinement); the
+    timed step is the sweep at the largest D.  value = split-BF16 sweep time."""
+    import tk_hip
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    sched = warmup_schedule(args.d0, args.dmax, args.ramp)
+    mpo = tk_hip.cylinder_mpo(6, 12)
+    n = len(mpo)
+    sym = tk_hip.SYM_U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (0, 0), sched[0], seed=0, min_sector_dim=0)
+    ctx = tk_hip.Context(0)
+    res = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        res[prec] = run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, args.warmup, args.steps, prec, args.tol,
+                               log)
+    s64 = res["fp64"][0]
+    per = {}
+    for prec, (st, wall, tim, _) in res.items():
+        per[prec] = {"seconds_per_sweep": round(wall / args.steps, 4), "energy": st[-1]["energy"],
+                     "abs_delta_e_vs_fp64": abs(st[-1]["energy"] - s64[-1]["energy"]),
+                     "max_truncation_error": st[-1]["max_truncation_error"],
+                     "max_bond_dim": st[-1]["max_bond_dim"],
+                     "matvecs": sum(x["matvecs"] for x in st[args.warmup:]),
+                     "matvecs_fp64": sum(x["matvecs_fp64"] for x in st[args.warmup:])}
+    slo, wlo, timlo, _ = res["bf16x3"]
+    tlo = wlo / args.steps
+    rec = {
+        "metric": "DMRG sweep time, J1 Heisenberg 6x12 cylinder D=4096, split-BF16 vs FP32-MFMA vs FP64",
+        "value": round(tlo, 4),
+        "unit": "s/sweep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * tlo, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3-f32acc (f64 Krylov, SVD, refinement)",
+        "data": "synthetic",
+        "config": {"workload": "dmrg_sweep_cylinder_6x12_D4096", "sites": n, "J": 1.0, "lattice": "cylinder 6 x 12",
+                   "ordering": "snake", "symmetry": "U(1)", "sz": 0, "schedule": sched,
+                   "timed_sweep_bond_dim": sched[-1], "davidson_tol": args.tol, "svd_cutoff": 1e-12,
+                   "initial_mps": "PCG64 seed 0, binomially concentrated charges"},
+        "roofline": roofline_of(timlo),
+        "precisions": per,
+        "kernels_bf16x3": kernel_table(timlo),
+    }
+    ctx.close()
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "heff", "heff-c4", "copy"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "sweep-c4", "heff", "heff-c4", "copy"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--sites", type=int, default=64)
     ap.add_argument("--d0", type=int, default=256)
-    ap.add_argument("--dmax", type=int, default=2048)
-    ap.add_argument("--ramp", type=int, default=7)
+    ap.add_argument("--dmax", type=int, default=None)
+    ap.add_argument("--ramp", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -404,7 +475,15 @@ def main():
         args.steps = 200 if args.steps is None else args.steps
         args.warmup = 20 if args.warmup is None else args.warmup
         rec = bench_heff(args)
+    elif args.workload == "sweep-c4":
+        args.dmax = 4096 if args.dmax is None else args.dmax
+        args.ramp = 4 if args.ramp is None else args.ramp
+        args.steps = 1 if args.steps is None else args.steps
+        args.warmup = args.ramp if args.warmup is None else args.warmup
+        rec = bench_sweep_c4(args)
     else:
+        args.dmax = 2048 if args.dmax is None else args.dmax
+        args.ramp = 7 if args.ramp is None else args.ramp
         args.steps = 1 if args.steps is None else args.steps
         args.warmup = args.ramp if args.warmup is None else args.warmup
         rec = bench_sweep(args)
