@@ -66,7 +66,9 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
   double* rn2 = w.scal.get() + 1;
   double* tn2 = w.scal.get() + 2;
   const double* dprec = (opts.precondition && diag_dev) ? diag_dev : nullptr;
-  bool low = low_precision;
+  // the first matvec and residual are FP64; low precision takes over only
+  // for a cold start (see DavidsonOpts::low_start_rel)
+  bool low = false;
 
   auto matvec = [&](const double* in, double* out) {
     mv(in, out, low ? 0 : 1, user);
@@ -127,6 +129,9 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     res.iterations = it + 1;
     res.eigenvalue = host[0];
     res.residual = std::sqrt(std::max(0.0, host[1]));
+    if (it == 0 && low_precision &&
+        res.residual > opts.low_start_rel * std::max(1.0, std::fabs(res.eigenvalue)))
+      low = true;  // far from converged: expand the basis with low-precision matvecs
     if (low) {
       if (res.residual < best * 0.7) {
         best = res.residual;

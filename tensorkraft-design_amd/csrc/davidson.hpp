@@ -30,6 +30,10 @@ struct DavidsonOpts {
   double tol = 1e-10;          // absolute 2-norm of the residual
   bool precondition = true;
   double switch_rel = 1e-6;    // low -> FP64 switch when ||r|| < switch_rel * |theta|
+  // A low-precision solve starts in FP64 and drops to low precision only
+  // when the first residual is above low_start_rel * |theta| (a warm start
+  // already that close is refined in FP64 without a basis restart).
+  double low_start_rel = 1e-4;
 };
 
 struct DavidsonResult {

@@ -105,7 +105,9 @@ def test_gpu_dmrg_mixed_precision_matches_fp64(ctx, prec):
     d64, s64 = _gpu(ctx, mpo, sym, bonds, data, (24, 48), 3, tol=1e-10)
     dlo, slo = _gpu(ctx, mpo, sym, bonds, data, (24, 48), 3, prec=prec, tol=1e-10)
     assert abs(slo[-1]["energy"] - s64[-1]["energy"]) <= 1e-10
-    assert slo[-1]["matvecs_fp64"] < slo[-1]["matvecs"]
+    # low precision expands the basis of the cold (first-sweep) solves; warm
+    # solves start close enough to be refined in FP64 directly
+    assert slo[0]["matvecs_fp64"] < slo[0]["matvecs"]
 
 
 def test_gpu_dmrg_cylinder_matches_ed(ctx):
