@@ -923,7 +923,7 @@ __global__ __launch_bounds__(NT) void bisect_kernel(const BisItem* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // 3a. Inverse iteration, one thread per wanted vector: T - lambda I = P L U
-// with partial pivoting (U has two superdiagonals), three solves from a
+// with partial pivoting (U has two superdiagonals), two solves from a
 // hashed start, each normalised by its largest entry; finally unit 2-norm.
 struct VecItem {
   int32_t job, c;
@@ -1025,7 +1025,7 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
   F(n - 1, 2) = 0.0;
   for (int i = 0; i < n; ++i) Y(i) = hash_unit(uint32_t(c) * 2654435761u + uint32_t(jb.job), uint32_t(i));
   double s = 1.0;
-  for (int itn = 0; itn < 3; ++itn) {
+  for (int itn = 0; itn < 2; ++itn) {
     // forward: row interchanges and multipliers applied to s * y
     double cur = s * Y(0);
     for (int i0 = 0; i0 + 1 < n; i0 += C) {
@@ -2296,8 +2296,8 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   const int ti = ctx_->timer.begin(ctx_->stream);
   hipLaunchKernelGGL(invit_kernel, dim3((nv + 63) / 64), dim3(64), 0, ctx_->stream, vid, nv, djd, D, E, W_,
                      tn_.get(), lu.get(), zi.get());
-  // factors written once and read three times, the vector written and read 8 times
-  ctx_->timer.end(ti, ctx_->stream, "inverse_iteration", 8.0 * double(z_tot) * (4.0 * kLuFields + 8.0), 0.0, 1);
+  // factors written once and read twice, the vector written and read 6 times
+  ctx_->timer.end(ti, ctx_->stream, "inverse_iteration", 8.0 * double(z_tot) * (3.0 * kLuFields + 6.0), 0.0, 1);
   TK_HIP(hipGetLastError());
   VecWork work;
   for (const DevJob& j : dj) {
