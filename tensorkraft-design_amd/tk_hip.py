@@ -746,6 +746,202 @@ def cylinder_mpo(width=6, length=12, J=1.0):
     return bond_heisenberg_mpo(width * length, cylinder_bonds(width, length), J)
 
 
+# ---------------------------------------------------------------- OpSum and its MPO
+@dataclass
+class SiteType:
+    """Local basis of a site: QIndex of the physical leg, named operator
+    matrices (row = out, column = in), the fermionic ones, and the string
+    operator carried between the two ends of a fermionic pair."""
+    phys: list
+    ops: dict
+    fermionic: frozenset
+    string: np.ndarray
+
+
+def _spin_half():
+    return SiteType(SPIN_PHYS, {"Id": S_ID, "Sz": S_Z, "S+": S_PLUS, "S-": S_MINUS}, frozenset(), S_ID)
+
+
+def _electron():
+    """Fermion basis |0>, |up>, |dn>, |updn> (tk-dsl/src/operators.rs:88, 117-200);
+    the down operators carry the on-site string (-1)^n_up."""
+    cu, cdu = F_C_UP, F_CDAG_UP
+    cd = F_UP_PARITY @ F_C_DN
+    cdd = cd.T.copy()
+    nu, nd = cdu @ cu, cdd @ cd
+    ops = {"Id": F_ID, "F": F_PARITY, "Cup": cu, "Cdagup": cdu, "Cdn": cd, "Cdagdn": cdd, "Nup": nu, "Ndn": nd,
+           "Ntot": nu + nd, "Nupdn": F_NUPNDN, "Sz": 0.5 * (nu - nd), "S+": cdu @ cd, "S-": cdd @ cu}
+    return SiteType(HUBBARD_PHYS, ops, frozenset({"Cup", "Cdagup", "Cdn", "Cdagdn"}), F_PARITY)
+
+
+SITE_TYPES = {"spin-1/2": _spin_half, "electron": _electron}
+
+
+class OpSum:
+    """Runtime accumulator of weighted operator products
+    (tk-dsl/src/opsum.rs:51-143): terms (coefficient, [(operator, site), ...])
+    in operator order.  `compile_mpo` turns sums of one- and two-site terms
+    into an exact finite-state MPO."""
+
+    def __init__(self, n_sites=None):
+        self.n_sites = n_sites
+        self.terms = []
+
+    def push_term(self, coeff, *factors):
+        """Add coeff * O1(site1) O2(site2) ... (factors (name, site))."""
+        for name, site in factors:
+            if self.n_sites is not None and not (0 <= site < self.n_sites):
+                raise ValueError(f"site {site} outside a lattice of {self.n_sites} sites")
+        self.terms.append((float(coeff), [(str(n), int(i)) for n, i in factors]))
+        return self
+
+    add = push_term
+
+    def n_terms(self):
+        return len(self.terms)
+
+    def iter_terms(self):
+        return iter(self.terms)
+
+    def scale(self, factor):
+        self.terms = [(c * factor, f) for c, f in self.terms]
+
+    def extend(self, other):
+        self.terms.extend(other.terms)
+
+
+def _flux(mat, phys):
+    """q(out) - q(in) of every nonzero element of a local operator (one value)."""
+    qs = [(_qkey(q)) for q, d in phys for _ in range(d)]
+    fl = {(qs[o][0] - qs[i][0], qs[o][1] - qs[i][1]) for o, i in zip(*np.nonzero(np.abs(mat) > 0))}
+    if len(fl) > 1:
+        raise ValueError("operator does not carry a definite charge")
+    return fl.pop() if fl else (0, 0)
+
+
+def compile_mpo(opsum, n_sites, site_type="spin-1/2"):
+    """Exact finite-state MPO of an OpSum of one- and two-site terms (the
+    reference's compile step, tk-dmrg/src/mpo/mod.rs:96-162, is a stub).
+    Factors on one site are multiplied in operator order.  A two-site term
+    c A_a B_b becomes a channel opened at min(a, b) and closed at max(a, b);
+    fermionic pairs are reordered to the left site first (one sign) and
+    carry the string between the ends (left factor A F), as in the Hubbard
+    MPO.  Channels with the same left site and left matrix share one state,
+    so the MPO bond on a cut is 2 + (open channels).  Returns one MpoSite
+    per site (first: left row `start`, last: right column `done`)."""
+    st = SITE_TYPES[site_type]() if isinstance(site_type, str) else site_type
+    d = len(st.string)
+    onsite = [np.zeros((d, d)) for _ in range(n_sites)]
+    chans = {}  # (left site, key) -> [left matrix, fermionic, {right site: right matrix sum}]
+    for coeff, factors in opsum.iter_terms():
+        by_site = {}
+        order = []
+        for name, site in factors:
+            if name not in st.ops:
+                raise ValueError(f"unknown operator {name}")
+            if site in by_site:
+                if name in st.fermionic:
+                    raise ValueError("fermionic operators on one site must be adjacent and are not supported")
+                by_site[site] = by_site[site] @ st.ops[name]
+            else:
+                by_site[site] = st.ops[name]
+                order.append((site, name in st.fermionic))
+        if len(order) == 1:
+            onsite[order[0][0]] += coeff * by_site[order[0][0]]
+            continue
+        if len(order) != 2:
+            raise ValueError("only one- and two-site terms compile")
+        (a, fa), (b, fb) = order
+        if fa != fb:
+            raise ValueError("a term with one fermionic factor changes the parity")
+        sign = 1.0
+        if a > b:
+            a, b = b, a
+            if fa:
+                sign = -1.0  # B_b A_a = -A_a B_b for two fermion operators
+        left = by_site[a] @ st.string if fa else by_site[a]
+        key = (a, left.tobytes(), fa)
+        ch = chans.setdefault(key, [left, fa, {}])
+        ch[2][b] = ch[2].get(b, 0.0) + sign * coeff * by_site[b]
+    # open channels on each cut (the cut after site c), in a fixed order
+    keys = sorted(chans, key=lambda k: (k[0], k[1], k[2]))
+    flux = {k: _flux(chans[k][0], st.phys) for k in keys}
+    cuts = [[]]
+    for c in range(n_sites):
+        cuts.append([k for k in keys if k[0] <= c and max(chans[k][2]) > c])
+
+    def states(open_keys):
+        chg = sorted({(-flux[k][0], -flux[k][1]) for k in open_keys}, key=lambda q: (q != (0, 0), q))
+        idx = {"start": 0, "done": 1}
+        q = [((0, 0), 2)]
+        nxt = 2
+        for qq in chg:
+            ks = [k for k in open_keys if (-flux[k][0], -flux[k][1]) == qq]
+            for k in ks:
+                idx[k] = nxt
+                nxt += 1
+            if qq == (0, 0):
+                q[0] = ((0, 0), 2 + len(ks))
+            else:
+                q.append((qq, len(ks)))
+        return idx, nxt, q
+
+    out = []
+    eye = np.eye(d)
+    for s in range(n_sites):
+        li, lw, lq = states(cuts[s])
+        ri, rw, rq = states(cuts[s + 1])
+        full = np.zeros((d, d, lw, rw))
+
+        def put(a, b, O):
+            full[:, :, a, b] += O.T
+
+        put(0, 0, eye)
+        put(1, 1, eye)
+        if np.any(onsite[s]):
+            put(0, 1, onsite[s])
+        for k in cuts[s]:
+            left, fa, closes = chans[k]
+            if k in ri:
+                put(li[k], ri[k], st.string if fa else eye)
+            if s in closes:
+                put(li[k], 1, closes[s])
+        for k in cuts[s + 1]:
+            if k[0] == s:
+                put(0, ri[k], chans[k][0])
+        if n_sites == 1:
+            out.append(MpoSite(st.phys, [((0, 0), 1)], [((0, 0), 1)], full[:, :, 0:1, 1:2].copy()))
+        elif s == 0:
+            out.append(MpoSite(st.phys, [((0, 0), 1)], rq, full[:, :, 0:1, :].copy()))
+        elif s == n_sites - 1:
+            out.append(MpoSite(st.phys, lq, [((0, 0), 1)], full[:, :, :, 1:2].copy()))
+        else:
+            out.append(MpoSite(st.phys, lq, rq, full))
+    return out
+
+
+def hubbard_opsum(bonds, n_sites, t=1.0, U=4.0):
+    """-t sum_{(i,j) in bonds, s} (c+_{is} c_{js} + h.c.) + U sum_i n_i,up n_i,dn."""
+    os_ = OpSum(n_sites)
+    for i, j in bonds:
+        for up, cd, c in (("up", "Cdagup", "Cup"), ("dn", "Cdagdn", "Cdn")):
+            os_.add(-t, (cd, i), (c, j))
+            os_.add(-t, (cd, j), (c, i))
+    for i in range(n_sites):
+        os_.add(U, ("Nupdn", i))
+    return os_
+
+
+def heisenberg_opsum(bonds, n_sites, J=1.0):
+    """J sum_{(i,j) in bonds} S_i . S_j."""
+    os_ = OpSum(n_sites)
+    for i, j in bonds:
+        os_.add(J, ("Sz", i), ("Sz", j))
+        os_.add(0.5 * J, ("S+", i), ("S-", j))
+        os_.add(0.5 * J, ("S-", i), ("S+", j))
+    return os_
+
+
 def synthetic_u1_bond(total=4096, qmax=30, width=8.0):
     """U(1) (2*Sz) bond of the cylinder microbenchmark: even charges
     -qmax..qmax, dims proportional to exp(-q^2/(2 width^2)), every sector at

@@ -90,3 +90,34 @@ def heisenberg_bonds_ground_energy(n, bonds, J=1.0, twice_sz=0):
     if len(states) <= 400:
         return float(np.linalg.eigvalsh(H.toarray())[0])
     return float(spl.eigsh(H, k=1, which="SA", tol=1e-14)[0][0])
+
+
+def hubbard_bonds_ground_energy(n, bonds, n_up, n_dn, t=1.0, U=4.0):
+    """H = -t sum_{(i,j) in bonds, s} (c+_{is} c_{js} + h.c.) + U sum n_up n_dn on
+    n sites, spin-resolved orbital order (all up orbitals, then all down): a
+    same-spin hop between i and j crosses the occupied orbitals of that spin
+    strictly between them."""
+    def sector(nparts):
+        confs = [sum(1 << i for i in c) for c in itertools.combinations(range(n), nparts)]
+        index = {c: k for k, c in enumerate(confs)}
+        rows, cols, vals = [], [], []
+        for k, c in enumerate(confs):
+            for i, j in bonds:
+                lo, hi = min(i, j), max(i, j)
+                between = bin(c & (((1 << hi) - 1) ^ ((1 << (lo + 1)) - 1))).count("1")
+                sgn = -1.0 if between % 2 else 1.0
+                for a, b in ((i, j), (j, i)):  # c+_a c_b
+                    if (c >> b) & 1 and not (c >> a) & 1:
+                        rows.append(index[c ^ (1 << a) ^ (1 << b)])
+                        cols.append(k)
+                        vals.append(-t * sgn)
+        return confs, sp.csr_matrix((vals, (rows, cols)), shape=(len(confs), len(confs)))
+
+    cu, Tu = sector(n_up)
+    cd, Td = sector(n_dn)
+    H = sp.kron(Tu, sp.identity(len(cd))) + sp.kron(sp.identity(len(cu)), Td)
+    dbl = np.array([bin(a & b).count("1") for a in cu for b in cd], dtype=np.float64)
+    H = H + U * sp.diags(dbl)
+    if H.shape[0] <= 500:
+        return float(np.linalg.eigvalsh(H.toarray())[0])
+    return float(spl.eigsh(H.tocsr(), k=1, which="SA", tol=1e-14)[0][0])

@@ -10,7 +10,8 @@ import pytest
 
 import oracle
 import tk_hip
-from ed import free_fermion_energy, heisenberg_bonds_ground_energy, heisenberg_ground_energy, hubbard_ground_energy
+from ed import (free_fermion_energy, heisenberg_bonds_ground_energy, heisenberg_ground_energy,
+                hubbard_bonds_ground_energy, hubbard_ground_energy)
 
 pytestmark = pytest.mark.gpu
 
@@ -156,3 +157,16 @@ def test_gpu_dmrg_checkpoint_resume(ctx, tmp_path):
     d2.prepare()
     st = [d2.sweep() for _ in range(2)]
     assert abs(st[-1]["energy"] - heisenberg_ground_energy(n)) < 1e-9
+
+
+def test_gpu_dmrg_compiled_hubbard_ladder_matches_ed(ctx):
+    """The OpSum compiler's Hubbard MPO on a 3 x 2 ladder in snake order
+    (hops across other sites carry the Jordan-Wigner string) on the GPU."""
+    print("case dmrg compiled hubbard ladder", flush=True)
+    n = 6
+    bonds = tk_hip.cylinder_bonds(2, 3)
+    mpo = tk_hip.compile_mpo(tk_hip.hubbard_opsum(bonds, n, 1.0, 4.0), n, "electron")
+    sym = tk_hip.SYM_U1U1
+    b, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (3, 3), 16, 0)
+    d, st = _gpu(ctx, mpo, sym, b, data, (64, 128), 5)
+    assert abs(st[-1]["energy"] - hubbard_bonds_ground_energy(n, bonds, 3, 3, 1.0, 4.0)) < 1e-9
