@@ -563,18 +563,17 @@ __global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restri
 
 // ---------------------------------------------------------------------------
 // 1b. Tridiagonalisation for large matrices: one launch per column step k,
-// batched over the jobs, each workgroup owning R rows of one job
-// (one wavefront per row) of the full symmetric matrix.  Every workgroup
-// recomputes the O(n) quantities it needs: the pending rank-2 update of step
-// k-1 (from the previous step's A22 v, double-buffered) and the reflector of
-// step k (from row k, which no workgroup writes during the step).  Reflector k
-// is stored in column k below the subdiagonal by the owners of those rows and
-// moved into row k afterwards.  VP holds per job: v (two buffers) and A22 v
-// (two buffers), each indexed like the eigenvalues.
-// R rows (wavefronts) per workgroup; the launch picks R per column step so
-// that the step's rows spread over enough workgroups to keep every CU busy.
-// Workgroups of a job at column step k; the job's column steps stop at k0
-// (DevJob::pad), where the single-workgroup kernel takes over its tail.
+// batched over the jobs, each workgroup of R wavefronts owning R * RW rows of
+// one job (RW rows per wavefront) of the full symmetric matrix.  Every
+// workgroup recomputes the O(n) quantities it needs: the pending rank-2
+// update of step k-1 (from the previous step's A22 v, double-buffered) and
+// the reflector of step k (from row k, which no workgroup writes during the
+// step).  Reflector k is stored in column k below the subdiagonal by the
+// owners of those rows and moved into row k afterwards.  VP holds per job: v
+// (two buffers) and A22 v (two buffers), each indexed like the eigenvalues.
+// Workgroups of a job at column step k (R here = rows per workgroup); the
+// job's column steps stop at k0 (DevJob::pad), where the single-workgroup
+// kernel takes over its tail.
 template <int R>
 __device__ __forceinline__ int step_count(int n, int k, int k0) {
   return k < k0 ? (n - k - 1 + R - 1) / R : 0;

@@ -1,18 +1,23 @@
 // Batched FP64 symmetric eigensolver for the truncated SVD and the Krylov
 // subspace problems (tdeig.hip).
 //
-// Method (one workgroup per matrix where a matrix must be walked in order):
+// Method:
 //   1. Householder tridiagonalisation A = Q T Q^T, the rank-2 update of step
 //      k fused with the symmetric product of step k+1, so the trailing matrix
-//      is read and written once per column; reflectors are kept in A.
-//   2. All eigenvalues of T by Sturm-count bisection, one thread per
-//      eigenvalue, sorted by construction.
+//      is read and written once per column; reflectors are kept in A.  One
+//      launch per column step over the whole batch while a trailing block is
+//      larger than 192, then one workgroup per matrix with the block in
+//      registers.
+//   2. All eigenvalues of T by Sturm-count multisection (16 lanes per
+//      eigenvalue), sorted by construction.
 //   3. For the k wanted eigenvalues only: inverse iteration on T (LU with
 //      partial pivoting, one thread per vector), blocked classical
-//      Gram-Schmidt applied twice in eigenvalue order (orthonormal to working
-//      precision even inside clusters), then back-transformation by Q with
-//      the reflectors grouped 128 at a time in compact WY form
-//      (I - V T V^T), applied as grouped MFMA contractions.
+//      Gram-Schmidt applied twice in eigenvalue order with Cholesky QR inside
+//      each 64-column block (orthonormal to working precision even inside
+//      clusters: a block with a dependent column falls back to column-by-column
+//      Gram-Schmidt), then back-transformation by Q with the reflectors grouped
+//      128 at a time in compact WY form (I - V T V^T), applied as grouped MFMA
+//      contractions.
 //
 // Reference call sites: the Gram eigen-decomposition behind the block-diagonal
 // truncated SVD (tk-dmrg/src/truncation/mod.rs:56-127 -> faer thin_svd,
