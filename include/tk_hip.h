@@ -215,6 +215,14 @@ tk_status tk_davidson(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, c
  * final residual). */
 tk_status tk_lanczos(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, double tol, int32_t max_krylov,
                      int32_t max_restarts, double* x_dev, double* eigenvalue, double* residual, int32_t* matvecs);
+/* exp_krylov_f64 + mat_exp_times_vec (tk-dmrg/src/tdvp/mod.rs:197-382):
+ * out_dev = exp(alpha A) v_dev for real alpha over the same matvec callback.
+ * min(krylov_dim, n) Arnoldi vectors (krylov_dim <= 48), happy breakdown at
+ * h_{j+1,j} < tol, the small exponential by the reference's Taylor series with
+ * scaling and squaring; |v| < 1e-30 gives zero.  out_dev may alias v_dev.
+ * *krylov_used (nullable) receives the Arnoldi vectors used. */
+tk_status tk_exp_krylov(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, const double* v_dev, double alpha,
+                        int32_t krylov_dim, double tol, double* out_dev, int32_t* krylov_used);
 
 /* ---- two-site effective Hamiltonian ------------------------------------- */
 /* build_heff_two_site (tk-dmrg/src/environments/mod.rs:491-602;
@@ -299,11 +307,40 @@ tk_status tk_dmrg_set_state(tk_dmrg* d, const tk_qindex* bonds, const double* co
 tk_status tk_dmrg_prepare(tk_dmrg* d);
 /* One left-to-right + right-to-left sweep (SweepSchedule::standard, sweep/mod.rs:59-81). */
 tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out);
+/* dmrg_step_single_site (tk-dmrg/src/sweep/mod.rs:451-509) over a whole sweep.
+ * The reference solves the one-site problem but neither writes the eigenvector
+ * back nor moves the centre; here every site is replaced by the lowest
+ * eigenpair of L W R (the config's eigensolver, warm-started from the site)
+ * and the centre moves on by an untruncated split with the neighbour, so the
+ * bond dimensions never grow.  Left to right then right to left; needs
+ * tk_dmrg_prepare.  Statistics as for tk_dmrg_sweep (truncation error 0 up to
+ * numerical zeros). */
+tk_status tk_dmrg_sweep_single_site(tk_dmrg* d, tk_sweep_stats* out);
 /* Per-step statistics of the last sweep (StepResult, sweep/mod.rs:35-50). */
 tk_status tk_dmrg_steps(tk_dmrg* d, int32_t capacity, int32_t* n, tk_step_stats* out);
 /* Current bond basis (sorted) and site tensor data. */
 tk_status tk_dmrg_bond(tk_dmrg* d, int32_t bond, int32_t capacity, int32_t* n_sectors, int32_t* qn, int64_t* dims);
 tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, double* data);
+
+/* TdvpStepResult (tk-dmrg/src/tdvp/mod.rs:20-33) plus the step's energy and norm factor. */
+typedef struct {
+  double energy;                /* <psi|H|psi>/<psi|psi> of the step's last two-site tensor */
+  double max_truncation_error;
+  double seconds;
+  double log_norm;              /* ln of the norm factor exp(-tau H) applied (before renormalisation) */
+  int64_t max_bond_dim;
+  int32_t matvecs, pad;
+} tk_tdvp_stats;
+/* TdvpDriver::step (tk-dmrg/src/tdvp/mod.rs:91-141), which in the reference
+ * leaves the MPS unchanged: here one second-order two-site TDVP step of
+ * psi <- exp(-tau H) psi for real tau (imaginary time), renormalised.  Bonds
+ * left to right then right to left: exp(-tau/2 H2) on the two-site tensor
+ * (exp_krylov_f64, tdvp/mod.rs:197-272), split truncated to max_bond (<= 0:
+ * the last bond-schedule value) with the config's cutoff, and away from the
+ * ends exp(+tau/2 H1) on the new centre site.  Needs tk_dmrg_prepare; the
+ * centre is back at site 0 afterwards. */
+tk_status tk_dmrg_tdvp_step(tk_dmrg* d, double tau, int64_t max_bond, int32_t krylov_dim, double krylov_tol,
+                            tk_tdvp_stats* out);
 
 /* ---- DMFT impurity pieces (tk-dmft) ---------------------------------------
  * BathParameters::hybridization_function (tk-dmft/src/impurity/bath.rs:61-83),

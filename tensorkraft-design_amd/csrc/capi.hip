@@ -11,6 +11,7 @@
 #include "dmft.hpp"
 #include "dmrg.hpp"
 #include "heff.hpp"
+#include "tdvp.hpp"
 #include "sym.hpp"
 
 struct tk_ctx {
@@ -463,6 +464,17 @@ tk_status tk_lanczos(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, do
   });
 }
 
+tk_status tk_exp_krylov(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, const double* v, double alpha,
+                        int32_t krylov_dim, double tol, double* out, int32_t* krylov_used) {
+  return guard(ctx, [&] {
+    if (!ctx || !matvec || !v || !out || n <= 0) throw tk::ArgError("bad exp_krylov arguments");
+    CallbackUser cb{matvec, user};
+    tk::ExpKrylovResult r =
+        tk::exp_krylov_generic(ctx->c.get(), n, callback_matvec, &cb, v, alpha, krylov_dim, tol, out);
+    if (krylov_used) *krylov_used = r.krylov_dim;
+  });
+}
+
 tk_status tk_heff_plan_create(tk_ctx* ctx, int32_t prec, int32_t sym, const tk_env* left, const tk_env* right,
                               const tk_mpo_site* wi, const tk_mpo_site* wj, tk_heff_plan** out) {
   return guard(ctx, [&] {
@@ -659,6 +671,31 @@ tk_status tk_dmrg_prepare(tk_dmrg* d) {
   });
 }
 
+namespace {
+void fill_sweep_stats(const tk::SweepStats& s, tk_sweep_stats* out) {
+  out->energy = s.energy;
+  out->max_truncation_error = s.max_truncation_error;
+  out->seconds = s.seconds;
+  out->t_plan = s.t_plan;
+  out->t_eig = s.t_eig;
+  out->t_svd = s.t_svd;
+  out->t_env = s.t_env;
+  out->max_bond_dim = s.max_bond_dim;
+  out->matvecs = s.matvecs;
+  out->matvecs_fp64 = s.matvecs_fp64;
+  out->steps = s.steps;
+  out->unconverged = s.unconverged;
+}
+}  // namespace
+
+tk_status tk_dmrg_sweep_single_site(tk_dmrg* d, tk_sweep_stats* out) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d) throw tk::ArgError("null argument");
+    tk::SweepStats s = d->e->sweep_single_site();
+    if (out) fill_sweep_stats(s, out);
+  });
+}
+
 tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out) {
   return guard(d ? d->ctx : nullptr, [&] {
     if (!d) throw tk::ArgError("null argument");
@@ -676,6 +713,24 @@ tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out) {
       out->matvecs_fp64 = s.matvecs_fp64;
       out->steps = s.steps;
       out->unconverged = s.unconverged;
+    }
+  });
+}
+
+tk_status tk_dmrg_tdvp_step(tk_dmrg* d, double tau, int64_t max_bond, int32_t krylov_dim, double krylov_tol,
+                            tk_tdvp_stats* out) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d) throw tk::ArgError("null argument");
+    if (krylov_dim <= 0) throw tk::ArgError("tdvp: krylov_dim must be positive");
+    tk::TdvpStats s = d->e->tdvp_step(tau, max_bond, krylov_dim, krylov_tol);
+    if (out) {
+      out->energy = s.energy;
+      out->max_truncation_error = s.max_truncation_error;
+      out->seconds = s.seconds;
+      out->log_norm = s.log_norm;
+      out->max_bond_dim = s.max_bond_dim;
+      out->matvecs = s.matvecs;
+      out->pad = 0;
     }
   });
 }

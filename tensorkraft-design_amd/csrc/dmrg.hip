@@ -5,11 +5,13 @@
 #include <chrono>
 #include <cmath>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <tuple>
 
 #include "gramsvd.hpp"
 #include "tdeig.hpp"
+#include "tdvp.hpp"
 #include "kernels.hpp"
 
 namespace tk {
@@ -118,6 +120,7 @@ void DmrgEngine::set_state(const std::vector<QIndex>& bonds, const std::vector<s
   L_.resize(n_ + 1);
   R_.resize(n_ + 1);
   sweeps_ = 0;
+  centre_ = -1;
 }
 
 std::vector<double> DmrgEngine::site_data(int i) const {
@@ -150,6 +153,23 @@ void DmrgEngine::prepare() {
   L_[0] = boundary_env(bonds_[0], cfg_.mpo[0].wl);
   R_[n_] = boundary_env(bonds_[n_], cfg_.mpo[n_ - 1].wr);
   for (int i = n_ - 1; i >= 2; --i) grow_right(i);
+  ctx_->sync();
+  centre_ = 0;
+}
+
+void DmrgEngine::centre_to_site0() {
+  if (centre_ < 0) throw ArgError("prepare() first");
+  const int c = centre_;
+  for (int j = c - 1; j >= 0; --j) {
+    const BlockTable pair = make_block_table(cfg_.sym, {cfg_.mpo[j].phys, cfg_.mpo[j + 1].phys, bonds_[j], bonds_[j + 2]},
+                                             {Dir::Out, Dir::Out, Dir::Out, Dir::In}, QN{});
+    DBuf<double> th(ctx_, size_t(std::max<int64_t>(1, pair.total)));
+    contract_two_site(j, pair, th.get());
+    StepStats sp;
+    split_two_site(j, false, pair, th.get(), INT64_MAX, 0.0, kLambdaZero, &sp);
+    grow_right(j + 1);
+  }
+  if (c == 0) grow_right(1);
   ctx_->sync();
 }
 
@@ -228,6 +248,186 @@ StepStats DmrgEngine::step(int i, bool lr) {
     cur_->t_env += t4 - t3;
   }
   return st;
+}
+
+struct DmrgEngine::OneSite {
+  MpoSite id;
+  std::unique_ptr<HeffPlan> plan;
+  std::vector<CopyJob> to_psi, to_site;
+};
+
+DmrgEngine::OneSite DmrgEngine::one_site_plan(int i, Prec prec) {
+  // a trivial second site: one state of charge 0 and W = identity on the MPO
+  // bond, so that psi(s, 0, a_L, a_R) is the site tensor (s, a_L, a_R) block
+  // for block
+  const MpoSite& W = cfg_.mpo[i];
+  OneSite os;
+  os.id.phys = QIndex({{QN{}, 1}});
+  os.id.wl = W.wr;
+  os.id.wr = W.wr;
+  os.id.w.assign(size_t(W.wr.total) * size_t(W.wr.total), 0.0);
+  for (int64_t a = 0; a < W.wr.total; ++a) os.id.w[size_t(a) * size_t(W.wr.total) + size_t(a)] = 1.0;
+  os.plan = std::make_unique<HeffPlan>(ctx_, prec, L_[i].t, L_[i].d.get(), R_[i + 1].t, R_[i + 1].d.get(), W, os.id);
+  const BlockTable& psi = os.plan->psi();
+  const Site& A = site_[i];
+  for (int64_t b = 0; b < psi.n_blocks(); ++b) {
+    int32_t secs[3] = {psi.s(b, 0), psi.s(b, 2), psi.s(b, 3)};
+    const int64_t ab = A.t.find(secs);
+    if (ab < 0) throw std::logic_error("one-site: site block missing from the one-site vector");
+    const int32_t rows = int32_t(W.phys.dim[secs[0]] * bonds_[i].dim[secs[1]]);
+    const int32_t cols = int32_t(bonds_[i + 1].dim[secs[2]]);
+    os.to_psi.push_back({A.t.offset[ab], psi.offset[b], rows, cols, cols, cols, 0, 0});
+    os.to_site.push_back({psi.offset[b], A.t.offset[ab], rows, cols, cols, cols, 0, 0});
+  }
+  if (int64_t(os.to_psi.size()) != A.t.n_blocks()) throw std::logic_error("one-site: block sets differ");
+  return os;
+}
+
+double DmrgEngine::evolve_one_site(int i, double alpha, int krylov_dim, double tol, int* matvecs) {
+  OneSite os = one_site_plan(i, Prec::FP64);
+  const BlockTable& psi = os.plan->psi();
+  Site& A = site_[i];
+  DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total))), y(ctx_, size_t(std::max<int64_t>(1, psi.total)));
+  launch_copies(ctx_, os.to_psi, A.d.get(), A.t.total, x.get(), psi.total);
+  const ExpKrylovResult r = exp_krylov(ctx_, *os.plan, x.get(), alpha, krylov_dim, tol, y.get());
+  *matvecs += r.matvecs;
+  if (!(r.norm_out > 0) || !(r.norm_in > 0)) throw ArgError("tdvp: the one-site tensor vanished");
+  scale(ctx_, y.get(), 1.0 / r.norm_out, psi.total);
+  launch_copies(ctx_, os.to_site, y.get(), psi.total, A.d.get(), A.t.total);
+  return std::log(r.norm_out) - std::log(r.norm_in);
+}
+
+StepStats DmrgEngine::step_single_site(int i, bool lr) {
+  StepStats st;
+  st.site = i;
+  st.left_to_right = lr;
+  double t0 = now();
+  OneSite os = one_site_plan(i, cfg_.prec);
+  const BlockTable& psi = os.plan->psi();
+  Site& A = site_[i];
+  DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total)));
+  launch_copies(ctx_, os.to_psi, A.d.get(), A.t.total, x.get(), psi.total);
+  double t1 = now();
+  DavidsonResult dr = cfg_.eigensolver == 1 ? lanczos(ctx_, *os.plan, x.get(), cfg_.lanczos)
+                                            : davidson(ctx_, *os.plan, x.get(), cfg_.davidson);
+  launch_copies(ctx_, os.to_site, x.get(), psi.total, A.d.get(), A.t.total);
+  double t2 = now();
+  st.energy = dr.eigenvalue;
+  st.iterations = dr.iterations;
+  st.matvecs = dr.matvecs;
+  st.matvecs_fp64 = dr.matvecs_fp64;
+  st.converged = dr.converged;
+  st.residual = dr.residual;
+  st.dim = psi.total;
+  // move the centre: untruncated split of the site with its neighbour
+  const int j = lr ? i : i - 1;  // left site of the pair
+  st.bond_dim = bonds_[lr ? i + 1 : i].total;
+  if (lr ? i + 1 < n_ : i > 0) {
+    const BlockTable pair = make_block_table(cfg_.sym, {cfg_.mpo[j].phys, cfg_.mpo[j + 1].phys, bonds_[j], bonds_[j + 2]},
+                                             {Dir::Out, Dir::Out, Dir::Out, Dir::In}, QN{});
+    DBuf<double> th(ctx_, size_t(std::max<int64_t>(1, pair.total)));
+    contract_two_site(j, pair, th.get());
+    StepStats sp;
+    split_two_site(j, lr, pair, th.get(), INT64_MAX, 0.0, kLambdaZero, &sp);
+    st.truncation_error = sp.truncation_error;
+    st.bond_dim = sp.bond_dim;
+    st.svd_nmax = sp.svd_nmax;
+    st.svd_n3 = sp.svd_n3;
+  }
+  double t3 = now();
+  if (lr && i + 1 < n_) grow_left(i);
+  if (!lr && i > 0) grow_right(i);
+  ctx_->sync();
+  double t4 = now();
+  if (cur_) {
+    cur_->t_plan += t1 - t0;
+    cur_->t_eig += t2 - t1;
+    cur_->t_svd += t3 - t2;
+    cur_->t_env += t4 - t3;
+  }
+  return st;
+}
+
+SweepStats DmrgEngine::sweep_single_site() {
+  if (site_.empty() || L_.empty() || L_[0].t.n_blocks() == 0) throw ArgError("single-site sweep: prepare() first");
+  SweepStats s;
+  cur_ = &s;
+  steps_.clear();
+  const double t0 = now();
+  centre_to_site0();
+  for (int i = 0; i + 1 < n_; ++i) steps_.push_back(step_single_site(i, true));
+  for (int i = n_ - 1; i >= 1; --i) steps_.push_back(step_single_site(i, false));
+  ctx_->sync();
+  s.seconds = now() - t0;
+  for (const StepStats& st : steps_) {
+    ++s.steps;
+    s.max_truncation_error = std::max(s.max_truncation_error, st.truncation_error);
+    s.max_bond_dim = std::max(s.max_bond_dim, st.bond_dim);
+    s.matvecs += st.matvecs;
+    s.matvecs_fp64 += st.matvecs_fp64;
+    if (!st.converged) ++s.unconverged;
+  }
+  energy_ = steps_.back().energy;
+  s.energy = energy_;
+  ++sweeps_;
+  cur_ = nullptr;
+  return s;
+}
+
+TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol) {
+  if (site_.empty() || L_.empty() || L_[0].t.n_blocks() == 0) throw ArgError("tdvp: prepare() first");
+  if (!std::isfinite(tau)) throw ArgError("tdvp: time step is not finite");
+  centre_to_site0();
+  if (max_bond <= 0) max_bond = cfg_.bond_schedule.back();
+  TdvpStats ts;
+  const double t0 = now();
+  const double floor = std::max(cfg_.svd_cutoff * cfg_.svd_cutoff, kLambdaZero);
+  // two-site update of bond (i, i+1): evolve, split, grow the environment
+  // behind the centre; the last bond of the step also gives the energy
+  auto two_site = [&](int i, bool lr, bool energy) {
+    HeffPlan plan(ctx_, Prec::FP64, L_[i].t, L_[i].d.get(), R_[i + 2].t, R_[i + 2].d.get(), cfg_.mpo[i],
+                  cfg_.mpo[i + 1]);
+    const BlockTable& psi = plan.psi();
+    const size_t len = size_t(std::max<int64_t>(1, psi.total));
+    DBuf<double> x(ctx_, len), y(ctx_, len);
+    contract_two_site(i, psi, x.get());
+    const ExpKrylovResult r = exp_krylov(ctx_, plan, x.get(), -0.5 * tau, krylov_dim, krylov_tol, y.get());
+    ts.matvecs += r.matvecs;
+    if (!(r.norm_out > 0) || !(r.norm_in > 0)) throw ArgError("tdvp: the two-site tensor vanished");
+    ts.log_norm += std::log(r.norm_out) - std::log(r.norm_in);
+    if (energy) {
+      DBuf<double> hy(ctx_, len), d(ctx_, 2);
+      plan.apply_fp64(y.get(), hy.get());
+      ++ts.matvecs;
+      dot_many(ctx_, y.get(), psi.total, 1, hy.get(), psi.total, d.get());
+      dot_many(ctx_, y.get(), psi.total, 1, y.get(), psi.total, d.get() + 1);
+      double h[2];
+      d.download(h, 2);
+      ctx_->sync();
+      ts.energy = h[0] / h[1];
+    }
+    StepStats st;
+    split_two_site(i, lr, psi, y.get(), max_bond, cfg_.svd_cutoff, floor, &st);
+    ts.max_truncation_error = std::max(ts.max_truncation_error, st.truncation_error);
+    ts.max_bond_dim = std::max(ts.max_bond_dim, st.bond_dim);
+    if (lr)
+      grow_left(i);
+    else
+      grow_right(i + 1);
+    ctx_->sync();
+  };
+  for (int i = 0; i + 1 < n_; ++i) {
+    two_site(i, true, false);
+    if (i + 2 < n_) ts.log_norm += evolve_one_site(i + 1, 0.5 * tau, krylov_dim, krylov_tol, &ts.matvecs);
+  }
+  for (int i = n_ - 2; i >= 0; --i) {
+    two_site(i, false, i == 0);
+    if (i > 0) ts.log_norm += evolve_one_site(i, 0.5 * tau, krylov_dim, krylov_tol, &ts.matvecs);
+  }
+  ctx_->sync();
+  energy_ = ts.energy;
+  ts.seconds = now() - t0;
+  return ts;
 }
 
 void DmrgEngine::contract_two_site(int i, const BlockTable& psi, double* x) {
@@ -487,6 +687,7 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
     run_list(ctx_, wl, psi.total, gtot, Ai.t.total, x, E, Ai.d.get());
   }
   ctx_->timer.end(ph_write, ctx_->stream, "phase_svd_writeback", 0.0, 0.0, 1);
+  centre_ = lr ? i + 1 : i;
   bonds_[i + 1] = newB;
   site_[i] = std::move(Ai);
   site_[i + 1] = std::move(Aj);

@@ -56,6 +56,16 @@ struct StepStats {
   double svd_n3 = 0;      // sum of n^3 over the Gram blocks
 };
 
+// One two-site TDVP step (DmrgEngine::tdvp_step).
+struct TdvpStats {
+  double energy = 0;  // <psi|H|psi> / <psi|psi> of the last two-site tensor of the step
+  double max_truncation_error = 0;
+  int64_t max_bond_dim = 0;
+  double seconds = 0;
+  double log_norm = 0;  // ln of the norm factor the step applied (before renormalisation)
+  int matvecs = 0;
+};
+
 struct SweepStats {
   double energy = 0;
   double max_truncation_error = 0;
@@ -83,6 +93,20 @@ class DmrgEngine {
   const BlockTable& site_table(int i) const { return site_[i].t; }
   std::vector<double> site_data(int i) const;
   const std::vector<StepStats>& last_steps() const { return steps_; }
+  // One second-order two-site TDVP step of psi <- exp(-tau H) psi (real tau:
+  // imaginary-time evolution), the state renormalised after every local
+  // update.  Left-to-right then right-to-left over every bond: the two-site
+  // tensor evolves by exp(-tau/2 H2), is split (truncated to max_bond, the
+  // sweep's cutoff), and, away from the ends, the new centre site evolves back
+  // by exp(+tau/2 H1).  Krylov exponentials of krylov_dim vectors.  Needs
+  // prepare() (centre at site 0) and leaves the centre at site 0.
+  TdvpStats tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol);
+  // One left-to-right + right-to-left single-site sweep: at every site the
+  // lowest eigenpair of H1 = L W R (the sweep's eigensolver, warm-started
+  // from the site tensor) replaces the site, and the centre moves on by an
+  // untruncated split of the site with its neighbour (the bond dimension
+  // cannot grow).  Needs prepare() (centre at site 0); leaves it at site 0.
+  SweepStats sweep_single_site();
 
  private:
   struct Site {
@@ -99,6 +123,17 @@ class DmrgEngine {
   // SVD of the two-site vector, truncation and write-back of A[i], A[i+1].
   void split_two_site(int i, bool lr, const BlockTable& psi, const double* x, int64_t max_bond, double cutoff,
                       double lambda_floor, StepStats* st);
+  // site i <- exp(alpha H1) site i / |...| with H1 = L[i] W_i R[i+1]; returns
+  // ln |exp(alpha H1) site i| - ln |site i| and adds the matvecs to *matvecs
+  double evolve_one_site(int i, double alpha, int krylov_dim, double tol, int* matvecs);
+  // H1 = L[i] W_i R[i+1] as a two-site plan with a trivial second site, and
+  // the block copies between site i and that plan's vector
+  struct OneSite;
+  OneSite one_site_plan(int i, Prec prec);
+  StepStats step_single_site(int i, bool lr);
+  // Move the orthogonality centre to site 0 by untruncated splits (the
+  // two-site sweep leaves it at site 1) and build R[1].
+  void centre_to_site0();
   void grow_left(int i);   // L[i+1] from L[i], A[i]
   void grow_right(int i);  // R[i] from R[i+1], A[i]
   Env boundary_env(const QIndex& bond, const QIndex& w);
@@ -112,6 +147,7 @@ class DmrgEngine {
   std::vector<Env> L_, R_;
   double energy_ = 0;
   int sweeps_ = 0;
+  int centre_ = -1;  // site of the orthogonality centre (-1 before prepare)
   std::vector<StepStats> steps_;
   SweepStats* cur_ = nullptr;
 };

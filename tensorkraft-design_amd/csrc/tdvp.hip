@@ -1,0 +1,232 @@
+// Krylov exponential on the device (see tdvp.hpp).
+#include "tdvp.hpp"
+
+#include <cmath>
+#include <limits>
+
+#include "kernels.hpp"
+
+namespace tk {
+namespace {
+
+constexpr int kExpThreads = 256;
+constexpr int kExpPer = (kExpKrylovMax * kExpKrylovMax + kExpThreads - 1) / kExpThreads;
+
+// Deterministic sum over the workgroup (tree in LDS); every thread gets it.
+__device__ double block_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int w = kExpThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// c = exp(alpha H[:m, :m]) (beta e_1), H row-major with leading dimension ldh
+// (mat_exp_times_vec, tdvp/mod.rs:278-382).  One workgroup.
+__global__ __launch_bounds__(kExpThreads) void krylov_expm_kernel(const double* __restrict__ H, int ldh, int m,
+                                                                  double alpha, double beta,
+                                                                  double* __restrict__ c) {
+  __shared__ double a[kExpKrylovMax * kExpKrylovMax];
+  __shared__ double e[kExpKrylovMax * kExpKrylovMax];
+  __shared__ double t[kExpKrylovMax * kExpKrylovMax];
+  __shared__ double red[kExpThreads];
+  const int tid = threadIdx.x, mm = m * m;
+  double p = 0;
+  for (int idx = tid; idx < mm; idx += kExpThreads) {
+    const int i = idx / m, j = idx - i * m;
+    a[idx] = alpha * H[int64_t(i) * ldh + j];
+    p += fabs(a[idx]);
+  }
+  const double norm = block_sum(p, red) / m;  // the reference's mean absolute row sum
+  if (!(norm <= 1e300)) {
+    for (int i = tid; i < m; i += kExpThreads) c[i] = __builtin_nan("");
+    return;
+  }
+  const int s = norm > 0 ? max(0, int(ceil(log2(norm / 0.5)))) : 0;
+  const double scale = ldexp(1.0, -s);
+  for (int idx = tid; idx < mm; idx += kExpThreads) a[idx] *= scale;
+  __syncthreads();
+  if (s == 0) {
+    // vector Taylor series: result and term in e[0..m), t[0..m)
+    for (int i = tid; i < m; i += kExpThreads) e[i] = t[i] = i == 0 ? beta : 0.0;
+    __syncthreads();
+    for (int k = 1; k <= 20; ++k) {
+      double nt = 0;
+      if (tid < m) {
+        for (int j = 0; j < m; ++j) nt += a[tid * m + j] * t[j];
+        nt /= k;
+      }
+      __syncthreads();
+      if (tid < m) {
+        t[tid] = nt;
+        e[tid] += nt;
+      }
+      const double tn = sqrt(block_sum(tid < m ? nt * nt : 0.0, red));
+      if (tn < 1e-16 * fabs(beta)) break;
+    }
+    for (int i = tid; i < m; i += kExpThreads) c[i] = e[i];
+    return;
+  }
+  // matrix Taylor series of a 2^-s, then s squarings
+  for (int idx = tid; idx < mm; idx += kExpThreads) {
+    const int i = idx / m, j = idx - i * m;
+    e[idx] = t[idx] = i == j ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  double r[kExpPer];
+  for (int k = 1; k <= 20; ++k) {
+    double sq = 0;
+#pragma unroll
+    for (int h = 0; h < kExpPer; ++h) {
+      const int idx = tid + kExpThreads * h;
+      r[h] = 0;
+      if (idx < mm) {
+        const int i = idx / m, j = idx - i * m;
+        double acc = 0;
+        for (int l = 0; l < m; ++l) acc += a[i * m + l] * t[l * m + j];
+        r[h] = acc / k;
+        sq += r[h] * r[h];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < kExpPer; ++h) {
+      const int idx = tid + kExpThreads * h;
+      if (idx < mm) {
+        t[idx] = r[h];
+        e[idx] += r[h];
+      }
+    }
+    const double tn = sqrt(block_sum(sq, red));
+    if (tn < 1e-16) break;
+  }
+  for (int q = 0; q < s; ++q) {
+#pragma unroll
+    for (int h = 0; h < kExpPer; ++h) {
+      const int idx = tid + kExpThreads * h;
+      r[h] = 0;
+      if (idx < mm) {
+        const int i = idx / m, j = idx - i * m;
+        double acc = 0;
+        for (int l = 0; l < m; ++l) acc += e[i * m + l] * e[l * m + j];
+        r[h] = acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < kExpPer; ++h) {
+      const int idx = tid + kExpThreads * h;
+      if (idx < mm) e[idx] = r[h];
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < m; i += kExpThreads) c[i] = e[i * m] * beta;
+}
+
+// H[i * ldh + j] (+)= h[i] for i < count
+__global__ void hcol_kernel(double* __restrict__ H, int ldh, int j, const double* __restrict__ h, int count,
+                            int accumulate) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x)
+    H[int64_t(i) * ldh + j] = (accumulate ? H[int64_t(i) * ldh + j] : 0.0) + h[i];
+}
+
+__global__ void sqrt_store_kernel(const double* __restrict__ nrm2, double* __restrict__ out) {
+  if (threadIdx.x == 0) *out = sqrt(fmax(0.0, *nrm2));
+}
+
+struct PlanUser {
+  HeffPlan* plan;
+};
+void plan_matvec_fp64(const double* x, double* y, int, void* user) {
+  static_cast<PlanUser*>(user)->plan->apply_fp64(x, y);
+}
+
+}  // namespace
+
+ExpKrylovResult exp_krylov_generic(Context* ctx, int64_t n, MatvecFn mv, void* user, const double* v, double alpha,
+                                   int krylov_dim, double tol, double* out) {
+  ExpKrylovResult res;
+  if (n < 0 || krylov_dim < 0) throw ArgError("exp_krylov: negative size");
+  if (krylov_dim > kExpKrylovMax) throw ArgError("exp_krylov: krylov_dim above 48");
+  hipStream_t s = ctx->stream;
+  const int m = int(std::min<int64_t>(krylov_dim, n));
+  if (m == 0) {
+    if (n > 0 && out != v) TK_HIP(hipMemcpyAsync(out, v, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+    TK_HIP(hipStreamSynchronize(s));
+    return res;
+  }
+  DBuf<double> Vb(ctx, size_t(m) * n), wb(ctx, n), hb(ctx, m), Hb(ctx, size_t(m + 1) * m), cb(ctx, m), sc(ctx, 2);
+  double* V = Vb.get();
+  double* w = wb.get();
+  double* H = Hb.get();
+  double* nrm2 = sc.get();
+  fill_f64(ctx, H, 0.0, int64_t(m + 1) * m);
+  dot_many(ctx, v, n, 1, v, n, nrm2);
+  double host = 0;
+  TK_HIP(hipMemcpyAsync(&host, nrm2, sizeof(double), hipMemcpyDeviceToHost, s));
+  TK_HIP(hipStreamSynchronize(s));
+  const double beta = std::sqrt(std::max(0.0, host));
+  res.norm_in = beta;
+  if (!std::isfinite(beta)) throw ArgError("exp_krylov: start vector is not finite");
+  if (beta < 1e-30) {
+    fill_f64(ctx, out, 0.0, n);
+    TK_HIP(hipStreamSynchronize(s));
+    return res;
+  }
+  TK_HIP(hipMemcpyAsync(V, v, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+  scale(ctx, V, 1.0 / beta, n);
+  int used = m;
+  for (int j = 0; j < m; ++j) {
+    const double* vj = V + int64_t(j) * n;
+    mv(vj, w, 1, user);
+    ++res.matvecs;
+    // column j of H: two classical Gram-Schmidt passes, coefficients summed
+    dot_many(ctx, V, n, j + 1, w, n, hb.get());
+    hipLaunchKernelGGL(hcol_kernel, dim3(1), dim3(64), 0, s, H, m, j, hb.get(), j + 1, 0);
+    subtract_projection(ctx, V, n, j + 1, hb.get(), w, n);
+    dot_many(ctx, V, n, j + 1, w, n, hb.get());
+    hipLaunchKernelGGL(hcol_kernel, dim3(1), dim3(64), 0, s, H, m, j, hb.get(), j + 1, 1);
+    subtract_projection(ctx, V, n, j + 1, hb.get(), w, n);
+    dot_many(ctx, w, n, 1, w, n, nrm2);
+    hipLaunchKernelGGL(sqrt_store_kernel, dim3(1), dim3(64), 0, s, nrm2, H + int64_t(j + 1) * m + j);
+    TK_HIP(hipMemcpyAsync(&host, nrm2, sizeof(double), hipMemcpyDeviceToHost, s));
+    TK_HIP(hipStreamSynchronize(s));
+    TK_HIP(hipGetLastError());
+    const double hn = std::sqrt(std::max(0.0, host));
+    if (!std::isfinite(hn)) throw ArgError("exp_krylov: the matvec produced a non-finite vector");
+    if (hn < tol) {  // happy breakdown: the Krylov space is invariant
+      used = j + 1;
+      break;
+    }
+    if (j + 1 < m) {
+      double* vn = V + int64_t(j + 1) * n;
+      TK_HIP(hipMemcpyAsync(vn, w, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
+      scale_by_inv_sqrt(ctx, vn, nrm2, n);
+    }
+  }
+  hipLaunchKernelGGL(krylov_expm_kernel, dim3(1), dim3(kExpThreads), 0, s, H, m, used, alpha, beta, cb.get());
+  combine(ctx, V, n, used, cb.get(), 0.0, out, n);
+  double c[kExpKrylovMax];
+  TK_HIP(hipMemcpyAsync(c, cb.get(), size_t(used) * sizeof(double), hipMemcpyDeviceToHost, s));
+  TK_HIP(hipStreamSynchronize(s));
+  TK_HIP(hipGetLastError());
+  double cn = 0;
+  for (int i = 0; i < used; ++i) cn += c[i] * c[i];
+  if (!std::isfinite(cn)) throw ArgError("exp_krylov: exp(alpha H) overflowed");
+  res.krylov_dim = used;
+  res.norm_out = std::sqrt(cn);
+  return res;
+}
+
+ExpKrylovResult exp_krylov(Context* ctx, HeffPlan& plan, const double* v, double alpha, int krylov_dim, double tol,
+                           double* out) {
+  PlanUser u{&plan};
+  return exp_krylov_generic(ctx, plan.psi().total, plan_matvec_fp64, &u, v, alpha, krylov_dim, tol, out);
+}
+
+}  // namespace tk

@@ -68,6 +68,11 @@ class tk_kernel_timing(C.Structure):
                 ("flops", C.c_double)]
 
 
+class tk_tdvp_stats(C.Structure):
+    _fields_ = [("energy", C.c_double), ("max_truncation_error", C.c_double), ("seconds", C.c_double),
+                ("log_norm", C.c_double), ("max_bond_dim", C.c_int64), ("matvecs", C.c_int32), ("pad", C.c_int32)]
+
+
 class tk_sweep_stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("max_truncation_error", C.c_double), ("seconds", C.c_double),
                 ("t_plan", C.c_double), ("t_eig", C.c_double), ("t_svd", C.c_double), ("t_env", C.c_double),
@@ -128,6 +133,9 @@ def lib():
         "tk_eigh": (I32, [P, I64, P, pD, P]),
         "tk_davidson": (I32, [P, I64, MATVEC_FN, P, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_lanczos": (I32, [P, I64, MATVEC_FN, P, D, I32, I32, P, pD, pD, pI32]),
+        "tk_exp_krylov": (I32, [P, I64, MATVEC_FN, P, P, D, I32, D, P, pI32]),
+        "tk_dmrg_tdvp_step": (I32, [P, D, I64, I32, D, C.POINTER(tk_tdvp_stats)]),
+        "tk_dmrg_sweep_single_site": (I32, [P, C.POINTER(tk_sweep_stats)]),
         "tk_dmrg_create": (I32, [P, C.POINTER(tk_dmrg_config), C.POINTER(tk_mpo_site), C.POINTER(P)]),
         "tk_dmrg_destroy": (I32, [P]),
         "tk_dmrg_set_state": (I32, [P, C.POINTER(tk_qindex), C.POINTER(C.POINTER(D))]),
@@ -171,7 +179,8 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time",
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
-            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_ctx_timing",
+            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
+            "tk_dmrg_sweep_single_site", "tk_ctx_timing",
             "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split", "tk_hybridization",
             "tk_bath_fit", "tk_lanczos_tridiagonalize", "tk_broyden_create", "tk_broyden_update",
             "tk_broyden_destroy", "tk_krylov_tridiag", "tk_mlp_create", "tk_mlp_destroy", "tk_mlp_n_params",
@@ -485,6 +494,20 @@ class HipBackend:
         if dd:
             dd.free()
         return ev.value, x, rn.value, nmv.value
+
+    def exp_krylov(self, matvec_dev, n, v, alpha, krylov_dim=30, tol=1e-12):
+        """exp(alpha A) v over a device matvec (exp_krylov_f64, tdvp/mod.rs:197-272);
+        returns (result, Krylov vectors used)."""
+        cb = MATVEC_FN(lambda x, y, u: matvec_dev(x, y))
+        dv = self.ctx.to_device(np.asarray(v, dtype=np.float64))
+        do = self.ctx.empty(n)
+        used = C.c_int32(0)
+        _check(self.ctx.L.tk_exp_krylov(self.ctx.h, n, cb, None, C.c_void_p(dv.ptr), float(alpha), int(krylov_dim),
+                                        float(tol), C.c_void_p(do.ptr), C.byref(used)), self.ctx.h)
+        out = do.get()
+        dv.free()
+        do.free()
+        return out, used.value
 
     def lanczos(self, matvec_dev, n, x0, tol=1e-10, max_krylov=100, max_restarts=1000):
         """Lowest eigenpair by Lanczos with full reorthogonalisation (LanczosSolver)."""
@@ -1267,6 +1290,20 @@ class Dmrg:
         s = tk_sweep_stats()
         _check(self.ctx.L.tk_dmrg_sweep(self.h, C.byref(s)), self.ctx.h)
         return {f: getattr(s, f) for f, _ in tk_sweep_stats._fields_}
+
+    def sweep_single_site(self):
+        """One single-site sweep (dmrg_step_single_site, sweep/mod.rs:451-509, with write-back)."""
+        s = tk_sweep_stats()
+        _check(self.ctx.L.tk_dmrg_sweep_single_site(self.h, C.byref(s)), self.ctx.h)
+        return {f: getattr(s, f) for f, _ in tk_sweep_stats._fields_}
+
+    def tdvp_step(self, tau, max_bond=0, krylov_dim=30, krylov_tol=1e-12):
+        """One second-order two-site TDVP step psi <- exp(-tau H) psi, renormalised
+        (TdvpDriver::step, tdvp/mod.rs:91-141)."""
+        s = tk_tdvp_stats()
+        _check(self.ctx.L.tk_dmrg_tdvp_step(self.h, float(tau), int(max_bond), int(krylov_dim), float(krylov_tol),
+                                            C.byref(s)), self.ctx.h)
+        return {f: getattr(s, f) for f, _ in tk_tdvp_stats._fields_ if f != "pad"}
 
     def steps(self):
         n = C.c_int32(0)

@@ -1,0 +1,156 @@
+"""Krylov exponential and two-site TDVP on the GPU, through the C ABI.
+
+exp_krylov: the reference's own tests (tk-dmrg/src/tdvp/mod.rs:410-484), parity
+with the oracle's restatement of exp_krylov_f64 (relative 1e-12, same number of
+Arnoldi vectors) on partial Krylov spaces, and scipy.linalg.expm where the
+Krylov space is complete.
+
+TDVP: the reference's TdvpDriver::step leaves the MPS unchanged, so the step is
+pinned by mathematics instead: with every bond at full dimension the two-site
+projector-splitting integrator is exact, so k steps must reproduce
+expm(-k tau H) psi_0 of the dense Hamiltonian (state to 1e-9, norm factor to
+1e-9, energy to 1e-10); long imaginary-time runs reach the ED ground energy.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import tdvp_oracle
+import tk_hip
+from ed import heisenberg_ground_energy
+from mps_dense import dense_mpo, dense_mps
+
+pytestmark = pytest.mark.gpu
+
+
+class _Dev:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+@pytest.fixture(scope="module")
+def be(ctx):
+    return tk_hip.HipBackend(ctx)
+
+
+def _dense_matvec(be, ctx, h):
+    n = h.shape[0]
+    dh = ctx.to_device(np.ascontiguousarray(h, dtype=np.float64))
+
+    def mv(x, y):
+        be.gemm_strided(n, 1, n, 1.0, dh, n, 1, _Dev(x), 1, 1, 0.0, _Dev(y), 1, 1)
+    return mv, dh
+
+
+def test_exp_krylov_reference_cases(be, ctx):
+    print("case exp_krylov reference tests", flush=True)
+    mv, keep = _dense_matvec(be, ctx, np.eye(5))
+    r, m = be.exp_krylov(mv, 5, np.array([1.0, 0, 0, 0, 0]), 0.5, 10, 1e-12)
+    assert abs(r[0] - np.exp(0.5)) < 1e-8 and np.all(np.abs(r[1:]) < 1e-10) and m == 1
+    d = np.array([1.0, 2.0, 3.0])
+    mv, keep = _dense_matvec(be, ctx, np.diag(d))
+    r, m = be.exp_krylov(mv, 3, np.ones(3), -0.1, 10, 1e-12)
+    assert np.max(np.abs(r - np.exp(-0.1 * d))) < 1e-14
+    a = 2 * np.eye(4) - np.eye(4, k=1) - np.eye(4, k=-1)
+    mv, keep = _dense_matvec(be, ctx, a)
+    v = np.array([1.0, 2, 3, 4])
+    r, m = be.exp_krylov(mv, 4, v, 0.0, 10, 1e-12)
+    assert np.max(np.abs(r - v)) < 1e-10
+
+
+# (64, 30, -1.0): |alpha A| ~ 11 takes the scaling-and-squaring branch, whose
+# six squarings amplify rounding, hence the looser 1e-11 there
+@pytest.mark.parametrize("n,kdim,alpha,rel", [(40, 12, -0.3, 1e-12), (40, 20, 0.15, 1e-12), (64, 30, -1.0, 1e-11),
+                                              (40, 40, -0.5, 1e-12)])
+def test_exp_krylov_matches_oracle(be, ctx, n, kdim, alpha, rel):
+    print(f"case exp_krylov n={n} kdim={kdim} alpha={alpha}", flush=True)
+    rng = np.random.default_rng(n + kdim)
+    g = rng.standard_normal((n, n))
+    a = (g + g.T) / 2
+    v = rng.standard_normal(n)
+    mv, keep = _dense_matvec(be, ctx, a)
+    r, m = be.exp_krylov(mv, n, v, alpha, kdim, 1e-13)
+    ro, mo = tdvp_oracle.exp_krylov(lambda x: a @ x, v, alpha, kdim, 1e-13)
+    assert m == mo
+    assert np.linalg.norm(r - ro) <= rel * np.linalg.norm(ro)
+    if kdim == n:
+        ref = scipy.linalg.expm(alpha * a) @ v
+        assert np.linalg.norm(r - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
+def test_exp_krylov_zero_vector_and_limits(be, ctx):
+    mv, keep = _dense_matvec(be, ctx, np.eye(6))
+    r, m = be.exp_krylov(mv, 6, np.zeros(6), -1.0, 6, 1e-12)
+    assert m == 0 and not np.any(r)
+    with pytest.raises(tk_hip.TkError, match="48"):
+        be.exp_krylov(mv, 6, np.ones(6), -1.0, 49, 1e-12)
+
+
+def _tdvp_case(ctx, kind, n, total, tau, steps, **mp):
+    mpo = tk_hip.chain_mpo(kind, n, **mp)
+    sym = tk_hip.SYM_U1 if kind == "heisenberg" else tk_hip.SYM_U1U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], total, 256, seed=7)
+    psi0 = dense_mps(sym, mpo, bonds, data)
+    psi0 /= np.linalg.norm(psi0)
+    h, _ = dense_mpo(mpo)
+    d = tk_hip.Dmrg(ctx, mpo, sym, schedule=(256,), svd_cutoff=1e-14)
+    d.set_state(bonds, data)
+    d.prepare()
+    log_norm, st = 0.0, None
+    for _ in range(steps):
+        st = d.tdvp_step(tau, max_bond=256, krylov_dim=40, krylov_tol=1e-14)
+        log_norm += st["log_norm"]
+    psi = dense_mps(sym, mpo, [d.bond(i) for i in range(n + 1)], [d.site(i) for i in range(n)])
+    exact = scipy.linalg.expm(-steps * tau * h) @ psi0
+    return psi, exact, log_norm, st, h
+
+
+@pytest.mark.parametrize("kind,n,total,mp", [("heisenberg", 6, (0, 0), {}), ("heisenberg", 7, (1, 0), {}),
+                                              ("hubbard", 4, (2, 2), {"t": 1.0, "U": 4.0}),
+                                              ("hubbard", 4, (3, 1), {"t": 1.0, "U": 2.0})])
+def test_gpu_tdvp_full_bonds_is_exact(ctx, kind, n, total, mp):
+    print(f"case tdvp exact {kind} n={n} total={total}", flush=True)
+    tau, steps = 0.1, 4
+    psi, exact, log_norm, st, h = _tdvp_case(ctx, kind, n, total, tau, steps, **mp)
+    nrm = np.linalg.norm(exact)
+    assert abs(log_norm - np.log(nrm)) < 1e-9
+    target = exact / nrm
+    assert abs(np.linalg.norm(psi) - 1) < 1e-10
+    sign = np.sign(psi @ target)
+    assert np.max(np.abs(sign * psi - target)) < 1e-9
+    assert abs(st["energy"] - target @ h @ target) < 1e-10
+    assert st["max_truncation_error"] < 1e-20
+
+
+def test_gpu_tdvp_imaginary_time_reaches_ground_state(ctx):
+    print("case tdvp imaginary time heisenberg n=10", flush=True)
+    n = 10
+    mpo = tk_hip.chain_mpo("heisenberg", n)
+    bonds, data = tk_hip.initial_mps(tk_hip.SYM_U1, [m.phys for m in mpo], (0, 0), 64, seed=3)
+    d = tk_hip.Dmrg(ctx, mpo, tk_hip.SYM_U1, schedule=(64,), svd_cutoff=1e-14)
+    d.set_state(bonds, data)
+    d.prepare()
+    # total time 45: the excited weight decays like exp(-gap * t) with the
+    # Sz = 0 triplet gap ~ 0.34 of the ten-site chain
+    e = [d.tdvp_step(0.75, krylov_dim=40)["energy"] for _ in range(60)]
+    assert all(e[k + 1] <= e[k] + 1e-12 for k in range(len(e) - 1))  # imaginary time lowers the energy
+    assert abs(e[-1] - heisenberg_ground_energy(n)) < 1e-9
+
+
+def test_gpu_tdvp_after_two_site_sweeps(ctx):
+    """A two-site sweep leaves the centre at site 1; the TDVP step first moves
+    it to site 0, so imaginary time from a converged state stays there."""
+    print("case tdvp after dmrg hubbard n=6", flush=True)
+    from ed import hubbard_ground_energy
+    n = 6
+    mpo = tk_hip.chain_mpo("hubbard", n, t=1.0, U=4.0)
+    bonds, data = tk_hip.initial_mps(tk_hip.SYM_U1U1, [m.phys for m in mpo], (3, 3), 64, seed=2)
+    d = tk_hip.Dmrg(ctx, mpo, tk_hip.SYM_U1U1, schedule=(64,), davidson_tol=1e-12, svd_cutoff=1e-14)
+    d.set_state(bonds, data)
+    d.prepare()
+    for _ in range(4):
+        d.sweep()
+    st = d.tdvp_step(0.2, krylov_dim=30)
+    e0 = hubbard_ground_energy(n, 3, 3, 1.0, 4.0)
+    assert abs(st["energy"] - e0) < 1e-9
+    assert abs(st["log_norm"] + 0.2 * e0) < 1e-8  # exp(-tau H) on an eigenstate scales it by exp(-tau E0)
