@@ -322,6 +322,15 @@ tk_status tk_dmrg_steps(tk_dmrg* d, int32_t capacity, int32_t* n, tk_step_stats*
 tk_status tk_dmrg_bond(tk_dmrg* d, int32_t bond, int32_t capacity, int32_t* n_sectors, int32_t* qn, int64_t* dims);
 tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, double* data);
 
+/* Step 2 of compute_greens_function_tdvp (tk-dmft/src/spectral/tdvp.rs:55-108, where
+ * it is described but not implemented): psi <- O psi for an operator O on site 0
+ * of definite charge dq[0..1] (U1: dq[1] = 0).  op is d x d row-major (row = out,
+ * column = in) in the ascending charge order of site 0's physical QIndex.  Bonds
+ * 1..N shift their charges by dq, the state is renormalised with the centre at
+ * site 0 and the environments are rebuilt; *norm2 = |O psi|^2 of the normalised
+ * input.  Site 0 carries no Jordan-Wigner string, so c and c+ act as given. */
+tk_status tk_dmrg_apply_site_op(tk_dmrg* d, const double* op, const int32_t* dq, double* norm2);
+
 /* TdvpStepResult (tk-dmrg/src/tdvp/mod.rs:20-33) plus the step's energy and norm factor. */
 typedef struct {
   double energy;                /* <psi|H|psi>/<psi|psi> of the step's last two-site tensor */

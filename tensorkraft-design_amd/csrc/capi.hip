@@ -717,6 +717,17 @@ tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out) {
   });
 }
 
+tk_status tk_dmrg_apply_site_op(tk_dmrg* d, const double* op, const int32_t* dq, double* norm2) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d || !op || !dq) throw tk::ArgError("null argument");
+    const int64_t dim = d->e->phys(0).total;
+    std::vector<double> o(op, op + dim * dim);
+    const tk::QN q{dq[0], d->e->sym() == tk::Sym::U1 ? 0 : dq[1]};
+    const double n2 = d->e->apply_site0_operator(o, q);
+    if (norm2) *norm2 = n2;
+  });
+}
+
 tk_status tk_dmrg_tdvp_step(tk_dmrg* d, double tau, int64_t max_bond, int32_t krylov_dim, double krylov_tol,
                             tk_tdvp_stats* out) {
   return guard(d ? d->ctx : nullptr, [&] {

@@ -374,6 +374,94 @@ SweepStats DmrgEngine::sweep_single_site() {
   return s;
 }
 
+double DmrgEngine::apply_site0_operator(const std::vector<double>& op, QN dq) {
+  centre_to_site0();
+  const QIndex& P = cfg_.mpo[0].phys;
+  const int64_t d = P.total;
+  if (int64_t(op.size()) != d * d) throw ArgError("site operator: expected a d x d matrix");
+  IndexMap pm(P);
+  for (int64_t so = 0; so < d; ++so)
+    for (int64_t si = 0; si < d; ++si)
+      if (op[size_t(so * d + si)] != 0.0 && P.q[pm.sector[so]] != fuse(P.q[pm.sector[si]], dq))
+        throw ArgError("site operator: an entry does not carry the stated charge");
+  auto shifted = [&](const QIndex& b) {
+    std::vector<std::pair<QN, int64_t>> v;
+    for (int s = 0; s < b.n(); ++s) v.push_back({fuse(b.q[s], dq), b.dim[s]});
+    return QIndex(v);
+  };
+  std::vector<QIndex> nb(bonds_);
+  for (int k = 1; k <= n_; ++k) nb[k] = shifted(bonds_[k]);
+  std::vector<Site> ns(n_);
+  for (int i = 0; i < n_; ++i) {
+    ns[i].t = make_block_table(cfg_.sym, {cfg_.mpo[i].phys, nb[i], nb[i + 1]}, {Dir::Out, Dir::Out, Dir::In}, QN{});
+    ns[i].d.reset(ctx_, size_t(std::max<int64_t>(1, ns[i].t.total)));
+    launch_fill(ctx_, ns[i].d.get(), 0.0, std::max<int64_t>(1, ns[i].t.total));
+  }
+  // sites 1.. keep their data: the sector order of a uniformly shifted bond
+  // is unchanged, so every block moves to the block of the same sectors
+  for (int i = 1; i < n_; ++i) {
+    std::vector<CopyJob> cj;
+    const Site& o = site_[i];
+    for (int64_t b = 0; b < o.t.n_blocks(); ++b) {
+      int32_t secs[3] = {o.t.s(b, 0), o.t.s(b, 1), o.t.s(b, 2)};
+      const int64_t nbk = ns[i].t.find(secs);
+      if (nbk < 0) throw std::logic_error("site operator: shifted block missing");
+      const int32_t rows = int32_t(cfg_.mpo[i].phys.dim[secs[0]] * bonds_[i].dim[secs[1]]);
+      const int32_t cols = int32_t(bonds_[i + 1].dim[secs[2]]);
+      cj.push_back({o.t.offset[b], ns[i].t.offset[nbk], rows, cols, cols, cols, 0, 0});
+    }
+    launch_copies(ctx_, cj, o.d.get(), o.t.total, ns[i].d.get(), ns[i].t.total);
+  }
+  // site 0: new row (so, r) = sum_si O[so][si] old row (si, r), one output per
+  // (so, right sector) and one term per contributing si (coefficient O)
+  {
+    const Site& o = site_[0];
+    DBuf<double> one(ctx_, 1);
+    launch_fill(ctx_, one.get(), 1.0, 1);
+    ContractList cl;
+    cl.flags = 0;
+    for (int64_t b = 0; b < ns[0].t.n_blocks(); ++b) {
+      const int s_out = ns[0].t.s(b, 0), l = ns[0].t.s(b, 1), r = ns[0].t.s(b, 2);
+      const int64_t dL = nb[0].dim[l], dR = nb[1].dim[r];
+      for (int64_t xo = 0; xo < P.dim[s_out]; ++xo) {
+        const int64_t so = P.off[s_out] + xo;
+        for (int64_t row = 0; row < dL; ++row) {
+          cl.begin_out(ns[0].t.offset[b] + (xo * dL + row) * dR, 1, int(dR), int(dR));
+          for (int64_t si = 0; si < d; ++si) {
+            const double c = op[size_t(so * d + si)];
+            if (c == 0.0) continue;
+            int32_t secs[3] = {pm.sector[si], l, r};
+            const int64_t ob = o.t.find(secs);
+            if (ob < 0) continue;
+            cl.begin_term(o.t.offset[ob] + (int64_t(pm.within[si]) * dL + row) * dR, int(dR), 1);
+            cl.add_src(0, 1, c);
+          }
+        }
+      }
+    }
+    if (!cl.outs.empty()) run_list(ctx_, cl, 1, o.t.total, ns[0].t.total, one.get(), o.d.get(), ns[0].d.get());
+  }
+  DBuf<double> nrm(ctx_, 1);
+  dot_many(ctx_, ns[0].d.get(), ns[0].t.total, 1, ns[0].d.get(), ns[0].t.total, nrm.get());
+  double norm2 = 0;
+  nrm.download(&norm2, 1);
+  ctx_->sync();
+  if (!(norm2 > 0) || !std::isfinite(norm2)) throw ArgError("site operator: the result is zero or not finite");
+  scale(ctx_, ns[0].d.get(), 1.0 / std::sqrt(norm2), ns[0].t.total);
+  bonds_ = std::move(nb);
+  site_ = std::move(ns);
+  L_.clear();
+  R_.clear();
+  L_.resize(n_ + 1);
+  R_.resize(n_ + 1);
+  L_[0] = boundary_env(bonds_[0], cfg_.mpo[0].wl);
+  R_[n_] = boundary_env(bonds_[n_], cfg_.mpo[n_ - 1].wr);
+  for (int i = n_ - 1; i >= 1; --i) grow_right(i);
+  ctx_->sync();
+  centre_ = 0;
+  return norm2;
+}
+
 TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol) {
   if (site_.empty() || L_.empty() || L_[0].t.n_blocks() == 0) throw ArgError("tdvp: prepare() first");
   if (!std::isfinite(tau)) throw ArgError("tdvp: time step is not finite");

@@ -107,6 +107,14 @@ class DmrgEngine {
   // untruncated split of the site with its neighbour (the bond dimension
   // cannot grow).  Needs prepare() (centre at site 0); leaves it at site 0.
   SweepStats sweep_single_site();
+  // psi <- O_0 psi for an operator O on site 0 of definite charge dq
+  // (O[so][si] != 0 only where q(so) = q(si) + dq; rows = out, columns = in,
+  // both in the sorted order of the site's physical QIndex).  Every bond
+  // from 1 on shifts its charges by dq (dimensions unchanged), the state is
+  // renormalised with the centre at site 0 and the environments rebuilt;
+  // returns |O psi|^2 for the normalised psi.  Site 0 carries no
+  // Jordan-Wigner string, so fermionic O act as given.
+  double apply_site0_operator(const std::vector<double>& op, QN dq);
 
  private:
   struct Site {

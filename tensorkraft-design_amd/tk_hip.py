@@ -136,6 +136,7 @@ def lib():
         "tk_exp_krylov": (I32, [P, I64, MATVEC_FN, P, P, D, I32, D, P, pI32]),
         "tk_dmrg_tdvp_step": (I32, [P, D, I64, I32, D, C.POINTER(tk_tdvp_stats)]),
         "tk_dmrg_sweep_single_site": (I32, [P, C.POINTER(tk_sweep_stats)]),
+        "tk_dmrg_apply_site_op": (I32, [P, pD, pI32, pD]),
         "tk_dmrg_create": (I32, [P, C.POINTER(tk_dmrg_config), C.POINTER(tk_mpo_site), C.POINTER(P)]),
         "tk_dmrg_destroy": (I32, [P]),
         "tk_dmrg_set_state": (I32, [P, C.POINTER(tk_qindex), C.POINTER(C.POINTER(D))]),
@@ -180,7 +181,7 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
             "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
-            "tk_dmrg_sweep_single_site", "tk_ctx_timing",
+            "tk_dmrg_sweep_single_site", "tk_dmrg_apply_site_op", "tk_ctx_timing",
             "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split", "tk_hybridization",
             "tk_bath_fit", "tk_lanczos_tridiagonalize", "tk_broyden_create", "tk_broyden_update",
             "tk_broyden_destroy", "tk_krylov_tridiag", "tk_mlp_create", "tk_mlp_destroy", "tk_mlp_n_params",
@@ -1246,6 +1247,7 @@ class Dmrg:
         self.ctx = ctx
         self.sym = sym
         self.n = len(mpo_sites)
+        self.phys0 = list(mpo_sites[0].phys)
         self._keep = []
         sites = []
         for m in mpo_sites:
@@ -1296,6 +1298,38 @@ class Dmrg:
         s = tk_sweep_stats()
         _check(self.ctx.L.tk_dmrg_sweep_single_site(self.h, C.byref(s)), self.ctx.h)
         return {f: getattr(s, f) for f, _ in tk_sweep_stats._fields_}
+
+    def apply_site_op(self, op, dq):
+        """psi <- O psi for O on site 0 (rows out, columns in, in the caller's order
+        of mpo[0].phys) of charge dq; returns |O psi|^2 of the normalised psi."""
+        op = np.asarray(op, dtype=np.float64)
+        secs = [(_qkey(q), int(dd)) for q, dd in self.phys0]
+        order = sorted(range(len(secs)), key=lambda i: secs[i][0])
+        off, acc = {}, 0
+        for i in order:
+            off[i] = acc
+            acc += secs[i][1]
+        perm = np.array([off[i] + x for i, (_, dd) in enumerate(secs) for x in range(dd)])
+        srt = np.zeros_like(op)
+        srt[np.ix_(perm, perm)] = op
+        srt = np.ascontiguousarray(srt)
+        q = np.array(_qkey(dq), dtype=np.int32)
+        n2 = C.c_double(0)
+        _check(self.ctx.L.tk_dmrg_apply_site_op(self.h, srt.ctypes.data_as(C.POINTER(C.c_double)),
+                                                q.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n2)), self.ctx.h)
+        return n2.value
+
+    def greens_function_tau(self, e0, dtau, n_steps, max_bond=0, krylov_dim=30, krylov_tol=1e-12):
+        """Imaginary-time G(tau_k) = -<phi| exp(-tau_k (H - e0)) |phi>, tau_k = 2 k dtau,
+        k = 0..n_steps, of the current state phi (for example O psi_0 after
+        apply_site_op; multiply by its norm^2): each TDVP step of dtau evolves phi to
+        exp(-k dtau H) phi, and G(2 k dtau) = -exp(2 sum log_norm + 2 k dtau e0)."""
+        taus, g, acc = [0.0], [-1.0], 0.0
+        for k in range(1, n_steps + 1):
+            acc += self.tdvp_step(dtau, max_bond, krylov_dim, krylov_tol)["log_norm"]
+            taus.append(2.0 * k * dtau)
+            g.append(-math.exp(2.0 * acc + 2.0 * k * dtau * e0))
+        return np.array(taus), np.array(g)
 
     def tdvp_step(self, tau, max_bond=0, krylov_dim=30, krylov_tol=1e-12):
         """One second-order two-site TDVP step psi <- exp(-tau H) psi, renormalised

@@ -18,7 +18,7 @@ import scipy.linalg
 import tdvp_oracle
 import tk_hip
 from ed import heisenberg_ground_energy
-from mps_dense import dense_mpo, dense_mps
+from mps_dense import dense_mpo, dense_mps, sorted_phys
 
 pytestmark = pytest.mark.gpu
 
@@ -154,3 +154,40 @@ def test_gpu_tdvp_after_two_site_sweeps(ctx):
     e0 = hubbard_ground_energy(n, 3, 3, 1.0, 4.0)
     assert abs(st["energy"] - e0) < 1e-9
     assert abs(st["log_norm"] + 0.2 * e0) < 1e-8  # exp(-tau H) on an eigenstate scales it by exp(-tau E0)
+
+
+@pytest.mark.parametrize("kind,n,total,opname,dq,mp", [
+    ("hubbard", 4, (2, 2), "Cdagup", (1, 0), {"t": 1.0, "U": 4.0}),
+    ("heisenberg", 6, (0, 0), "S+", (2, 0), {}),
+])
+def test_gpu_greens_function_tau_matches_ed(ctx, kind, n, total, opname, dq, mp):
+    """G(tau) = -<psi0| O+ exp(-tau (H - E0)) O |psi0> from the DMRG ground state,
+    O applied on site 0 and imaginary-time TDVP (exact at full bond dimension),
+    against the same correlator of the densified MPO's ground state."""
+    print(f"case greens tau {kind} n={n} op={opname}", flush=True)
+    mpo = tk_hip.chain_mpo(kind, n, **mp)
+    sym = tk_hip.SYM_U1 if kind == "heisenberg" else tk_hip.SYM_U1U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], total, 256, seed=4)
+    d = tk_hip.Dmrg(ctx, mpo, sym, schedule=(256,), davidson_tol=1e-12, svd_cutoff=1e-14)
+    d.set_state(bonds, data)
+    d.prepare()
+    for _ in range(4):
+        e0 = d.sweep()["energy"]
+    site_type = tk_hip.SITE_TYPES["electron" if kind == "hubbard" else "spin-1/2"]()
+    op = site_type.ops[opname]
+    norm2 = d.apply_site_op(op, dq)
+    taus, g = d.greens_function_tau(e0, 0.05, 6, max_bond=256, krylov_dim=40, krylov_tol=1e-14)
+    # dense reference in the sorted basis of the densified MPO
+    h, ch = dense_mpo(mpo)
+    idx = [k for k, q in enumerate(ch) if q == tuple(total)]
+    w, v = np.linalg.eigh(h[np.ix_(idx, idx)])
+    assert abs(w[0] - e0) < 1e-10 and w[1] - w[0] > 1e-3
+    psi0 = np.zeros(h.shape[0])
+    psi0[idx] = v[:, 0]
+    perm, _ = sorted_phys(mpo[0].phys)
+    ops = np.zeros_like(op)
+    ops[np.ix_(perm, perm)] = op
+    phi = np.kron(ops, np.eye(h.shape[0] // ops.shape[0])) @ psi0
+    assert abs(norm2 - phi @ phi) < 1e-10
+    ref = np.array([-(phi @ scipy.linalg.expm(-t * (h - w[0] * np.eye(h.shape[0]))) @ phi) for t in taus])
+    assert np.max(np.abs(norm2 * g - ref)) < 1e-9 * np.max(np.abs(ref))
