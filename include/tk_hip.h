@@ -331,6 +331,12 @@ tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, d
  * input.  Site 0 carries no Jordan-Wigner string, so c and c+ act as given. */
 tk_status tk_dmrg_apply_site_op(tk_dmrg* d, const double* op, const int32_t* dq, double* norm2);
 
+/* mps_overlap (named in the Green's function plan of
+ * tk-dmft/src/spectral/tdvp.rs:85-99): *out = <bra|ket> of the two engines'
+ * current states (same site count, symmetry and physical legs; norms not
+ * divided out).  Both engines must live on one device. */
+tk_status tk_dmrg_overlap(tk_dmrg* bra, tk_dmrg* ket, double* out);
+
 /* TdvpStepResult (tk-dmrg/src/tdvp/mod.rs:20-33) plus the step's energy and norm factor. */
 typedef struct {
   double energy;                /* <psi|H|psi>/<psi|psi> of the step's last two-site tensor */

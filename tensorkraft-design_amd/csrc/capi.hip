@@ -700,20 +700,7 @@ tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out) {
   return guard(d ? d->ctx : nullptr, [&] {
     if (!d) throw tk::ArgError("null argument");
     tk::SweepStats s = d->e->sweep();
-    if (out) {
-      out->energy = s.energy;
-      out->max_truncation_error = s.max_truncation_error;
-      out->seconds = s.seconds;
-      out->t_plan = s.t_plan;
-      out->t_eig = s.t_eig;
-      out->t_svd = s.t_svd;
-      out->t_env = s.t_env;
-      out->max_bond_dim = s.max_bond_dim;
-      out->matvecs = s.matvecs;
-      out->matvecs_fp64 = s.matvecs_fp64;
-      out->steps = s.steps;
-      out->unconverged = s.unconverged;
-    }
+    if (out) fill_sweep_stats(s, out);
   });
 }
 
@@ -725,6 +712,13 @@ tk_status tk_dmrg_apply_site_op(tk_dmrg* d, const double* op, const int32_t* dq,
     const tk::QN q{dq[0], d->e->sym() == tk::Sym::U1 ? 0 : dq[1]};
     const double n2 = d->e->apply_site0_operator(o, q);
     if (norm2) *norm2 = n2;
+  });
+}
+
+tk_status tk_dmrg_overlap(tk_dmrg* bra, tk_dmrg* ket, double* out) {
+  return guard(ket ? ket->ctx : nullptr, [&] {
+    if (!bra || !ket || !out) throw tk::ArgError("null argument");
+    *out = tk::mps_overlap(*bra->e, *ket->e);
   });
 }
 

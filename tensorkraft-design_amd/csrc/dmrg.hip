@@ -462,6 +462,105 @@ double DmrgEngine::apply_site0_operator(const std::vector<double>& op, QN dq) {
   return norm2;
 }
 
+double mps_overlap(const DmrgEngine& bra, const DmrgEngine& ket) {
+  Context* ctx = ket.context();
+  const int n = ket.n_sites();
+  if (bra.n_sites() != n || bra.sym() != ket.sym()) throw ArgError("overlap: the two states differ in length or symmetry");
+  if (bra.bonds().empty() || ket.bonds().empty()) throw ArgError("overlap: set_state first");
+  for (int i = 0; i < n; ++i)
+    if (!same_qindex(bra.phys(i), ket.phys(i))) throw ArgError("overlap: physical legs differ at site " + std::to_string(i));
+  // E on one bond: one block per charge present in both bond bases
+  struct EBlock {
+    int sb, sk;  // sector in the bra / ket bond
+    int64_t off, db, dk;
+  };
+  auto layout = [&](const QIndex& qb, const QIndex& qk, std::vector<EBlock>& blocks) {
+    blocks.clear();
+    int64_t tot = 0;
+    for (int a = 0; a < qb.n(); ++a) {
+      const int b = qk.find(qb.q[a]);
+      if (b < 0) continue;
+      blocks.push_back({a, b, tot, qb.dim[a], qk.dim[b]});
+      tot += qb.dim[a] * qk.dim[b];
+    }
+    return tot;
+  };
+  std::vector<EBlock> eb, nb;
+  int64_t etot = layout(bra.bonds()[0], ket.bonds()[0], eb);
+  if (etot != 1) throw ArgError("overlap: the left boundary bonds differ");
+  DBuf<double> E(ctx, 1);
+  launch_fill(ctx, E.get(), 1.0, 1);
+  for (int i = 0; i < n; ++i) {
+    const QIndex& P = ket.phys(i);
+    const QIndex& bbL = bra.bonds()[i];
+    const QIndex& bbR = bra.bonds()[i + 1];
+    const QIndex& bkL = ket.bonds()[i];
+    const QIndex& bkR = ket.bonds()[i + 1];
+    const BlockTable& tb = bra.site_table(i);
+    const BlockTable& tk = ket.site_table(i);
+    const int64_t ntot = layout(bbR, bkR, nb);
+    std::map<int, const EBlock*> e_of_ket;  // ket bond sector -> E block
+    for (const EBlock& x : eb) e_of_ket[x.sk] = &x;
+    std::map<int, const EBlock*> n_of_ket;
+    for (const EBlock& x : nb) n_of_ket[x.sk] = &x;
+    // stage 1: T = E[q] A_ket[s, x](q, q') for every ket block and x
+    struct TSlice {
+      int64_t off, db, dkr;
+      int s, x, bsec_l, ksec_r;
+    };
+    std::vector<TSlice> ts;
+    int64_t ttot = 0;
+    ContractList c1;
+    c1.flags = 0;
+    for (int64_t b = 0; b < tk.n_blocks(); ++b) {
+      const int s = tk.s(b, 0), l = tk.s(b, 1), r = tk.s(b, 2);
+      auto ei = e_of_ket.find(l);
+      if (ei == e_of_ket.end() || n_of_ket.find(r) == n_of_ket.end()) continue;
+      const EBlock& e = *ei->second;
+      const int64_t dkl = bkL.dim[l], dkr = bkR.dim[r];
+      for (int x = 0; x < P.dim[s]; ++x) {
+        ts.push_back({ttot, e.db, dkr, s, x, e.sb, r});
+        c1.begin_out(ttot, int(e.db), int(dkr), int(dkr));
+        c1.begin_term(tk.offset[b] + int64_t(x) * dkl * dkr, int(dkr), int(dkl));
+        c1.add_src(e.off, int(dkl), 1.0);
+        ttot += e.db * dkr;
+      }
+    }
+    DBuf<double> T(ctx, size_t(std::max<int64_t>(1, ttot)));
+    DBuf<double> En(ctx, size_t(std::max<int64_t>(1, ntot)));
+    launch_fill(ctx, En.get(), 0.0, std::max<int64_t>(1, ntot));
+    if (!c1.outs.empty()) run_list(ctx, c1, etot, tk.total, ttot, E.get(), ket.site_device(i), T.get());
+    // stage 2: E'[q'] = sum over (s, x, q) of A_bra[s, x](q, q')^T T
+    ContractList c2;
+    c2.flags = kTransA;
+    for (const EBlock& o : nb) {
+      bool opened = false;
+      for (const TSlice& t : ts) {
+        if (t.ksec_r != o.sk) continue;
+        int32_t secs[3] = {t.s, t.bsec_l, o.sb};
+        const int64_t bb = tb.find(secs);
+        if (bb < 0) continue;
+        if (!opened) {
+          c2.begin_out(o.off, int(o.db), int(o.dk), int(o.dk));
+          opened = true;
+        }
+        const int64_t dbl = bbL.dim[t.bsec_l], dbr = bbR.dim[o.sb];
+        c2.begin_term(t.off, int(t.dkr), int(dbl));
+        c2.add_src(tb.offset[bb] + int64_t(t.x) * dbl * dbr, int(dbr), 1.0);
+      }
+    }
+    if (!c2.outs.empty()) run_list(ctx, c2, tb.total, ttot, ntot, bra.site_device(i), T.get(), En.get());
+    E = std::move(En);
+    eb = nb;
+    etot = ntot;
+  }
+  if (etot != 1) throw ArgError("overlap: the states end in different total charges");
+  double v = 0;
+  E.download(&v, 1);
+  ctx->sync();
+  return v;
+}
+
 TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol) {
   if (site_.empty() || L_.empty() || L_[0].t.n_blocks() == 0) throw ArgError("tdvp: prepare() first");
   if (!std::isfinite(tau)) throw ArgError("tdvp: time step is not finite");

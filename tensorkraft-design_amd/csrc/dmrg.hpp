@@ -92,6 +92,8 @@ class DmrgEngine {
   const std::vector<QIndex>& bonds() const { return bonds_; }
   const BlockTable& site_table(int i) const { return site_[i].t; }
   std::vector<double> site_data(int i) const;
+  const double* site_device(int i) const { return site_[i].d.get(); }
+  Context* context() const { return ctx_; }
   const std::vector<StepStats>& last_steps() const { return steps_; }
   // One second-order two-site TDVP step of psi <- exp(-tau H) psi (real tau:
   // imaginary-time evolution), the state renormalised after every local
@@ -159,5 +161,13 @@ class DmrgEngine {
   std::vector<StepStats> steps_;
   SweepStats* cur_ = nullptr;
 };
+
+// <bra|ket> of two MPS over the same physical legs: the transfer matrix
+// E[q](bra, ket), block diagonal in the bond charge, carried from the left
+// end to the right by two grouped contractions per site,
+//   T = E[q] A_ket[s](q, q'),  E'[q'] = sum_{s, q} A_bra[s](q, q')^T T.
+// The state norm is not divided out.  Reference: mps_overlap, named in the
+// Green's function plan of tk-dmft/src/spectral/tdvp.rs:85-99.
+double mps_overlap(const DmrgEngine& bra, const DmrgEngine& ket);
 
 }  // namespace tk

@@ -137,6 +137,7 @@ def lib():
         "tk_dmrg_tdvp_step": (I32, [P, D, I64, I32, D, C.POINTER(tk_tdvp_stats)]),
         "tk_dmrg_sweep_single_site": (I32, [P, C.POINTER(tk_sweep_stats)]),
         "tk_dmrg_apply_site_op": (I32, [P, pD, pI32, pD]),
+        "tk_dmrg_overlap": (I32, [P, P, pD]),
         "tk_dmrg_create": (I32, [P, C.POINTER(tk_dmrg_config), C.POINTER(tk_mpo_site), C.POINTER(P)]),
         "tk_dmrg_destroy": (I32, [P]),
         "tk_dmrg_set_state": (I32, [P, C.POINTER(tk_qindex), C.POINTER(C.POINTER(D))]),
@@ -181,7 +182,8 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
             "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
-            "tk_dmrg_sweep_single_site", "tk_dmrg_apply_site_op", "tk_ctx_timing",
+            "tk_dmrg_sweep_single_site", "tk_dmrg_apply_site_op", "tk_dmrg_overlap",
+            "tk_ctx_timing",
             "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split", "tk_hybridization",
             "tk_bath_fit", "tk_lanczos_tridiagonalize", "tk_broyden_create", "tk_broyden_update",
             "tk_broyden_destroy", "tk_krylov_tridiag", "tk_mlp_create", "tk_mlp_destroy", "tk_mlp_n_params",
@@ -674,6 +676,60 @@ def heisenberg_w(J=1.0, position="bulk"):
            (2, 1): J * S_Z, (3, 1): 0.5 * J * S_MINUS, (4, 1): 0.5 * J * S_PLUS}
     wq = [((0, 0), 3), ((-2, 0), 1), ((2, 0), 1)]
     return _assemble(ops, 2, 5, SPIN_PHYS, wq, position, start=0, done=1)
+
+
+# Real-time evolution in real arithmetic.  A complex state psi = psi_R + i psi_I
+# is stored as the real state Phi = |R> psi_R + |I> psi_I with one extra
+# charge-0 site of two states (the "ancilla") in front of the chain.  With
+# J|R> = -|I>, J|I> = |R>, exp(t J (x) H) Phi carries exp(-i H t) psi, so the
+# two-site TDVP step with tau = -t evolves in real time; J (x) H is real
+# antisymmetric and the Krylov exponential (Arnoldi) needs no symmetry.
+ANCILLA_PHYS = [((0, 0), 2)]
+
+
+def complex_time_mpo(mpo):
+    """MPO of J (x) H on [ancilla] + the sites of `mpo` (W[s_in][s_out] = <s_out|J|s_in>)."""
+    w = np.zeros((2, 2, 1, 1))
+    w[0, 1, 0, 0] = -1.0  # <I|J|R>
+    w[1, 0, 0, 0] = 1.0   # <R|J|I>
+    return [MpoSite(ANCILLA_PHYS, list(mpo[0].wl), list(mpo[0].wl), w)] + list(mpo)
+
+
+def complex_state(bonds, sites, part="R"):
+    """(bonds, sites) of |part> (x) phi on [ancilla] + sites for a real state phi."""
+    anc = np.array([1.0, 0.0]) if part == "R" else np.array([0.0, 1.0])
+    return [bonds[0], bonds[0]] + list(bonds[1:]), [anc] + [np.asarray(x, dtype=np.float64) for x in sites]
+
+
+def greens_function_real_time(ctx, mpo, sym, bonds, sites, e0, dt, n_steps, max_bond=256, krylov_dim=30,
+                              krylov_tol=1e-12, svd_cutoff=1e-12, start=None):
+    """G(t_k) = -i exp(i e0 t_k) <phi| exp(-i H t_k) |phi>, t_k = k dt, k = 0..n_steps,
+    for the real normalised state phi = (bonds, sites) on the sites of `mpo`
+    (for example O psi_0 from Dmrg.apply_site_op, then multiply by its norm^2).
+
+    Steps 3-4 of compute_greens_function_tdvp (tk-dmft/src/spectral/tdvp.rs:55-108):
+    two-site TDVP in real time on the ancilla form (tau = -dt per step) and the
+    overlaps <R phi|Phi(t)>, <I phi|Phi(t)> (mps_overlap).  `start` replaces the
+    initial complex state |R> phi (bonds, sites on [ancilla] + sites)."""
+    cmpo = complex_time_mpo(mpo)
+    ket = Dmrg(ctx, cmpo, sym, schedule=(max_bond,), svd_cutoff=svd_cutoff)
+    ket.set_state(*(start if start is not None else complex_state(bonds, sites, "R")))
+    ket.prepare()
+    bra_r = Dmrg(ctx, cmpo, sym, schedule=(max_bond,))
+    bra_r.set_state(*complex_state(bonds, sites, "R"))
+    bra_i = Dmrg(ctx, cmpo, sym, schedule=(max_bond,))
+    bra_i.set_state(*complex_state(bonds, sites, "I"))
+    ts, g, acc = [], [], 0.0
+    for k in range(n_steps + 1):
+        if k > 0:
+            acc += ket.tdvp_step(-dt, max_bond, krylov_dim, krylov_tol)["log_norm"]
+        t = k * dt
+        amp = (bra_r.overlap(ket) + 1j * bra_i.overlap(ket)) * math.exp(acc)
+        ts.append(t)
+        g.append(-1j * complex(math.cos(e0 * t), math.sin(e0 * t)) * amp)
+    for e in (ket, bra_r, bra_i):
+        e.close()
+    return np.array(ts), np.array(g)
 
 
 def snake_path(lx, ly):
@@ -1318,6 +1374,12 @@ class Dmrg:
         _check(self.ctx.L.tk_dmrg_apply_site_op(self.h, srt.ctypes.data_as(C.POINTER(C.c_double)),
                                                 q.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(n2)), self.ctx.h)
         return n2.value
+
+    def overlap(self, ket):
+        """<self|ket> of two states over the same physical legs (norms not divided out)."""
+        v = C.c_double(0)
+        _check(self.ctx.L.tk_dmrg_overlap(self.h, ket.h, C.byref(v)), self.ctx.h)
+        return v.value
 
     def greens_function_tau(self, e0, dtau, n_steps, max_bond=0, krylov_dim=30, krylov_tol=1e-12):
         """Imaginary-time G(tau_k) = -<phi| exp(-tau_k (H - e0)) |phi>, tau_k = 2 k dtau,

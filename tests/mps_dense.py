@@ -47,3 +47,64 @@ def dense_mps(sym, mpo, bonds, sites):
         A = oracle.to_dense(sym, [W.phys, bonds[i], bonds[i + 1]], [OUT, OUT, IN], (0, 0), sites[i])
         psi = np.einsum("xa,sab->xsb", psi, A).reshape(psi.shape[0] * A.shape[0], A.shape[2])
     return psi[:, 0]
+
+
+def from_dense(sym, legs, dirs, flux, dense):
+    """Inverse of oracle.to_dense: block data in block-table order."""
+    offs = []
+    for leg in legs:
+        o, acc = {}, 0
+        for q, d in oracle._sorted_leg(leg):
+            o[q] = (acc, d)
+            acc += d
+        offs.append(o)
+    keys, boffs, tot = oracle.block_table(sym, legs, dirs, flux)
+    out = np.zeros(tot)
+    for k, off in zip(keys, boffs):
+        qs = oracle.unpack_key(sym, k, len(legs))
+        sl = tuple(slice(offs[i][qs[i]][0], offs[i][qs[i]][0] + offs[i][qs[i]][1]) for i in range(len(legs)))
+        blk = dense[sl]
+        out[off:off + blk.size] = blk.ravel()
+    return out
+
+
+def _bond_union(a, b):
+    """Direct sum of two bond bases: per charge the a-part, then the b-part."""
+    da = {_key(q): int(d) for q, d in a}
+    db = {_key(q): int(d) for q, d in b}
+    qs = sorted(set(da) | set(db))
+    bond = [(q, da.get(q, 0) + db.get(q, 0)) for q in qs]
+    pa, pb, acc = [], [], 0
+    for q in qs:
+        pa += list(range(acc, acc + da.get(q, 0)))
+        acc += da.get(q, 0)
+        pb += list(range(acc, acc + db.get(q, 0)))
+        acc += db.get(q, 0)
+    return bond, np.array(pa, dtype=int), np.array(pb, dtype=int)
+
+
+def mps_sum(sym, phys, a, b, cb=1.0):
+    """(bonds, sites) of |a> + cb |b> as a direct-sum MPS (bond dimensions add;
+    the end bonds, of dimension one, are shared)."""
+    (ba, sa), (bb, sb) = a, b
+    n = len(phys)
+    bonds, pos = [], []
+    for i in range(n + 1):
+        if i in (0, n):
+            bonds.append(list(ba[i]))
+            pos.append((np.array([0]), np.array([0])))
+        else:
+            bond, pa, pb = _bond_union(ba[i], bb[i])
+            bonds.append(bond)
+            pos.append((pa, pb))
+    sites = []
+    for i in range(n):
+        Aa = oracle.to_dense(sym, [phys[i], ba[i], ba[i + 1]], [OUT, OUT, IN], (0, 0), sa[i])
+        Ab = oracle.to_dense(sym, [phys[i], bb[i], bb[i + 1]], [OUT, OUT, IN], (0, 0), sb[i])
+        dl = sum(d for _, d in bonds[i])
+        dr = sum(d for _, d in bonds[i + 1])
+        S = np.zeros((Aa.shape[0], dl, dr))
+        S[np.ix_(np.arange(Aa.shape[0]), pos[i][0], pos[i + 1][0])] += Aa
+        S[np.ix_(np.arange(Ab.shape[0]), pos[i][1], pos[i + 1][1])] += (cb if i == 0 else 1.0) * Ab
+        sites.append(from_dense(sym, [phys[i], bonds[i], bonds[i + 1]], [OUT, OUT, IN], (0, 0), S))
+    return bonds, sites

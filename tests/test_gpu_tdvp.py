@@ -191,3 +191,92 @@ def test_gpu_greens_function_tau_matches_ed(ctx, kind, n, total, opname, dq, mp)
     assert abs(norm2 - phi @ phi) < 1e-10
     ref = np.array([-(phi @ scipy.linalg.expm(-t * (h - w[0] * np.eye(h.shape[0]))) @ phi) for t in taus])
     assert np.max(np.abs(norm2 * g - ref)) < 1e-9 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("kind,n,total,mp", [("heisenberg", 6, (0, 0), {}), ("hubbard", 4, (2, 2), {"t": 1.0, "U": 4.0})])
+def test_gpu_mps_overlap_matches_dense(ctx, kind, n, total, mp):
+    print(f"case overlap {kind} n={n}", flush=True)
+    mpo = tk_hip.chain_mpo(kind, n, **mp)
+    sym = tk_hip.SYM_U1 if kind == "heisenberg" else tk_hip.SYM_U1U1
+    phys = [m.phys for m in mpo]
+    a = tk_hip.initial_mps(sym, phys, total, 5, seed=11)
+    b = tk_hip.initial_mps(sym, phys, total, 12, seed=12)
+    da, db = tk_hip.Dmrg(ctx, mpo, sym), tk_hip.Dmrg(ctx, mpo, sym)
+    da.set_state(*a)
+    db.set_state(*b)
+    pa, pb = dense_mps(sym, mpo, *a), dense_mps(sym, mpo, *b)
+    assert abs(da.overlap(db) - pa @ pb) <= 1e-12 * np.linalg.norm(pa) * np.linalg.norm(pb)
+    assert abs(db.overlap(db) - pb @ pb) <= 1e-12 * (pb @ pb)
+    db.prepare()  # canonical gauge, norm 1: the overlap is gauge invariant
+    assert abs(da.overlap(db) - pa @ pb / np.linalg.norm(pb)) <= 1e-12 * np.linalg.norm(pa)
+
+
+def _complex_dense(sym, cmpo, bonds, sites):
+    phi = dense_mps(sym, cmpo, bonds, sites)
+    half = phi.shape[0] // 2  # the ancilla is the most significant site: |R> block, then |I>
+    return phi[:half] + 1j * phi[half:]
+
+
+def test_gpu_real_time_tdvp_full_bonds_is_exact(ctx):
+    """exp(-i H t) psi through the ancilla form: every bond (the ancilla's
+    included) at full dimension, so the integrator is exact."""
+    print("case real-time tdvp hubbard n=4", flush=True)
+    mpo = tk_hip.chain_mpo("hubbard", 4, t=1.0, U=4.0)
+    sym = tk_hip.SYM_U1U1
+    cmpo = tk_hip.complex_time_mpo(mpo)
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in cmpo], (2, 2), 256, seed=9)
+    psi0 = _complex_dense(sym, cmpo, bonds, data)
+    psi0 /= np.linalg.norm(psi0)
+    d = tk_hip.Dmrg(ctx, cmpo, sym, schedule=(256,), svd_cutoff=1e-14)
+    d.set_state(bonds, data)
+    d.prepare()
+    dt, steps, acc = 0.1, 4, 0.0
+    for _ in range(steps):
+        acc += d.tdvp_step(-dt, 256, 40, 1e-14)["log_norm"]
+    assert abs(acc) < 1e-10  # unitary
+    psi = _complex_dense(sym, cmpo, [d.bond(i) for i in range(len(cmpo) + 1)], [d.site(i) for i in range(len(cmpo))])
+    h, _ = dense_mpo(mpo)
+    exact = scipy.linalg.expm(-1j * steps * dt * h) @ psi0
+    assert np.max(np.abs(psi - exact)) < 1e-9
+
+
+def test_gpu_greens_function_real_time_matches_ed(ctx):
+    """G(t) = -i e^{i E0 t} <phi|e^{-i H t}|phi>, phi = c+_up psi0 / |.|, from the
+    DMRG ground state of a 4-site Hubbard chain.  The start state gets a
+    component eps = 1e-6 of a random full-rank state so that every bond is at
+    full dimension (the integrator is then exact); that perturbs G by at most
+    about 2 eps."""
+    print("case greens real time hubbard n=4", flush=True)
+    from mps_dense import mps_sum
+    mpo = tk_hip.chain_mpo("hubbard", 4, t=1.0, U=4.0)
+    sym = tk_hip.SYM_U1U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (2, 2), 256, seed=4)
+    d = tk_hip.Dmrg(ctx, mpo, sym, schedule=(256,), davidson_tol=1e-12, svd_cutoff=1e-14)
+    d.set_state(bonds, data)
+    d.prepare()
+    for _ in range(4):
+        e0 = d.sweep()["energy"]
+    op = tk_hip.SITE_TYPES["electron"]().ops["Cdagup"]
+    d.apply_site_op(op, (1, 0))
+    pb = [d.bond(i) for i in range(5)]
+    ps = [d.site(i) for i in range(4)]
+    cmpo = tk_hip.complex_time_mpo(mpo)
+    cphys = [m.phys for m in cmpo]
+    chi = tk_hip.initial_mps(sym, cphys, (3, 2), 256, seed=21)
+    eps = 1e-6 / np.linalg.norm(dense_mps(sym, cmpo, *chi))
+    start = mps_sum(sym, cphys, tk_hip.complex_state(pb, ps, "R"), chi, cb=eps)
+    t, g = tk_hip.greens_function_real_time(ctx, mpo, sym, pb, ps, e0, 0.1, 10, max_bond=256, krylov_dim=40,
+                                            krylov_tol=1e-14, svd_cutoff=1e-14, start=start)
+    h, ch = dense_mpo(mpo)
+    idx = [k for k, q in enumerate(ch) if q == (2, 2)]
+    w, v = np.linalg.eigh(h[np.ix_(idx, idx)])
+    psi0 = np.zeros(h.shape[0])
+    psi0[idx] = v[:, 0]
+    perm, _ = sorted_phys(mpo[0].phys)
+    ops = np.zeros_like(op)
+    ops[np.ix_(perm, perm)] = op
+    phi = np.kron(ops, np.eye(h.shape[0] // 4)) @ psi0
+    phi /= np.linalg.norm(phi)
+    ref = np.array([-1j * np.exp(1j * w[0] * tt) * (phi @ scipy.linalg.expm(-1j * tt * h) @ phi) for tt in t])
+    assert abs(g[0] + 1j) < 3e-6
+    assert np.max(np.abs(g - ref)) < 3e-6

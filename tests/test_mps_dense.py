@@ -33,3 +33,19 @@ def test_dense_mps_lives_in_its_charge_sector():
         _, ch = dense_mpo(mpo)
         outside = [k for k, q in enumerate(ch) if q != tuple(total)]
         assert np.linalg.norm(psi) > 0 and np.abs(psi[outside]).max() == 0
+
+
+def test_mps_sum_and_from_dense_round_trip():
+    from mps_dense import from_dense, mps_sum
+    import oracle
+    mpo = tk_hip.chain_mpo("hubbard", 4, t=1.0, U=4.0)
+    phys = [m.phys for m in mpo]
+    sym = tk_hip.SYM_U1U1
+    a = tk_hip.initial_mps(sym, phys, (2, 2), 6, seed=1)
+    b = tk_hip.initial_mps(sym, phys, (2, 2), 9, seed=2)
+    s = mps_sum(sym, phys, a, b, cb=-0.3)
+    pa, pb, ps = (dense_mps(sym, mpo, *x) for x in (a, b, s))
+    assert np.abs(ps - (pa - 0.3 * pb)).max() < 1e-12
+    legs = [phys[1], a[0][1], a[0][2]]
+    d = oracle.to_dense(sym, legs, [1, 1, 0], (0, 0), a[1][1])
+    assert np.array_equal(from_dense(sym, legs, [1, 1, 0], (0, 0), d), np.asarray(a[1][1]))
