@@ -393,6 +393,127 @@ def bench_sweep(args):
     return rec
 
 
+def _ground_state(tk_hip, ctx, args, log):
+    """Two-site FP64 DMRG ground state of the Hubbard chain (t=1, U=4, half
+    filling) ramped to args.dmax; returns (engine, mpo, sym, energy)."""
+    n = args.sites
+    sched = warmup_schedule(min(args.d0, args.dmax), args.dmax, args.ramp)
+    mpo = tk_hip.chain_mpo("hubbard", n, t=1.0, U=4.0)
+    sym = tk_hip.SYM_U1U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (n // 2, n // 2), sched[0], seed=0,
+                                     min_sector_dim=0)
+    d = tk_hip.Dmrg(ctx, mpo, sym, schedule=sched, davidson_tol=args.tol, svd_cutoff=1e-12)
+    d.set_state(bonds, data)
+    d.prepare()
+    e = 0.0
+    for k in range(len(sched)):
+        t0 = time.perf_counter()
+        s = d.sweep()
+        e = s["energy"]
+        log(f"[ground state] sweep {k + 1}/{len(sched)} D={sched[k]} E={e:.13f} t={time.perf_counter() - t0:.3f}s")
+    return d, mpo, sym, e, sched
+
+
+def _timed_record(metric, value, unit, args, workload, timing, extra):
+    rec = {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * value, 3), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic", "config": workload,
+           "roofline": roofline_of(timing), "kernels": kernel_table(timing)}
+    rec.update(extra)
+    return rec
+
+
+def bench_tdvp(args):
+    """Real-time two-site TDVP of c+_up psi_0 on the Hubbard chain (the G(t)
+    workload of the impurity solver): ground state by two-site DMRG ramped to
+    D = dmax, c+_up on site 0, then real-time steps dt = 0.05 in the ancilla
+    form at maximum bond dimension dmax (Krylov dimension 30), the last
+    args.steps of them timed.  No CPU baseline: the oracle has no TDVP."""
+    import tk_hip
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    ctx = tk_hip.Context(0)
+    d, mpo, sym, e0, sched = _ground_state(tk_hip, ctx, args, log)
+    n2 = d.apply_site_op(tk_hip.SITE_TYPES["electron"]().ops["Cdagup"], (1, 0))
+    phi = ([d.bond(i) for i in range(args.sites + 1)], [d.site(i) for i in range(args.sites)])
+    d.close()
+    ket = tk_hip.Dmrg(ctx, tk_hip.complex_time_mpo(mpo), sym, schedule=(args.dmax,), svd_cutoff=1e-12)
+    ket.set_state(*tk_hip.complex_state(*phi, "R"))
+    ket.prepare()
+    dt, wall, steps = 0.05, 0.0, []
+    for k in range(args.warmup + args.steps):
+        timed = k >= args.warmup
+        if timed and k == args.warmup:
+            ctx.sync()
+            ctx.timing(True)
+        t0 = time.perf_counter()
+        st = ket.tdvp_step(-dt, args.dmax, 30, 1e-12)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        if timed:
+            wall += el
+            steps.append(st)
+        log(f"[tdvp] step {k + 1}/{args.warmup + args.steps}{' (timed)' if timed else ''} t={el:.3f}s "
+            f"maxD={st['max_bond_dim']} trunc={st['max_truncation_error']:.2e} matvecs={st['matvecs']} "
+            f"log_norm={st['log_norm']:.2e}")
+    timing = ctx.timing_read()
+    ctx.timing(False)
+    ket.close()
+    ctx.close()
+    workload = {"workload": f"tdvp_real_time_hubbard_L{args.sites}_D{args.dmax}", "sites": args.sites, "t": 1.0,
+                "U": 4.0, "filling": "half, Sz=0", "operator": "c+_up on site 0", "dt": dt, "krylov_dim": 30,
+                "max_bond_dim": args.dmax, "ground_state_schedule": sched}
+    return _timed_record(f"TDVP real-time step time, Hubbard L={args.sites}, c+ psi0, D<={args.dmax}",
+                         wall / args.steps, "s/step", args, workload, timing,
+                         {"ground_state_energy": e0, "norm2_cdag_psi0": n2,
+                          "max_bond_dim": max(s["max_bond_dim"] for s in steps),
+                          "max_truncation_error": max(s["max_truncation_error"] for s in steps),
+                          "matvecs_per_step": sum(s["matvecs"] for s in steps) / len(steps),
+                          "cpu_baseline": None})
+
+
+def bench_sweep1(args):
+    """Single-site sweep at the two-site ground state's bond dimensions
+    (Hubbard chain ramped to D = dmax by two-site sweeps, then args.warmup
+    untimed and args.steps timed single-site sweeps).  No CPU baseline."""
+    import tk_hip
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    ctx = tk_hip.Context(0)
+    d, mpo, sym, e2, sched = _ground_state(tk_hip, ctx, args, log)
+    wall, out = 0.0, []
+    for k in range(args.warmup + args.steps):
+        timed = k >= args.warmup
+        if timed and k == args.warmup:
+            ctx.sync()
+            ctx.timing(True)
+        t0 = time.perf_counter()
+        s = d.sweep_single_site()
+        ctx.sync()
+        el = time.perf_counter() - t0
+        if timed:
+            wall += el
+            out.append(s)
+        log(f"[single-site] sweep {k + 1}{' (timed)' if timed else ''} E={s['energy']:.13f} t={el:.3f}s "
+            f"matvecs={s['matvecs']} maxD={s['max_bond_dim']}")
+    timing = ctx.timing_read()
+    ctx.timing(False)
+    d.close()
+    ctx.close()
+    workload = {"workload": f"dmrg_single_site_sweep_hubbard_L{args.sites}_D{args.dmax}", "sites": args.sites,
+                "t": 1.0, "U": 4.0, "filling": "half, Sz=0", "two_site_schedule": sched}
+    return _timed_record(f"single-site DMRG sweep time, Hubbard L={args.sites}, D={args.dmax}",
+                         wall / args.steps, "s/sweep", args, workload, timing,
+                         {"energy_two_site": e2, "energy_single_site": out[-1]["energy"],
+                          "energy_not_raised": out[-1]["energy"] <= e2 + 1e-10,
+                          "max_bond_dim": out[-1]["max_bond_dim"], "matvecs": out[-1]["matvecs"],
+                          "cpu_baseline": None})
+
+
 def bench_sweep_c4(args):
     """Config-4 DMRG: J1 Heisenberg on the 6 x 12 cylinder (snake order, MPO
     bond dims up to 20, U(1), Sz = 0), D ramped to 4096, the same seeded MPS
@@ -454,7 +575,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "sweep-c4", "heff", "heff-c4", "copy"])
+    ap.add_argument("--workload", default="sweep",
+                    choices=["sweep", "sweep-c4", "heff", "heff-c4", "copy", "tdvp", "sweep1"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--sites", type=int, default=64)
@@ -475,6 +597,12 @@ def main():
         args.steps = 200 if args.steps is None else args.steps
         args.warmup = 20 if args.warmup is None else args.warmup
         rec = bench_heff(args)
+    elif args.workload in ("tdvp", "sweep1"):
+        args.dmax = (256 if args.workload == "tdvp" else 2048) if args.dmax is None else args.dmax
+        args.ramp = (4 if args.workload == "tdvp" else 7) if args.ramp is None else args.ramp
+        args.steps = (4 if args.workload == "tdvp" else 1) if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        rec = bench_tdvp(args) if args.workload == "tdvp" else bench_sweep1(args)
     elif args.workload == "sweep-c4":
         args.dmax = 4096 if args.dmax is None else args.dmax
         args.ramp = 4 if args.ramp is None else args.ramp
