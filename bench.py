@@ -428,7 +428,10 @@ def bench_tdvp(args):
     workload of the impurity solver): ground state by two-site DMRG ramped to
     D = dmax, c+_up on site 0, then real-time steps dt = 0.05 in the ancilla
     form at maximum bond dimension dmax (Krylov dimension 30), the last
-    args.steps of them timed.  No CPU baseline: the oracle has no TDVP."""
+    args.steps of them timed, once with the reference's fixed Krylov dimension
+    and once with the early stop (adaptive_tol 1e-12), both from the same
+    state; value is the early-stop step time.  No CPU baseline: the oracle
+    has no TDVP."""
     import tk_hip
 
     def log(msg):
@@ -439,38 +442,49 @@ def bench_tdvp(args):
     n2 = d.apply_site_op(tk_hip.SITE_TYPES["electron"]().ops["Cdagup"], (1, 0))
     phi = ([d.bond(i) for i in range(args.sites + 1)], [d.site(i) for i in range(args.sites)])
     d.close()
-    ket = tk_hip.Dmrg(ctx, tk_hip.complex_time_mpo(mpo), sym, schedule=(args.dmax,), svd_cutoff=1e-12)
-    ket.set_state(*tk_hip.complex_state(*phi, "R"))
-    ket.prepare()
-    dt, wall, steps = 0.05, 0.0, []
-    for k in range(args.warmup + args.steps):
-        timed = k >= args.warmup
-        if timed and k == args.warmup:
+    dt, runs = 0.05, {}
+    for mode, atol in (("fixed", 0.0), ("adaptive", 1e-12)):
+        ket = tk_hip.Dmrg(ctx, tk_hip.complex_time_mpo(mpo), sym, schedule=(args.dmax,), svd_cutoff=1e-12)
+        ket.set_state(*tk_hip.complex_state(*phi, "R"))
+        ket.prepare()
+        wall, steps = 0.0, []
+        for k in range(args.warmup + args.steps):
+            timed = k >= args.warmup
+            if timed and k == args.warmup:
+                ctx.sync()
+                ctx.timing(True)
+            t0 = time.perf_counter()
+            st = ket.tdvp_step(-dt, args.dmax, 30, 1e-12, atol)
             ctx.sync()
-            ctx.timing(True)
-        t0 = time.perf_counter()
-        st = ket.tdvp_step(-dt, args.dmax, 30, 1e-12)
-        ctx.sync()
-        el = time.perf_counter() - t0
-        if timed:
-            wall += el
-            steps.append(st)
-        log(f"[tdvp] step {k + 1}/{args.warmup + args.steps}{' (timed)' if timed else ''} t={el:.3f}s "
-            f"maxD={st['max_bond_dim']} trunc={st['max_truncation_error']:.2e} matvecs={st['matvecs']} "
-            f"log_norm={st['log_norm']:.2e}")
-    timing = ctx.timing_read()
-    ctx.timing(False)
-    ket.close()
+            el = time.perf_counter() - t0
+            if timed:
+                wall += el
+                steps.append(st)
+            log(f"[tdvp {mode}] step {k + 1}/{args.warmup + args.steps}{' (timed)' if timed else ''} t={el:.3f}s "
+                f"maxD={st['max_bond_dim']} trunc={st['max_truncation_error']:.2e} matvecs={st['matvecs']} "
+                f"log_norm={st['log_norm']:.2e}")
+        runs[mode] = (ket, wall / args.steps, steps, ctx.timing_read())
+        ctx.timing(False)
+    fixed, adapt = runs["fixed"], runs["adaptive"]
+    # the two runs' states after the same steps: |<fixed|adaptive>| should be 1
+    fidelity = abs(fixed[0].overlap(adapt[0]))
+    for r in runs.values():
+        r[0].close()
     ctx.close()
+    timing, steps = adapt[3], adapt[2]
     workload = {"workload": f"tdvp_real_time_hubbard_L{args.sites}_D{args.dmax}", "sites": args.sites, "t": 1.0,
                 "U": 4.0, "filling": "half, Sz=0", "operator": "c+_up on site 0", "dt": dt, "krylov_dim": 30,
                 "max_bond_dim": args.dmax, "ground_state_schedule": sched}
+    workload["krylov_stop"] = "adaptive 1e-12 (value); fixed dimension 30 as in the reference (fixed_*)"
     return _timed_record(f"TDVP real-time step time, Hubbard L={args.sites}, c+ psi0, D<={args.dmax}",
-                         wall / args.steps, "s/step", args, workload, timing,
+                         adapt[1], "s/step", args, workload, timing,
                          {"ground_state_energy": e0, "norm2_cdag_psi0": n2,
                           "max_bond_dim": max(s["max_bond_dim"] for s in steps),
                           "max_truncation_error": max(s["max_truncation_error"] for s in steps),
                           "matvecs_per_step": sum(s["matvecs"] for s in steps) / len(steps),
+                          "fixed_seconds_per_step": round(fixed[1], 4),
+                          "fixed_matvecs_per_step": sum(s["matvecs"] for s in fixed[2]) / len(fixed[2]),
+                          "fidelity_fixed_vs_adaptive": fidelity,
                           "cpu_baseline": None})
 
 

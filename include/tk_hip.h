@@ -220,9 +220,11 @@ tk_status tk_lanczos(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, do
  * min(krylov_dim, n) Arnoldi vectors (krylov_dim <= 48), happy breakdown at
  * h_{j+1,j} < tol, the small exponential by the reference's Taylor series with
  * scaling and squaring; |v| < 1e-30 gives zero.  out_dev may alias v_dev.
- * *krylov_used (nullable) receives the Arnoldi vectors used. */
+ * adaptive_tol > 0 (not in the reference; 0 keeps its fixed dimension) stops
+ * after step m once h_{m+1,m} |c_m| <= adaptive_tol |c| for c = exp(alpha H_m)
+ * |v| e_1.  *krylov_used (nullable) receives the Arnoldi vectors used. */
 tk_status tk_exp_krylov(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, const double* v_dev, double alpha,
-                        int32_t krylov_dim, double tol, double* out_dev, int32_t* krylov_used);
+                        int32_t krylov_dim, double tol, double adaptive_tol, double* out_dev, int32_t* krylov_used);
 
 /* ---- two-site effective Hamiltonian ------------------------------------- */
 /* build_heff_two_site (tk-dmrg/src/environments/mod.rs:491-602;
@@ -352,10 +354,11 @@ typedef struct {
  * left to right then right to left: exp(-tau/2 H2) on the two-site tensor
  * (exp_krylov_f64, tdvp/mod.rs:197-272), split truncated to max_bond (<= 0:
  * the last bond-schedule value) with the config's cutoff, and away from the
- * ends exp(+tau/2 H1) on the new centre site.  Needs tk_dmrg_prepare; the
- * centre is back at site 0 afterwards. */
+ * ends exp(+tau/2 H1) on the new centre site.  krylov_adaptive_tol as in
+ * tk_exp_krylov (0: the reference's fixed Krylov dimension).  Needs
+ * tk_dmrg_prepare; the centre is at site 0 afterwards. */
 tk_status tk_dmrg_tdvp_step(tk_dmrg* d, double tau, int64_t max_bond, int32_t krylov_dim, double krylov_tol,
-                            tk_tdvp_stats* out);
+                            double krylov_adaptive_tol, tk_tdvp_stats* out);
 
 /* ---- DMFT impurity pieces (tk-dmft) ---------------------------------------
  * BathParameters::hybridization_function (tk-dmft/src/impurity/bath.rs:61-83),

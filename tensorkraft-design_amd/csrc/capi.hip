@@ -465,12 +465,12 @@ tk_status tk_lanczos(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, do
 }
 
 tk_status tk_exp_krylov(tk_ctx* ctx, int64_t n, tk_matvec_fn matvec, void* user, const double* v, double alpha,
-                        int32_t krylov_dim, double tol, double* out, int32_t* krylov_used) {
+                        int32_t krylov_dim, double tol, double adaptive_tol, double* out, int32_t* krylov_used) {
   return guard(ctx, [&] {
     if (!ctx || !matvec || !v || !out || n <= 0) throw tk::ArgError("bad exp_krylov arguments");
     CallbackUser cb{matvec, user};
     tk::ExpKrylovResult r =
-        tk::exp_krylov_generic(ctx->c.get(), n, callback_matvec, &cb, v, alpha, krylov_dim, tol, out);
+        tk::exp_krylov_generic(ctx->c.get(), n, callback_matvec, &cb, v, alpha, krylov_dim, tol, out, adaptive_tol);
     if (krylov_used) *krylov_used = r.krylov_dim;
   });
 }
@@ -723,11 +723,11 @@ tk_status tk_dmrg_overlap(tk_dmrg* bra, tk_dmrg* ket, double* out) {
 }
 
 tk_status tk_dmrg_tdvp_step(tk_dmrg* d, double tau, int64_t max_bond, int32_t krylov_dim, double krylov_tol,
-                            tk_tdvp_stats* out) {
+                            double krylov_adaptive_tol, tk_tdvp_stats* out) {
   return guard(d ? d->ctx : nullptr, [&] {
     if (!d) throw tk::ArgError("null argument");
     if (krylov_dim <= 0) throw tk::ArgError("tdvp: krylov_dim must be positive");
-    tk::TdvpStats s = d->e->tdvp_step(tau, max_bond, krylov_dim, krylov_tol);
+    tk::TdvpStats s = d->e->tdvp_step(tau, max_bond, krylov_dim, krylov_tol, krylov_adaptive_tol);
     if (out) {
       out->energy = s.energy;
       out->max_truncation_error = s.max_truncation_error;

@@ -283,13 +283,14 @@ DmrgEngine::OneSite DmrgEngine::one_site_plan(int i, Prec prec) {
   return os;
 }
 
-double DmrgEngine::evolve_one_site(int i, double alpha, int krylov_dim, double tol, int* matvecs) {
+double DmrgEngine::evolve_one_site(int i, double alpha, int krylov_dim, double tol, double adaptive_tol,
+                                   int* matvecs) {
   OneSite os = one_site_plan(i, Prec::FP64);
   const BlockTable& psi = os.plan->psi();
   Site& A = site_[i];
   DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total))), y(ctx_, size_t(std::max<int64_t>(1, psi.total)));
   launch_copies(ctx_, os.to_psi, A.d.get(), A.t.total, x.get(), psi.total);
-  const ExpKrylovResult r = exp_krylov(ctx_, *os.plan, x.get(), alpha, krylov_dim, tol, y.get());
+  const ExpKrylovResult r = exp_krylov(ctx_, *os.plan, x.get(), alpha, krylov_dim, tol, y.get(), adaptive_tol);
   *matvecs += r.matvecs;
   if (!(r.norm_out > 0) || !(r.norm_in > 0)) throw ArgError("tdvp: the one-site tensor vanished");
   scale(ctx_, y.get(), 1.0 / r.norm_out, psi.total);
@@ -561,7 +562,8 @@ double mps_overlap(const DmrgEngine& bra, const DmrgEngine& ket) {
   return v;
 }
 
-TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol) {
+TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol,
+                                double adaptive_tol) {
   if (site_.empty() || L_.empty() || L_[0].t.n_blocks() == 0) throw ArgError("tdvp: prepare() first");
   if (!std::isfinite(tau)) throw ArgError("tdvp: time step is not finite");
   centre_to_site0();
@@ -578,7 +580,8 @@ TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, do
     const size_t len = size_t(std::max<int64_t>(1, psi.total));
     DBuf<double> x(ctx_, len), y(ctx_, len);
     contract_two_site(i, psi, x.get());
-    const ExpKrylovResult r = exp_krylov(ctx_, plan, x.get(), -0.5 * tau, krylov_dim, krylov_tol, y.get());
+    const ExpKrylovResult r =
+        exp_krylov(ctx_, plan, x.get(), -0.5 * tau, krylov_dim, krylov_tol, y.get(), adaptive_tol);
     ts.matvecs += r.matvecs;
     if (!(r.norm_out > 0) || !(r.norm_in > 0)) throw ArgError("tdvp: the two-site tensor vanished");
     ts.log_norm += std::log(r.norm_out) - std::log(r.norm_in);
@@ -605,11 +608,11 @@ TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, do
   };
   for (int i = 0; i + 1 < n_; ++i) {
     two_site(i, true, false);
-    if (i + 2 < n_) ts.log_norm += evolve_one_site(i + 1, 0.5 * tau, krylov_dim, krylov_tol, &ts.matvecs);
+    if (i + 2 < n_) ts.log_norm += evolve_one_site(i + 1, 0.5 * tau, krylov_dim, krylov_tol, adaptive_tol, &ts.matvecs);
   }
   for (int i = n_ - 2; i >= 0; --i) {
     two_site(i, false, i == 0);
-    if (i > 0) ts.log_norm += evolve_one_site(i, 0.5 * tau, krylov_dim, krylov_tol, &ts.matvecs);
+    if (i > 0) ts.log_norm += evolve_one_site(i, 0.5 * tau, krylov_dim, krylov_tol, adaptive_tol, &ts.matvecs);
   }
   ctx_->sync();
   energy_ = ts.energy;

@@ -100,9 +100,10 @@ class DmrgEngine {
   // update.  Left-to-right then right-to-left over every bond: the two-site
   // tensor evolves by exp(-tau/2 H2), is split (truncated to max_bond, the
   // sweep's cutoff), and, away from the ends, the new centre site evolves back
-  // by exp(+tau/2 H1).  Krylov exponentials of krylov_dim vectors.  Needs
+  // by exp(+tau/2 H1).  Krylov exponentials of krylov_dim vectors (at most,
+  // with adaptive_tol > 0: the early stop of exp_krylov_generic).  Needs
   // prepare() (centre at site 0) and leaves the centre at site 0.
-  TdvpStats tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol);
+  TdvpStats tdvp_step(double tau, int64_t max_bond, int krylov_dim, double krylov_tol, double adaptive_tol = 0.0);
   // One left-to-right + right-to-left single-site sweep: at every site the
   // lowest eigenpair of H1 = L W R (the sweep's eigensolver, warm-started
   // from the site tensor) replaces the site, and the centre moves on by an
@@ -135,7 +136,7 @@ class DmrgEngine {
                       double lambda_floor, StepStats* st);
   // site i <- exp(alpha H1) site i / |...| with H1 = L[i] W_i R[i+1]; returns
   // ln |exp(alpha H1) site i| - ln |site i| and adds the matvecs to *matvecs
-  double evolve_one_site(int i, double alpha, int krylov_dim, double tol, int* matvecs);
+  double evolve_one_site(int i, double alpha, int krylov_dim, double tol, double adaptive_tol, int* matvecs);
   // H1 = L[i] W_i R[i+1] as a two-site plan with a trivial second site, and
   // the block copies between site i and that plan's vector
   struct OneSite;

@@ -149,7 +149,7 @@ void plan_matvec_fp64(const double* x, double* y, int, void* user) {
 }  // namespace
 
 ExpKrylovResult exp_krylov_generic(Context* ctx, int64_t n, MatvecFn mv, void* user, const double* v, double alpha,
-                                   int krylov_dim, double tol, double* out) {
+                                   int krylov_dim, double tol, double* out, double adaptive_tol) {
   ExpKrylovResult res;
   if (n < 0 || krylov_dim < 0) throw ArgError("exp_krylov: negative size");
   if (krylov_dim > kExpKrylovMax) throw ArgError("exp_krylov: krylov_dim above 48");
@@ -181,6 +181,8 @@ ExpKrylovResult exp_krylov_generic(Context* ctx, int64_t n, MatvecFn mv, void* u
   TK_HIP(hipMemcpyAsync(V, v, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
   scale(ctx, V, 1.0 / beta, n);
   int used = m;
+  const bool adaptive = adaptive_tol > 0;
+  double c[kExpKrylovMax];  // host copy of the small exponential's result
   for (int j = 0; j < m; ++j) {
     const double* vj = V + int64_t(j) * n;
     mv(vj, w, 1, user);
@@ -194,6 +196,10 @@ ExpKrylovResult exp_krylov_generic(Context* ctx, int64_t n, MatvecFn mv, void* u
     subtract_projection(ctx, V, n, j + 1, hb.get(), w, n);
     dot_many(ctx, w, n, 1, w, n, nrm2);
     hipLaunchKernelGGL(sqrt_store_kernel, dim3(1), dim3(64), 0, s, nrm2, H + int64_t(j + 1) * m + j);
+    if (adaptive) {
+      hipLaunchKernelGGL(krylov_expm_kernel, dim3(1), dim3(kExpThreads), 0, s, H, m, j + 1, alpha, beta, cb.get());
+      TK_HIP(hipMemcpyAsync(c, cb.get(), size_t(j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
     TK_HIP(hipMemcpyAsync(&host, nrm2, sizeof(double), hipMemcpyDeviceToHost, s));
     TK_HIP(hipStreamSynchronize(s));
     TK_HIP(hipGetLastError());
@@ -203,15 +209,24 @@ ExpKrylovResult exp_krylov_generic(Context* ctx, int64_t n, MatvecFn mv, void* u
       used = j + 1;
       break;
     }
+    if (adaptive) {
+      double cn = 0;
+      for (int i = 0; i <= j; ++i) cn += c[i] * c[i];
+      if (hn * std::fabs(c[j]) <= adaptive_tol * std::sqrt(cn)) {
+        used = j + 1;
+        break;
+      }
+    }
     if (j + 1 < m) {
       double* vn = V + int64_t(j + 1) * n;
       TK_HIP(hipMemcpyAsync(vn, w, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice, s));
       scale_by_inv_sqrt(ctx, vn, nrm2, n);
     }
   }
-  hipLaunchKernelGGL(krylov_expm_kernel, dim3(1), dim3(kExpThreads), 0, s, H, m, used, alpha, beta, cb.get());
+  // with the early stop on, cb already holds exp(alpha H_used) beta e_1
+  if (!adaptive)
+    hipLaunchKernelGGL(krylov_expm_kernel, dim3(1), dim3(kExpThreads), 0, s, H, m, used, alpha, beta, cb.get());
   combine(ctx, V, n, used, cb.get(), 0.0, out, n);
-  double c[kExpKrylovMax];
   TK_HIP(hipMemcpyAsync(c, cb.get(), size_t(used) * sizeof(double), hipMemcpyDeviceToHost, s));
   TK_HIP(hipStreamSynchronize(s));
   TK_HIP(hipGetLastError());
@@ -224,9 +239,9 @@ ExpKrylovResult exp_krylov_generic(Context* ctx, int64_t n, MatvecFn mv, void* u
 }
 
 ExpKrylovResult exp_krylov(Context* ctx, HeffPlan& plan, const double* v, double alpha, int krylov_dim, double tol,
-                           double* out) {
+                           double* out, double adaptive_tol) {
   PlanUser u{&plan};
-  return exp_krylov_generic(ctx, plan.psi().total, plan_matvec_fp64, &u, v, alpha, krylov_dim, tol, out);
+  return exp_krylov_generic(ctx, plan.psi().total, plan_matvec_fp64, &u, v, alpha, krylov_dim, tol, out, adaptive_tol);
 }
 
 }  // namespace tk

@@ -29,10 +29,14 @@ struct ExpKrylovResult {
 };
 
 // out = exp(alpha A) v; v and out are device vectors of length n (may alias).
+// adaptive_tol > 0 adds an early stop the reference does not have: after
+// Arnoldi step m the small exponential c = exp(alpha H_m) |v| e_1 is formed
+// and the iteration ends once h_{m+1,m} |c_m| <= adaptive_tol |c| (the size
+// of the next Krylov direction's contribution); krylov_dim is then a maximum.
 ExpKrylovResult exp_krylov_generic(Context* ctx, int64_t n, MatvecFn mv, void* user, const double* v, double alpha,
-                                   int krylov_dim, double tol, double* out);
+                                   int krylov_dim, double tol, double* out, double adaptive_tol = 0.0);
 // Over an H_eff plan, with FP64 matvecs.
 ExpKrylovResult exp_krylov(Context* ctx, HeffPlan& plan, const double* v, double alpha, int krylov_dim, double tol,
-                           double* out);
+                           double* out, double adaptive_tol = 0.0);
 
 }  // namespace tk

@@ -133,8 +133,8 @@ def lib():
         "tk_eigh": (I32, [P, I64, P, pD, P]),
         "tk_davidson": (I32, [P, I64, MATVEC_FN, P, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_lanczos": (I32, [P, I64, MATVEC_FN, P, D, I32, I32, P, pD, pD, pI32]),
-        "tk_exp_krylov": (I32, [P, I64, MATVEC_FN, P, P, D, I32, D, P, pI32]),
-        "tk_dmrg_tdvp_step": (I32, [P, D, I64, I32, D, C.POINTER(tk_tdvp_stats)]),
+        "tk_exp_krylov": (I32, [P, I64, MATVEC_FN, P, P, D, I32, D, D, P, pI32]),
+        "tk_dmrg_tdvp_step": (I32, [P, D, I64, I32, D, D, C.POINTER(tk_tdvp_stats)]),
         "tk_dmrg_sweep_single_site": (I32, [P, C.POINTER(tk_sweep_stats)]),
         "tk_dmrg_apply_site_op": (I32, [P, pD, pI32, pD]),
         "tk_dmrg_overlap": (I32, [P, P, pD]),
@@ -498,15 +498,17 @@ class HipBackend:
             dd.free()
         return ev.value, x, rn.value, nmv.value
 
-    def exp_krylov(self, matvec_dev, n, v, alpha, krylov_dim=30, tol=1e-12):
-        """exp(alpha A) v over a device matvec (exp_krylov_f64, tdvp/mod.rs:197-272);
-        returns (result, Krylov vectors used)."""
+    def exp_krylov(self, matvec_dev, n, v, alpha, krylov_dim=30, tol=1e-12, adaptive_tol=0.0):
+        """exp(alpha A) v over a device matvec (exp_krylov_f64, tdvp/mod.rs:197-272;
+        adaptive_tol > 0: early stop, not in the reference); returns (result,
+        Krylov vectors used)."""
         cb = MATVEC_FN(lambda x, y, u: matvec_dev(x, y))
         dv = self.ctx.to_device(np.asarray(v, dtype=np.float64))
         do = self.ctx.empty(n)
         used = C.c_int32(0)
         _check(self.ctx.L.tk_exp_krylov(self.ctx.h, n, cb, None, C.c_void_p(dv.ptr), float(alpha), int(krylov_dim),
-                                        float(tol), C.c_void_p(do.ptr), C.byref(used)), self.ctx.h)
+                                        float(tol), float(adaptive_tol), C.c_void_p(do.ptr), C.byref(used)),
+               self.ctx.h)
         out = do.get()
         dv.free()
         do.free()
@@ -702,7 +704,7 @@ def complex_state(bonds, sites, part="R"):
 
 
 def greens_function_real_time(ctx, mpo, sym, bonds, sites, e0, dt, n_steps, max_bond=256, krylov_dim=30,
-                              krylov_tol=1e-12, svd_cutoff=1e-12, start=None):
+                              krylov_tol=1e-12, svd_cutoff=1e-12, start=None, adaptive_tol=0.0):
     """G(t_k) = -i exp(i e0 t_k) <phi| exp(-i H t_k) |phi>, t_k = k dt, k = 0..n_steps,
     for the real normalised state phi = (bonds, sites) on the sites of `mpo`
     (for example O psi_0 from Dmrg.apply_site_op, then multiply by its norm^2).
@@ -722,7 +724,7 @@ def greens_function_real_time(ctx, mpo, sym, bonds, sites, e0, dt, n_steps, max_
     ts, g, acc = [], [], 0.0
     for k in range(n_steps + 1):
         if k > 0:
-            acc += ket.tdvp_step(-dt, max_bond, krylov_dim, krylov_tol)["log_norm"]
+            acc += ket.tdvp_step(-dt, max_bond, krylov_dim, krylov_tol, adaptive_tol)["log_norm"]
         t = k * dt
         amp = (bra_r.overlap(ket) + 1j * bra_i.overlap(ket)) * math.exp(acc)
         ts.append(t)
@@ -1381,24 +1383,25 @@ class Dmrg:
         _check(self.ctx.L.tk_dmrg_overlap(self.h, ket.h, C.byref(v)), self.ctx.h)
         return v.value
 
-    def greens_function_tau(self, e0, dtau, n_steps, max_bond=0, krylov_dim=30, krylov_tol=1e-12):
+    def greens_function_tau(self, e0, dtau, n_steps, max_bond=0, krylov_dim=30, krylov_tol=1e-12, adaptive_tol=0.0):
         """Imaginary-time G(tau_k) = -<phi| exp(-tau_k (H - e0)) |phi>, tau_k = 2 k dtau,
         k = 0..n_steps, of the current state phi (for example O psi_0 after
         apply_site_op; multiply by its norm^2): each TDVP step of dtau evolves phi to
         exp(-k dtau H) phi, and G(2 k dtau) = -exp(2 sum log_norm + 2 k dtau e0)."""
         taus, g, acc = [0.0], [-1.0], 0.0
         for k in range(1, n_steps + 1):
-            acc += self.tdvp_step(dtau, max_bond, krylov_dim, krylov_tol)["log_norm"]
+            acc += self.tdvp_step(dtau, max_bond, krylov_dim, krylov_tol, adaptive_tol)["log_norm"]
             taus.append(2.0 * k * dtau)
             g.append(-math.exp(2.0 * acc + 2.0 * k * dtau * e0))
         return np.array(taus), np.array(g)
 
-    def tdvp_step(self, tau, max_bond=0, krylov_dim=30, krylov_tol=1e-12):
+    def tdvp_step(self, tau, max_bond=0, krylov_dim=30, krylov_tol=1e-12, adaptive_tol=0.0):
         """One second-order two-site TDVP step psi <- exp(-tau H) psi, renormalised
-        (TdvpDriver::step, tdvp/mod.rs:91-141)."""
+        (TdvpDriver::step, tdvp/mod.rs:91-141); adaptive_tol > 0 lets each Krylov
+        exponential stop early (krylov_dim becomes a maximum)."""
         s = tk_tdvp_stats()
         _check(self.ctx.L.tk_dmrg_tdvp_step(self.h, float(tau), int(max_bond), int(krylov_dim), float(krylov_tol),
-                                            C.byref(s)), self.ctx.h)
+                                            float(adaptive_tol), C.byref(s)), self.ctx.h)
         return {f: getattr(s, f) for f, _ in tk_tdvp_stats._fields_ if f != "pad"}
 
     def steps(self):

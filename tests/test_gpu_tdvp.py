@@ -280,3 +280,44 @@ def test_gpu_greens_function_real_time_matches_ed(ctx):
     ref = np.array([-1j * np.exp(1j * w[0] * tt) * (phi @ scipy.linalg.expm(-1j * tt * h) @ phi) for tt in t])
     assert abs(g[0] + 1j) < 3e-6
     assert np.max(np.abs(g - ref)) < 3e-6
+
+
+def test_exp_krylov_adaptive_stop(be, ctx):
+    """The early stop (not in the reference): fewer Arnoldi vectors, same result."""
+    print("case exp_krylov adaptive", flush=True)
+    rng = np.random.default_rng(64)
+    g = rng.standard_normal((64, 64))
+    a = (g + g.T) / 2
+    v = rng.standard_normal(64)
+    mv, keep = _dense_matvec(be, ctx, a)
+    ref = scipy.linalg.expm(-0.05 * a) @ v
+    r, m = be.exp_krylov(mv, 64, v, -0.05, 40, 1e-14, adaptive_tol=1e-13)
+    assert m < 40
+    assert np.linalg.norm(r - ref) <= 1e-11 * np.linalg.norm(ref)
+    r0, m0 = be.exp_krylov(mv, 64, v, -0.05, 40, 1e-14)
+    assert m0 == 40 and np.linalg.norm(r0 - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_gpu_tdvp_adaptive_krylov_is_exact(ctx):
+    print("case tdvp adaptive krylov heisenberg n=6", flush=True)
+    mpo = tk_hip.chain_mpo("heisenberg", 6)
+    sym = tk_hip.SYM_U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (0, 0), 256, seed=7)
+    psi0 = dense_mps(sym, mpo, bonds, data)
+    psi0 /= np.linalg.norm(psi0)
+    h, _ = dense_mpo(mpo)
+    out = {}
+    for tol in (0.0, 1e-13):
+        d = tk_hip.Dmrg(ctx, mpo, sym, schedule=(256,), svd_cutoff=1e-14)
+        d.set_state(bonds, data)
+        d.prepare()
+        mv = 0
+        for _ in range(4):
+            mv += d.tdvp_step(0.1, 256, 40, 1e-14, adaptive_tol=tol)["matvecs"]
+        psi = dense_mps(sym, mpo, [d.bond(i) for i in range(7)], [d.site(i) for i in range(6)])
+        out[tol] = (psi, mv)
+    target = scipy.linalg.expm(-0.4 * h) @ psi0
+    target /= np.linalg.norm(target)
+    for tol, (psi, mv) in out.items():
+        assert np.max(np.abs(np.sign(psi @ target) * psi - target)) < 1e-9
+    assert out[1e-13][1] < out[0.0][1]
