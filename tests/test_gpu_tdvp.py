@@ -321,3 +321,27 @@ def test_gpu_tdvp_adaptive_krylov_is_exact(ctx):
     for tol, (psi, mv) in out.items():
         assert np.max(np.abs(np.sign(psi @ target) * psi - target)) < 1e-9
     assert out[1e-13][1] < out[0.0][1]
+
+
+def test_gpu_tdvp_error_paths(ctx):
+    """Misuse is reported through the C ABI's status, not by a crash."""
+    print("case tdvp error paths", flush=True)
+    mpo = tk_hip.chain_mpo("heisenberg", 4)
+    sym = tk_hip.SYM_U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (0, 0), 8, seed=1)
+    d = tk_hip.Dmrg(ctx, mpo, sym)
+    d.set_state(bonds, data)
+    with pytest.raises(tk_hip.TkError, match="prepare"):
+        d.tdvp_step(0.1)
+    d.prepare()
+    with pytest.raises(tk_hip.TkError, match="charge"):
+        d.apply_site_op(tk_hip.SITE_TYPES["spin-1/2"]().ops["S+"], (0, 0))  # S+ raises 2Sz by 2, not 0
+    with pytest.raises(tk_hip.TkError, match="krylov_dim"):
+        d.tdvp_step(0.1, krylov_dim=0)
+    other = tk_hip.Dmrg(ctx, tk_hip.chain_mpo("heisenberg", 6), sym)
+    b6, d6 = tk_hip.initial_mps(sym, [m.phys for m in tk_hip.chain_mpo("heisenberg", 6)], (0, 0), 8, seed=1)
+    other.set_state(b6, d6)
+    with pytest.raises(tk_hip.TkError, match="length"):
+        d.overlap(other)
+    st = d.tdvp_step(0.1)  # the engine still works after the rejected calls
+    assert np.isfinite(st["energy"])
