@@ -485,6 +485,8 @@ def bench_tdvp(args):
                           "fixed_seconds_per_step": round(fixed[1], 4),
                           "fixed_matvecs_per_step": sum(s["matvecs"] for s in fixed[2]) / len(fixed[2]),
                           "fidelity_fixed_vs_adaptive": fidelity,
+                          "phases": {k: round(sum(s[k] for s in steps) / len(steps), 4)
+                                     for k in ("t_plan", "t_exp", "t_svd", "t_env")},
                           "cpu_baseline": None})
 
 

@@ -345,6 +345,7 @@ typedef struct {
   double max_truncation_error;
   double seconds;
   double log_norm;              /* ln of the norm factor exp(-tau H) applied (before renormalisation) */
+  double t_plan, t_exp, t_svd, t_env;  /* phase wall times: plans, Krylov exponentials, splits, environments */
   int64_t max_bond_dim;
   int32_t matvecs, pad;
 } tk_tdvp_stats;

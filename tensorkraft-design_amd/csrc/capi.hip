@@ -733,6 +733,10 @@ tk_status tk_dmrg_tdvp_step(tk_dmrg* d, double tau, int64_t max_bond, int32_t kr
       out->max_truncation_error = s.max_truncation_error;
       out->seconds = s.seconds;
       out->log_norm = s.log_norm;
+      out->t_plan = s.t_plan;
+      out->t_exp = s.t_exp;
+      out->t_svd = s.t_svd;
+      out->t_env = s.t_env;
       out->max_bond_dim = s.max_bond_dim;
       out->matvecs = s.matvecs;
       out->pad = 0;

@@ -284,17 +284,21 @@ DmrgEngine::OneSite DmrgEngine::one_site_plan(int i, Prec prec) {
 }
 
 double DmrgEngine::evolve_one_site(int i, double alpha, int krylov_dim, double tol, double adaptive_tol,
-                                   int* matvecs) {
+                                   TdvpStats* ts) {
+  const double t0 = now();
   OneSite os = one_site_plan(i, Prec::FP64);
+  const double t1 = now();
   const BlockTable& psi = os.plan->psi();
   Site& A = site_[i];
   DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total))), y(ctx_, size_t(std::max<int64_t>(1, psi.total)));
   launch_copies(ctx_, os.to_psi, A.d.get(), A.t.total, x.get(), psi.total);
   const ExpKrylovResult r = exp_krylov(ctx_, *os.plan, x.get(), alpha, krylov_dim, tol, y.get(), adaptive_tol);
-  *matvecs += r.matvecs;
+  ts->matvecs += r.matvecs;
   if (!(r.norm_out > 0) || !(r.norm_in > 0)) throw ArgError("tdvp: the one-site tensor vanished");
   scale(ctx_, y.get(), 1.0 / r.norm_out, psi.total);
   launch_copies(ctx_, os.to_site, y.get(), psi.total, A.d.get(), A.t.total);
+  ts->t_plan += t1 - t0;
+  ts->t_exp += now() - t1;
   return std::log(r.norm_out) - std::log(r.norm_in);
 }
 
@@ -574,8 +578,10 @@ TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, do
   // two-site update of bond (i, i+1): evolve, split, grow the environment
   // behind the centre; the last bond of the step also gives the energy
   auto two_site = [&](int i, bool lr, bool energy) {
+    const double t0 = now();
     HeffPlan plan(ctx_, Prec::FP64, L_[i].t, L_[i].d.get(), R_[i + 2].t, R_[i + 2].d.get(), cfg_.mpo[i],
                   cfg_.mpo[i + 1]);
+    const double t1 = now();
     const BlockTable& psi = plan.psi();
     const size_t len = size_t(std::max<int64_t>(1, psi.total));
     DBuf<double> x(ctx_, len), y(ctx_, len);
@@ -596,23 +602,29 @@ TdvpStats DmrgEngine::tdvp_step(double tau, int64_t max_bond, int krylov_dim, do
       ctx_->sync();
       ts.energy = h[0] / h[1];
     }
+    const double t2 = now();
     StepStats st;
     split_two_site(i, lr, psi, y.get(), max_bond, cfg_.svd_cutoff, floor, &st);
     ts.max_truncation_error = std::max(ts.max_truncation_error, st.truncation_error);
     ts.max_bond_dim = std::max(ts.max_bond_dim, st.bond_dim);
+    const double t3 = now();
     if (lr)
       grow_left(i);
     else
       grow_right(i + 1);
     ctx_->sync();
+    ts.t_plan += t1 - t0;
+    ts.t_exp += t2 - t1;
+    ts.t_svd += t3 - t2;
+    ts.t_env += now() - t3;
   };
   for (int i = 0; i + 1 < n_; ++i) {
     two_site(i, true, false);
-    if (i + 2 < n_) ts.log_norm += evolve_one_site(i + 1, 0.5 * tau, krylov_dim, krylov_tol, adaptive_tol, &ts.matvecs);
+    if (i + 2 < n_) ts.log_norm += evolve_one_site(i + 1, 0.5 * tau, krylov_dim, krylov_tol, adaptive_tol, &ts);
   }
   for (int i = n_ - 2; i >= 0; --i) {
     two_site(i, false, i == 0);
-    if (i > 0) ts.log_norm += evolve_one_site(i, 0.5 * tau, krylov_dim, krylov_tol, adaptive_tol, &ts.matvecs);
+    if (i > 0) ts.log_norm += evolve_one_site(i, 0.5 * tau, krylov_dim, krylov_tol, adaptive_tol, &ts);
   }
   ctx_->sync();
   energy_ = ts.energy;

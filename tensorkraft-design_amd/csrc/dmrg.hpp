@@ -64,6 +64,10 @@ struct TdvpStats {
   double seconds = 0;
   double log_norm = 0;  // ln of the norm factor the step applied (before renormalisation)
   int matvecs = 0;
+  // host wall time of the phases (stream-synchronised): plan construction,
+  // Krylov exponentials (with the two-site contraction and block copies),
+  // splits, environment updates
+  double t_plan = 0, t_exp = 0, t_svd = 0, t_env = 0;
 };
 
 struct SweepStats {
@@ -135,8 +139,8 @@ class DmrgEngine {
   void split_two_site(int i, bool lr, const BlockTable& psi, const double* x, int64_t max_bond, double cutoff,
                       double lambda_floor, StepStats* st);
   // site i <- exp(alpha H1) site i / |...| with H1 = L[i] W_i R[i+1]; returns
-  // ln |exp(alpha H1) site i| - ln |site i| and adds the matvecs to *matvecs
-  double evolve_one_site(int i, double alpha, int krylov_dim, double tol, double adaptive_tol, int* matvecs);
+  // ln |exp(alpha H1) site i| - ln |site i| and adds matvecs and phase times to *ts
+  double evolve_one_site(int i, double alpha, int krylov_dim, double tol, double adaptive_tol, TdvpStats* ts);
   // H1 = L[i] W_i R[i+1] as a two-site plan with a trivial second site, and
   // the block copies between site i and that plan's vector
   struct OneSite;

@@ -70,7 +70,8 @@ class tk_kernel_timing(C.Structure):
 
 class tk_tdvp_stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("max_truncation_error", C.c_double), ("seconds", C.c_double),
-                ("log_norm", C.c_double), ("max_bond_dim", C.c_int64), ("matvecs", C.c_int32), ("pad", C.c_int32)]
+                ("log_norm", C.c_double), ("t_plan", C.c_double), ("t_exp", C.c_double), ("t_svd", C.c_double),
+                ("t_env", C.c_double), ("max_bond_dim", C.c_int64), ("matvecs", C.c_int32), ("pad", C.c_int32)]
 
 
 class tk_sweep_stats(C.Structure):
