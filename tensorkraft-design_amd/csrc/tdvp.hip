@@ -11,6 +11,7 @@ namespace {
 
 constexpr int kExpThreads = 256;
 constexpr int kExpPer = (kExpKrylovMax * kExpKrylovMax + kExpThreads - 1) / kExpThreads;
+static_assert(kExpKrylovMax <= 64, "the vector series keeps one row per lane of one wavefront");
 
 // Deterministic sum over the workgroup (tree in LDS); every thread gets it.
 __device__ double block_sum(double v, double* red) {
@@ -52,24 +53,27 @@ __global__ __launch_bounds__(kExpThreads) void krylov_expm_kernel(const double* 
   for (int idx = tid; idx < mm; idx += kExpThreads) a[idx] *= scale;
   __syncthreads();
   if (s == 0) {
-    // vector Taylor series: result and term in e[0..m), t[0..m)
-    for (int i = tid; i < m; i += kExpThreads) e[i] = t[i] = i == 0 ? beta : 0.0;
-    __syncthreads();
+    // vector Taylor series in wavefront 0, lane i holding row i (m <= 64):
+    // the term's entries are broadcast by shuffles, its norm by a butterfly,
+    // so the loop needs no workgroup barrier
+    if (tid >= 64) return;
+    double ti = tid == 0 ? beta : 0.0;  // current term
+    double ei = ti;                     // partial sum
     for (int k = 1; k <= 20; ++k) {
       double nt = 0;
-      if (tid < m) {
-        for (int j = 0; j < m; ++j) nt += a[tid * m + j] * t[j];
-        nt /= k;
+      for (int j = 0; j < m; ++j) {
+        const double tj = __shfl(ti, j, 64);
+        nt += (tid < m ? a[tid * m + j] : 0.0) * tj;
       }
-      __syncthreads();
-      if (tid < m) {
-        t[tid] = nt;
-        e[tid] += nt;
-      }
-      const double tn = sqrt(block_sum(tid < m ? nt * nt : 0.0, red));
-      if (tn < 1e-16 * fabs(beta)) break;
+      nt = tid < m ? nt / k : 0.0;
+      ti = nt;
+      ei += nt;
+      double sq = nt * nt;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o, 64);
+      if (sqrt(sq) < 1e-16 * fabs(beta)) break;
     }
-    for (int i = tid; i < m; i += kExpThreads) c[i] = e[i];
+    if (tid < m) c[tid] = ei;
     return;
   }
   // matrix Taylor series of a 2^-s, then s squarings
