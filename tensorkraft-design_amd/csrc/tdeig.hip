@@ -76,12 +76,23 @@ __device__ double block_sum(double x, double* red, int& parity) {
 }
 
 // Number of eigenvalues of the tridiagonal (d, e^2) smaller than x.
+// a / q through the hardware reciprocal and one Newton step (the Sturm
+// recurrence keeps |q| >= pivmin, so it stays finite): a few ulps from the
+// rounded quotient, a perturbation of the same order as the rounding of T, at a
+// fraction of the IEEE division's dependent latency on the recurrence's
+// critical path.
+__device__ __forceinline__ double fast_quot(double e2, double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return e2 * r;
+}
+
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
   double q = d[0] - x;
   if (fabs(q) <= pivmin) q = -pivmin;
   int cnt = q < 0.0;
   for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
+    q = (d[i] - x) - fast_quot(e2[i - 1], q);
     if (fabs(q) <= pivmin) q = -pivmin;
     cnt += q < 0.0;
   }
