@@ -364,8 +364,9 @@ __global__ __launch_bounds__(NT) void tridiag_kernel(const TailJob* __restrict__
 // 1a'. The same trailing-block tridiagonalisation with the block in registers
 // (m <= MT = 64 CG): wavefront wv holds rows wv + NW r (r < RPW) and lane l
 // columns l + 64 g (g < CG) of the full symmetric block.  Per column step the
-// owner of row k publishes it (pending update applied) to LDS; every thread
-// forms the reflector; each wavefront applies the pending rank-2 update to
+// owner of row k publishes it (pending update applied) to LDS together with
+// the squared norm beyond the subdiagonal; every thread forms the reflector
+// entries it needs from that row; each wavefront applies the pending rank-2 update to
 // its rows and forms their parts of A22 v, reduced across the lanes through
 // LDS (RPW partials per lane, summed by groups of lanes, no per-row
 // reduction chains).  Reflector k goes to row k, columns k+2.., of A.
@@ -390,8 +391,7 @@ __global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restri
   extern __shared__ double sm[];
   double* v = sm;             // pending reflector (v[k] = 1 at its step)
   double* w = v + MT;         // pending w
-  double* vn = w + MT;        // this step's reflector
-  double* p = vn + MT;        // A22 vn
+  double* p = w + MT;         // A22 vn (vn: this step's reflector, formed from rowk)
   double* rowk = p + MT;      // row k with the pending update
   double* red = rowk + MT;    // 2 NW
   double* fin = red + 2 * NW; // the last 2 x 2 block
@@ -447,25 +447,32 @@ __global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restri
   }
   __syncthreads();
   for (int k = 0; k + 2 < m; ++k) {
-    // (a) the owner publishes row k with the pending update
+    // (a) the owner publishes row k with the pending update, and the squared
+    // norm of its part beyond the subdiagonal (fin[4])
     if (wv == k % NW) {
       const int r = k / NW;
       const double vk = v[k], wk = w[k];
+      double ss = 0;
 #pragma unroll
       for (int rr = 0; rr < RPW; ++rr)
         if (rr == r) {
 #pragma unroll
           for (int g = 0; g < CG; ++g) {
             const int j = lane + 64 * g;
-            if (j >= k && j < m) rowk[j] = x[rr][g] - (vk * w[j] + wk * v[j]);
+            if (j >= k && j < m) {
+              const double xk = x[rr][g] - (vk * w[j] + wk * v[j]);
+              rowk[j] = xk;
+              if (j >= k + 2) ss += xk * xk;
+            }
           }
         }
+      ss = wave_sum(ss);
+      if (lane == 0) fin[4] = ss;
     }
     __syncthreads();
-    // (b) reflector of row k
-    double ss = 0;
-    for (int j = k + 2 + tid; j < m; j += NT) ss += rowk[j] * rowk[j];
-    const double sigma = block_sum<NT>(ss, red, parity);
+    // (b) reflector of row k; every thread forms the entries it needs from
+    // rowk, so no barrier separates this from (c)
+    const double sigma = fin[4];
     const double alpha = rowk[k + 1];
     double tk = 0, beta = alpha, scale = 0;
     if (sigma > 0) {
@@ -474,17 +481,14 @@ __global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restri
       tk = (beta - alpha) / beta;
       scale = 1.0 / (alpha - beta);
     }
-    for (int j = k + 1 + tid; j < m; j += NT) {
-      const double vj = j == k + 1 ? 1.0 : rowk[j] * scale;
-      vn[j] = vj;
-      if (j >= k + 2) ag[int64_t(k) * ld + j] = vj;
-    }
+    for (int j = k + 2 + tid; j < m; j += NT) ag[int64_t(k) * ld + j] = rowk[j] * scale;
     if (tid == 0) {
       d[k] = rowk[k];
       e[k] = beta;
       tau[k] = tk;
     }
-    __syncthreads();
+    // this step's reflector entry j (j > k)
+    auto refl = [&](int j) { return j == k + 1 ? 1.0 : rowk[j] * scale; };
     // (c) rows k+1..m-1: pending update, then the lane partials of A22 vn
     double vj[CG], wj[CG], vnj[CG];
 #pragma unroll
@@ -493,7 +497,7 @@ __global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restri
       const bool in = j > k && j < m;
       vj[g] = in ? v[j] : 0.0;
       wj[g] = in ? w[j] : 0.0;
-      vnj[g] = in ? vn[j] : 0.0;
+      vnj[g] = in ? refl(j) : 0.0;
     }
     double* pw = part + (wv * 64 + lane) * (RPW + 1);
 #pragma unroll
@@ -529,12 +533,13 @@ __global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restri
     __syncthreads();
     // (d) w = tau p - tau^2/2 (p.vn) vn; v <- vn
     double pv = 0;
-    for (int j = k + 1 + tid; j < m; j += NT) pv += p[j] * vn[j];
+    for (int j = k + 1 + tid; j < m; j += NT) pv += p[j] * refl(j);
     pv = block_sum<NT>(pv, red, parity);
     const double alw = -0.5 * tk * tk * pv;
     for (int j = k + 1 + tid; j < m; j += NT) {
-      w[j] = tk * p[j] + alw * vn[j];
-      v[j] = vn[j];
+      const double vnj2 = refl(j);
+      w[j] = tk * p[j] + alw * vnj2;
+      v[j] = vnj2;
     }
     if (tid == 0) {
       v[k] = 0;
@@ -1650,7 +1655,7 @@ constexpr int kRegTailNT = 512, kRegTail = 192;
 void launch_tridiag_reg(Context* ctx, const TailJob* jobs, int n_jobs, double* A, double* D, double* E, double* TAU,
                         const double* VP, int64_t w_n) {
   constexpr int NW = kRegTailNT / 64, RPW = kRegTail / NW;
-  const size_t lds = (5 * size_t(kRegTail) + 2 * NW + 8 + size_t(NW) * 64 * (RPW + 1)) * sizeof(double);
+  const size_t lds = (4 * size_t(kRegTail) + 2 * NW + 8 + size_t(NW) * 64 * (RPW + 1)) * sizeof(double);
   allow_lds(tridiag_reg_kernel<kRegTailNT, kRegTail>, lds);
   hipLaunchKernelGGL((tridiag_reg_kernel<kRegTailNT, kRegTail>), dim3(n_jobs), dim3(kRegTailNT), lds, ctx->stream,
                      jobs, A, D, E, TAU, VP, w_n);
