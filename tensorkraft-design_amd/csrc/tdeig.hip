@@ -528,13 +528,18 @@ __global__ __launch_bounds__(NT) void tridiag_reg_kernel(const TailJob* __restri
 #pragma unroll
       for (int o = 1; o < LPR; o <<= 1) s2 += __shfl_xor(s2, o, 64);
       const int i = wv + NW * r;
-      if (r < RPW && q == 0 && i > k && i < m) p[i] = s2;
+      const bool own = r < RPW && q == 0 && i > k && i < m;
+      if (own) p[i] = s2;
+      // this wavefront's rows' share of p.vn, summed over the wavefronts below
+      const double pvw = wave_sum(own ? s2 * refl(i) : 0.0);
+      if (lane == 0) red[parity * NW + wv] = pvw;
     }
     __syncthreads();
     // (d) w = tau p - tau^2/2 (p.vn) vn; v <- vn
     double pv = 0;
-    for (int j = k + 1 + tid; j < m; j += NT) pv += p[j] * refl(j);
-    pv = block_sum<NT>(pv, red, parity);
+#pragma unroll
+    for (int q = 0; q < NW; ++q) pv += red[parity * NW + q];
+    parity ^= 1;
     const double alw = -0.5 * tk * tk * pv;
     for (int j = k + 1 + tid; j < m; j += NT) {
       const double vnj2 = refl(j);
