@@ -1908,42 +1908,52 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   for (int q = tid; q < nb * nb; q += 256) L[(q / nb) * LD + q % nb] = G[off + q];
   if (tid == 0) good = test ? 1 : ok[blockIdx.x];
   __syncthreads();
-  for (int a = 0; a < nb && good; ++a) {
-    if (tid == 0) {
-      const double dd = L[a * LD + a];
-      if (!(dd > 0) || (test && sqrt(dd) <= 1e-3 * norms[jb.w_off + c0 + a]))
-        good = 0;
-      else
-        L[a * LD + a] = sqrt(dd);
+  // Right-looking Cholesky with one barrier per column: every thread reads
+  // the pivot (the same value, so the same decision); step a updates the
+  // rows below a with row a unscaled (u_b u_c / d_aa) and scales row a - 1,
+  // which no thread reads any more, by the previous pivot's square root.
+  bool gl = good != 0;
+  double rprev = 1.0;
+  for (int a = 0; a < nb && gl; ++a) {
+    const double dd = L[a * LD + a];
+    if (!(dd > 0) || (test && sqrt(dd) <= 1e-3 * norms[jb.w_off + c0 + a])) {
+      gl = false;
+      break;
     }
-    __syncthreads();
-    if (!good) break;
-    const double r = L[a * LD + a];
-    for (int b = a + 1 + tid; b < nb; b += 256) L[a * LD + b] /= r;
-    __syncthreads();
+    if (a > 0) {
+      const double ir = 1.0 / rprev;
+      for (int b = a + tid; b < nb; b += 256) L[(a - 1) * LD + b] *= ir;
+      if (tid == 0) L[(a - 1) * LD + (a - 1)] = rprev;
+    }
+    rprev = sqrt(dd);
+    const double idd = 1.0 / dd;
     const int rem = nb - a - 1;
     for (int q = tid; q < rem * rem; q += 256) {
       const int b = a + 1 + q / rem, c = a + 1 + q % rem;
-      if (c >= b) L[b * LD + c] -= L[a * LD + b] * L[a * LD + c];
+      if (c >= b) L[b * LD + c] -= (L[a * LD + b] * idd) * L[a * LD + c];
     }
     __syncthreads();
   }
+  if (gl && nb > 0 && tid == 0) L[(nb - 1) * LD + (nb - 1)] = rprev;  // the last row is its pivot alone
   __syncthreads();
-  const bool g = good != 0;
-  // column j of R^-1 by thread j: X_jj = 1 / R_jj, X_ij = -(sum_{l=i+1..j} R_il X_lj) / R_ii
-  if (g && tid < nb) {
-    const int j = tid;
-    X[j * LD + j] = 1.0 / L[j * LD + j];
-    for (int i = j - 1; i >= 0; --i) {
-      const double* li = L + i * LD;
-      double t0 = 0, t1 = 0;
-      int l = i + 1;
-      for (; l + 1 <= j; l += 2) {
-        t0 += li[l] * X[l * LD + j];
-        t1 += li[l + 1] * X[(l + 1) * LD + j];
-      }
-      if (l <= j) t0 += li[l] * X[l * LD + j];
-      X[i * LD + j] = -(t0 + t1) / L[i * LD + i];
+  const bool g = gl;
+  // column j of R^-1 by the four lanes 4j..4j+3 (one wavefront holds 16
+  // columns): X_jj = 1 / R_jj, X_ij = -(sum_{l=i+1..j} R_il X_lj) / R_ii, the
+  // sum split over the four lanes and closed by two butterfly steps; the
+  // loop over i is uniform so that every lane takes part in the shuffles
+  if (g) {
+    const int j = tid >> 2, q = tid & 3;
+    const bool live = j < nb;
+    if (live && q == 0) X[j * LD + j] = 1.0 / L[j * LD + j];
+    __builtin_amdgcn_wave_barrier();
+    for (int i = nb - 2; i >= 0; --i) {
+      double t = 0;
+      if (live && i < j)
+        for (int l = i + 1 + q; l <= j; l += 4) t += L[i * LD + l] * X[l * LD + j];
+      t += __shfl_xor(t, 1, 64);
+      t += __shfl_xor(t, 2, 64);
+      if (live && i < j && q == 0) X[i * LD + j] = -t / L[i * LD + i];
+      __builtin_amdgcn_wave_barrier();  // entry (i, j) is read by the column's other lanes next
     }
   }
   __syncthreads();
