@@ -76,11 +76,11 @@ __device__ double block_sum(double x, double* red, int& parity) {
 }
 
 // Number of eigenvalues of the tridiagonal (d, e^2) smaller than x.
-// a / q through the hardware reciprocal and one Newton step (the Sturm
-// recurrence keeps |q| >= pivmin, so it stays finite): a few ulps from the
-// rounded quotient, a perturbation of the same order as the rounding of T, at a
-// fraction of the IEEE division's dependent latency on the recurrence's
-// critical path.
+// a / q through the hardware reciprocal and one Newton step (callers keep
+// |q| away from zero: the Sturm recurrence by pivmin, inverse iteration by its
+// pivot guard): a few ulps from the rounded quotient, a perturbation of the
+// same order as the rounding of T, in a third of the IEEE division's
+// instructions and dependent latency.
 __device__ __forceinline__ double fast_quot(double e2, double q) {
   double r = __builtin_amdgcn_rcp(q);
   r = fma(fma(-q, r, 1.0), r, r);
@@ -1020,7 +1020,7 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
       const double dd = dn[u] - lam;
       if (fabs(c0) >= fabs(ei[u])) {
         c0 = guard(c0);
-        const double m = ei[u] / c0;
+        const double m = fast_quot(ei[u], c0);
         F(i, 0) = c0;
         F(i, 1) = c1;
         F(i, 2) = 0.0;
@@ -1029,7 +1029,7 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
         c0 = dd - m * c1;
         c1 = en[u];
       } else {
-        const double m = c0 / ei[u];
+        const double m = fast_quot(c0, ei[u]);
         F(i, 0) = ei[u];
         F(i, 1) = dd;
         F(i, 2) = en[u];
@@ -1091,7 +1091,7 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
       for (int u = 0; u < C; ++u) {
         const int i = i1 - u;
         if (i < 0) break;
-        const double zi = (yv[u] - u1[u] * z1 - u2[u] * z2) / u0[u];
+        const double zi = fast_quot(yv[u] - u1[u] * z1 - u2[u] * z2, u0[u]);
         Y(i) = zi;
         zmax = fmax(zmax, fabs(zi));
         z2 = z1;
