@@ -49,3 +49,31 @@ def test_mps_sum_and_from_dense_round_trip():
     legs = [phys[1], a[0][1], a[0][2]]
     d = oracle.to_dense(sym, legs, [1, 1, 0], (0, 0), a[1][1])
     assert np.array_equal(from_dense(sym, legs, [1, 1, 0], (0, 0), d), np.asarray(a[1][1]))
+
+
+def test_complex_time_mpo_is_j_kron_h():
+    """The ancilla MPO equals J (x) H with J|R> = -|I>, J|I> = |R>, so that
+    exp(t J (x) H) carries exp(-i H t) on psi_R + i psi_I."""
+    mpo = tk_hip.chain_mpo("hubbard", 3, t=1.0, U=4.0)
+    h, _ = dense_mpo(mpo)
+    hc, ch = dense_mpo(tk_hip.complex_time_mpo(mpo))
+    J = np.array([[0.0, 1.0], [-1.0, 0.0]])  # rows out (R, I), columns in (R, I)
+    assert np.abs(hc - np.kron(J, h)).max() < 1e-14
+    # d/dt (psi_R, psi_I) = (H psi_I, -H psi_R): the ancilla form of exp(-i H t)
+    rng = np.random.default_rng(0)
+    pr, pi = rng.standard_normal(h.shape[0]), rng.standard_normal(h.shape[0])
+    flow = hc @ np.concatenate([pr, pi])
+    assert np.allclose(flow[:h.shape[0]], h @ pi) and np.allclose(flow[h.shape[0]:], -(h @ pr))
+
+
+def test_complex_state_layout():
+    mpo = tk_hip.chain_mpo("heisenberg", 4)
+    sym = tk_hip.SYM_U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (0, 0), 8, seed=2)
+    phi = dense_mps(sym, mpo, bonds, data)
+    cmpo = tk_hip.complex_time_mpo(mpo)
+    for part, half in (("R", 0), ("I", 1)):
+        b, s = tk_hip.complex_state(bonds, data, part)
+        full = dense_mps(sym, cmpo, b, s)
+        n = phi.shape[0]
+        assert np.array_equal(full[half * n:(half + 1) * n], phi) and not np.any(full[(1 - half) * n:(2 - half) * n])
