@@ -3,9 +3,11 @@
 
 Workloads
   sweep  (default) two-site DMRG on the Hubbard chain L=64, t=1, U=4, half
-         filling, Sz=0, U(1)xU(1): bond dimension 256 -> 2048 over 8 sweeps
-         (the first 7 untimed), run in FP64 and in mixed precision from the
-         same seeded MPS; a step is one full sweep at D=2048.
+         filling, Sz=0, U(1)xU(1): bond dimension 256 -> 2048 over 8 sweeps,
+         run in FP64 and in mixed precision from the same seeded MPS; a step
+         is one full sweep at D=2048.  The ramp sweeps (D < 2048) are never
+         timed; --warmup untimed sweeps at D=2048 follow them, then --steps
+         timed ones; value is the median timed mixed sweep.
   heff   one H_eff*psi on the U(1)xU(1) Hubbard microbenchmark bond
          (d=4, MPO bond 6, D=1024 on both sides, 145 sectors); a step is one
          matvec with inputs resident in HBM.
@@ -186,42 +188,37 @@ def bench_copy(args):
             "config": {"workload": "copy_4096x4096_f64", "bytes_read": 8 * x.size, "bytes_written": 8 * x.size}}
 
 
-def warmup_schedule(d0, d_max, n):
-    """BondDimensionSchedule::warmup (tk-dmrg/src/truncation/mod.rs:137-151):
-    round(d0 * r^i) half away from zero, r = (d_max/d0)^(1/n), clamped."""
-    import math
-
-    r = (d_max / d0) ** (1.0 / n)
-    out = []
-    for i in range(n + 1):
-        v = d0 * r ** i
-        out.append(max(1, min(d_max, int(math.floor(v + 0.5)))))
-    return out
-
-
 def run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, warmup, steps, prec, tol, log):
-    """warmup + steps sweeps from the same seeded MPS; the last `steps` are
-    timed (device synchronised on both sides) with kernel timing enabled."""
+    """The bond-dimension ramp (sweeps at D < D_max, never timed), then
+    `warmup` untimed and `steps` timed sweeps at D_max, all from the same
+    seeded MPS; kernel timing is on for the timed sweeps only.  Returns every
+    sweep's statistics, the timed sweeps' wall times, the kernel record and
+    the centre bonds."""
     d = tk_hip.Dmrg(ctx, mpo, sym, prec=prec, schedule=sched, davidson_tol=tol, svd_cutoff=1e-12)
     d.set_state(bonds, data)
     t0 = time.perf_counter()
     d.prepare()
     log(f"[{prec}] prepared in {time.perf_counter() - t0:.2f}s")
-    out, wall = [], 0.0
-    for k in range(warmup + steps):
-        timed = k >= warmup
-        if timed and k == warmup:
+    ramp = len(sched) - 1
+    first_timed = ramp + warmup
+    out, times = [], []
+    for k in range(first_timed + steps):
+        timed = k >= first_timed
+        if timed and k == first_timed:
             ctx.sync()
             ctx.timing(True)
         t0 = time.perf_counter()
         s = d.sweep()
         ctx.sync()
         el = time.perf_counter() - t0
+        s["wall"] = el
+        s["bond_dim"] = sched[min(k, ramp)]
         if timed:
-            wall += el
+            times.append(el)
         out.append(s)
-        log(f"[{prec}] sweep {k + 1}/{warmup + steps}{' (timed)' if timed else ''} "
-            f"D={sched[min(k, len(sched) - 1)]} E={s['energy']:.13f} t={el:.3f}s (plan {s['t_plan']:.2f} "
+        tag = " (timed)" if timed else (" (ramp)" if k < ramp else " (warmup)")
+        log(f"[{prec}] sweep {k + 1}/{first_timed + steps}{tag} "
+            f"D={s['bond_dim']} E={s['energy']:.13f} t={el:.3f}s (plan {s['t_plan']:.2f} "
             f"eig {s['t_eig']:.2f} svd {s['t_svd']:.2f} env {s['t_env']:.2f}) matvecs={s['matvecs']} "
             f"fp64={s['matvecs_fp64']} maxD={s['max_bond_dim']} trunc={s['max_truncation_error']:.2e}")
     timing = ctx.timing_read()
@@ -229,7 +226,7 @@ def run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, warmup, steps, prec, t
     n = d.n
     centre = (d.bond(n // 2 - 1), d.bond(n // 2 + 1))
     d.close()
-    return out, wall, timing, centre
+    return out, times, timing, centre
 
 
 # Measured HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated
@@ -289,50 +286,54 @@ def kernel_table(timing):
     return out
 
 
-def cpu_baseline_sweep(tk_hip, ctx, centre, heff_flops, budget_s, log):
-    """The oracle's FP64 H_eff*psi (OpenMP over the host cores) on a two-site
-    problem with the timed sweep's centre-bond sectors (random entries), timed
-    for about budget_s, scaled to the timed sweep by its summed matvec flops."""
+def cpu_baseline_sweep(tk_hip, sites, log):
+    """The oracle's FP64 two-site sweeps (matvecs, SVD, environments; OpenMP
+    over the host's cores) on the same chain from the same kind of seeded
+    MPS: one sweep at D=64 (untimed ramp), then the timed sweep at D=128,
+    the largest bond dimension of the schedule family whose sweep fits the
+    CPU budget; and the first D=64 sweep on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    W = tk_hip.hubbard_w(1.0, 4.0)
-    p = tk_hip.heff_problem(centre[0], centre[1], W, seed=1)
-    L, R = ctx.to_device(p["L"]), ctx.to_device(p["R"])
-    plan = tk_hip.HeffPlan(ctx, "fp64", p["sym"], p["bond_l"], W.wl, L, p["bond_r"], W.wr, R, W, W)
-    f_centre = plan.stats()["flops"]
-    plan.close()
-    L.free()
-    R.free()
-    oracle.heff_apply(p, p["x"])
-    n, t0 = 0, time.perf_counter()
-    while True:
-        oracle.heff_apply(p, p["x"])
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    per = el / n
-    rate = f_centre / per
+    mpo = tk_hip.chain_mpo("hubbard", sites, t=1.0, U=4.0)
+    sym = tk_hip.SYM_U1U1
+    bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (sites // 2, sites // 2), 64, seed=0,
+                                     min_sector_dim=0)
     cores = os.cpu_count()
-    log(f"[cpu] oracle H_eff*psi at the centre bonds ({sum(d for _, d in centre[0])}, "
-        f"{sum(d for _, d in centre[1])}): {per * 1e3:.1f} ms/matvec, {rate / 1e9:.1f} GFLOP/s on {cores} threads")
-    return {"value": round(heff_flops / rate, 3), "unit": "s/sweep (H_eff*psi matvecs only)", "cores": cores,
-            "kind": "port",
-            "sample": f"{n} oracle FP64 H_eff*psi on the timed sweep's centre bonds (D="
-                      f"{sum(d for _, d in centre[0])}, random entries), {per * 1e3:.1f} ms each = "
-                      f"{rate / 1e9:.1f} GFLOP/s, scaled by the sweep's {heff_flops / 1e12:.2f} TFLOP of matvecs; "
-                      f"SVD, environments and Krylov vector work excluded"}
+    res = {}
+    for nthreads, sched in ((0, (64, 128)), (1, (64,))):
+        o = oracle.Dmrg(mpo, sym, schedule=sched, davidson_tol=1e-10, svd_cutoff=1e-12, nthreads=nthreads)
+        o.set_state(bonds, data)
+        o.prepare()
+        for dk in sched:
+            t0 = time.perf_counter()
+            st = o.sweep()
+            res[(nthreads, dk)] = (time.perf_counter() - t0, st)
+            log(f"[cpu] oracle sweep D={dk} on {cores if nthreads == 0 else 1} thread(s): "
+                f"{res[(nthreads, dk)][0]:.2f}s E={st['energy']:.12f}")
+        o.close()
+    t128, st128 = res[(0, 128)]
+    t64_1, _ = res[(1, 64)]
+    return {"value": round(t128, 3), "unit": "s/sweep at D=128", "cores": cores, "kind": "port",
+            "sample": f"one oracle FP64 two-site sweep (Davidson tol 1e-10, cutoff 1e-12: matvecs "
+                      f"{st128['t_matvec']:.1f}s, SVD {st128['t_svd']:.1f}s, environments {st128['t_env']:.1f}s, "
+                      f"the rest Davidson vector work) of the same Hubbard chain L={sites} at D=128 after one at "
+                      f"D=64, from the PCG64-seeded MPS, on {cores} OpenMP threads",
+            "single_thread": {"value": round(t64_1, 3), "unit": "s/sweep at D=64", "cores": 1,
+                              "sample": "the first oracle sweep at D=64 from the same seeded MPS on one thread"},
+            "scaled_to_d2048_estimate_s": round(t128 * (2048 / 128) ** 3, 1)}
 
 
 def bench_sweep(args):
+    import statistics
+
     import tk_hip
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
     n = args.sites
-    sched = warmup_schedule(args.d0, args.dmax, args.ramp)
+    sched = tk_hip.BondDimensionSchedule.warmup(args.d0, args.dmax, args.ramp).dims
     mpo = tk_hip.chain_mpo("hubbard", n, t=1.0, U=4.0)
     sym = tk_hip.SYM_U1U1
     bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (n // 2, n // 2), sched[0], seed=0,
@@ -342,33 +343,46 @@ def bench_sweep(args):
     for prec in ("fp64", args.mixed):
         res[prec] = run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, args.warmup, args.steps, prec, args.tol,
                                log)
-    s64, w64, tim64, centre = res["fp64"]
-    slo, wlo, timlo, _ = res[args.mixed]
-    t64 = w64 / args.steps
-    tlo = wlo / args.steps
+    s64, times64, tim64, centre = res["fp64"]
+    slo, timeslo, timlo, _ = res[args.mixed]
+    ramp = len(sched) - 1
+    timed64, timedlo = s64[ramp + args.warmup:], slo[ramp + args.warmup:]
+    assert all(s["bond_dim"] == args.dmax for s in timedlo + timed64)
+    med64, medlo = statistics.median(times64), statistics.median(timeslo)
+    mean_lo = sum(timeslo) / len(timeslo)
     de = abs(slo[-1]["energy"] - s64[-1]["energy"])
-    timed64, timedlo = s64[args.warmup:], slo[args.warmup:]
+    mv_lo = sum(s["matvecs"] for s in timedlo)
+    mv_lo64 = sum(s["matvecs_fp64"] for s in timedlo)
+    lp_frac = 1.0 - mv_lo64 / mv_lo if mv_lo else 0.0
+    dtype = "f64" if mv_lo64 == mv_lo else f"{args.mixed}/f64"
     rec = {
         "metric": METRIC,
-        "value": round(tlo, 4),
+        "value": round(medlo, 4),
         "unit": "s/sweep",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * tlo, 3),
+        "ms_per_step": round(1e3 * mean_lo, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32 (f64 Krylov, SVD, refinement)", "bf16x3": "bf16x3-f32acc (f64 Krylov, SVD, "
-                                                                           "refinement)"}[args.mixed],
+        "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": "dmrg_sweep_hubbard_L64_D2048", "sites": n, "t": 1.0, "U": 4.0,
+        "config": {"workload": f"dmrg_sweep_hubbard_L{n}_D{args.dmax}", "sites": n, "t": 1.0, "U": 4.0,
                    "filling": "half, Sz=0", "symmetry": "U(1)xU(1)", "schedule": sched,
-                   "timed_sweep_bond_dim": sched[-1], "davidson_tol": args.tol, "svd_cutoff": 1e-12,
-                   "mixed_precision": args.mixed, "initial_mps": "PCG64 seed 0, binomially concentrated charges"},
+                   "timed_sweep_bond_dim": args.dmax, "davidson_tol": args.tol, "svd_cutoff": 1e-12,
+                   "mixed_precision": args.mixed, "initial_mps": "PCG64 seed 0, binomially concentrated charges",
+                   "value_is": f"median wall time of the {args.steps} timed sweeps at D={args.dmax} of the mixed "
+                               f"run (ramp sweeps and {args.warmup} warmup sweeps untimed); ms_per_step is their "
+                               f"mean"},
         "roofline": roofline_of(timlo),
-        "fp64_seconds_per_sweep": round(t64, 4),
-        "speedup_mixed_vs_fp64": round(t64 / tlo, 3),
+        "fp64_seconds_per_sweep": round(med64, 4),
+        "speedup_mixed_vs_fp64": round(med64 / medlo, 3),
+        "first_dmax_sweep_s": {"mixed": round(slo[ramp]["wall"], 4), "fp64": round(s64[ramp]["wall"], 4)},
+        "timed_sweeps_mixed": [{"D": s["bond_dim"], "s": round(s["wall"], 4), "matvecs": s["matvecs"],
+                                "matvecs_fp64": s["matvecs_fp64"]} for s in timedlo],
+        "timed_sweeps_fp64": [{"D": s["bond_dim"], "s": round(s["wall"], 4), "matvecs": s["matvecs"]}
+                              for s in timed64],
         "energy_fp64": s64[-1]["energy"],
         "energy_mixed": slo[-1]["energy"],
         "abs_delta_e": de,
@@ -379,17 +393,16 @@ def bench_sweep(args):
                          for k in ("t_plan", "t_eig", "t_svd", "t_env")},
         "phases_fp64": {k: round(sum(s[k] for s in timed64) / len(timed64), 4)
                         for k in ("t_plan", "t_eig", "t_svd", "t_env")},
-        "matvecs_mixed": sum(s["matvecs"] for s in timedlo),
-        "matvecs_mixed_fp64": sum(s["matvecs_fp64"] for s in timedlo),
+        "matvecs_mixed": mv_lo,
+        "matvecs_mixed_fp64": mv_lo64,
+        "mixed_low_precision_matvec_fraction": round(lp_frac, 4),
         "matvecs_fp64": sum(s["matvecs"] for s in timed64),
         "kernels_mixed": kernel_table(timlo),
         "kernels_fp64": kernel_table(tim64),
     }
-    if not args.no_cpu_baseline:
-        heff_flops = sum(t["flops"] for k, t in tim64.items() if k.startswith("heff_"))
-        rec["cpu_baseline"] = cpu_baseline_sweep(tk_hip, ctx, centre, heff_flops / args.steps, args.cpu_seconds,
-                                                 log)
     ctx.close()
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline_sweep(tk_hip, n, log)
     return rec
 
 
@@ -397,7 +410,7 @@ def _ground_state(tk_hip, ctx, args, log):
     """Two-site FP64 DMRG ground state of the Hubbard chain (t=1, U=4, half
     filling) ramped to args.dmax; returns (engine, mpo, sym, energy)."""
     n = args.sites
-    sched = warmup_schedule(min(args.d0, args.dmax), args.dmax, args.ramp)
+    sched = tk_hip.BondDimensionSchedule.warmup(min(args.d0, args.dmax), args.dmax, args.ramp).dims
     mpo = tk_hip.chain_mpo("hubbard", n, t=1.0, U=4.0)
     sym = tk_hip.SYM_U1U1
     bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (n // 2, n // 2), sched[0], seed=0,
@@ -540,7 +553,7 @@ def bench_sweep_c4(args):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    sched = warmup_schedule(args.d0, args.dmax, args.ramp)
+    sched = tk_hip.BondDimensionSchedule.warmup(args.d0, args.dmax, args.ramp).dims
     mpo = tk_hip.cylinder_mpo(6, 12)
     n = len(mpo)
     sym = tk_hip.SYM_U1
@@ -552,15 +565,16 @@ def bench_sweep_c4(args):
                                log)
     s64 = res["fp64"][0]
     per = {}
-    for prec, (st, wall, tim, _) in res.items():
-        per[prec] = {"seconds_per_sweep": round(wall / args.steps, 4), "energy": st[-1]["energy"],
+    first = len(sched) - 1 + args.warmup
+    for prec, (st, times, tim, _) in res.items():
+        per[prec] = {"seconds_per_sweep": round(sum(times) / len(times), 4), "energy": st[-1]["energy"],
                      "abs_delta_e_vs_fp64": abs(st[-1]["energy"] - s64[-1]["energy"]),
                      "max_truncation_error": st[-1]["max_truncation_error"],
                      "max_bond_dim": st[-1]["max_bond_dim"],
-                     "matvecs": sum(x["matvecs"] for x in st[args.warmup:]),
-                     "matvecs_fp64": sum(x["matvecs_fp64"] for x in st[args.warmup:])}
-    slo, wlo, timlo, _ = res["bf16x3"]
-    tlo = wlo / args.steps
+                     "matvecs": sum(x["matvecs"] for x in st[first:]),
+                     "matvecs_fp64": sum(x["matvecs_fp64"] for x in st[first:])}
+    slo, tlos, timlo, _ = res["bf16x3"]
+    tlo = sum(tlos) / len(tlos)
     rec = {
         "metric": "DMRG sweep time, J1 Heisenberg 6x12 cylinder D=4096, split-BF16 vs FP32-MFMA vs FP64",
         "value": round(tlo, 4),
@@ -623,13 +637,15 @@ def main():
         args.dmax = 4096 if args.dmax is None else args.dmax
         args.ramp = 4 if args.ramp is None else args.ramp
         args.steps = 1 if args.steps is None else args.steps
-        args.warmup = args.ramp if args.warmup is None else args.warmup
+        args.warmup = 0 if args.warmup is None else args.warmup
         rec = bench_sweep_c4(args)
     else:
+        # the ramp's sweeps (D < D_max) are never timed; --warmup counts
+        # untimed sweeps at D_max after them
         args.dmax = 2048 if args.dmax is None else args.dmax
         args.ramp = 7 if args.ramp is None else args.ramp
-        args.steps = 1 if args.steps is None else args.steps
-        args.warmup = args.ramp if args.warmup is None else args.warmup
+        args.steps = 3 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
         rec = bench_sweep(args)
     print(json.dumps(rec))
 

@@ -184,17 +184,86 @@ tk_status tk_block_gemm(tk_ctx* ctx, int32_t sym, const tk_block_matrix* a, cons
  * y over the full sorted index ranges of cols / rows.  Synchronous. */
 tk_status tk_spmv(tk_ctx* ctx, int32_t sym, const tk_block_matrix* a, const double* x_dev, double* y_dev);
 /* LinAlgBackend::svd_truncated (traits.rs:38-88; faer.rs:127-240): row-major
- * m x n A; keeps at most max_rank singular values with sigma >= cutoff *
- * sigma_max (numerical zeros, sigma^2 <= 1e-15 sigma_max^2, are dropped);
- * U (m x rank), Vt (rank x n) row-major on the device, s (host) descending,
- * truncation_error = sum of discarded sigma^2 (results.rs:18-23).  min(m, n)
- * must not exceed 2048. */
+ * m x n A, min(m, n) <= 2048 (else TK_ERR_INVALID_ARG); keeps at most
+ * max_rank singular values with sigma >= cutoff * sigma_max (the
+ * truncate_svd! rule, faer.rs:135-147).  Two Gram levels resolve sigma to
+ * ~eps sigma_max^2 / sigma above 3.2e-5 sigma_max and to ~1e-13 sigma_max
+ * below, so the default cutoff 1e-12 holds except for values within a factor
+ * ~3 of it; values no level resolves count as zero.  U (m x rank), Vt
+ * (rank x n) row-major on the device (capacity m x min(m, n, max_rank) and
+ * min(m, n, max_rank) x n), s (host) descending, truncation_error = sum of
+ * discarded sigma^2 (results.rs:18-23). */
 tk_status tk_svd_truncated(tk_ctx* ctx, int64_t m, int64_t n, const double* a_dev, int64_t max_rank, double cutoff,
                            double* u_dev, double* s_host, double* vt_dev, int64_t* rank, double* truncation_error);
+/* truncate_svd (tk-dmrg/src/truncation/mod.rs:56-127): the full SVD of a
+ * row-major rows x cols matrix (min <= 2048) cut by TruncationConfig
+ * (truncation/mod.rs:12-30): keep while count < max_bond_dim and sigma >=
+ * svd_cutoff * sigma_max, then at least min(min_bond_dim, min(rows, cols)).
+ * U (rows x bond_dim_new) and Vt (bond_dim_new x cols) on the device, s
+ * (host, bond_dim_new values), n_discarded = min(rows, cols) - bond_dim_new,
+ * truncation_error relative: 1 - sum kept sigma^2 / sum sigma^2. */
+tk_status tk_truncate_svd(tk_ctx* ctx, int64_t rows, int64_t cols, const double* a_dev, int64_t max_bond_dim,
+                          double svd_cutoff, int64_t min_bond_dim, double* u_dev, double* s_host, double* vt_dev,
+                          int64_t* bond_dim_new, int64_t* n_discarded, double* truncation_error);
+/* The selection rule of truncate_svd (truncation/mod.rs:72-98) over the
+ * singular values of several blocks at once, as the two-site split applies
+ * it (host only, no device work).  sigma holds n_blocks runs of counts[b]
+ * values, each descending.  Candidates are ordered by sigma descending (ties:
+ * block, then position); keep while count < max_bond and sigma >= cutoff *
+ * sigma_max and sigma^2 >= sigma2_floor * sigma_max^2; when degeneracy_tol > 0
+ * the cut moves up to the start of a group sigma_k - sigma_{k+1} <=
+ * degeneracy_tol * sigma_max (unless nothing would be kept); then at least
+ * min(min_bond, candidates).  Outputs: keep[b] per block, the totals and the
+ * relative truncation error.  The reference's single-matrix rule is
+ * n_blocks = 1, degeneracy_tol = 0, sigma2_floor = 0. */
+tk_status tk_truncation_select(int32_t n_blocks, const int64_t* counts, const double* sigma, int64_t max_bond,
+                               double cutoff, int64_t min_bond, double degeneracy_tol, double sigma2_floor,
+                               int64_t* keep, int64_t* n_kept, int64_t* n_discarded, double* truncation_error);
+/* The block-diagonal truncated SVD of the two-site split (the reference's
+ * truncate_svd on the whole matricised two-site tensor, sweep/mod.rs:401-421,
+ * here per middle charge): blocks b = 0..n_blocks-1 of rows[b] x cols[b]
+ * (rows[b] <= 2048) lie back to back, row-major, in x_dev; the selection of
+ * tk_truncation_select over all blocks' singular values.  Outputs: keep[b];
+ * sigma_host: the kept values block after block (each run descending); u_dev:
+ * per block the rows[b] x keep[b] left singular vectors, back to back; c_dev:
+ * per block the keep[b] x cols[b] matrix U_b^T X_b (= diag(sigma) Vt, the
+ * next centre of a left-to-right step), back to back.  The capacities bound
+ * sigma_host (values), u_dev and c_dev (elements); TK_ERR_CAPACITY when one
+ * is too small.  Synchronous. */
+tk_status tk_block_svd_truncate(tk_ctx* ctx, int32_t n_blocks, const int64_t* rows, const int64_t* cols,
+                                const double* x_dev, int64_t max_bond, double cutoff, int64_t min_bond,
+                                double degeneracy_tol, double sigma2_floor, int64_t* keep, double* sigma_host,
+                                int64_t sigma_capacity, double* u_dev, int64_t u_capacity, double* c_dev,
+                                int64_t c_capacity, int64_t* n_kept, int64_t* n_discarded, double* truncation_error);
 /* LinAlgBackend::eigh_lowest (traits.rs:134-138), all pairs: eigenvalues
  * ascending (host), eigenvectors as columns of a row-major n x n (device);
  * n <= 2048. */
 tk_status tk_eigh(tk_ctx* ctx, int64_t n, const double* a_dev, double* evals_host, double* evecs_dev);
+/* LinAlgBackend::eigh_lowest (traits.rs:134-138; faer.rs:243-290): the k
+ * lowest eigenpairs of a symmetric n x n (n <= 2048), eigenvalues ascending
+ * (host, k), eigenvectors as the columns of a row-major n x k (device).
+ * k > n is LinAlgError::EighKTooLarge: TK_ERR_INVALID_ARG. */
+tk_status tk_eigh_lowest(tk_ctx* ctx, int64_t n, const double* a_dev, int64_t k, double* evals_host,
+                         double* evecs_dev);
+/* LinAlgBackend::qr (traits.rs:148; faer.rs:292-324): thin QR of a row-major
+ * m x n A: Q (m x k, orthonormal columns) and R (k x n, upper triangular),
+ * k = min(m, n), row-major on the device (QrResult, results.rs:34-40);
+ * Householder reflectors, so R's diagonal may be negative.  A is not
+ * modified.  Synchronous. */
+tk_status tk_qr(tk_ctx* ctx, int64_t m, int64_t n, const double* a_dev, double* q_dev, double* r_dev);
+/* LinAlgBackend::regularized_svd_inverse (traits.rs:150-183): out (n x m,
+ * device) = V diag(s_i / (s_i^2 + delta^2)) U^T for U (m x k) and Vt (k x n)
+ * on the device and s (k, host); delta > 0. */
+tk_status tk_regularized_svd_inverse(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, const double* s_host,
+                                     const double* u_dev, const double* vt_dev, double delta, double* out_dev);
+/* BondDimensionSchedule::warmup (tk-dmrg/src/truncation/mod.rs:137-151):
+ * round(d_init * r^i) (half away from zero), r = (d_max / d_init)^(1/n),
+ * clamped to [1, d_max], i = 0..n (n + 1 values; n = 0 gives [d_max]).
+ * bond_dim_at_sweep (truncation/mod.rs:163-171) is the value at the sweep
+ * index, the last one beyond the end: tk_dmrg_config.bond_schedule takes the
+ * list with that meaning.  Host only. */
+tk_status tk_bond_schedule_warmup(int64_t d_init, int64_t d_max, int64_t n_warmup_sweeps, int64_t capacity,
+                                  int64_t* dims, int64_t* n_dims);
 
 /* IterativeEigensolver::lowest_eigenpair (tk-dmrg/src/eigensolver/mod.rs:38-55)
  * with DavidsonSolver semantics (eigensolver/mod.rs:300-516) on a caller
@@ -309,6 +378,15 @@ tk_status tk_dmrg_set_state(tk_dmrg* d, const tk_qindex* bonds, const double* co
 tk_status tk_dmrg_prepare(tk_dmrg* d);
 /* One left-to-right + right-to-left sweep (SweepSchedule::standard, sweep/mod.rs:59-81). */
 tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out);
+/* The cancel flag of DMRGEngine::run_with_cancel_flag (sweep/mod.rs:274-291):
+ * while the int32 at `flag` (host memory, may be null to clear) reads
+ * non-zero, tk_dmrg_sweep and tk_dmrg_sweep_single_site stop before their
+ * next two-site (or one-site) step and return TK_ERR_CANCELLED
+ * (DmrgError::Cancelled, tk-dmrg/src/error.rs:53).  Another thread may set
+ * the flag at any time; it is read with a relaxed atomic load.  Steps
+ * already done are kept (the state stays a valid MPS); the engine must be
+ * prepared again (tk_dmrg_prepare) before the next sweep. */
+tk_status tk_dmrg_set_cancel_flag(tk_dmrg* d, const int32_t* flag);
 /* dmrg_step_single_site (tk-dmrg/src/sweep/mod.rs:451-509) over a whole sweep.
  * The reference solves the one-site problem but neither writes the eigenvector
  * back nor moves the centre; here every site is replaced by the lowest

@@ -397,6 +397,26 @@ def svd(a):
     return u.reshape(m, r), s, v.reshape(n, r)
 
 
+def truncation_select(sigmas, max_bond, cutoff, min_bond=0, degeneracy_tol=0.0, sigma2_floor=0.0):
+    """truncate_svd's selection (truncation/mod.rs:56-98) over the singular
+    values of several blocks (a list of descending arrays): (keep per block,
+    n_kept, n_discarded, relative truncation error)."""
+    counts = np.array([len(s) for s in sigmas], dtype=np.int64)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(s, dtype=np.float64) for s in sigmas])
+                                if len(sigmas) else np.zeros(0))
+    keep = np.zeros(max(1, len(sigmas)), dtype=np.int64)
+    nk, nd, te = C.c_int64(0), C.c_int64(0), C.c_double(0)
+    L = lib()
+    L.or_truncation_select.argtypes = [C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_int64,
+                                       C.c_double, C.c_int64, C.c_double, C.c_double, C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    if L.or_truncation_select(len(sigmas), _p(counts, C.c_int64), _p(flat, C.c_double), int(max_bond),
+                              float(cutoff), int(min_bond), float(degeneracy_tol), float(sigma2_floor),
+                              _p(keep, C.c_int64), C.byref(nk), C.byref(nd), C.byref(te)):
+        raise RuntimeError("or_truncation_select failed")
+    return keep[:len(sigmas)].tolist(), nk.value, nd.value, te.value
+
+
 class Dmrg:
     """The oracle's two-site DMRG (CPU, FP64) on the same MPO and initial state."""
 

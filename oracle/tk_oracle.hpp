@@ -63,6 +63,12 @@ int or_heff_diag(int32_t sym, const or_qindex* phys_i, const or_qindex* phys_j, 
 // The reference's dense H_eff (environments/mod.rs:505-599) restated loop by
 // loop on densified environments, then y = H x in the dense (s_i, s_j, a_L, a_R)
 // layout (full-index order).  Tiny sizes only: dim^2 doubles.
+// truncate_svd's selection rule (truncation/mod.rs:56-98) over the singular
+// values of n_blocks blocks (counts[b] values each, descending), with the
+// design's degeneracy rule (deg_tol > 0) and numerical-zero floor on sigma^2.
+int or_truncation_select(int32_t n_blocks, const int64_t* counts, const double* sigma, int64_t max_bond,
+                         double cutoff, int64_t min_bond, double deg_tol, double floor, int64_t* keep,
+                         int64_t* n_kept, int64_t* n_disc, double* terr);
 int or_heff_dense_reference(int32_t sym, const or_qindex* phys_i, const or_qindex* phys_j, const or_qindex* bond_l,
                             const or_qindex* bond_r, const or_qindex* w_l, const or_qindex* w_m,
                             const or_qindex* w_r, const double* wi, const double* wj, const double* L,

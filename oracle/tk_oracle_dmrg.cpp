@@ -43,36 +43,73 @@ double wall() {
 // off-diagonal pivot, threshold 1e-15, at most 100 n^2 rotations.  Eigenvectors
 // column-major-by-index v[i*n + j] = component i of vector j.
 void jacobi_small(const std::vector<double>& mat, int n, std::vector<double>& evals, std::vector<double>& v) {
+  // Same rotations in the same order as the reference: the pivot is the first
+  // largest |a_ij| (i < j) in row-major order.  Instead of rescanning the
+  // triangle and copying the matrix for every rotation, the first maximum of
+  // every row is tracked (rm, rc) and only rows p, q and columns p, q are
+  // rewritten, from the values before the rotation as in the reference.
   std::vector<double> a = mat;
   v.assign(size_t(n) * n, 0.0);
   for (int i = 0; i < n; ++i) v[i * n + i] = 1.0;
+  std::vector<double> rm(size_t(std::max(n, 1)), 0.0);
+  std::vector<int> rc(size_t(std::max(n, 1)), -1);
+  auto rescan = [&](int i) {
+    rm[i] = 0.0;
+    rc[i] = -1;
+    for (int j = i + 1; j < n; ++j)
+      if (std::fabs(a[i * n + j]) > rm[i]) {
+        rm[i] = std::fabs(a[i * n + j]);
+        rc[i] = j;
+      }
+  };
+  // a changed entry (i, j), i < j, of a row whose maximum is elsewhere
+  auto touch = [&](int i, int j) {
+    const double x = std::fabs(a[i * n + j]);
+    if (x > rm[i] || (x == rm[i] && x > 0.0 && j < rc[i])) {
+      rm[i] = x;
+      rc[i] = j;
+    }
+  };
+  for (int i = 0; i < n; ++i) rescan(i);
+  std::vector<double> colp(size_t(std::max(n, 1))), colq(size_t(std::max(n, 1)));
   for (long it = 0; it < 100L * n * n; ++it) {
     double mx = 0.0;
     int p = 0, q = 1;
     for (int i = 0; i < n; ++i)
-      for (int j = i + 1; j < n; ++j)
-        if (std::fabs(a[i * n + j]) > mx) {
-          mx = std::fabs(a[i * n + j]);
-          p = i;
-          q = j;
-        }
+      if (rm[i] > mx) {
+        mx = rm[i];
+        p = i;
+        q = rc[i];
+      }
     if (mx < 1e-15 || n < 2) break;
     const double app = a[p * n + p], aqq = a[q * n + q], apq = a[p * n + q];
     const double theta = std::fabs(app - aqq) < 1e-30 ? M_PI / 4 : 0.5 * std::atan(2.0 * apq / (app - aqq));
     const double c = std::cos(theta), s = std::sin(theta);
-    std::vector<double> na = a;
     for (int i = 0; i < n; ++i)
       if (i != p && i != q) {
-        na[i * n + p] = c * a[i * n + p] + s * a[i * n + q];
-        na[p * n + i] = na[i * n + p];
-        na[i * n + q] = -s * a[i * n + p] + c * a[i * n + q];
-        na[q * n + i] = na[i * n + q];
+        colp[i] = c * a[i * n + p] + s * a[i * n + q];
+        colq[i] = -s * a[i * n + p] + c * a[i * n + q];
       }
-    na[p * n + p] = c * c * app + 2.0 * s * c * apq + s * s * aqq;
-    na[q * n + q] = s * s * app - 2.0 * s * c * apq + c * c * aqq;
-    na[p * n + q] = 0.0;
-    na[q * n + p] = 0.0;
-    a.swap(na);
+    for (int i = 0; i < n; ++i)
+      if (i != p && i != q) {
+        a[i * n + p] = a[p * n + i] = colp[i];
+        a[i * n + q] = a[q * n + i] = colq[i];
+      }
+    a[p * n + p] = c * c * app + 2.0 * s * c * apq + s * s * aqq;
+    a[q * n + q] = s * s * app - 2.0 * s * c * apq + c * c * aqq;
+    a[p * n + q] = 0.0;
+    a[q * n + p] = 0.0;
+    rescan(p);
+    rescan(q);
+    for (int i = 0; i < q; ++i) {
+      if (i == p) continue;
+      if (rc[i] == p || rc[i] == q) {
+        rescan(i);
+      } else {
+        if (i < p) touch(i, p);
+        touch(i, q);
+      }
+    }
     for (int i = 0; i < n; ++i) {
       const double vip = v[i * n + p], viq = v[i * n + q];
       v[i * n + p] = c * vip + s * viq;
@@ -174,11 +211,21 @@ double davidson_core(const std::function<void(const std::vector<double>&, std::v
     matvec(basis[0], ab[0]);
     double eig = 0.0;
     std::vector<double> y = basis[0];
+    // The reference rebuilds the projected matrix from scratch every iteration
+    // (eigensolver/mod.rs:372-384); entries of earlier iterations are the same
+    // dot products of unchanged vectors, so they are kept (bit-identical
+    // values) and only the new column is formed.  A restart clears them.
+    std::vector<std::vector<double>> hc;
     for (int it = 0; it < max_iter; ++it) {
       const int ns = int(basis.size());
+      while (int(hc.size()) < ns) {
+        const int b = int(hc.size());
+        hc.emplace_back(size_t(b + 1));
+        for (int a = 0; a <= b; ++a) hc[b][a] = dot(basis[a], ab[b]);
+      }
       std::vector<double> h(size_t(ns) * ns);
-      for (int a = 0; a < ns; ++a)
-        for (int b = a; b < ns; ++b) h[a * ns + b] = h[b * ns + a] = dot(basis[a], ab[b]);
+      for (int b = 0; b < ns; ++b)
+        for (int a = 0; a <= b; ++a) h[a * ns + b] = h[b * ns + a] = hc[b][a];
       std::vector<double> ev, vv;
       jacobi_small(h, ns, ev, vv);
       int mi = int(std::min_element(ev.begin(), ev.end()) - ev.begin());
@@ -221,6 +268,7 @@ double davidson_core(const std::function<void(const std::vector<double>&, std::v
         if (ns >= dim) break;
         basis.assign(1, y);
         ab.assign(1, ay);
+        hc.clear();
       }
       basis.push_back(t);
       ab.emplace_back();
@@ -232,6 +280,52 @@ double davidson_core(const std::function<void(const std::vector<double>&, std::v
 }
 
 }  // namespace
+
+// truncate_svd's selection (truncation/mod.rs:56-98) over the singular values
+// of several blocks (each run descending): candidates by sigma descending
+// (ties by block, then position); keep while count < max_bond and sigma >=
+// cutoff * sigma_max, treating sigma^2 < floor * sigma_max^2 as zero; with
+// deg_tol > 0 never cut between two values closer than deg_tol * sigma_max
+// (the cut moves up to the group's start unless nothing would remain); at
+// least min(min_bond, candidates).  Returns per-block kept counts, the kept
+// and discarded totals and the summed sigma^2 of all and of the kept values.
+void select_keep(const std::vector<std::vector<double>>& sig, long max_bond, double cutoff, long min_bond,
+                 double deg_tol, double floor, std::vector<long>& keep_of, long* n_kept, long* n_disc,
+                 double* sum_all, double* sum_kept) {
+  struct Cand { double l; int mid; int idx; };
+  std::vector<Cand> all;
+  for (int k = 0; k < int(sig.size()); ++k)
+    for (int j = 0; j < int(sig[k].size()); ++j) all.push_back({sig[k][j] * sig[k][j], k, j});
+  std::stable_sort(all.begin(), all.end(), [](const Cand& a, const Cand& b) {
+    if (a.l != b.l) return a.l > b.l;
+    if (a.mid != b.mid) return a.mid < b.mid;
+    return a.idx < b.idx;
+  });
+  keep_of.assign(sig.size(), 0);
+  const double lmax = all.empty() ? 0 : all[0].l;
+  const double thr = std::max(cutoff * cutoff, floor) * lmax;
+  size_t keep = 0;
+  while (keep < all.size() && long(keep) < max_bond && all[keep].l >= thr) ++keep;
+  if (deg_tol > 0 && keep < all.size() && keep > 0) {
+    size_t k2 = keep;
+    const double smax = std::sqrt(lmax);
+    while (k2 > 0 && std::sqrt(all[k2 - 1].l) - std::sqrt(all[k2].l) <= deg_tol * smax) --k2;
+    if (k2 > 0) keep = k2;
+  }
+  keep = std::max<size_t>(keep, size_t(std::min<long>(std::max<long>(min_bond, 0), long(all.size()))));
+  double sa = 0, sk = 0;
+  for (size_t k = 0; k < all.size(); ++k) {
+    sa += all[k].l;
+    if (k < keep) {
+      sk += all[k].l;
+      ++keep_of[all[k].mid];
+    }
+  }
+  *n_kept = long(keep);
+  *n_disc = long(all.size() - keep);
+  *sum_all = sa;
+  *sum_kept = sk;
+}
 
 struct DmrgCfg {
   int sym = 1;
@@ -355,34 +449,13 @@ struct Dmrg {
         }
       svd_jacobi(M, m.nr, m.nc, m.U, m.S, m.V);
     }
-    struct Cand { double l; int mid; int idx; };
-    std::vector<Cand> all;
-    for (int k = 0; k < int(mids.size()); ++k)
-      for (int j = 0; j < int(mids[k].S.size()); ++j) all.push_back({mids[k].S[j] * mids[k].S[j], k, j});
-    std::stable_sort(all.begin(), all.end(), [](const Cand& a, const Cand& b) {
-      if (a.l != b.l) return a.l > b.l;
-      if (a.mid != b.mid) return a.mid < b.mid;
-      return a.idx < b.idx;
-    });
-    const double lmax = all.empty() ? 0 : all[0].l;
-    const double thr = std::max(cutoff * cutoff, floor) * lmax;
-    size_t keep = 0;
-    while (keep < all.size() && long(keep) < max_bond_dim && all[keep].l >= thr && all[keep].l > 0) ++keep;
-    if (keep < all.size() && keep > 0) {
-      size_t k2 = keep;
-      const double smax = std::sqrt(lmax);
-      while (k2 > 0 && std::sqrt(all[k2 - 1].l) - std::sqrt(all[k2].l) <= cfg.deg_tol * smax) --k2;
-      if (k2 > 0) keep = k2;
-    }
-    keep = std::max<size_t>(keep, size_t(std::min<long>(cfg.min_bond, long(all.size()))));
+    std::vector<std::vector<double>> sv(mids.size());
+    for (size_t k = 0; k < mids.size(); ++k) sv[k] = mids[k].S;
+    std::vector<long> kb;
+    long nk = 0, nd = 0;
     double sa = 0, sk = 0;
-    for (size_t k = 0; k < all.size(); ++k) {
-      sa += all[k].l;
-      if (k < keep) {
-        sk += all[k].l;
-        ++mids[all[k].mid].keep;
-      }
-    }
+    select_keep(sv, max_bond_dim, cutoff, cfg.min_bond, cfg.deg_tol, floor, kb, &nk, &nd, &sa, &sk);
+    for (size_t k = 0; k < mids.size(); ++k) mids[k].keep = kb[k];
     *terr = sa > 0 ? (sa - sk) / sa : 0.0;
     const double rn = sk > 0 ? 1.0 / std::sqrt(sk) : 1.0;
     std::vector<std::pair<Q, long>> nb;
@@ -695,6 +768,28 @@ int or_davidson_dense(int32_t n, const double* h, const double* diag, const doub
   if (x_out) std::copy(x.begin(), x.end(), x_out);
   if (n_matvec) *n_matvec = nmv;
   return 0;
+}
+
+int or_truncation_select(int32_t n_blocks, const int64_t* counts, const double* sigma, int64_t max_bond,
+                         double cutoff, int64_t min_bond, double deg_tol, double floor, int64_t* keep,
+                         int64_t* n_kept, int64_t* n_disc, double* terr) {
+  try {
+    std::vector<std::vector<double>> sig(static_cast<size_t>(n_blocks));
+    long at = 0;
+    for (int b = 0; b < n_blocks; ++b)
+      for (long j = 0; j < counts[b]; ++j) sig[size_t(b)].push_back(sigma[at++]);
+    std::vector<long> kb;
+    long nk = 0, nd = 0;
+    double sa = 0, sk = 0;
+    orc::select_keep(sig, max_bond, cutoff, min_bond, deg_tol, floor, kb, &nk, &nd, &sa, &sk);
+    for (int b = 0; b < n_blocks; ++b) keep[b] = kb[size_t(b)];
+    *n_kept = nk;
+    *n_disc = nd;
+    *terr = sa > 0 ? (sa - sk) / sa : 0.0;
+    return 0;
+  } catch (...) {
+    return 1;
+  }
 }
 
 int or_svd(int64_t m, int64_t n, const double* a, double* u, double* s, double* v) {

@@ -3,6 +3,10 @@
 // grouped-contraction kernel and the batched Jacobi eigensolver.
 #include "backend.hpp"
 
+#include <string>
+
+#include "truncate.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -156,73 +160,31 @@ void block_spmv(Context* ctx, Sym sym, const QIndex& rows, const QIndex& cols, Q
 
 SvdOut svd_truncated(Context* ctx, int64_t m, int64_t n, const double* A, int64_t max_rank, double cutoff,
                      double* U, double* Vt) {
-  if (m <= 0 || n <= 0) throw ArgError("svd: empty matrix");
-  const bool rows_side = m < n;  // Gram of the smaller side
-  const int64_t g = rows_side ? m : n;
-  if (g > kSymEigMaxN) throw ArgError("svd: the smaller dimension exceeds the eigensolver limit");
-  DBuf<double> G(ctx, size_t(g * g)), E(ctx, size_t(g * g)), W(ctx, size_t(g));
-  if (rows_side)  // G = A A^T
-    dense_gemm(ctx, m, m, n, 1.0, A, n, 1, A, 1, n, 0.0, G.get(), m, 1);
-  else  // G = A^T A
-    dense_gemm(ctx, n, n, m, 1.0, A, 1, n, A, n, 1, 0.0, G.get(), n, 1);
-  SymEigBatch eig(ctx);
-  eig.values({SymJob{0, 0, int32_t(g)}}, G.get(), g * g, W.get(), g, true);
-  std::vector<double> lam(static_cast<size_t>(g));
-  W.download(lam.data(), lam.size());
-  ctx->sync();
+  // the truncate_svd! rule (faer.rs:135-147): keep while count < max_rank and
+  // sigma >= cutoff * sigma_max; no minimum, no floor, no degeneracy rule
+  TruncRule rule;
+  rule.max_bond = std::max<int64_t>(0, max_rank);
+  rule.cutoff = cutoff;
+  DenseSvd d = dense_svd_truncate(ctx, m, n, A, rule, U, Vt);
   SvdOut out;
-  const double lmax = std::max(0.0, lam[0]);
-  double total = 0;
-  for (double l : lam) total += std::max(0.0, l);
-  int64_t r = 0;
-  // keep while count < max_rank and sigma >= cutoff * sigma_max (faer.rs:135-147);
-  // singular values at the Gram noise floor (sigma^2 <= 1e-15 sigma_max^2) count as zero
-  while (r < g && r < max_rank && lam[r] > 1e-15 * lmax && std::sqrt(lam[r]) >= cutoff * std::sqrt(lmax)) ++r;
-  double kept = 0;
-  for (int64_t j = 0; j < r; ++j) {
-    out.s.push_back(std::sqrt(lam[j]));
-    kept += lam[j];
-  }
-  out.rank = r;
-  out.truncation_error = std::max(0.0, total - kept);  // SvdResult: sum of discarded sigma^2
-  if (r == 0) return out;
-  eig.vectors({r}, E.get(), g * g);
-  // scale columns of the eigenvectors by 1/sigma to form the other factor
-  std::vector<double> inv(static_cast<size_t>(r));
-  for (int64_t j = 0; j < r; ++j) inv[j] = 1.0 / out.s[j];
-  if (rows_side) {
-    // U = E[:, :r] (m x r, row-major, ld g); Vt = diag(1/s) U^T A  (r x n)
-    for (int64_t i = 0; i < m; ++i)
-      TK_HIP(hipMemcpyAsync(U + i * r, E.get() + i * g, size_t(r) * sizeof(double), hipMemcpyDeviceToDevice,
-                            ctx->stream));
-    dense_gemm(ctx, r, n, m, 1.0, U, 1, r, A, n, 1, 0.0, Vt, n, 1);
-    // row j of Vt scaled by 1/s_j: Vt = diag(inv) Vt via a diagonal-matrix product
-    DBuf<double> D(ctx, size_t(r * r)), T(ctx, size_t(r * n));
-    std::vector<double> dh(size_t(r * r), 0.0);
-    for (int64_t j = 0; j < r; ++j) dh[size_t(j * r + j)] = inv[j];
-    D.upload(dh.data(), dh.size());
-    TK_HIP(hipMemcpyAsync(T.get(), Vt, size_t(r * n) * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    dense_gemm(ctx, r, n, r, 1.0, D.get(), r, 1, T.get(), n, 1, 0.0, Vt, n, 1);
-  } else {
-    // Vt = E[:, :r]^T (r x n); U = A E[:, :r] diag(1/s) (m x r)
-    DBuf<double> V(ctx, size_t(n * r));
-    for (int64_t i = 0; i < n; ++i)
-      TK_HIP(hipMemcpyAsync(V.get() + i * r, E.get() + i * g, size_t(r) * sizeof(double), hipMemcpyDeviceToDevice,
-                            ctx->stream));
-    std::vector<double> dh(size_t(r * r), 0.0);
-    for (int64_t j = 0; j < r; ++j) dh[size_t(j * r + j)] = inv[j];
-    DBuf<double> D(ctx, size_t(r * r)), T(ctx, size_t(m * r));
-    D.upload(dh.data(), dh.size());
-    dense_gemm(ctx, m, r, n, 1.0, A, n, 1, V.get(), r, 1, 0.0, T.get(), r, 1);
-    dense_gemm(ctx, m, r, r, 1.0, T.get(), r, 1, D.get(), r, 1, 0.0, U, r, 1);
-    // Vt = I V^T
-    std::vector<double> ih(size_t(r * r), 0.0);
-    for (int64_t j = 0; j < r; ++j) ih[size_t(j * r + j)] = 1.0;
-    DBuf<double> I(ctx, size_t(r * r));
-    I.upload(ih.data(), ih.size());
-    dense_gemm(ctx, r, n, r, 1.0, I.get(), r, 1, V.get(), 1, r, 0.0, Vt, n, 1);
-  }
-  ctx->sync();
+  out.rank = d.rank;
+  out.s = d.s;
+  out.truncation_error = d.sel.abs_error();  // SvdResult: sum of discarded sigma^2
+  return out;
+}
+
+TruncateOut truncate_svd(Context* ctx, int64_t rows, int64_t cols, const double* A, int64_t max_bond_dim,
+                         double svd_cutoff, int64_t min_bond_dim, double* U, double* Vt) {
+  TruncRule rule;
+  rule.max_bond = std::max<int64_t>(0, max_bond_dim);
+  rule.cutoff = svd_cutoff;
+  rule.min_bond = min_bond_dim;
+  DenseSvd d = dense_svd_truncate(ctx, rows, cols, A, rule, U, Vt);
+  TruncateOut out;
+  out.bond_dim_new = d.rank;
+  out.n_discarded = d.sel.discarded;
+  out.s = d.s;
+  out.truncation_error = d.sel.rel_error();
   return out;
 }
 
@@ -237,6 +199,51 @@ std::vector<double> eigh(Context* ctx, int64_t n, const double* A, double* evecs
   W.download(w.data(), w.size());
   ctx->sync();
   return w;
+}
+
+std::vector<double> eigh_lowest(Context* ctx, int64_t n, const double* A, int64_t k, double* evecs) {
+  if (n <= 0 || n > kSymEigMaxN) throw ArgError("eigh: dimension outside 1..2048");
+  if (k < 0 || k > n) throw ArgError("eigh: k = " + std::to_string(k) + " exceeds the dimension n = " +
+                                     std::to_string(n) + " (EighKTooLarge)");
+  if (k == 0) return {};
+  DBuf<double> G(ctx, size_t(n * n)), W(ctx, size_t(n)), Z(ctx, size_t(n * n));
+  TK_HIP(hipMemcpyAsync(G.get(), A, size_t(n * n) * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  SymEigBatch eig(ctx);
+  eig.values({SymJob{0, 0, int32_t(n)}}, G.get(), n * n, W.get(), n, false);
+  eig.vectors({k}, Z.get(), n * n);
+  launch_copies(ctx, {CopyJob{0, 0, int32_t(n), int32_t(k), int32_t(n), int32_t(k), 0, 0}}, Z.get(), n * n, evecs,
+                n * k);
+  std::vector<double> w(static_cast<size_t>(n));
+  W.download(w.data(), w.size());
+  ctx->sync();
+  w.resize(size_t(k));
+  return w;
+}
+
+namespace {
+// Ud (m x k) = U diag(f)
+__global__ void scale_columns_kernel(const double* __restrict__ U, const double* __restrict__ f, int64_t m,
+                                     int64_t k, double* __restrict__ Ud) {
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < m * k; e += int64_t(gridDim.x) * blockDim.x)
+    Ud[e] = U[e] * f[e % k];
+}
+}  // namespace
+
+void regularized_svd_inverse(Context* ctx, int64_t m, int64_t n, int64_t k, const double* s, const double* U,
+                             const double* Vt, double delta, double* out) {
+  if (m <= 0 || n <= 0 || k <= 0) throw ArgError("regularized inverse: empty factors");
+  if (!(delta > 0)) throw ArgError("regularized inverse: delta must be positive");
+  const double d2 = delta * delta;
+  std::vector<double> f(static_cast<size_t>(k));
+  for (int64_t i = 0; i < k; ++i) f[size_t(i)] = s[i] / (s[i] * s[i] + d2);
+  DBuf<double> fd(ctx, size_t(k)), Ud(ctx, size_t(m * k));
+  fd.upload(f.data(), f.size());
+  const int grid = int(std::max<int64_t>(1, std::min<int64_t>((m * k + 255) / 256, int64_t(ctx->n_cu) * 8)));
+  hipLaunchKernelGGL(scale_columns_kernel, dim3(grid), dim3(256), 0, ctx->stream, U, fd.get(), m, k, Ud.get());
+  TK_HIP(hipGetLastError());
+  // out (n x m) = Vt^T (U diag(f))^T
+  dense_gemm(ctx, n, m, k, 1.0, Vt, 1, n, Ud.get(), 1, k, 0.0, out, m, 1);
+  ctx->sync();
 }
 
 }  // namespace tk

@@ -58,13 +58,39 @@ struct SvdOut {
   std::vector<double> s;
   double truncation_error = 0;  // sum of discarded sigma^2 (results.rs:18-23)
 };
-// svd_truncated (traits.rs:38-88 with faer.rs:127-240): row-major m x n A;
-// U (m x rank) and Vt (rank x n) row-major on the device.
+// svd_truncated (traits.rs:38-88 with faer.rs:127-240): row-major m x n A,
+// min(m, n) <= 2048; U (m x rank) and Vt (rank x n) row-major on the device.
+// Every sigma >= cutoff * sigma_max is resolved (two Gram levels).
 SvdOut svd_truncated(Context* ctx, int64_t m, int64_t n, const double* A, int64_t max_rank, double cutoff,
                      double* U, double* Vt);
 
+// truncate_svd (tk-dmrg/src/truncation/mod.rs:56-127): the full SVD with the
+// TruncationConfig rule (max_bond_dim, relative svd_cutoff, min_bond_dim);
+// truncation_error relative (1 - kept / total sigma^2).
+struct TruncateOut {
+  int64_t bond_dim_new = 0, n_discarded = 0;
+  std::vector<double> s;
+  double truncation_error = 0;
+};
+TruncateOut truncate_svd(Context* ctx, int64_t rows, int64_t cols, const double* A, int64_t max_bond_dim,
+                         double svd_cutoff, int64_t min_bond_dim, double* U, double* Vt);
+
+// qr (traits.rs:148; faer.rs:292-324): thin Householder QR of a row-major m x n
+// A: Q (m x k) with orthonormal columns and upper-triangular R (k x n),
+// k = min(m, n), both row-major on the device; A is not modified.
+void qr(Context* ctx, int64_t m, int64_t n, const double* A, double* Q, double* R);
+
+// regularized_svd_inverse (traits.rs:150-183): V diag(s / (s^2 + delta^2)) U^T
+// for U (m x k), Vt (k x n) on the device, s on the host; out is n x m.
+void regularized_svd_inverse(Context* ctx, int64_t m, int64_t n, int64_t k, const double* s, const double* U,
+                             const double* Vt, double delta, double* out);
+
 // All eigenpairs of a symmetric n x n matrix, ascending (eigh_lowest, traits.rs:134-138).
 std::vector<double> eigh(Context* ctx, int64_t n, const double* A, double* evecs);
+// eigh_lowest (traits.rs:134-138, faer.rs:243-290): the k lowest eigenpairs,
+// eigenvalues ascending, eigenvectors as the columns of a row-major n x k;
+// k > n is EighKTooLarge (an ArgError here).
+std::vector<double> eigh_lowest(Context* ctx, int64_t n, const double* A, int64_t k, double* evecs);
 
 // The trait surface as one object (DeviceAPI<HipBackend, HipBackend>).
 class HipBackend {
@@ -79,6 +105,10 @@ class HipBackend {
     return tk::svd_truncated(ctx_, m, n, A, max_rank, cutoff, U, Vt);
   }
   std::vector<double> eigh(int64_t n, const double* A, double* evecs) const { return tk::eigh(ctx_, n, A, evecs); }
+  std::vector<double> eigh_lowest(int64_t n, const double* A, int64_t k, double* evecs) const {
+    return tk::eigh_lowest(ctx_, n, A, k, evecs);
+  }
+  void qr(int64_t m, int64_t n, const double* A, double* Q, double* R) const { tk::qr(ctx_, m, n, A, Q, R); }
   void spmv(Sym sym, const QIndex& rows, const QIndex& cols, QN flux, const double* a, const double* x,
             double* y) const {
     block_spmv(ctx_, sym, rows, cols, flux, a, x, y);

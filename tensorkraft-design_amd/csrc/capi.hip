@@ -13,6 +13,7 @@
 #include "heff.hpp"
 #include "tdvp.hpp"
 #include "sym.hpp"
+#include "truncate.hpp"
 
 struct tk_ctx {
   std::unique_ptr<tk::Context> c;
@@ -53,6 +54,12 @@ tk_status guard(tk_ctx* ctx, F&& f) {
   } catch (const tk::ArgError& e) {
     set_error(ctx, e.what());
     return TK_ERR_INVALID_ARG;
+  } catch (const tk::CancelledError& e) {
+    set_error(ctx, e.what());
+    return TK_ERR_CANCELLED;
+  } catch (const tk::CapacityError& e) {
+    set_error(ctx, e.what());
+    return TK_ERR_CAPACITY;
   } catch (const tk::HipError& e) {
     set_error(ctx, e.what());
     return TK_ERR_HIP;
@@ -260,7 +267,7 @@ tk_status tk_block_table(int32_t sym, int32_t rank, const tk_qindex* legs, const
     tk::BlockTable t = tk::make_block_table(s, ql, dl, fq);
     *n_blocks = t.n_blocks();
     if (total) *total = t.total;
-    if (t.n_blocks() > capacity) throw tk::ArgError("capacity");
+    if (t.n_blocks() > capacity) throw tk::CapacityError("output capacity too small");
     for (int64_t b = 0; b < t.n_blocks(); ++b) {
       if (keys) keys[b] = t.keys[b];
       if (offsets) offsets[b] = t.offset[b];
@@ -337,7 +344,7 @@ tk_status tk_fuse_map(int32_t sym, int32_t n_legs, const tk_qindex* legs, const 
     tk::FuseMap m = tk::make_fuse_map(ql, dl);
     *n_fused = m.fused.n();
     *n_combos = int64_t(m.combo_offset.size());
-    if (m.fused.n() > capacity || int64_t(m.combo_offset.size()) > capacity) throw tk::ArgError("capacity");
+    if (m.fused.n() > capacity || int64_t(m.combo_offset.size()) > capacity) throw tk::CapacityError("output capacity too small");
     for (int i = 0; i < m.fused.n(); ++i) {
       if (fused_qn) {
         fused_qn[2 * i] = m.fused.q[i].a;
@@ -371,7 +378,7 @@ tk_status tk_block_gemm_table(int32_t sym, const tk_block_matrix* a, const tk_bl
                         to_qindex(s, b->rows), to_qindex(s, b->cols), tk::QN{b->flux[0], b->flux[1]});
     *n_blocks = int64_t(p.keys().size());
     if (total) *total = p.total();
-    if (int64_t(p.keys().size()) > capacity) throw tk::ArgError("capacity");
+    if (int64_t(p.keys().size()) > capacity) throw tk::CapacityError("output capacity too small");
     for (size_t i = 0; i < p.keys().size(); ++i) {
       if (keys) keys[i] = p.keys()[i];
       if (offsets) offsets[i] = p.offsets()[i];
@@ -410,11 +417,168 @@ tk_status tk_svd_truncated(tk_ctx* ctx, int64_t m, int64_t n, const double* a, i
   });
 }
 
+tk_status tk_truncate_svd(tk_ctx* ctx, int64_t rows, int64_t cols, const double* a, int64_t max_bond_dim,
+                          double svd_cutoff, int64_t min_bond_dim, double* u, double* s, double* vt,
+                          int64_t* bond_dim_new, int64_t* n_discarded, double* truncation_error) {
+  return guard(ctx, [&] {
+    if (!ctx || !a || !u || !vt || !bond_dim_new) throw tk::ArgError("null argument");
+    tk::TruncateOut o =
+        tk::truncate_svd(ctx->c.get(), rows, cols, a, max_bond_dim, svd_cutoff, min_bond_dim, u, vt);
+    *bond_dim_new = o.bond_dim_new;
+    if (s) std::copy(o.s.begin(), o.s.end(), s);
+    if (n_discarded) *n_discarded = o.n_discarded;
+    if (truncation_error) *truncation_error = o.truncation_error;
+  });
+}
+
+namespace {
+tk::TruncRule make_rule(int64_t max_bond, double cutoff, int64_t min_bond, double degeneracy_tol,
+                        double sigma2_floor) {
+  if (max_bond < 0 || min_bond < 0) throw tk::ArgError("negative bond dimension");
+  if (!(cutoff >= 0) || !(degeneracy_tol >= 0) || !(sigma2_floor >= 0))
+    throw tk::ArgError("cutoff, degeneracy tolerance and floor must be non-negative");
+  tk::TruncRule r;
+  r.max_bond = max_bond;
+  r.cutoff = cutoff;
+  r.min_bond = min_bond;
+  r.degeneracy_tol = degeneracy_tol;
+  r.lambda_floor = sigma2_floor;
+  return r;
+}
+}  // namespace
+
+tk_status tk_truncation_select(int32_t n_blocks, const int64_t* counts, const double* sigma, int64_t max_bond,
+                               double cutoff, int64_t min_bond, double degeneracy_tol, double sigma2_floor,
+                               int64_t* keep, int64_t* n_kept, int64_t* n_discarded, double* truncation_error) {
+  return guard(nullptr, [&] {
+    if (n_blocks < 0 || (n_blocks > 0 && (!counts || !keep))) throw tk::ArgError("null argument");
+    const tk::TruncRule rule = make_rule(max_bond, cutoff, min_bond, degeneracy_tol, sigma2_floor);
+    std::vector<std::vector<double>> lam(static_cast<size_t>(n_blocks));
+    int64_t at = 0;
+    for (int32_t b = 0; b < n_blocks; ++b) {
+      if (counts[b] < 0) throw tk::ArgError("negative count");
+      if (counts[b] > 0 && !sigma) throw tk::ArgError("null argument");
+      for (int64_t j = 0; j < counts[b]; ++j) {
+        const double x = sigma[at + j];
+        if (!(x >= 0) || !std::isfinite(x)) throw tk::ArgError("singular values must be finite and >= 0");
+        if (j > 0 && x > sigma[at + j - 1]) throw tk::ArgError("singular values of a block must descend");
+        lam[size_t(b)].push_back(x * x);
+      }
+      at += counts[b];
+    }
+    tk::TruncSelection sel = tk::select_truncation(lam, rule);
+    for (int32_t b = 0; b < n_blocks; ++b) keep[b] = sel.keep[size_t(b)];
+    if (n_kept) *n_kept = sel.kept;
+    if (n_discarded) *n_discarded = sel.discarded;
+    if (truncation_error) *truncation_error = sel.rel_error();
+  });
+}
+
+tk_status tk_block_svd_truncate(tk_ctx* ctx, int32_t n_blocks, const int64_t* rows, const int64_t* cols,
+                                const double* x, int64_t max_bond, double cutoff, int64_t min_bond,
+                                double degeneracy_tol, double sigma2_floor, int64_t* keep, double* sigma,
+                                int64_t sigma_capacity, double* u, int64_t u_capacity, double* c,
+                                int64_t c_capacity, int64_t* n_kept, int64_t* n_discarded, double* truncation_error) {
+  return guard(ctx, [&] {
+    if (!ctx || n_blocks < 0 || (n_blocks > 0 && (!rows || !cols || !x || !keep)))
+      throw tk::ArgError("null argument");
+    tk::Context* cx = ctx->c.get();
+    const tk::TruncRule rule = make_rule(max_bond, cutoff, min_bond, degeneracy_tol, sigma2_floor);
+    std::vector<tk::GramBlock> gb;
+    int64_t xtot = 0;
+    for (int32_t b = 0; b < n_blocks; ++b) {
+      if (rows[b] < 1 || cols[b] < 1 || rows[b] > tk::kSymEigMaxN || cols[b] > INT32_MAX)
+        throw tk::ArgError("block shape outside 1..2048 rows");
+      gb.push_back({xtot, int32_t(rows[b]), int32_t(cols[b])});
+      xtot += rows[b] * cols[b];
+    }
+    tk::BlockSvd bs(cx);
+    const tk::TruncSelection& sel = bs.run(gb, x, xtot, rule);
+    if (!std::isfinite(sel.lambda_max)) throw tk::ArgError("block svd: the blocks are not finite");
+    int64_t s_need = 0, u_need = 0, c_need = 0;
+    for (int32_t b = 0; b < n_blocks; ++b) {
+      s_need += sel.keep[size_t(b)];
+      u_need += rows[b] * sel.keep[size_t(b)];
+      c_need += sel.keep[size_t(b)] * cols[b];
+    }
+    if ((s_need > 0 && (!sigma || !u || !c)) || s_need > sigma_capacity || u_need > u_capacity ||
+        c_need > c_capacity)
+      throw tk::CapacityError("block svd: output capacity too small");
+    std::vector<tk::CopyJob> uc;
+    tk::ContractList cl;
+    cl.flags = tk::kTransA;
+    int64_t so = 0, uo = 0, co = 0;
+    for (int32_t b = 0; b < n_blocks; ++b) {
+      const int64_t k = sel.keep[size_t(b)], m = rows[b], n = cols[b];
+      keep[b] = k;
+      for (int64_t j = 0; j < k; ++j) sigma[so + j] = std::sqrt(std::max(0.0, bs.lambda()[size_t(b)][size_t(j)]));
+      if (k > 0) {
+        uc.push_back({bs.u_offset(size_t(b)), uo, int32_t(m), int32_t(k), int32_t(m), int32_t(k), 0, 0});
+        // C_b = U_b^T X_b: the kept columns of the n_b x n_b vector matrix, transposed
+        cl.begin_out(co, int(k), int(n), int(n));
+        cl.begin_term(gb[size_t(b)].x_off, int(n), int(m));
+        cl.add_src(bs.u_offset(size_t(b)), int(m), 1.0);
+      }
+      so += k;
+      uo += m * k;
+      co += k * n;
+    }
+    if (!uc.empty() && sel.lambda_max > 0) {
+      tk::launch_copies(cx, uc, bs.u(), bs.u_size(), u, u_need);
+      cl.finalize();
+      cl.validate(bs.u_size(), xtot, c_need);
+      tk::DeviceContractList d;
+      d.upload(cx, cl);
+      tk::launch_contract(cx, d, bs.u(), x, c);
+      cx->sync();
+    } else if (!uc.empty()) {
+      throw tk::ArgError("block svd: all blocks are zero");
+    }
+    if (n_kept) *n_kept = sel.kept;
+    if (n_discarded) *n_discarded = sel.discarded;
+    if (truncation_error) *truncation_error = sel.rel_error();
+  });
+}
+
 tk_status tk_eigh(tk_ctx* ctx, int64_t n, const double* a, double* evals, double* evecs) {
   return guard(ctx, [&] {
     if (!ctx || !a || !evecs) throw tk::ArgError("null argument");
     std::vector<double> w = tk::eigh(ctx->c.get(), n, a, evecs);
     if (evals) std::copy(w.begin(), w.end(), evals);
+  });
+}
+
+tk_status tk_eigh_lowest(tk_ctx* ctx, int64_t n, const double* a, int64_t k, double* evals, double* evecs) {
+  return guard(ctx, [&] {
+    if (!ctx || !a || (k > 0 && !evecs)) throw tk::ArgError("null argument");
+    std::vector<double> w = tk::eigh_lowest(ctx->c.get(), n, a, k, evecs);
+    if (evals) std::copy(w.begin(), w.end(), evals);
+  });
+}
+
+tk_status tk_qr(tk_ctx* ctx, int64_t m, int64_t n, const double* a, double* q, double* r) {
+  return guard(ctx, [&] {
+    if (!ctx || !a || !q || !r) throw tk::ArgError("null argument");
+    tk::qr(ctx->c.get(), m, n, a, q, r);
+  });
+}
+
+tk_status tk_regularized_svd_inverse(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, const double* s, const double* u,
+                                     const double* vt, double delta, double* out) {
+  return guard(ctx, [&] {
+    if (!ctx || !s || !u || !vt || !out) throw tk::ArgError("null argument");
+    tk::regularized_svd_inverse(ctx->c.get(), m, n, k, s, u, vt, delta, out);
+  });
+}
+
+tk_status tk_bond_schedule_warmup(int64_t d_init, int64_t d_max, int64_t n_warmup_sweeps, int64_t capacity,
+                                  int64_t* dims, int64_t* n_dims) {
+  return guard(nullptr, [&] {
+    if (!dims || !n_dims) throw tk::ArgError("null argument");
+    tk::BondSchedule b = tk::BondSchedule::warmup(d_init, d_max, n_warmup_sweeps);
+    *n_dims = int64_t(b.dims.size());
+    if (int64_t(b.dims.size()) > capacity) throw tk::CapacityError("bond schedule: capacity too small");
+    std::copy(b.dims.begin(), b.dims.end(), dims);
   });
 }
 
@@ -504,7 +668,7 @@ tk_status tk_heff_psi_table(tk_heff_plan* plan, int64_t capacity, int64_t* n_blo
     const tk::BlockTable& t = plan->p->psi();
     *n_blocks = t.n_blocks();
     if (total) *total = t.total;
-    if (t.n_blocks() > capacity) throw tk::ArgError("capacity");
+    if (t.n_blocks() > capacity) throw tk::CapacityError("output capacity too small");
     for (int64_t b = 0; b < t.n_blocks(); ++b) {
       if (keys) keys[b] = t.keys[b];
       if (offsets) offsets[b] = t.offset[b];
@@ -704,6 +868,13 @@ tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out) {
   });
 }
 
+tk_status tk_dmrg_set_cancel_flag(tk_dmrg* d, const int32_t* flag) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d) throw tk::ArgError("null argument");
+    d->e->set_cancel_flag(flag);
+  });
+}
+
 tk_status tk_dmrg_apply_site_op(tk_dmrg* d, const double* op, const int32_t* dq, double* norm2) {
   return guard(d ? d->ctx : nullptr, [&] {
     if (!d || !op || !dq) throw tk::ArgError("null argument");
@@ -749,7 +920,7 @@ tk_status tk_dmrg_steps(tk_dmrg* d, int32_t capacity, int32_t* n, tk_step_stats*
     if (!d || !n) throw tk::ArgError("null argument");
     const auto& st = d->e->last_steps();
     *n = int32_t(st.size());
-    if (int32_t(st.size()) > capacity) throw tk::ArgError("capacity");
+    if (int32_t(st.size()) > capacity) throw tk::CapacityError("output capacity too small");
     for (size_t k = 0; k < st.size(); ++k) {
       out[k].site = st[k].site;
       out[k].left_to_right = st[k].left_to_right;
@@ -774,7 +945,7 @@ tk_status tk_dmrg_bond(tk_dmrg* d, int32_t bond, int32_t capacity, int32_t* n_se
     if (bond < 0 || bond > d->e->n_sites()) throw tk::ArgError("bond out of range");
     const tk::QIndex& q = d->e->bonds()[bond];
     *n_sectors = q.n();
-    if (q.n() > capacity) throw tk::ArgError("capacity");
+    if (q.n() > capacity) throw tk::CapacityError("output capacity too small");
     for (int i = 0; i < q.n(); ++i) {
       if (qn) {
         qn[2 * i] = q.q[i].a;
@@ -791,7 +962,7 @@ tk_status tk_dmrg_site(tk_dmrg* d, int32_t site, int64_t capacity, int64_t* n, d
     if (site < 0 || site >= d->e->n_sites()) throw tk::ArgError("site out of range");
     std::vector<double> h = d->e->site_data(site);
     *n = int64_t(h.size());
-    if (int64_t(h.size()) > capacity) throw tk::ArgError("capacity");
+    if (int64_t(h.size()) > capacity) throw tk::CapacityError("output capacity too small");
     if (data) std::copy(h.begin(), h.end(), data);
   });
 }

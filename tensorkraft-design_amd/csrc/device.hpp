@@ -19,6 +19,14 @@ struct HipError : std::runtime_error {
 struct ArgError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+// A caller-provided output array is too small (TK_ERR_CAPACITY).
+struct CapacityError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+// DmrgError::Cancelled (tk-dmrg/src/error.rs:53): a cancel flag was raised.
+struct CancelledError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 
 #define TK_HIP(call)                                                                      \
   do {                                                                                    \

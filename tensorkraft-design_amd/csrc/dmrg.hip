@@ -1,5 +1,6 @@
 // Two-site DMRG engine (see dmrg.hpp).
 #include "dmrg.hpp"
+#include "truncate.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -173,13 +174,28 @@ void DmrgEngine::centre_to_site0() {
   ctx_->sync();
 }
 
+void DmrgEngine::check_cancel() {
+  if (cancel_ && __atomic_load_n(cancel_, __ATOMIC_RELAXED) != 0) {
+    cur_ = nullptr;
+    centre_ = -1;  // prepare() again before sweeping
+    throw CancelledError("computation cancelled");
+  }
+}
+
 SweepStats DmrgEngine::sweep() {
+  if (centre_ < 0) throw ArgError("sweep: prepare the state first");
   SweepStats s;
   cur_ = &s;
   steps_.clear();
   const double t0 = now();
-  for (int i = 0; i + 1 < n_; ++i) steps_.push_back(step(i, true));
-  for (int i = n_ - 1; i >= 1; --i) steps_.push_back(step(i, false));
+  for (int i = 0; i + 1 < n_; ++i) {
+    check_cancel();
+    steps_.push_back(step(i, true));
+  }
+  for (int i = n_ - 1; i >= 1; --i) {
+    check_cancel();
+    steps_.push_back(step(i, false));
+  }
   ctx_->sync();
   s.seconds = now() - t0;
   for (const StepStats& st : steps_) {
@@ -360,8 +376,14 @@ SweepStats DmrgEngine::sweep_single_site() {
   steps_.clear();
   const double t0 = now();
   centre_to_site0();
-  for (int i = 0; i + 1 < n_; ++i) steps_.push_back(step_single_site(i, true));
-  for (int i = n_ - 1; i >= 1; --i) steps_.push_back(step_single_site(i, false));
+  for (int i = 0; i + 1 < n_; ++i) {
+    check_cancel();
+    steps_.push_back(step_single_site(i, true));
+  }
+  for (int i = n_ - 1; i >= 1; --i) {
+    check_cancel();
+    steps_.push_back(step_single_site(i, false));
+  }
   ctx_->sync();
   s.seconds = now() - t0;
   for (const StepStats& st : steps_) {
@@ -738,66 +760,31 @@ void DmrgEngine::split_two_site(int i, bool lr, const BlockTable& psi, const dou
     }
     launch_copies(ctx_, xc, x, psi.total, X.get(), xtot);
   }
-  // squared singular values of every block (descending), two-level Gram
+  // squared singular values of every block (descending), two-level Gram,
+  // the global selection of truncate_svd (truncation/mod.rs:72-82) with the
+  // degeneracy and numerical-zero rules, then the kept vectors
   std::vector<GramBlock> gb;
   for (size_t k = 0; k < mids.size(); ++k)
     gb.push_back({x_off[k], int32_t(mids[k].n), int32_t(lr ? mids[k].ncol : mids[k].nrow)});
-  GramSvd gs(ctx_);
-  const double lam_min_rel = std::max(cutoff * cutoff, lambda_floor);
-  const int ph_values = ctx_->timer.begin(ctx_->stream);
-  const std::vector<std::vector<double>> lam = gs.values(gb, X.get(), xtot, lam_min_rel, max_bond);
-  ctx_->timer.end(ph_values, ctx_->stream, "phase_svd_values", 0.0, 0.0, 1);
-  for (size_t k = 0; k < mids.size(); ++k) mids[k].g_off = gs.u_offset(k);
-  const int64_t gtot = gs.u_size();
-  // global selection: sigma descending, keep while count < max_bond and
-  // sigma >= cutoff * sigma_max (truncation/mod.rs:72-82), numerical-zero floor
-  // on sigma^2, no cut inside a group |sigma_k - sigma_k+1| <= tol * sigma_max,
-  // min_bond_dim floor
-  struct Cand {
-    double l;
-    int mid;
-    int idx;
-  };
-  std::vector<Cand> all;
-  for (int k = 0; k < int(mids.size()); ++k)
-    for (size_t j = 0; j < lam[k].size(); ++j) all.push_back({std::max(0.0, lam[k][j]), k, int(j)});
-  std::stable_sort(all.begin(), all.end(), [](const Cand& a, const Cand& b) {
-    if (a.l != b.l) return a.l > b.l;
-    if (a.mid != b.mid) return a.mid < b.mid;
-    return a.idx < b.idx;
-  });
-  const double lmax = all.empty() ? 0.0 : all[0].l;
-  if (!(lmax > 0) || !std::isfinite(lmax))
+  TruncRule rule;
+  rule.max_bond = max_bond;
+  rule.cutoff = cutoff;
+  rule.min_bond = cfg_.min_bond_dim;
+  rule.degeneracy_tol = cfg_.degeneracy_tol;
+  rule.lambda_floor = lambda_floor;
+  BlockSvd bs(ctx_);
+  const TruncSelection& sel = bs.run(gb, X.get(), xtot, rule);
+  if (!(sel.lambda_max > 0) || !std::isfinite(sel.lambda_max))
     throw ArgError("split: the two-site tensor is zero or not finite (zero-norm MPS?)");
-  const double thr = lam_min_rel * lmax;
-  size_t keep = 0;
-  while (keep < all.size() && int64_t(keep) < max_bond && all[keep].l >= thr && all[keep].l > 0) ++keep;
-  if (keep < all.size() && keep > 0) {
-    const double smax = std::sqrt(lmax);
-    size_t k2 = keep;
-    while (k2 > 0 && std::sqrt(all[k2 - 1].l) - std::sqrt(all[k2].l) <= cfg_.degeneracy_tol * smax) --k2;
-    if (k2 > 0) keep = k2;
+  for (size_t k = 0; k < mids.size(); ++k) {
+    mids[k].g_off = bs.u_offset(k);
+    mids[k].keep = sel.keep[k];
   }
-  keep = std::max<size_t>(keep, size_t(std::min<int64_t>(cfg_.min_bond_dim, int64_t(all.size()))));
-  double sum_all = 0, sum_kept = 0;
-  for (size_t k = 0; k < all.size(); ++k) {
-    sum_all += all[k].l;
-    if (k < keep) {
-      sum_kept += all[k].l;
-      ++mids[all[k].mid].keep;
-    }
-  }
-  st->truncation_error = sum_all > 0 ? (sum_all - sum_kept) / sum_all : 0.0;
-  const double renorm = sum_kept > 0 ? 1.0 / std::sqrt(sum_kept) : 1.0;
-  {
-    std::vector<int64_t> keep_of;
-    for (const Mid& m : mids) keep_of.push_back(m.keep);
-    const int ph_vec = ctx_->timer.begin(ctx_->stream);
-    gs.vectors(keep_of);
-    ctx_->timer.end(ph_vec, ctx_->stream, "phase_svd_vectors", 0.0, 0.0, 1);
-  }
+  const int64_t gtot = bs.u_size();
+  st->truncation_error = sel.rel_error();
+  const double renorm = sel.sum_kept > 0 ? 1.0 / std::sqrt(sel.sum_kept) : 1.0;
   const int ph_write = ctx_->timer.begin(ctx_->stream);
-  const double* E = gs.u();
+  const double* E = bs.u();
   // new bond
   std::vector<std::pair<QN, int64_t>> nb;
   std::map<QN, int> mid_of;

@@ -91,6 +91,11 @@ class DmrgEngine {
   // Right-canonicalise (centre at site 0, norm 1) and build the right environments.
   void prepare();
   SweepStats sweep();  // one left-to-right + right-to-left sweep
+  // run_with_cancel_flag (sweep/mod.rs:274-291): while *flag reads non-zero,
+  // sweep() and sweep_single_site() throw CancelledError before their next
+  // step (null clears the flag).  The state stays a valid MPS; prepare()
+  // again before the next sweep.
+  void set_cancel_flag(const int32_t* flag) { cancel_ = flag; }
   double energy() const { return energy_; }
   int sweeps_done() const { return sweeps_; }
   const std::vector<QIndex>& bonds() const { return bonds_; }
@@ -165,6 +170,8 @@ class DmrgEngine {
   int centre_ = -1;  // site of the orthogonality centre (-1 before prepare)
   std::vector<StepStats> steps_;
   SweepStats* cur_ = nullptr;
+  const int32_t* cancel_ = nullptr;
+  void check_cancel();
 };
 
 // <bra|ket> of two MPS over the same physical legs: the transfer matrix

@@ -1029,7 +1029,9 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
         c0 = dd - m * c1;
         c1 = en[u];
       } else {
-        const double m = fast_quot(c0, ei[u]);
+        // |c0| < |e_i|; below the normal range the hardware reciprocal of e_i
+        // overflows, so that rare case takes the IEEE quotient (|m| < 1)
+        const double m = fabs(ei[u]) >= DBL_MIN ? fast_quot(c0, ei[u]) : c0 / ei[u];
         F(i, 0) = ei[u];
         F(i, 1) = dd;
         F(i, 2) = en[u];

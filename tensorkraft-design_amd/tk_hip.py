@@ -22,6 +22,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtk_hip.so")
 
 TK_OK = 0
+TK_ERR_INVALID_ARG = 1
+TK_ERR_CANCELLED = 5
+TK_ERR_CAPACITY = 7
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 SYM_U1, SYM_U1U1 = 1, 2
 DIR_IN, DIR_OUT = 0, 1
@@ -132,6 +135,15 @@ def lib():
         "tk_spmv": (I32, [P, I32, C.POINTER(tk_block_matrix), P, P]),
         "tk_svd_truncated": (I32, [P, I64, I64, P, I64, D, P, pD, P, pI64, pD]),
         "tk_eigh": (I32, [P, I64, P, pD, P]),
+        "tk_truncate_svd": (I32, [P, I64, I64, P, I64, D, I64, P, pD, P, pI64, pI64, pD]),
+        "tk_truncation_select": (I32, [I32, pI64, pD, I64, D, I64, D, D, pI64, pI64, pI64, pD]),
+        "tk_block_svd_truncate": (I32, [P, I32, pI64, pI64, P, I64, D, I64, D, D, pI64, pD, I64, P, I64, P, I64,
+                                        pI64, pI64, pD]),
+        "tk_eigh_lowest": (I32, [P, I64, P, I64, pD, P]),
+        "tk_qr": (I32, [P, I64, I64, P, P, P]),
+        "tk_regularized_svd_inverse": (I32, [P, I64, I64, I64, pD, P, P, D, P]),
+        "tk_bond_schedule_warmup": (I32, [I64, I64, I64, I64, pI64, pI64]),
+        "tk_dmrg_set_cancel_flag": (I32, [P, pI32]),
         "tk_davidson": (I32, [P, I64, MATVEC_FN, P, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_lanczos": (I32, [P, I64, MATVEC_FN, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_exp_krylov": (I32, [P, I64, MATVEC_FN, P, P, D, I32, D, D, P, pI32]),
@@ -182,13 +194,67 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time",
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
             "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_block_gemm_table", "tk_block_gemm",
-            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
+            "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_truncate_svd", "tk_truncation_select",
+            "tk_block_svd_truncate", "tk_eigh_lowest", "tk_qr", "tk_regularized_svd_inverse",
+            "tk_bond_schedule_warmup", "tk_dmrg_set_cancel_flag", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
             "tk_dmrg_sweep_single_site", "tk_dmrg_apply_site_op", "tk_dmrg_overlap",
             "tk_ctx_timing",
             "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split", "tk_hybridization",
             "tk_bath_fit", "tk_lanczos_tridiagonalize", "tk_broyden_create", "tk_broyden_update",
             "tk_broyden_destroy", "tk_krylov_tridiag", "tk_mlp_create", "tk_mlp_destroy", "tk_mlp_n_params",
             "tk_mlp_get_params", "tk_mlp_set_params", "tk_mlp_forward", "tk_mlp_loss_grad", "tk_mlp_adam"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def truncation_select(sigmas, max_bond, cutoff, min_bond=0, degeneracy_tol=0.0, sigma2_floor=0.0):
+    """The library's truncation rule (tk_truncation_select; truncate_svd's
+    selection, truncation/mod.rs:72-98, over several blocks): (keep per
+    block, n_kept, n_discarded, relative truncation error).  Host only."""
+    counts = np.array([len(x) for x in sigmas], dtype=np.int64)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in sigmas])
+                                if len(sigmas) else np.zeros(1))
+    keep = np.zeros(max(1, len(sigmas)), dtype=np.int64)
+    nk, nd, te = C.c_int64(0), C.c_int64(0), C.c_double(0)
+    _check(lib().tk_truncation_select(len(sigmas), _ip(counts), _dp(flat), int(max_bond), float(cutoff),
+                                      int(min_bond), float(degeneracy_tol), float(sigma2_floor), _ip(keep),
+                                      C.byref(nk), C.byref(nd), C.byref(te)))
+    return keep[:len(sigmas)].tolist(), nk.value, nd.value, te.value
+
+
+class BondDimensionSchedule:
+    """BondDimensionSchedule (tk-dmrg/src/truncation/mod.rs:131-172); warmup
+    comes from the library (tk_bond_schedule_warmup)."""
+
+    def __init__(self, dims):
+        self.dims = [int(d) for d in dims]
+
+    @classmethod
+    def warmup(cls, d_init, d_max, n_warmup_sweeps):
+        out = np.zeros(int(n_warmup_sweeps) + 1, dtype=np.int64)
+        n = C.c_int64(0)
+        _check(lib().tk_bond_schedule_warmup(int(d_init), int(d_max), int(n_warmup_sweeps), len(out), _ip(out),
+                                             C.byref(n)))
+        return cls(out[:n.value].tolist())
+
+    @classmethod
+    def fixed(cls, d):
+        return cls([d])
+
+    @classmethod
+    def custom(cls, dims):
+        return cls(dims)
+
+    def bond_dim_at_sweep(self, sweep_index):
+        if 0 <= sweep_index < len(self.dims):
+            return self.dims[sweep_index]
+        return self.dims[-1] if self.dims else 1
 
 
 def _check(st, ctx=None):
@@ -243,7 +309,7 @@ def block_table(sym, legs, dirs, flux=(0, 0)):
     L = lib()
     st = L.tk_block_table(sym, len(qs), arr, d.ctypes.data_as(C.POINTER(C.c_int32)),
                           f.ctypes.data_as(C.POINTER(C.c_int32)), 0, C.byref(n), None, None, None, C.byref(tot))
-    if st not in (TK_OK, 1):
+    if st not in (TK_OK, TK_ERR_CAPACITY):
         _check(st)
     cap = n.value
     keys = np.zeros(cap, dtype=np.uint64)
@@ -469,6 +535,100 @@ class HipBackend:
         for x in (da, du, dv):
             x.free()
         return u, s[:k], vt, te.value
+
+    def truncate_svd(self, a, max_bond_dim=200, svd_cutoff=1e-12, min_bond_dim=1):
+        """truncate_svd (tk-dmrg/src/truncation/mod.rs:56-127) with a
+        TruncationConfig: (U, s, Vt, bond_dim_new, n_discarded, relative
+        truncation error)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        m, n = a.shape
+        r = min(m, n)
+        da = self.ctx.to_device(a)
+        du, dv = self.ctx.empty(m * r), self.ctx.empty(r * n)
+        s = np.zeros(r)
+        bd, nd, te = C.c_int64(0), C.c_int64(0), C.c_double(0)
+        _check(self.ctx.L.tk_truncate_svd(self.ctx.h, m, n, C.c_void_p(da.ptr), int(max_bond_dim), float(svd_cutoff),
+                                          int(min_bond_dim), C.c_void_p(du.ptr), _dp(s), C.c_void_p(dv.ptr),
+                                          C.byref(bd), C.byref(nd), C.byref(te)), self.ctx.h)
+        k = bd.value
+        u = du.get()[:m * k].reshape(m, k)
+        vt = dv.get()[:k * n].reshape(k, n)
+        for x in (da, du, dv):
+            x.free()
+        return u, s[:k], vt, k, nd.value, te.value
+
+    def block_svd_truncate(self, blocks, max_bond, cutoff, min_bond=1, degeneracy_tol=1e-12, sigma2_floor=1e-26):
+        """The two-site split's block-diagonal truncated SVD: per block (U_b,
+        sigma_b, C_b = U_b^T X_b) and (n_kept, n_discarded, relative error)."""
+        blocks = [np.ascontiguousarray(b, dtype=np.float64) for b in blocks]
+        rows = np.array([b.shape[0] for b in blocks], dtype=np.int64)
+        cols = np.array([b.shape[1] for b in blocks], dtype=np.int64)
+        flat = np.concatenate([b.ravel() for b in blocks]) if blocks else np.zeros(1)
+        cap_s = int(sum(min(r, c) for r, c in zip(rows, cols)))
+        cap_u = int(sum(r * min(r, c) for r, c in zip(rows, cols)))
+        cap_c = int(sum(c * min(r, c) for r, c in zip(rows, cols)))
+        dx = self.ctx.to_device(flat)
+        du, dc = self.ctx.empty(max(cap_u, 1)), self.ctx.empty(max(cap_c, 1))
+        keep = np.zeros(max(1, len(blocks)), dtype=np.int64)
+        sig = np.zeros(max(1, cap_s))
+        nk, nd, te = C.c_int64(0), C.c_int64(0), C.c_double(0)
+        _check(self.ctx.L.tk_block_svd_truncate(self.ctx.h, len(blocks), _ip(rows), _ip(cols), C.c_void_p(dx.ptr),
+                                                int(max_bond), float(cutoff), int(min_bond), float(degeneracy_tol),
+                                                float(sigma2_floor), _ip(keep), _dp(sig), cap_s, C.c_void_p(du.ptr),
+                                                cap_u, C.c_void_p(dc.ptr), cap_c, C.byref(nk), C.byref(nd),
+                                                C.byref(te)), self.ctx.h)
+        uh, ch = du.get(), dc.get()
+        out, so, uo, co = [], 0, 0, 0
+        for b in range(len(blocks)):
+            k, m, n = int(keep[b]), int(rows[b]), int(cols[b])
+            out.append((uh[uo:uo + m * k].reshape(m, k), sig[so:so + k].copy(), ch[co:co + k * n].reshape(k, n)))
+            so, uo, co = so + k, uo + m * k, co + k * n
+        for x in (dx, du, dc):
+            x.free()
+        return out, nk.value, nd.value, te.value
+
+    def eigh_lowest(self, a, k):
+        """The k lowest eigenpairs (eigh_lowest, traits.rs:134-138): (values, n x k vectors)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = a.shape[0]
+        da = self.ctx.to_device(a)
+        dv = self.ctx.empty(max(1, n * k))
+        w = np.zeros(max(1, k))
+        _check(self.ctx.L.tk_eigh_lowest(self.ctx.h, n, C.c_void_p(da.ptr), int(k), _dp(w), C.c_void_p(dv.ptr)),
+               self.ctx.h)
+        v = dv.get()[:n * k].reshape(n, k)
+        da.free()
+        dv.free()
+        return w[:k], v
+
+    def qr(self, a):
+        """Thin QR (qr, traits.rs:148): (Q m x k, R k x n), k = min(m, n)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        m, n = a.shape
+        k = min(m, n)
+        da = self.ctx.to_device(a)
+        dq, dr = self.ctx.empty(m * k), self.ctx.empty(k * n)
+        _check(self.ctx.L.tk_qr(self.ctx.h, m, n, C.c_void_p(da.ptr), C.c_void_p(dq.ptr), C.c_void_p(dr.ptr)),
+               self.ctx.h)
+        q, r = dq.get().reshape(m, k), dr.get().reshape(k, n)
+        for x in (da, dq, dr):
+            x.free()
+        return q, r
+
+    def regularized_svd_inverse(self, s, u, vt, delta):
+        """V diag(s / (s^2 + delta^2)) U^T (regularized_svd_inverse, traits.rs:150-183)."""
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        u, vt = np.ascontiguousarray(u, dtype=np.float64), np.ascontiguousarray(vt, dtype=np.float64)
+        m, k = u.shape
+        n = vt.shape[1]
+        du, dv, do = self.ctx.to_device(u), self.ctx.to_device(vt), self.ctx.empty(n * m)
+        _check(self.ctx.L.tk_regularized_svd_inverse(self.ctx.h, m, n, k, _dp(s), C.c_void_p(du.ptr),
+                                                     C.c_void_p(dv.ptr), float(delta), C.c_void_p(do.ptr)),
+               self.ctx.h)
+        out = do.get().reshape(n, m)
+        for x in (du, dv, do):
+            x.free()
+        return out
 
     def eigh(self, a):
         a = np.ascontiguousarray(a, dtype=np.float64)
@@ -1324,7 +1484,16 @@ class Dmrg:
         h = C.c_void_p()
         _check(ctx.L.tk_dmrg_create(ctx.h, C.byref(cfg), arr, C.byref(h)), ctx.h)
         self.h = h
+        # the cancel flag of run_with_cancel_flag (sweep/mod.rs:274-291): the
+        # library reads this int32 before every step
+        self.cancel_flag = np.zeros(1, dtype=np.int32)
+        _check(ctx.L.tk_dmrg_set_cancel_flag(self.h, self.cancel_flag.ctypes.data_as(C.POINTER(C.c_int32))), ctx.h)
         ctx.adopt(self)
+
+    def cancel(self, on=True):
+        """Raise (or clear) the cancel flag; a running sweep stops before its
+        next step with TK_ERR_CANCELLED.  Safe from another thread."""
+        self.cancel_flag[0] = 1 if on else 0
 
     def close(self):
         if self.h and self.ctx.h:

@@ -64,15 +64,6 @@ def test_svd_faer_matrix():
     assert np.abs(u @ np.diag(s) @ v.T - a).max() < 1e-12
 
 
-def test_truncation_rule_known_answers():
-    # truncation/mod.rs:185-236: sigma (3, 2) kept with error 1/14 when the third
-    # (sigma 1) is dropped at max_bond_dim 2: (1^2) / (9 + 4 + 1)
-    s = np.array([3.0, 2.0, 1.0])
-    lam = s ** 2
-    keep = 2
-    assert abs((lam.sum() - lam[:keep].sum()) / lam.sum() - 1.0 / 14.0) < 1e-15
-
-
 def _run_oracle(kind, n, sched, sweeps, total, d0=16, seed=0, tol=1e-12, **mp):
     mpo = tk_hip.chain_mpo(kind, n, **mp)
     sym = tk_hip.SYM_U1 if kind == "heisenberg" else tk_hip.SYM_U1U1
