@@ -139,11 +139,18 @@ def test_mlp_forward_gradient_adam_vs_oracle(ctx):
     loss_ref, g_ref = O.mlp_loss_grad(p, dims, x.astype(np.float64), t.astype(np.float64))
     assert loss == pytest.approx(loss_ref, rel=1e-5)
     assert np.max(np.abs(g - g_ref)) <= 1e-4 * np.max(np.abs(g_ref))
-    # one Adam step from zero moments
+    # one Adam step from zero moments, then a second with a new gradient (the
+    # oracle's Adam is pinned by the two-step closed form in test_oracle_dmft)
     net.adam(1e-3)
     p1 = net.params().astype(np.float64)
-    p_ref, _, _ = O.adam(p, g.astype(np.float64), np.zeros_like(p), np.zeros_like(p), 1, 1e-3)
+    p_ref, m_ref, v_ref = O.adam(p, g.astype(np.float64), np.zeros_like(p), np.zeros_like(p), 1, 1e-3)
     assert np.max(np.abs(p1 - p_ref)) <= 1e-6
+    x2 = rng.normal(size=(16, 512)).astype(np.float32)
+    _, g2 = net.loss_grad(x2, t)
+    net.adam(1e-3)
+    p2 = net.params().astype(np.float64)
+    p_ref2, _, _ = O.adam(p1, g2.astype(np.float64), m_ref, v_ref, 2, 1e-3)
+    assert np.max(np.abs(p2 - p_ref2)) <= 1e-6
     # training lowers the loss
     hist = net.train(x, t, epochs=60, lr=1e-3)
     assert hist[-1] < 0.2 * hist[0]

@@ -129,3 +129,29 @@ def test_mlp_gradient_finite_difference():
         e[i] = h
         fd = (O.mlp_loss_grad(p + e, dims, x, t)[0] - O.mlp_loss_grad(p - e, dims, x, t)[0]) / (2 * h)
         assert abs(fd - g[i]) < 1e-8 * max(1.0, abs(fd))
+
+
+def test_adam_two_steps_closed_form():
+    """The oracle's Adam (Kingma & Ba, bias-corrected) against the two-step
+    closed form: from zero moments with gradients g1 then g2,
+      step 1: m^ = g1, v^ = g1^2, p1 = p0 - lr g1 / (|g1| + eps)
+      step 2: m^ = (b1 g1 + g2) / (1 + b1), v^ = (b2 g1^2 + g2^2) / (1 + b2),
+              p2 = p1 - lr m^ / (sqrt(v^) + eps)
+    (the bias corrections 1 - b^t with the moments' geometric sums); a zero
+    gradient leaves its parameter, eps dominates a tiny one."""
+    lr, b1, b2, eps = 0.1, 0.9, 0.999, 1e-8
+    p0 = np.array([1.0, -2.0, 0.5, 3.0])
+    g1 = np.array([0.5, -0.25, 0.0, 1e-12])
+    g2 = np.array([-0.25, 2.0, 0.0, 1e-12])
+    p, m, v = O.adam(p0, g1, np.zeros(4), np.zeros(4), 1, lr, b1, b2, eps)
+    want1 = p0 - lr * g1 / (np.abs(g1) + eps)
+    assert np.abs(p - want1).max() < 1e-15
+    assert abs(want1[0] - (1.0 - 0.1 * 0.5 / (0.5 + 1e-8))) < 1e-16 and want1[2] == 0.5
+    p, m, v = O.adam(p, g2, m, v, 2, lr, b1, b2, eps)
+    mh = (b1 * g1 + g2) / (1 + b1)
+    vh = (b2 * g1 ** 2 + g2 ** 2) / (1 + b2)
+    want2 = want1 - lr * mh / (np.sqrt(vh) + eps)
+    assert np.abs(p - want2).max() < 1e-14
+    assert p[2] == 0.5
+    # a gradient far below eps moves its parameter by ~lr |g| / eps
+    assert abs((p0[3] - p[3]) - lr * 1e-12 / (1e-12 + eps) * 2) < 1e-6 * lr
