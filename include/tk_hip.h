@@ -163,6 +163,18 @@ tk_status tk_gemm(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha, co
                   int64_t cs_a, const double* b_dev, int64_t rs_b, int64_t cs_b, double beta, double* c_dev,
                   int64_t rs_c, int64_t cs_c);
 
+/* No counterpart in the reference: tk_gemm for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops).  Same
+ * arguments, results bit-identical to tk_gemm's; the call's work list (16
+ * bytes per 32 x 32 output tile) lives in a static 4 MiB device array per
+ * GPU instead of a per-call allocation, so outputs above about 262,000
+ * tiles are rejected with TK_ERR_INVALID_ARG.  Calls that overlap on one GPU
+ * (contexts of the same device on different threads) race on that array.
+ * Synchronous. */
+tk_status tk_gemm_noalloc(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha, const double* a_dev,
+                          int64_t rs_a, int64_t cs_a, const double* b_dev, int64_t rs_b, int64_t cs_b, double beta,
+                          double* c_dev, int64_t rs_c, int64_t cs_c);
+
 /* Rank-2 block-sparse matrix, legs [Incoming, Outgoing] (block_gemm_realistic.rs:26). */
 typedef struct {
   tk_qindex rows;

@@ -44,37 +44,45 @@ void run(Context* ctx, ContractList& cl, int64_t a_n, int64_t b_n, int64_t c_n, 
 
 }  // namespace
 
-void dense_gemm(Context* ctx, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t rsA,
-                int64_t csA, const double* B, int64_t rsB, int64_t csB, double beta, double* C, int64_t rsC,
-                int64_t csC) {
+bool dense_gemm_list(int64_t m, int64_t n, int64_t k, double alpha, int64_t rsA, int64_t csA, int64_t rsB,
+                     int64_t csB, double beta, int64_t rsC, int64_t csC, ContractList* cl) {
   if (m < 0 || n < 0 || k < 0) throw ArgError("gemm: negative dimension");
-  if (m == 0 || n == 0) return;
+  if (m == 0 || n == 0) return false;
+  if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) throw ArgError("gemm: dimension above 2^31 - 1");
   int64_t lda, ldb, ldc;
   bool ta, tb, tc;
   if (!as_slice(m, k, rsA, csA, &lda, &ta) || !as_slice(k, n, rsB, csB, &ldb, &tb) ||
       !as_slice(m, n, rsC, csC, &ldc, &tc))
     throw ArgError("gemm: operand without a unit stride (NonContiguousForBlas)");
-  ContractList cl;
-  cl.flags = (ta ? kTransA : 0) | (tb ? kTransB : 0) | (tc ? kTransC : 0);
-  cl.beta = beta;
-  cl.begin_out(0, int(m), int(n), int(ldc));
+  cl->clear();
+  cl->flags = (ta ? kTransA : 0) | (tb ? kTransB : 0) | (tc ? kTransC : 0);
+  cl->beta = beta;
+  cl->begin_out(0, int(m), int(n), int(ldc));
+  // C = beta C when there is nothing to add: an output with no terms scales (beta = 0 zeroes)
   if (k > 0 && alpha != 0.0) {
-    cl.begin_term(0, int(ldb), int(k));
-    cl.add_src(0, int(lda), alpha);
+    cl->begin_term(0, int(ldb), int(k));
+    cl->add_src(0, int(lda), alpha);
   }
-  const int64_t a_n = ta ? (k - 1) * lda + m : (m - 1) * lda + k;
-  const int64_t b_n = tb ? (n - 1) * ldb + k : (k - 1) * ldb + n;
+  cl->finalize();
   const int64_t c_n = tc ? (n - 1) * ldc + m : (m - 1) * ldc + n;
-  if (k == 0 || alpha == 0.0) {
-    // C = beta C: an output with no terms under beta scales (beta = 0 zeroes)
-    cl.finalize();
-    DeviceContractList d;
-    d.upload(ctx, cl);
-    launch_contract(ctx, d, A, B, C);
-    ctx->sync();
-    return;
+  int64_t a_n = 0, b_n = 0;
+  if (k > 0 && alpha != 0.0) {
+    a_n = ta ? (k - 1) * lda + m : (m - 1) * lda + k;
+    b_n = tb ? (n - 1) * ldb + k : (k - 1) * ldb + n;
   }
-  run(ctx, cl, a_n, b_n, c_n, A, B, C);
+  cl->validate(a_n, b_n, c_n);
+  return true;
+}
+
+void dense_gemm(Context* ctx, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t rsA,
+                int64_t csA, const double* B, int64_t rsB, int64_t csB, double beta, double* C, int64_t rsC,
+                int64_t csC) {
+  ContractList cl;
+  if (!dense_gemm_list(m, n, k, alpha, rsA, csA, rsB, csB, beta, rsC, csC, &cl)) return;
+  DeviceContractList d;
+  d.upload(ctx, cl);
+  launch_contract(ctx, d, A, B, C);
+  ctx->sync();
 }
 
 BlockGemmPlan::BlockGemmPlan(Sym sym, const QIndex& ar, const QIndex& ac, QN af, const QIndex& br, const QIndex& bc,

@@ -25,6 +25,17 @@ void dense_gemm(Context* ctx, int64_t m, int64_t n, int64_t k, double alpha, con
                 int64_t csA, const double* B, int64_t rsB, int64_t csB, double beta, double* C, int64_t rsC,
                 int64_t csC);
 
+// dense_gemm with its work list in a static device array instead of an
+// allocation (noalloc.hip); lists above that array's size are an ArgError.
+void dense_gemm_noalloc(Context* ctx, int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t rsA,
+                        int64_t csA, const double* B, int64_t rsB, int64_t csB, double beta, double* C, int64_t rsC,
+                        int64_t csC);
+
+// The work list of one dense_gemm call (finalized and validated against the
+// operand extents); false when m or n is 0 (nothing to do).
+bool dense_gemm_list(int64_t m, int64_t n, int64_t k, double alpha, int64_t rsA, int64_t csA, int64_t rsB,
+                     int64_t csB, double beta, int64_t rsC, int64_t csC, ContractList* cl);
+
 // Rank-2 block_gemm (faer.rs:390-575): output holds the blocks that receive at
 // least one sector task, keys ascending; equal output keys accumulate in the
 // fixed order (flops descending, key_a, key_b).

@@ -369,6 +369,15 @@ tk_status tk_gemm(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha, co
   });
 }
 
+tk_status tk_gemm_noalloc(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t rs_a,
+                          int64_t cs_a, const double* b, int64_t rs_b, int64_t cs_b, double beta, double* c,
+                          int64_t rs_c, int64_t cs_c) {
+  return guard(ctx, [&] {
+    if (!ctx) throw tk::ArgError("null context");
+    tk::dense_gemm_noalloc(ctx->c.get(), m, n, k, alpha, a, rs_a, cs_a, b, rs_b, cs_b, beta, c, rs_c, cs_c);
+  });
+}
+
 tk_status tk_block_gemm_table(int32_t sym, const tk_block_matrix* a, const tk_block_matrix* b, int64_t capacity,
                               int64_t* n_blocks, uint64_t* keys, int64_t* offsets, int64_t* total) {
   return guard(nullptr, [&] {

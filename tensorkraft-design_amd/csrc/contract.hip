@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <string>
 
 #include "contract.hpp"
 #include "device.hpp"
@@ -368,35 +369,55 @@ void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const 
 
 }  // namespace
 
-void DeviceContractList::upload(Context* ctx, const ContractList& h) {
-  auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
-  size_t o_items = 0;
-  size_t o_witems = align(o_items + h.items.size() * sizeof(GItem));
-  size_t o_outs = align(o_witems + h.witems.size() * sizeof(GItem));
-  size_t o_terms = align(o_outs + h.outs.size() * sizeof(GOut));
-  size_t o_srcs = align(o_terms + h.terms.size() * sizeof(GTerm));
-  size_t total = align(o_srcs + h.srcs.size() * sizeof(GSrc));
-  host.assign(total, 0);
-  std::copy_n(reinterpret_cast<const char*>(h.items.data()), h.items.size() * sizeof(GItem), host.data() + o_items);
+namespace {
+struct Packing {
+  size_t o_items, o_witems, o_outs, o_terms, o_srcs, total;
+  explicit Packing(const ContractList& h) {
+    auto align = [](size_t x) { return (x + 255) & ~size_t(255); };
+    o_items = 0;
+    o_witems = align(o_items + h.items.size() * sizeof(GItem));
+    o_outs = align(o_witems + h.witems.size() * sizeof(GItem));
+    o_terms = align(o_outs + h.outs.size() * sizeof(GOut));
+    o_srcs = align(o_terms + h.terms.size() * sizeof(GTerm));
+    total = align(o_srcs + h.srcs.size() * sizeof(GSrc));
+  }
+};
+}  // namespace
+
+size_t DeviceContractList::packed_bytes(const ContractList& h) { return Packing(h).total; }
+
+void DeviceContractList::upload_into(Context* ctx, const ContractList& h, char* base, size_t capacity) {
+  const Packing pk(h);
+  if (pk.total > capacity)
+    throw ArgError("contraction list of " + std::to_string(pk.total) + " bytes exceeds the " +
+                   std::to_string(capacity) + "-byte buffer");
+  host.assign(pk.total, 0);
+  std::copy_n(reinterpret_cast<const char*>(h.items.data()), h.items.size() * sizeof(GItem),
+              host.data() + pk.o_items);
   std::copy_n(reinterpret_cast<const char*>(h.witems.data()), h.witems.size() * sizeof(GItem),
-              host.data() + o_witems);
-  std::copy_n(reinterpret_cast<const char*>(h.outs.data()), h.outs.size() * sizeof(GOut), host.data() + o_outs);
-  std::copy_n(reinterpret_cast<const char*>(h.terms.data()), h.terms.size() * sizeof(GTerm), host.data() + o_terms);
-  std::copy_n(reinterpret_cast<const char*>(h.srcs.data()), h.srcs.size() * sizeof(GSrc), host.data() + o_srcs);
-  buf.reset(ctx, total);
-  buf.upload(host.data(), total);
-  char* base = buf.get();
-  items = reinterpret_cast<GItem*>(base + o_items);
-  witems = reinterpret_cast<GItem*>(base + o_witems);
-  outs = reinterpret_cast<GOut*>(base + o_outs);
-  terms = reinterpret_cast<GTerm*>(base + o_terms);
-  srcs = reinterpret_cast<GSrc*>(base + o_srcs);
+              host.data() + pk.o_witems);
+  std::copy_n(reinterpret_cast<const char*>(h.outs.data()), h.outs.size() * sizeof(GOut), host.data() + pk.o_outs);
+  std::copy_n(reinterpret_cast<const char*>(h.terms.data()), h.terms.size() * sizeof(GTerm),
+              host.data() + pk.o_terms);
+  std::copy_n(reinterpret_cast<const char*>(h.srcs.data()), h.srcs.size() * sizeof(GSrc), host.data() + pk.o_srcs);
+  if (pk.total) TK_HIP(hipMemcpyAsync(base, host.data(), pk.total, hipMemcpyHostToDevice, ctx->stream));
+  items = reinterpret_cast<GItem*>(base + pk.o_items);
+  witems = reinterpret_cast<GItem*>(base + pk.o_witems);
+  outs = reinterpret_cast<GOut*>(base + pk.o_outs);
+  terms = reinterpret_cast<GTerm*>(base + pk.o_terms);
+  srcs = reinterpret_cast<GSrc*>(base + pk.o_srcs);
   n_items = int(h.items.size());
   n_narrow = h.witems.empty() ? n_items : h.n_narrow;
   n_witems = int(h.witems.size());
   flags = h.flags;
   flops = h.flops;
   beta = h.beta;
+}
+
+void DeviceContractList::upload(Context* ctx, const ContractList& h) {
+  const size_t total = Packing(h).total;
+  buf.reset(ctx, total);
+  upload_into(ctx, h, buf.get(), total);
 }
 
 void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C) {

@@ -24,6 +24,11 @@ struct DeviceContractList {
   double flops = 0;
   double beta = 0;  // C = sum + beta * C
   void upload(Context* ctx, const ContractList& h);
+  // The same packing into caller-owned device memory of `capacity` bytes
+  // (no allocation); throws ArgError when the list does not fit.
+  void upload_into(Context* ctx, const ContractList& h, char* dst, size_t capacity);
+  // bytes the packed list of h takes
+  static size_t packed_bytes(const ContractList& h);
 };
 
 void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C);
