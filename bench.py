@@ -324,6 +324,19 @@ def cpu_baseline_sweep(tk_hip, sites, log):
             "scaled_to_d2048_estimate_s": round(t128 * (2048 / 128) ** 3, 1)}
 
 
+def tune_report(tk_hip, args):
+    """The contraction tilings the autotuner measured (signature -> choice and
+    the launch times of the choice and of the default 32 x 128 form)."""
+    if args.no_tune:
+        return {"tuned": False}
+    kinds = {0: "32x32 wavefront", 1: "32x128 workgroup", 2: "64x64 workgroup"}
+    t = tk_hip.tune_table()
+    return {"tuned": True, "entries": len(t),
+            "choices": [{"signature": hex(e["signature"]), "kind": kinds[e["wg_kind"]],
+                         "tile_min": e["tile_min"] if e["wg_kind"] == 2 else None,
+                         "us_best": round(e["us_best"], 2), "us_default": round(e["us_default"], 2)} for e in t]}
+
+
 def bench_sweep(args):
     import statistics
 
@@ -339,6 +352,7 @@ def bench_sweep(args):
     bonds, data = tk_hip.initial_mps(sym, [m.phys for m in mpo], (n // 2, n // 2), sched[0], seed=0,
                                      min_sector_dim=0)
     ctx = tk_hip.Context(0)
+    ctx.tune(not args.no_tune)
     res = {}
     for prec in ("fp64", args.mixed):
         res[prec] = run_sweeps(tk_hip, ctx, mpo, sym, bonds, data, sched, args.warmup, args.steps, prec, args.tol,
@@ -399,6 +413,7 @@ def bench_sweep(args):
         "matvecs_fp64": sum(s["matvecs"] for s in timed64),
         "kernels_mixed": kernel_table(timlo),
         "kernels_fp64": kernel_table(tim64),
+        "tiling": tune_report(tk_hip, args),
     }
     ctx.close()
     if not args.no_cpu_baseline:
@@ -615,6 +630,7 @@ def main():
     ap.add_argument("--ramp", type=int, default=None)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="run the default contraction tiling (no autotuning)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -109,6 +109,31 @@ typedef struct {
 tk_status tk_ctx_timing(tk_ctx* ctx, int32_t enable);
 tk_status tk_ctx_timing_read(tk_ctx* ctx, int32_t max, tk_kernel_timing* out, int32_t* n_out);
 
+/* Contraction tiling autotuner (no counterpart in the reference, which
+ * batches sector GEMMs by dimension without a measured choice,
+ * ARCHITECTURE.md:1196-1218).  Each grouped contraction of an H_eff plan can
+ * run as 32 x 32 wavefront tiles, 32 x 128 workgroup tiles sharing the A
+ * chunk through LDS, or 64 x 64 workgroup tiles with both chunks in LDS (for
+ * outputs from 32, 48 or 64 rows and columns on).  With mode 1 a plan whose
+ * stage signature (precision, operand layout, log2 buckets of the block
+ * sizes, terms and sources per output, half-octaves of the flops) has no cached choice times every
+ * candidate with HIP events at its first apply and caches the fastest for
+ * the process; mode 0 (the default) runs the 32 x 128 form.  Results of the
+ * forms differ only in rounding. */
+typedef struct {
+  int32_t device;
+  int32_t wg_kind;   /* 0 wavefront tiles, 1 32 x 128, 2 64 x 64 */
+  int32_t tile_min;  /* 64 x 64 form: smallest output rows and columns it takes */
+  int32_t pad;
+  uint64_t signature;
+  double us_best, us_default;  /* measured launch time of the choice and of the 32 x 128 form */
+} tk_tune_entry;
+tk_status tk_ctx_tune(tk_ctx* ctx, int32_t mode);
+tk_status tk_tune_table(int32_t max, tk_tune_entry* out, int32_t* n_out);
+tk_status tk_tune_save(const char* path);
+tk_status tk_tune_load(const char* path);
+tk_status tk_tune_clear(void);
+
 /* ---- sector tables (host only) ------------------------------------------ */
 /* QIndex::new / iter_sectors / total_dim (tk-symmetry/src/sector_key.rs:108-158):
  * sectors sorted by quantum number, prefix offsets.  Outputs have n_sectors
@@ -320,6 +345,14 @@ tk_status tk_heff_psi_table(tk_heff_plan* plan, int64_t capacity, int64_t* n_blo
                             int64_t* offsets, int64_t* total);
 /* y = H_eff x (the matvec closure, environments/mod.rs:592-599). */
 tk_status tk_heff_apply(tk_heff_plan* plan, const double* x_dev, double* y_dev);
+/* The same product from a HIP graph of the apply's kernel launches (built at
+ * the first call; later calls re-point its x and y and replay it).  The
+ * Davidson and Lanczos solvers of the sweep multiply through this path.
+ * Split-BF16 plans run tk_heff_apply. */
+tk_status tk_heff_apply_graph(tk_heff_plan* plan, const double* x_dev, double* y_dev);
+/* The tiling stage 1 or 3 of the plan runs with (after its first apply when
+ * the context tunes): wg_kind and tile_min as in tk_tune_entry. */
+tk_status tk_heff_tiling(tk_heff_plan* plan, int32_t stage, int32_t* wg_kind, int32_t* tile_min);
 /* y = H_eff x with FP64 contractions whatever the plan precision (refinement). */
 tk_status tk_heff_apply_fp64(tk_heff_plan* plan, const double* x_dev, double* y_dev);
 /* d = diag(H_eff), the Davidson preconditioner (eigensolver/mod.rs:425-435). */

@@ -14,6 +14,7 @@
 #include "tdvp.hpp"
 #include "sym.hpp"
 #include "truncate.hpp"
+#include "tune.hpp"
 
 struct tk_ctx {
   std::unique_ptr<tk::Context> c;
@@ -162,6 +163,44 @@ tk_status tk_ctx_sync(tk_ctx* ctx) {
     if (!ctx) throw tk::ArgError("null context");
     ctx->c->sync();
   });
+}
+
+tk_status tk_ctx_tune(tk_ctx* ctx, int32_t mode) {
+  return guard(ctx, [&] {
+    if (!ctx) throw tk::ArgError("null context");
+    if (mode != 0 && mode != 1) throw tk::ArgError("tune mode must be 0 or 1");
+    ctx->c->tune_mode = mode;
+  });
+}
+
+tk_status tk_tune_table(int32_t max, tk_tune_entry* out, int32_t* n_out) {
+  return guard(nullptr, [&] {
+    if (!n_out || (max > 0 && !out)) throw tk::ArgError("null argument");
+    const std::vector<tk::TuneEntry> e = tk::tune_entries();
+    *n_out = int32_t(e.size());
+    if (int64_t(e.size()) > max) throw tk::CapacityError("tune table: capacity too small");
+    for (size_t i = 0; i < e.size(); ++i)
+      out[i] = tk_tune_entry{e[i].device, e[i].choice.wg_kind, e[i].choice.tile_min, 0, e[i].sig, e[i].us_best,
+                             e[i].us_default};
+  });
+}
+
+tk_status tk_tune_save(const char* path) {
+  return guard(nullptr, [&] {
+    if (!path) throw tk::ArgError("null path");
+    tk::tune_save(path);
+  });
+}
+
+tk_status tk_tune_load(const char* path) {
+  return guard(nullptr, [&] {
+    if (!path) throw tk::ArgError("null path");
+    tk::tune_load(path);
+  });
+}
+
+tk_status tk_tune_clear(void) {
+  return guard(nullptr, [&] { tk::tune_clear(); });
 }
 
 tk_status tk_ctx_timing(tk_ctx* ctx, int32_t enable) {
@@ -689,6 +728,23 @@ tk_status tk_heff_apply(tk_heff_plan* plan, const double* x, double* y) {
   return guard(plan ? plan->ctx : nullptr, [&] {
     if (!plan || !x || !y) throw tk::ArgError("null argument");
     plan->p->apply(x, y);
+  });
+}
+
+tk_status tk_heff_apply_graph(tk_heff_plan* plan, const double* x, double* y) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !x || !y) throw tk::ArgError("null argument");
+    plan->p->apply_graph(x, y);
+  });
+}
+
+tk_status tk_heff_tiling(tk_heff_plan* plan, int32_t stage, int32_t* wg_kind, int32_t* tile_min) {
+  return guard(plan ? plan->ctx : nullptr, [&] {
+    if (!plan || !wg_kind || !tile_min) throw tk::ArgError("null argument");
+    if (stage != 1 && stage != 3) throw tk::ArgError("stage must be 1 or 3");
+    const tk::TileChoice c = plan->p->tiling(stage);
+    *wg_kind = c.wg_kind;
+    *tile_min = c.tile_min;
   });
 }
 

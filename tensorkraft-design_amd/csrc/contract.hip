@@ -34,18 +34,21 @@ void ContractList::finalize() {
       work += int64_t(tm.k) * ns;
       flops += 2.0 * g.m * g.n * tm.k + 2.0 * g.m * tm.k * (ns - 1);
     }
-    const bool w = wide && g.n >= kWideMinN;
+    const bool w = (wg_kind == kWgWide && g.n >= kWideMinN) ||
+                   (wg_kind == kWgTile64 && g.m >= tile_min && g.n >= tile_min);
     for (int m0 = 0; m0 < g.m; m0 += 32)
       for (int n0 = 0; n0 < g.n; n0 += 32) {
         int64_t c = std::max<int64_t>(1, work * std::min(32, g.m - m0) * std::min(32, g.n - n0) / 64);
         (w ? expand : items).push_back({int32_t(o), m0, n0, int32_t(std::min<int64_t>(c, INT32_MAX))});
       }
-    if (w)
-      for (int m0 = 0; m0 < g.m; m0 += 32)
-        for (int n0 = 0; n0 < g.n; n0 += 128) {
-          int64_t c = std::max<int64_t>(1, work * std::min(32, g.m - m0) * std::min(128, g.n - n0) / 64);
+    if (w) {
+      const int tm = wg_kind == kWgWide ? 32 : 64, tn = wg_kind == kWgWide ? 128 : 64;
+      for (int m0 = 0; m0 < g.m; m0 += tm)
+        for (int n0 = 0; n0 < g.n; n0 += tn) {
+          int64_t c = std::max<int64_t>(1, work * std::min(tm, g.m - m0) * std::min(tn, g.n - n0) / 64);
           witems.push_back({int32_t(o), m0, n0, int32_t(std::min<int64_t>(c, INT32_MAX))});
         }
+    }
   }
   auto lpt = [](const GItem& a, const GItem& b) { return a.cost > b.cost; };
   std::stable_sort(items.begin(), items.end(), lpt);
@@ -327,6 +330,134 @@ __global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restr
   }
 }
 
+
+// Workgroup items of 64 x 64: four wavefronts in a 2 x 2 arrangement, each
+// owning a 32 x 32 quarter (2 x 2 MFMA tiles).  Per inner chunk of KC = 4U
+// indices the workgroup stages the source-summed A chunk (64 x KC) and the B
+// chunk (KC x 64) in LDS in MFMA fragment order ([16-row group][step][lane],
+// so every fragment read is 64 consecutive elements), double-buffered: the
+// global loads of chunk c+1 are issued before the MFMAs of chunk c and
+// written to the other buffer after them, one barrier per chunk.  Chunks run
+// over the terms of the output in list order and over k within a term; the k
+// order inside a chunk is the wavefront kernel's (lane group lk supplies
+// k = U lk + u at MFMA step u), so A and B see the same permutation.
+template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
+__global__ __launch_bounds__(256, 2) void contract_tile64_kernel(const GItem* __restrict__ items, int n_items,
+                                                                 const GOut* __restrict__ outs,
+                                                                 const GTerm* __restrict__ terms,
+                                                                 const GSrc* __restrict__ srcs,
+                                                                 const TI* __restrict__ A, const TI* __restrict__ B,
+                                                                 TO* __restrict__ C, double beta) {
+  using M = Mfma<TI>;
+  using acc_t = typename M::acc_t;
+  constexpr int KC = 4 * U;
+  constexpr int PA = 64 * KC / 256;  // staged elements per thread and chunk, A and B alike
+  __shared__ TI As[2][4][U][64];
+  __shared__ TI Bs[2][4][U][64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = wv >> 1, wc = wv & 1;  // this wavefront's 32 x 32 quarter
+  const int li = lane & 15, lk = lane >> 4;
+  // the (row or column, k) element each thread stages, and its fragment slot
+  int a_r[PA], a_k[PA], a_slot[PA], b_c[PA], b_k[PA], b_slot[PA];
+#pragma unroll
+  for (int h = 0; h < PA; ++h) {
+    const int e = tid + 256 * h;
+    a_r[h] = TA ? (e & 63) : e / KC;
+    a_k[h] = TA ? e / 64 : e % KC;
+    b_c[h] = TB ? e / KC : (e & 63);
+    b_k[h] = TB ? e % KC : e / 64;
+    a_slot[h] = ((a_r[h] >> 4) * U + a_k[h] % U) * 64 + (a_r[h] & 15) + 16 * (a_k[h] / U);
+    b_slot[h] = ((b_c[h] >> 4) * U + b_k[h] % U) * 64 + (b_c[h] & 15) + 16 * (b_k[h] / U);
+  }
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const GItem item = items[it];
+    const GOut o = outs[item.out];
+    acc_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
+    // chunk cursor over (term, k0)
+    int t = o.t_begin, k0 = 0;
+    TI av[PA], bv[PA];
+    auto load = [&](int tt, int kk0) {
+      const GTerm tm = terms[tt];
+      const TI* __restrict__ Bp = B + tm.b_off;
+#pragma unroll
+      for (int h = 0; h < PA; ++h) {
+        const int gc = item.n0 + b_c[h], gk = kk0 + b_k[h];
+        bv[h] = (gc < o.n && gk < tm.k) ? (TB ? Bp[int64_t(gc) * tm.ldb + gk] : Bp[int64_t(gk) * tm.ldb + gc])
+                                        : TI(0);
+        av[h] = TI(0);
+      }
+      for (int s = tm.s_begin; s < tm.s_end; ++s) {
+        const GSrc sr = srcs[s];
+        const TI* __restrict__ Ap = A + sr.a_off;
+        const TI cf = TI(sr.coef);
+#pragma unroll
+        for (int h = 0; h < PA; ++h) {
+          const int gr = item.m0 + a_r[h], gk = kk0 + a_k[h];
+          const TI x = (gr < o.m && gk < tm.k) ? (TA ? Ap[int64_t(gk) * sr.lda + gr] : Ap[int64_t(gr) * sr.lda + gk])
+                                               : TI(0);
+          av[h] += cf * x;
+        }
+      }
+    };
+    auto store = [&](int buf) {
+      TI* as = &As[buf][0][0][0];
+      TI* bs = &Bs[buf][0][0][0];
+#pragma unroll
+      for (int h = 0; h < PA; ++h) {
+        as[a_slot[h]] = av[h];
+        bs[b_slot[h]] = bv[h];
+      }
+    };
+    if (t < o.t_end) {
+      load(t, k0);
+      store(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    while (t < o.t_end) {
+      // advance the cursor and request the next chunk before this one's MFMAs
+      int tn = t, kn = k0 + KC;
+      if (kn >= terms[t].k) {
+        ++tn;
+        kn = 0;
+      }
+      const bool more = tn < o.t_end;
+      if (more) load(tn, kn);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const TI a0 = As[buf][2 * wr][u][lane], a1 = As[buf][2 * wr + 1][u][lane];
+        const TI b0 = Bs[buf][2 * wc][u][lane], b1 = Bs[buf][2 * wc + 1][u][lane];
+        c00 = M::run(a0, b0, c00);
+        c01 = M::run(a0, b1, c01);
+        c10 = M::run(a1, b0, c10);
+        c11 = M::run(a1, b1, c11);
+      }
+      if (more) store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+      t = tn;
+      k0 = kn;
+    }
+    TO* __restrict__ Cp = C + o.c_off;
+    auto put = [&](int row, int col, TI v) {
+      if (row < o.m && col < o.n) {
+        TO* dst = TC ? &Cp[int64_t(col) * o.ldc + row] : &Cp[int64_t(row) * o.ldc + col];
+        *dst = beta == 0.0 ? TO(v) : TO(v) + TO(beta) * *dst;
+      }
+    };
+    const int rb = item.m0 + 32 * wr, cb = item.n0 + 32 * wc;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = rb + M::row(lk, r);
+      const int col = cb + li;
+      put(row, col, c00[r]);
+      put(row, col + 16, c01[r]);
+      put(row + 16, col, c10[r]);
+      put(row + 16, col + 16, c11[r]);
+    }
+  }
+}
+
 template <typename T> struct KUnroll;
 template <> struct KUnroll<double> { static constexpr int value = 4; };
 template <> struct KUnroll<float> { static constexpr int value = 8; };
@@ -351,7 +482,7 @@ void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const 
     }
 #undef TK_CASE
   }
-  if (L.n_witems > 0) {
+  if (L.n_witems > 0 && L.wg_kind == kWgWide) {
     const int blocks = std::min(L.n_witems, ctx->n_cu * 4);
 #define TK_CASE(F)                                                                                     \
   case F:                                                                                              \
@@ -363,8 +494,73 @@ void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const 
       TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
     }
 #undef TK_CASE
+  } else if (L.n_witems > 0 && L.wg_kind == kWgTile64) {
+    const int blocks = std::min(L.n_witems, ctx->n_cu * 4);
+#define TK_CASE(F)                                                                                     \
+  case F:                                                                                              \
+    hipLaunchKernelGGL((contract_tile64_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC), KUnroll<TI>::value>), \
+                       dim3(blocks), dim3(256), 0, ctx->stream, L.witems, L.n_witems, L.outs, L.terms, L.srcs, A, B, C, \
+                       L.beta);                                                                        \
+    break;
+    switch (L.flags & 7) {
+      TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+    }
+#undef TK_CASE
   }
   TK_HIP(hipGetLastError());
+}
+
+
+template <typename TI, typename TO>
+KernelNode make_node(const void* f, dim3 grid, const GItem* items, int n, const DeviceContractList& L, const TI* A,
+                     const TI* B, TO* C) {
+  KernelNode k;
+  k.func = f;
+  k.grid = grid;
+  k.block = dim3(256);
+  k.nargs = 9;
+  k.set(0, items);
+  k.set(1, n);
+  k.set(2, L.outs);
+  k.set(3, L.terms);
+  k.set(4, L.srcs);
+  k.set(5, A);
+  k.set(6, B);
+  k.set(7, C);
+  k.set(8, L.beta);
+  k.bind();
+  return k;
+}
+
+// The same kernels, shapes and arguments as launch_typed, as graph nodes.
+template <typename TI, typename TO>
+std::vector<KernelNode> nodes_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
+  std::vector<KernelNode> out;
+  const int n_wave = L.n_witems > 0 ? L.n_narrow : L.n_items;
+  const int F = L.flags & 7;
+  if (n_wave > 0) {
+    const int blocks = std::min((n_wave + 3) / 4, ctx->n_cu * 8);
+    const void* f = nullptr;
+#define TK_CASE(G) \
+  case G: f = reinterpret_cast<const void*>(&contract_kernel<TI, TO, bool(G & kTransA), bool(G & kTransB), bool(G & kTransC), KUnroll<TI>::value>); break;
+    switch (F) { TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7) }
+#undef TK_CASE
+    out.push_back(make_node(f, dim3(blocks), L.items, n_wave, L, A, B, C));
+  }
+  if (L.n_witems > 0) {
+    const bool wide = L.wg_kind == kWgWide;
+    const int blocks = std::min(L.n_witems, ctx->n_cu * 4);
+    const void* f = nullptr;
+#define TK_CASE(G)                                                                                                  \
+  case G:                                                                                                           \
+    f = wide ? reinterpret_cast<const void*>(&contract_wide_kernel<TI, TO, bool(G & kTransA), bool(G & kTransB), bool(G & kTransC), KUnroll<TI>::value>) \
+             : reinterpret_cast<const void*>(&contract_tile64_kernel<TI, TO, bool(G & kTransA), bool(G & kTransB), bool(G & kTransC), KUnroll<TI>::value>); \
+    break;
+    switch (F) { TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7) }
+#undef TK_CASE
+    out.push_back(make_node(f, dim3(blocks), L.witems, L.n_witems, L, A, B, C));
+  }
+  return out;
 }
 
 }  // namespace
@@ -409,6 +605,7 @@ void DeviceContractList::upload_into(Context* ctx, const ContractList& h, char* 
   n_items = int(h.items.size());
   n_narrow = h.witems.empty() ? n_items : h.n_narrow;
   n_witems = int(h.witems.size());
+  wg_kind = h.witems.empty() ? kWgNone : h.wg_kind;
   flags = h.flags;
   flops = h.flops;
   beta = h.beta;
@@ -428,6 +625,18 @@ void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, 
 }
 void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C) {
   launch_typed<float, double>(ctx, L, A, B, C);
+}
+std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const double* A, const double* B,
+                                       double* C) {
+  return nodes_typed<double, double>(ctx, L, A, B, C);
+}
+std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const float* A, const float* B,
+                                       double* C) {
+  return nodes_typed<float, double>(ctx, L, A, B, C);
+}
+std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const float* A, const float* B,
+                                       float* C) {
+  return nodes_typed<float, float>(ctx, L, A, B, C);
 }
 
 }  // namespace tk

@@ -45,19 +45,24 @@ enum ContractFlags : int { kTransA = 1, kTransB = 2, kTransC = 4 };
 // Narrowest output (columns) tiled into workgroup items when a list is wide.
 constexpr int kWideMinN = 96;
 
+// Workgroup item kinds: 32 x 128 items with the A chunk shared through LDS
+// (kWgWide), or 64 x 64 items with both operand chunks staged in LDS
+// (kWgTile64).
+enum WgKind : int { kWgNone = 0, kWgWide = 1, kWgTile64 = 2 };
+
 // Host-side list builder.
 struct ContractList {
   std::vector<GOut> outs;
   std::vector<GTerm> terms;
   std::vector<GSrc> srcs;
   std::vector<GItem> items;
-  // With `wide`, outputs of at least kWideMinN columns are also tiled into
-  // 32 x 128 workgroup items (`witems`); items[0, n_narrow) then tile only the
-  // other outputs, and the rest of `items` tiles the wide outputs in 32 x 32
-  // for kernels without a workgroup form.  The H_eff stages set it; on the
-  // single-source Gram and projection lists of the eigensolver it measured no
-  // faster.
-  bool wide = false;
+  // With wg_kind != kWgNone, the outputs the kind takes (kWgWide: at least
+  // kWideMinN columns; kWgTile64: at least tile_min rows and columns) are
+  // tiled into workgroup items (`witems`); items[0, n_narrow) then tile only
+  // the other outputs, and the rest of `items` tiles the workgroup outputs in
+  // 32 x 32 for kernels without that workgroup form.
+  int wg_kind = kWgNone;
+  int tile_min = 48;
   std::vector<GItem> witems;
   int n_narrow = 0;
   int flags = 0;

@@ -231,7 +231,10 @@ __global__ __launch_bounds__(256) void contract_bf16x3_wide_kernel(const GItem* 
 
 template <typename TO>
 void launch_bf16(Context* ctx, const DeviceContractList& L, const float* A, const float* B, TO* C) {
-  const int n_wave = L.n_witems > 0 ? L.n_narrow : L.n_items;
+  // 32 x 128 workgroup items have a split-BF16 form; 64 x 64 items run as
+  // their 32 x 32 expansion on the wavefront kernel
+  const bool wide = L.n_witems > 0 && L.wg_kind == kWgWide;
+  const int n_wave = wide ? L.n_narrow : L.n_items;
   if (n_wave > 0) {
     int blocks = std::min((n_wave + 3) / 4, ctx->n_cu * 8);
     dim3 grid(blocks), block(256);
@@ -245,7 +248,7 @@ void launch_bf16(Context* ctx, const DeviceContractList& L, const float* A, cons
     }
 #undef TK_CASE
   }
-  if (L.n_witems > 0) {
+  if (wide) {
     const int blocks = std::min(L.n_witems, ctx->n_cu * 4);
 #define TK_CASE(F)                                                                                         \
   case F:                                                                                                  \

@@ -300,10 +300,12 @@ namespace {
 struct PlanUser {
   HeffPlan* plan;
 };
+// The plan's own precision replays its graph; FP64 products of a
+// low-precision plan (the refinement) take the eager FP64 path.
 void plan_matvec(const double* x, double* y, int fp64, void* user) {
   HeffPlan* p = static_cast<PlanUser*>(user)->plan;
-  if (fp64) p->apply_fp64(x, y);
-  else p->apply(x, y);
+  if (fp64 && p->prec() != Prec::FP64) p->apply_fp64(x, y);
+  else p->apply_graph(x, y);
 }
 }  // namespace
 

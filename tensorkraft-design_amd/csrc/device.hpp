@@ -182,6 +182,9 @@ class KernelTimer {
 
 struct Context {
   int device = 0;
+  // contraction tiling: 0 the default tiling, 1 measured per list signature
+  // on first use and cached (tune.hpp)
+  int tune_mode = 0;
   KernelTimer timer;
   hipStream_t stream = nullptr;
   DevicePool pool;

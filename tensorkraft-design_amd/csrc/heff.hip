@@ -1,6 +1,8 @@
 // HeffPlan: sector tables -> two grouped-contraction work lists + diagonal.
 #include "heff.hpp"
 
+#include "tune.hpp"
+
 #include <cmath>
 #include <map>
 #include <tuple>
@@ -189,8 +191,6 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
         dslices_.push_back(ds);
       }
   }
-  h3_.wide = true;
-  h3_.finalize();
 
   h1_.clear();
   h1_.flags = 0;
@@ -199,10 +199,16 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
     h1_.begin_term(r.b_off, r.n, r.k);
     h1_.add_src(r.a_off, r.lda, 1.0);
   }
-  h1_.wide = true;
-  h1_.finalize();
-  h1_.validate(l_total_, psi_.total, t1_total);
-  h3_.validate(t1_total, r_total_, psi_.total);
+  // tiling: the cached measurement for these signatures when tuning is on
+  // (a miss is measured at the first apply), else the default
+  sig1_ = tile_signature(h1_, int(prec_));
+  sig3_ = tile_signature(h3_, int(prec_));
+  TileChoice c1, c3;
+  tune_pending_ = false;
+  if (ctx_->tune_mode)
+    tune_pending_ = !(tune_lookup(ctx_->device, sig1_, &c1) && tune_lookup(ctx_->device, sig3_, &c3));
+  if (tune_pending_) c1 = c3 = TileChoice{};
+  set_tiling(c1, c3, t1_total);
   for (const DiagSlice& s : dslices_)
     for (int t = s.t_begin; t < s.t_end; ++t) {
       const DiagTerm& d = dterms_[t];
@@ -210,8 +216,6 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
         throw std::out_of_range("heff diag term outside the environments");
     }
 
-  d1_.upload(ctx_, h1_);
-  d3_.upload(ctx_, h3_);
   t1_.reset(ctx_, size_t(std::max<int64_t>(t1_total, 1)));
 
   // diagonal tables in one device allocation
@@ -245,6 +249,65 @@ void HeffPlan::build(const BlockTable& env_l, const BlockTable& env_r, const Mpo
   stats_.bytes = stats_.s1_bytes + stats_.s3_bytes;
 }
 
+void HeffPlan::set_tiling(const TileChoice& c1, const TileChoice& c3, int64_t t1_total) {
+  h1_.wg_kind = c1.wg_kind;
+  h1_.tile_min = c1.tile_min;
+  h1_.finalize();
+  h1_.validate(l_total_, psi_.total, t1_total);
+  h3_.wg_kind = c3.wg_kind;
+  h3_.tile_min = c3.tile_min;
+  h3_.finalize();
+  h3_.validate(t1_total, r_total_, psi_.total);
+  d1_.upload(ctx_, h1_);
+  d3_.upload(ctx_, h3_);
+  tiling1_ = c1;
+  tiling3_ = c3;
+}
+
+void HeffPlan::launch_stage(int stage, const DeviceContractList& d, const double* x, double* y) {
+  const bool s1 = stage == 1;
+  switch (prec_) {
+    case Prec::FP64:
+      if (s1) launch_contract(ctx_, d, L_, x, t1_.get());
+      else launch_contract(ctx_, d, t1_.get(), R_, y);
+      return;
+    case Prec::FP32:
+      if (s1) launch_contract(ctx_, d, Lf_.get(), xf_.get(), t1f_.get());
+      else launch_contract(ctx_, d, t1f_.get(), Rf_.get(), y);
+      return;
+    case Prec::BF16X3:
+      if (s1) launch_contract_bf16x3(ctx_, d, Lf_.get(), xf_.get(), t1f_.get());
+      else launch_contract_bf16x3(ctx_, d, t1f_.get(), Rf_.get(), y);
+      return;
+  }
+}
+
+void HeffPlan::tune(const double* x, double* y) {
+  tune_pending_ = false;
+  if (prec_ != Prec::FP64) {
+    ensure_low();
+    convert_f64_to_f32(ctx_, x, xf_.get(), psi_.total);
+  }
+  const std::vector<TileChoice> cands = tile_candidates(int(prec_));
+  TileChoice best[2];
+  for (int stage : {1, 3}) {
+    ContractList& proto = stage == 1 ? h1_ : h3_;
+    std::vector<DeviceContractList> dl(cands.size());
+    for (size_t i = 0; i < cands.size(); ++i) {
+      ContractList l = proto;
+      l.wg_kind = cands[i].wg_kind;
+      l.tile_min = cands[i].tile_min;
+      l.finalize();
+      dl[i].upload(ctx_, l);
+    }
+    ctx_->sync();
+    best[stage == 1 ? 0 : 1] = tune_measure(ctx_, stage == 1 ? sig1_ : sig3_, cands,
+                                            [&](size_t i) { launch_stage(stage, dl[i], x, y); });
+  }
+  set_tiling(best[0], best[1], int64_t(t1_.size()));
+  drop_graph();
+}
+
 void HeffPlan::ensure_low() {
   if (low_ready_) return;
   Lf_.reset(ctx_, size_t(std::max<int64_t>(l_total_, 1)));
@@ -259,6 +322,7 @@ void HeffPlan::ensure_low() {
 void HeffPlan::rebind(const double* L, const double* R) {
   L_ = L;
   R_ = R;
+  drop_graph();  // its nodes point at the old environments
   if (low_ready_) {
     convert_f64_to_f32(ctx_, L_, Lf_.get(), l_total_);
     convert_f64_to_f32(ctx_, R_, Rf_.get(), r_total_);
@@ -266,6 +330,10 @@ void HeffPlan::rebind(const double* L, const double* R) {
 }
 
 void HeffPlan::apply_fp64(const double* x, double* y) {
+  if (prec_ == Prec::FP64) {
+    apply(x, y);
+    return;
+  }
   int tk = ctx_->timer.begin(ctx_->stream);
   launch_contract(ctx_, d1_, L_, x, t1_.get());
   ctx_->timer.end(tk, ctx_->stream, "heff_stage1_fp64", stats_.s1_bytes_fp64, stats_.s1_flops, 1);
@@ -320,8 +388,100 @@ void HeffPlan::stage3(double* y) {
 }
 
 void HeffPlan::apply(const double* x, double* y) {
+  if (tune_pending_) tune(x, y);
   stage1(x);
   stage3(y);
+}
+
+HeffPlan::~HeffPlan() { drop_graph(); }
+
+void HeffPlan::drop_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  gnodes_.clear();
+  ghandles_.clear();
+  gx_arg_.clear();
+  gy_arg_.clear();
+}
+
+void HeffPlan::build_graph(const double* x, double* y) {
+  drop_graph();
+  std::vector<KernelNode> s1, s3;
+  if (prec_ == Prec::FP64) {
+    s1 = contract_nodes(ctx_, d1_, L_, x, t1_.get());
+    s3 = contract_nodes(ctx_, d3_, t1_.get(), R_, y);
+  } else {
+    ensure_low();
+    gnodes_.push_back(convert_f64_to_f32_node(ctx_, x, xf_.get(), psi_.total));
+    gx_arg_.push_back(0);
+    gy_arg_.push_back(-1);
+    s1 = contract_nodes(ctx_, d1_, Lf_.get(), xf_.get(), t1f_.get());
+    s3 = contract_nodes(ctx_, d3_, t1f_.get(), Rf_.get(), y);
+  }
+  for (const KernelNode& k : s1) {
+    gnodes_.push_back(k);
+    gx_arg_.push_back(prec_ == Prec::FP64 ? 6 : -1);  // argument 6 is B = x
+    gy_arg_.push_back(-1);
+  }
+  for (const KernelNode& k : s3) {
+    gnodes_.push_back(k);
+    gx_arg_.push_back(-1);
+    gy_arg_.push_back(7);  // argument 7 is C = y
+  }
+  for (KernelNode& k : gnodes_) k.bind();  // slots moved with the vector
+  TK_HIP(hipGraphCreate(&graph_, 0));
+  hipGraphNode_t prev = nullptr;
+  for (KernelNode& k : gnodes_) {
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(k.func);
+    p.gridDim = k.grid;
+    p.blockDim = k.block;
+    p.sharedMemBytes = k.shmem;
+    p.kernelParams = k.args;
+    p.extra = nullptr;
+    hipGraphNode_t h;
+    TK_HIP(hipGraphAddKernelNode(&h, graph_, prev ? &prev : nullptr, prev ? 1 : 0, &p));
+    ghandles_.push_back(h);
+    prev = h;
+  }
+  TK_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+}
+
+void HeffPlan::apply_graph(const double* x, double* y) {
+  if (prec_ == Prec::BF16X3) {
+    apply(x, y);
+    return;
+  }
+  if (tune_pending_) tune(x, y);
+  if (!graph_exec_) build_graph(x, y);
+  for (size_t i = 0; i < gnodes_.size(); ++i) {
+    KernelNode& k = gnodes_[i];
+    bool changed = false;
+    if (gx_arg_[i] >= 0 && k.slot[gx_arg_[i]] != reinterpret_cast<uint64_t>(x)) {
+      k.set(gx_arg_[i], x);
+      changed = true;
+    }
+    if (gy_arg_[i] >= 0 && k.slot[gy_arg_[i]] != reinterpret_cast<uint64_t>(y)) {
+      k.set(gy_arg_[i], y);
+      changed = true;
+    }
+    if (changed) {
+      hipKernelNodeParams p{};
+      p.func = const_cast<void*>(k.func);
+      p.gridDim = k.grid;
+      p.blockDim = k.block;
+      p.sharedMemBytes = k.shmem;
+      p.kernelParams = k.args;
+      p.extra = nullptr;
+      TK_HIP(hipGraphExecKernelNodeSetParams(graph_exec_, ghandles_[i], &p));
+    }
+  }
+  const int tk = ctx_->timer.begin(ctx_->stream);
+  TK_HIP(hipGraphLaunch(graph_exec_, ctx_->stream));
+  ctx_->timer.end(tk, ctx_->stream, prec_ == Prec::FP64 ? "heff_apply_graph_fp64" : "heff_apply_graph_fp32",
+                  stats_.s1_bytes + stats_.s3_bytes, stats_.s1_flops + stats_.s3_flops, 1);
 }
 
 namespace {

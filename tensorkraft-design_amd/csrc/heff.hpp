@@ -22,6 +22,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "sym.hpp"
+#include "tune.hpp"
 
 namespace tk {
 
@@ -76,8 +77,16 @@ class HeffPlan {
   Prec prec() const { return prec_; }
   // y = H_eff x; x, y device FP64 vectors of length psi().total (y overwritten).
   void apply(const double* x, double* y);
+  // The same product replayed from a HIP graph of the apply's launches,
+  // built at the first call (after any tuning) with explicit kernel nodes;
+  // later calls re-point the nodes' x and y arguments and launch the graph.
+  // Split-BF16 plans run apply().
+  void apply_graph(const double* x, double* y);
   // y = H_eff x with every contraction in FP64 regardless of the plan precision.
   void apply_fp64(const double* x, double* y);
+  ~HeffPlan();
+  HeffPlan(const HeffPlan&) = delete;
+  HeffPlan& operator=(const HeffPlan&) = delete;
   // The two launches of apply(), separately (for per-stage timing).
   void stage1(const double* x);
   void stage3(double* y);
@@ -85,6 +94,9 @@ class HeffPlan {
   void diagonal(double* d);
   // Re-bind environment data (same tables) without re-planning.
   void rebind(const double* L, const double* R);
+  // The tiling each stage runs with (tune.hpp): the cached or measured one
+  // when the context tunes, else the default.
+  TileChoice tiling(int stage) const { return stage == 1 ? tiling1_ : tiling3_; }
 
  private:
   void build(const BlockTable& env_l, const BlockTable& env_r, const MpoSite& wi, const MpoSite& wj);
@@ -106,7 +118,22 @@ class HeffPlan {
   std::vector<DiagTerm> dterms_;
   DBuf<char> ddev_;
   HeffStats stats_;
+  uint64_t sig1_ = 0, sig3_ = 0;
+  TileChoice tiling1_, tiling3_;
+  bool tune_pending_ = false;
+  // graph replay of apply(): nodes (with their argument slots), the index
+  // of the argument that is x or y per node (-1: neither), the instance
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  std::vector<KernelNode> gnodes_;
+  std::vector<hipGraphNode_t> ghandles_;
+  std::vector<int> gx_arg_, gy_arg_;
+  void build_graph(const double* x, double* y);
+  void drop_graph();
   void ensure_low();
+  void set_tiling(const TileChoice& c1, const TileChoice& c3, int64_t t1_total);
+  void launch_stage(int stage, const DeviceContractList& d, const double* x, double* y);
+  void tune(const double* x, double* y);
 };
 
 void launch_heff_diag(Context* ctx, const void* slices, int n_slices, const void* terms, const double* L,

@@ -1,6 +1,7 @@
 // Host-callable launchers for the HIP kernels of the library.
 #pragma once
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "contract.hpp"
@@ -20,6 +21,7 @@ struct DeviceContractList {
   int n_items = 0;
   int n_narrow = 0;
   int n_witems = 0;
+  int wg_kind = kWgNone;  // what the workgroup items are (WgKind)
   int flags = 0;
   double flops = 0;
   double beta = 0;  // C = sum + beta * C
@@ -31,7 +33,38 @@ struct DeviceContractList {
   static size_t packed_bytes(const ContractList& h);
 };
 
+// One kernel launch held as a graph node: the function, its shape and its
+// argument values, each in an 8-byte slot that args[i] points to (an int
+// occupies the slot's low bytes).
+struct KernelNode {
+  const void* func = nullptr;
+  dim3 grid, block;
+  unsigned shmem = 0;
+  int nargs = 0;
+  uint64_t slot[12] = {};
+  void* args[12] = {};
+  template <typename T>
+  void set(int i, T v) {
+    static_assert(sizeof(T) <= sizeof(uint64_t), "argument slot");
+    slot[i] = 0;
+    std::memcpy(&slot[i], &v, sizeof(T));
+  }
+  void bind() {
+    for (int i = 0; i < nargs; ++i) args[i] = &slot[i];
+  }
+};
+
 void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C);
+// The launches launch_contract would make, as graph nodes (argument 6 is B,
+// argument 7 is C).
+std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const double* A, const double* B,
+                                       double* C);
+std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const float* A, const float* B,
+                                       double* C);
+std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const float* A, const float* B,
+                                       float* C);
+// convert_f64_to_f32 as a graph node (argument 0 is x)
+KernelNode convert_f64_to_f32_node(Context* ctx, const double* x, float* y, int64_t n);
 void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C);
 void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C);
 // split-BF16 (hi+lo, three products, FP32 accumulation) on FP32 operands

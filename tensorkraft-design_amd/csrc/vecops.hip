@@ -275,6 +275,18 @@ void convert_f64_to_f32(Context* ctx, const double* x, float* y, int64_t n) {
   hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, x, y, n);
   TK_HIP(hipGetLastError());
 }
+KernelNode convert_f64_to_f32_node(Context* ctx, const double* x, float* y, int64_t n) {
+  KernelNode k;
+  k.func = reinterpret_cast<const void*>(&f64_to_f32_kernel);
+  k.grid = dim3(grid_for(ctx, n));
+  k.block = dim3(kThreads);
+  k.nargs = 3;
+  k.set(0, x);
+  k.set(1, y);
+  k.set(2, n);
+  k.bind();
+  return k;
+}
 void convert_f32_to_f64(Context* ctx, const float* x, double* y, int64_t n) {
   hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, ctx->stream, x, y, n);
   TK_HIP(hipGetLastError());

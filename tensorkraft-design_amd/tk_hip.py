@@ -84,6 +84,11 @@ class tk_sweep_stats(C.Structure):
                 ("steps", C.c_int32), ("unconverged", C.c_int32)]
 
 
+class tk_tune_entry(C.Structure):
+    _fields_ = [("device", C.c_int32), ("wg_kind", C.c_int32), ("tile_min", C.c_int32), ("pad", C.c_int32),
+                ("signature", C.c_uint64), ("us_best", C.c_double), ("us_default", C.c_double)]
+
+
 class tk_step_stats(C.Structure):
     _fields_ = [("site", C.c_int32), ("left_to_right", C.c_int32), ("iterations", C.c_int32),
                 ("matvecs", C.c_int32), ("matvecs_fp64", C.c_int32), ("converged", C.c_int32),
@@ -161,6 +166,13 @@ def lib():
         "tk_dmrg_bond": (I32, [P, I32, I32, pI32, pI32, pI64]),
         "tk_dmrg_site": (I32, [P, I32, I64, pI64, pD]),
         "tk_ctx_timing": (I32, [P, I32]),
+        "tk_ctx_tune": (I32, [P, I32]),
+        "tk_tune_table": (I32, [I32, C.POINTER(tk_tune_entry), pI32]),
+        "tk_tune_save": (I32, [C.c_char_p]),
+        "tk_tune_load": (I32, [C.c_char_p]),
+        "tk_tune_clear": (I32, []),
+        "tk_heff_tiling": (I32, [P, I32, pI32, pI32]),
+        "tk_heff_apply_graph": (I32, [P, P, P]),
         "tk_block_permute": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, pI32, P, I64, P, I64]),
         "tk_block_fuse": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, I32, I32, P, I64, P, I64]),
         "tk_block_split": (I32, [P, I32, I32, C.POINTER(tk_qindex), pI32, pI32, I32, I32, P, I64, P, I64]),
@@ -200,11 +212,33 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_block_svd_truncate", "tk_eigh_lowest", "tk_qr", "tk_regularized_svd_inverse",
             "tk_bond_schedule_warmup", "tk_dmrg_set_cancel_flag", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
             "tk_dmrg_sweep_single_site", "tk_dmrg_apply_site_op", "tk_dmrg_overlap",
-            "tk_ctx_timing",
+            "tk_ctx_timing", "tk_ctx_tune", "tk_tune_table", "tk_tune_save", "tk_tune_load", "tk_tune_clear",
+            "tk_heff_tiling", "tk_heff_apply_graph",
             "tk_ctx_timing_read", "tk_block_permute", "tk_block_fuse", "tk_block_split", "tk_hybridization",
             "tk_bath_fit", "tk_lanczos_tridiagonalize", "tk_broyden_create", "tk_broyden_update",
             "tk_broyden_destroy", "tk_krylov_tridiag", "tk_mlp_create", "tk_mlp_destroy", "tk_mlp_n_params",
             "tk_mlp_get_params", "tk_mlp_set_params", "tk_mlp_forward", "tk_mlp_loss_grad", "tk_mlp_adam"]
+
+
+def tune_table():
+    """The tuner's cache: [{device, wg_kind, tile_min, signature, us_best, us_default}]."""
+    n = C.c_int32(0)
+    lib().tk_tune_table(0, None, C.byref(n))
+    buf = (tk_tune_entry * max(1, n.value))()
+    _check(lib().tk_tune_table(len(buf), buf, C.byref(n)))
+    return [{f: getattr(buf[i], f) for f, _ in tk_tune_entry._fields_ if f != "pad"} for i in range(n.value)]
+
+
+def tune_save(path):
+    _check(lib().tk_tune_save(os.fsencode(path)))
+
+
+def tune_load(path):
+    _check(lib().tk_tune_load(os.fsencode(path)))
+
+
+def tune_clear():
+    _check(lib().tk_tune_clear())
 
 
 def _dp(a):
@@ -385,6 +419,10 @@ class Context:
         """Enable (and clear) or disable the HIP-event timing of the H_eff and
         eigensolver kernels on this context's stream."""
         _check(self.L.tk_ctx_timing(self.h, 1 if enable else 0), self.h)
+
+    def tune(self, on=True):
+        """Measure the contraction tiling per signature at first use (tk_ctx_tune)."""
+        _check(self.L.tk_ctx_tune(self.h, 1 if on else 0), self.h)
 
     def timing_read(self):
         """Per kernel: launches, event milliseconds, algorithmic bytes and flops
@@ -737,6 +775,16 @@ class HeffPlan:
 
     def apply(self, x_dev, y_dev):
         _check(self.ctx.L.tk_heff_apply(self.h, C.c_void_p(x_dev.ptr), C.c_void_p(y_dev.ptr)), self.ctx.h)
+
+    def apply_graph(self, x_dev, y_dev):
+        """y = H_eff x replayed from the plan's HIP graph (tk_heff_apply_graph)."""
+        _check(self.ctx.L.tk_heff_apply_graph(self.h, C.c_void_p(x_dev.ptr), C.c_void_p(y_dev.ptr)), self.ctx.h)
+
+    def tiling(self, stage):
+        """(wg_kind, tile_min) stage 1 or 3 runs with (tk_heff_tiling)."""
+        k, t = C.c_int32(0), C.c_int32(0)
+        _check(self.ctx.L.tk_heff_tiling(self.h, stage, C.byref(k), C.byref(t)), self.ctx.h)
+        return k.value, t.value
 
     def apply_fp64(self, x_dev, y_dev):
         _check(self.ctx.L.tk_heff_apply_fp64(self.h, C.c_void_p(x_dev.ptr), C.c_void_p(y_dev.ptr)), self.ctx.h)
