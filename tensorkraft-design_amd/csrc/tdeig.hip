@@ -945,9 +945,37 @@ __global__ __launch_bounds__(NT) void bisect_kernel(const BisItem* __restrict__ 
 // 3a. Inverse iteration, one thread per wanted vector: T - lambda I = P L U
 // with partial pivoting (U has two superdiagonals), two solves from a
 // hashed start, each normalised by its largest entry; finally unit 2-norm.
+//
+// Equal eigenvalues: an eigenvalue of multiplicity m of the reduced matrix
+// makes T split into unreduced blocks (off-diagonals at the rounding level,
+// |e_i| <= kSplitTol eps |T|), each holding one copy.  Inverse iteration
+// with the common shift on the whole of T would send every copy to the
+// block whose eigenvalue happens to lie closest, and orthogonalising those
+// m nearly parallel vectors leaves rounding noise.  So for a run of
+// eigenvalues within kClusterTol eps |T| of each other, the off-diagonals at
+// the split points are taken as zero, the blocks holding the run's values
+// are counted by Sturm sequences, and the run's r-th vector starts (and, T
+// being block diagonal then, stays) on the r-th of them.
+constexpr double kSplitTol = 16.0;
+constexpr double kClusterTol = 64.0;
+
 struct VecItem {
   int32_t job, c;
 };
+
+// Eigenvalues below x of the unreduced block rows [a, b) of T with the
+// split off-diagonals (|e_i| <= tol_e) taken as zero.
+__device__ int sturm_count_block(const double* d, const double* e, int a, int b, double x, double pivmin) {
+  double q = d[a] - x;
+  if (fabs(q) <= pivmin) q = -pivmin;
+  int cnt = q < 0.0;
+  for (int i = a + 1; i < b; ++i) {
+    q = (d[i] - x) - e[i - 1] * e[i - 1] / q;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
 
 // One solve with the factors of T - lambda I stored by invit_kernel for
 // vector c (row i, field f at lu[(i * kLuFields + f) * k + c]): y <- U^-1 L^-1 P y
@@ -1001,17 +1029,53 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
     Y(0) = 1.0;
     return;
   }
-  const double piv = kEps * fmax(TN[jb.job], DBL_MIN);
+  const double tn = fmax(TN[jb.job], DBL_MIN);
+  const double piv = kEps * tn;
   auto guard = [&](double x) { return fabs(x) < piv ? (x < 0 ? -piv : piv) : x; };
-  double c0 = d[0] - lam, c1 = e[0];
+  // a run of equal eigenvalues: this vector's split block [s0, s1)
+  const double tol_e = kSplitTol * kEps * tn;
+  bool split = false;
+  int s0 = 0, s1 = n;
+  {
+    const double* w = W + jb.w_off;
+    const double tol_c = kClusterTol * kEps * tn;
+    int r0 = c, r1 = c;
+    while (r0 > 0 && fabs(w[r0 - 1] - w[r0]) <= tol_c) --r0;
+    while (r1 + 1 < n && fabs(w[r1 + 1] - w[r1]) <= tol_c) ++r1;
+    if (r1 > r0) {
+      const double lo = fmin(w[r0], w[r1]) - tol_c, hi = fmax(w[r0], w[r1]) + tol_c;
+      double e2max = 0;
+      for (int i = 0; i + 1 < n; ++i) e2max = fmax(e2max, e[i] * e[i]);
+      const double pivmin = DBL_MIN * fmax(1.0, e2max);
+      const int rank = c - r0;
+      int a = 0, cum = 0, p0 = -1, p1 = -1;
+      for (int i = 0; i < n; ++i) {
+        if (i + 1 < n && fabs(e[i]) > tol_e) continue;
+        const int cnt = sturm_count_block(d, e, a, i + 1, hi, pivmin) - sturm_count_block(d, e, a, i + 1, lo, pivmin);
+        if (p0 < 0 && rank < cum + cnt) {
+          p0 = a;
+          p1 = i + 1;
+        }
+        cum += cnt;
+        a = i + 1;
+      }
+      if (cum == r1 - r0 + 1 && p0 >= 0) {
+        split = true;
+        s0 = p0;
+        s1 = p1;
+      }
+    }
+  }
+  auto Ev = [&](int i) { return split && fabs(e[i]) <= tol_e ? 0.0 : e[i]; };
+  double c0 = d[0] - lam, c1 = Ev(0);
   for (int i0 = 0; i0 + 1 < n; i0 += C) {
     double dn[C], ei[C], en[C];
 #pragma unroll
     for (int u = 0; u < C; ++u) {
       const int i = i0 + u;
       dn[u] = i + 1 < n ? d[i + 1] : 0.0;
-      ei[u] = i + 1 < n ? e[i] : 0.0;
-      en[u] = i + 2 < n ? e[i + 1] : 0.0;
+      ei[u] = i + 1 < n ? Ev(i) : 0.0;
+      en[u] = i + 2 < n ? Ev(i + 1) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < C; ++u) {
@@ -1045,7 +1109,8 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
   F(n - 1, 0) = guard(c0);
   F(n - 1, 1) = 0.0;
   F(n - 1, 2) = 0.0;
-  for (int i = 0; i < n; ++i) Y(i) = hash_unit(uint32_t(c) * 2654435761u + uint32_t(jb.job), uint32_t(i));
+  for (int i = 0; i < n; ++i)
+    Y(i) = i >= s0 && i < s1 ? hash_unit(uint32_t(c) * 2654435761u + uint32_t(jb.job), uint32_t(i)) : 0.0;
   double s = 1.0;
   for (int itn = 0; itn < 2; ++itn) {
     // forward: row interchanges and multipliers applied to s * y
@@ -1282,6 +1347,7 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
     double* R = H + 64;  // NB x NB, upper triangle
     if (tid == 0) {
       bool ok = true;
+      double minr = 1.0;
       double g[NB][NB];
       int u = 0;
       for (int a = 0; a < NB; ++a)
@@ -1298,6 +1364,7 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
           break;
         }
         const double r = sqrt(d);
+        if (test) minr = fmin(minr, r / fmax(nrm0[a], DBL_MIN));
         R[a * NB + a] = r;
         for (int b = a + 1; b < nb; ++b) {
           double t = g[a][b];
@@ -1306,6 +1373,7 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
         }
       }
       hs[0] = ok ? 1.0 : 0.0;
+      if (NB > 1) hs[1] = minr;
     }
     __syncthreads();
     if (hs[0] == 0.0) return false;
@@ -1343,8 +1411,17 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
       project(c_begin, b0, 0, nb);
     }
     // fast path: Cholesky QR twice on the sub-block; the column-by-column
-    // path below runs only when a column is numerically dependent
-    if (cholqr(nb, true) && cholqr(nb, false)) {
+    // path below runs only when a column is numerically dependent.  A block
+    // that lost most of a column's norm to the earlier blocks had R^-1 scale
+    // up what the two projections left of them: project and factor once more.
+    bool fast = cholqr(nb, true);
+    const bool heavy = fast && NB > 1 && hs[1] < 0.5;
+    fast = fast && cholqr(nb, false);
+    if (fast && heavy && b0 > 0) {
+      project(0, b0, 0, nb);
+      fast = cholqr(nb, false);
+    }
+    if (fast) {
       for (int e = tid; e < n * NB; e += NT) {
         const int i = e / NB, c = e - i * NB;
         if (c < nb) z[int64_t(i) * k + b0 + c] = Zb[e];
@@ -1352,28 +1429,32 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
       __syncthreads();
       continue;
     }
+    // column c of Zb minus its projection on the block's columns q < c (two passes)
+    auto inblock = [&](int c) {
+      for (int pass = 0; pass < 2 && c > 0; ++pass) {
+        double part[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) part[q] = 0;
+        for (int i = tid; i < n; i += NT) {
+          const double zc = Zb[i * NB + c];
+#pragma unroll
+          for (int q = 0; q < NB; ++q)
+            if (q < c) part[q] += Zb[i * NB + q] * zc;
+        }
+        block_sums(part, c);
+        for (int i = tid; i < n; i += NT) {
+          double s = 0;
+#pragma unroll
+          for (int q = 0; q < NB; ++q)
+            if (q < c) s += hs[q] * Zb[i * NB + q];
+          Zb[i * NB + c] -= s;
+        }
+        __syncthreads();
+      }
+    };
     for (int c = 0; c < nb; ++c) {
       for (int attempt = 0;; ++attempt) {
-        for (int pass = 0; pass < 2 && c > 0; ++pass) {
-          double part[NB];
-#pragma unroll
-          for (int q = 0; q < NB; ++q) part[q] = 0;
-          for (int i = tid; i < n; i += NT) {
-            const double zc = Zb[i * NB + c];
-#pragma unroll
-            for (int q = 0; q < NB; ++q)
-              if (q < c) part[q] += Zb[i * NB + q] * zc;
-          }
-          block_sums(part, c);
-          for (int i = tid; i < n; i += NT) {
-            double s = 0;
-#pragma unroll
-            for (int q = 0; q < NB; ++q)
-              if (q < c) s += hs[q] * Zb[i * NB + q];
-            Zb[i * NB + c] -= s;
-          }
-          __syncthreads();
-        }
+        inblock(c);
         double part[NB];
 #pragma unroll
         for (int q = 0; q < NB; ++q) part[q] = 0;
@@ -1384,16 +1465,36 @@ __global__ __launch_bounds__(NT) void orth_kernel(const DevJob* __restrict__ job
           const double f = nu > 0 ? 1.0 / nu : 0.0;
           for (int i = tid; i < n; i += NT) Zb[i * NB + c] *= f;
           __syncthreads();
+          if (nu > 0 && nu < 0.5 * nrm0[c]) {
+            // most of the column went into the projections, which leave
+            // ~eps nrm0 / nu of the earlier columns in it: one more pass
+            if (b0 > 0) project(0, b0, c, c + 1);
+            inblock(c);
+            double p3[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) p3[q] = 0;
+            for (int i = tid; i < n; i += NT) p3[0] += Zb[i * NB + c] * Zb[i * NB + c];
+            block_sums(p3, 1);
+            const double g = hs[0] > 0 ? 1.0 / sqrt(hs[0]) : 0.0;
+            for (int i = tid; i < n; i += NT) Zb[i * NB + c] *= g;
+            __syncthreads();
+          }
           break;
         }
         // numerically dependent: restart from a hashed vector, pulled back into
         // the eigenspace by one solve with this vector's factors when the
-        // columns are inverse-iteration vectors (LU != null)
+        // columns are inverse-iteration vectors (LU != null).  The start is
+        // made orthogonal to every earlier column first, so that the solve,
+        // which amplifies all eigenvectors of a cluster alike, returns a
+        // direction of the cluster outside their span (inverse iteration
+        // with reorthogonalisation).
         for (int i = tid; i < n; i += NT)
           Zb[i * NB + c] =
               hash_unit(uint32_t(b0 + c) * 40503u + uint32_t(attempt + 1) * 977u + uint32_t(jb.job), uint32_t(i));
         __syncthreads();
         if (LU != nullptr) {
+          if (b0 > 0) project(0, b0, c, c + 1);
+          inblock(c);
           if (tid == 0) {
             const double zmax = lu_solve_inplace(LU + jb.lu_off, k, b0 + c, n, Zb + c, NB);
             nrm0[c] = zmax > 0 ? 1.0 / zmax : 1.0;
@@ -1887,9 +1988,11 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
 
 // Cholesky factor and inverse of the nb x nb Gram block G = Z_b^T Z_b of each
 // job (one workgroup per job, goff[job] < 0: no block): G = R^T R and
-// Rinv = R^-1 (upper triangular, nb x nb at goff).  With `test`, column c is
-// numerically dependent when R_cc <= 1e-3 of its norm on entry (norms at
-// w_off + c0 + c); a job with a dependent column or a non-positive pivot, or
+// Rinv = R^-1 (upper triangular, nb x nb at goff).  With `test`, the block is
+// left to the column-by-column path when some R_cc <= 0.1 of its column's
+// norm on entry (norms at w_off + c0 + c): R^-1 would scale what the two
+// projections left of the earlier columns up past ~10 eps.  A job with such
+// a column or a non-positive pivot, or
 // (without `test`) one whose ok flag is already 0, gets Rinv = I and ok = 0.
 template <int NB>
 __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__ jobs, const int64_t* __restrict__ goff,
@@ -1918,7 +2021,7 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   double rprev = 1.0;
   for (int a = 0; a < nb && gl; ++a) {
     const double dd = L[a * LD + a];
-    if (!(dd > 0) || (test && sqrt(dd) <= 1e-3 * norms[jb.w_off + c0 + a])) {
+    if (!(dd > 0) || (test && sqrt(dd) <= 0.1 * norms[jb.w_off + c0 + a])) {
       gl = false;
       break;
     }

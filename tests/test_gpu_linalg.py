@@ -159,6 +159,59 @@ def test_block_svd_matches_lapack(be):
                 assert np.abs(pu - pl).max() < 1e-8
 
 
+@pytest.mark.parametrize("n,mults", [(10, (4,)), (64, (7, 3)), (200, (7, 5, 2)), (700, (12, 4))])
+def test_eigh_exactly_degenerate(be, n, mults):
+    """Exactly repeated eigenvalues (their copies sit in separate split blocks
+    of the tridiagonal form): every vector is an eigenvector and the set is
+    orthonormal (LAPACK-level: residual <= 1e-12 |A|)."""
+    print(f"case eigh degenerate n={n} {mults}", flush=True)
+    rng = np.random.default_rng(n)
+    lam = list(rng.uniform(-1, 1, n - sum(mults)))
+    for j, m in enumerate(mults):
+        lam += [0.25 * (j + 1) - 0.6] * m
+    q = _orth(rng, n, n)
+    a = (q * np.array(lam)) @ q.T
+    a = (a + a.T) / 2
+    w, v = be.eigh_lowest(a, n)
+    assert np.abs(w - np.sort(lam)).max() < 1e-13 * n
+    assert np.abs(a @ v - v * w).max() < 1e-12 * np.abs(a).max() * np.sqrt(n)
+    assert np.abs(v.T @ v - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("nb,m", [(8, 4), (33, 3), (5, 40)])
+def test_eigh_block_diagonal_copies(be, nb, m):
+    """m identical diagonal blocks, rows permuted: the tridiagonal form splits
+    exactly (zero off-diagonals) into copies with bit-equal eigenvalues."""
+    print(f"case eigh block copies {nb}x{m}", flush=True)
+    rng = np.random.default_rng(nb * 100 + m)
+    b = rng.standard_normal((nb, nb))
+    b = b + b.T
+    a = np.kron(np.eye(m), b)
+    p = rng.permutation(nb * m)
+    a = a[np.ix_(p, p)]
+    n = nb * m
+    w, v = be.eigh_lowest(a, n)
+    assert np.abs(w - np.linalg.eigvalsh(a)).max() < 1e-13 * np.abs(a).max() * n
+    assert np.abs(a @ v - v * w).max() < 1e-12 * np.abs(a).max() * np.sqrt(n)
+    assert np.abs(v.T @ v - np.eye(n)).max() < 1e-12
+
+
+def test_block_svd_degenerate_reconstructs(be):
+    """Blocks whose singular values repeat exactly (the bond blocks of a sum of
+    identical operators): U has orthonormal columns and U (U^T X) = X."""
+    print("case block svd degenerate", flush=True)
+    rng = np.random.default_rng(5)
+    blocks = []
+    for m, n, s in [(10, 6, [1, .5, .5, .5, .5, .298]), (16, 9, [1] + [.5] * 7 + [.444]),
+                    (40, 30, [2.0] * 10 + list(np.geomspace(1, 1e-6, 20)))]:
+        k = len(s)
+        blocks.append(_orth(rng, m, k) @ np.diag(s) @ _orth(rng, n, k).T)
+    out, nk, nd, err = be.block_svd_truncate(blocks, 10 ** 6, 0.0, degeneracy_tol=0.0)
+    for b, (u, s, c) in zip(blocks, out):
+        assert np.abs(u.T @ u - np.eye(len(s))).max() < 1e-12
+        assert np.abs(u @ c - b).max() < 1e-13 * np.abs(b).max() * 10
+
+
 def test_eigh_lowest(be):
     print("case eigh_lowest", flush=True)
     # faer.rs:846-864

@@ -83,3 +83,17 @@ def test_ed_bonds_matches_chain_ed():
     chain = [(i, i + 1) for i in range(n - 1)]
     assert abs(hubbard_bonds_ground_energy(n, chain, 3, 3, 1.0, 4.0) - hubbard_ground_energy(n, 3, 3, 1.0, 4.0)) < 1e-12
     assert abs(hubbard_bonds_ground_energy(n, chain, 3, 3, 1.0, 0.0) - free_fermion_energy(n, 3, 3)) < 1e-12
+
+
+def test_add_mpo_is_the_weighted_sum():
+    n = 5
+    a = tk_hip.chain_mpo("heisenberg", n)
+    b = tk_hip.compile_mpo(tk_hip.heisenberg_opsum(tk_hip.cylinder_bonds(1, 5) + [(0, 2)], n, J=0.7), n)
+    s = tk_hip.add_mpo(a, b, 0.5, -2.0)
+    want = 0.5 * _dense_from_mpo(a) - 2.0 * _dense_from_mpo(b)
+    assert np.abs(_dense_from_mpo(s) - want).max() < 1e-13
+    for W, Wa, Wb in zip(s[1:], a[1:], b[1:]):
+        assert W.w.shape[2] == Wa.w.shape[2] + Wb.w.shape[2]
+        assert len({tk_hip._qkey(q) for q, _ in W.wl}) == len(W.wl)  # one sector per charge
+    with pytest.raises(ValueError):
+        tk_hip.add_mpo(a, a[:-1])
