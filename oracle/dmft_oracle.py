@@ -213,6 +213,26 @@ def aim_green(eps_imp, U, eps_chain, v_chain, n_up, n_dn, z):
     return e0, g
 
 
+def aim_green_time(eps_imp, U, eps_chain, v_chain, n_up, n_dn, t):
+    """Retarded G_up(t) = -i <{c_0up(t), c+_0up}> from the same full
+    diagonalisation of the N+-1 sectors as aim_green:
+    -i sum_k a_k exp(-i p_k t) over the poles p_k and weights a_k of G(z)
+    (the real-time object of tk-dmft/src/spectral/tdvp.rs:55-108)."""
+    e0, psi, st = aim_ground_state(eps_imp, U, eps_chain, v_chain, n_up, n_dn)
+    t = np.asarray(t, dtype=np.float64)
+    g = np.zeros(t.shape, dtype=np.complex128)
+    for dagger, sec, sgn in ((True, (n_up + 1, n_dn), 1), (False, (n_up - 1, n_dn), -1)):
+        if sec[0] < 0 or sec[0] > len(eps_chain) + 1:
+            continue
+        st2, H2 = aim_sector_hamiltonian(eps_imp, U, eps_chain, v_chain, *sec)
+        phi = _apply_c(st, psi, st2, 0, dagger)
+        w, V = np.linalg.eigh(H2)
+        amp = (V.T @ phi) ** 2
+        p = sgn * (w - e0)
+        g += -1j * (np.exp(-1j * np.multiply.outer(t, p)) @ amp)
+    return e0, g
+
+
 # ---------------------------------------------------------------- MLP
 def mlp_unpack(p, dims):
     out, o = [], 0

@@ -322,7 +322,9 @@ class ImpuritySolver:
         self.bond_dim, self.sweeps, self.krylov = bond_dim, sweeps, krylov
         self.n_solves = 0
 
-    def ground_state(self, eps_chain, v_chain):
+    def _dmrg(self, eps_chain, v_chain):
+        """Two-site DMRG ground state at half filling (N_up, N_dn) =
+        (ceil(n/2), floor(n/2)): (energy, open engine, mpo, total charge)."""
         n = self.n_bath + 1
         mpo = aim_chain_mpo(self.eps_imp, self.U, eps_chain, v_chain)
         phys = [m.phys for m in mpo]
@@ -338,10 +340,56 @@ class ImpuritySolver:
             if e_prev is not None and abs(e - e_prev) < 1e-12 * max(1.0, abs(e)):
                 break
             e_prev = e
+        self.n_solves += 1
+        return e, d, mpo, total
+
+    def ground_state(self, eps_chain, v_chain):
+        n = self.n_bath + 1
+        e, d, _, total = self._dmrg(eps_chain, v_chain)
         psi = mps_to_dense(self.be, d, n)
         d.close()
-        self.n_solves += 1
         return e, psi, total
+
+    def green_time(self, eps_chain, v_chain, dt=0.05, t_max=20.0, max_bond=None, krylov_dim=30, krylov_tol=1e-12,
+                   adaptive_tol=1e-12):
+        """Retarded G_up(t) = -i <{c_0up(t), c+_0up}> of the impurity at
+        t_k = k dt, k = 0..round(t_max / dt), from the DMRG ground state by
+        real-time two-site TDVP (the G(t) stage of the reference's spectral
+        pipeline, tk-dmft/src/spectral/tdvp.rs:55-108): the particle part
+        n_p g_p(t) from phi_p = c+ psi0 / |c+ psi0| and the hole part
+        -n_h conj(g_h(t)) from phi_h = c psi0 / |c psi0|, with
+        g(t) = -i e^{i E0 t} <phi| e^{-i H t} |phi> (tk_hip.greens_function_real_time).
+        Everything stays in MPS form on the device.  Returns (t, G(t), info)."""
+        import time as _time
+        t0 = _time.perf_counter()
+        e0, d, mpo, total = self._dmrg(eps_chain, v_chain)
+        n = self.n_bath + 1
+        bonds = [d.bond(i) for i in range(n + 1)]
+        sites = [d.site(i) for i in range(n)]
+        d.close()
+        t_gs = _time.perf_counter() - t0
+        n_steps = int(round(t_max / dt))
+        mb = self.bond_dim if max_bond is None else max_bond
+        sym = tk_hip.SYM_U1U1
+        g = None
+        info = {"e0": e0, "t_ground_state": t_gs}
+        for name, op, dq in (("particle", tk_hip.F_CDAG_UP, (1, 0)), ("hole", tk_hip.F_C_UP, (-1, 0))):
+            t1 = _time.perf_counter()
+            x = tk_hip.Dmrg(self.ctx, mpo, sym, schedule=[mb])
+            x.set_state(bonds, sites)
+            x.prepare()
+            w = x.apply_site_op(op, dq)
+            pb = [x.bond(i) for i in range(n + 1)]
+            ps = [x.site(i) for i in range(n)]
+            x.close()
+            ts, gp = tk_hip.greens_function_real_time(self.ctx, mpo, sym, pb, ps, e0, dt, n_steps, max_bond=mb,
+                                                      krylov_dim=krylov_dim, krylov_tol=krylov_tol,
+                                                      svd_cutoff=1e-12, adaptive_tol=adaptive_tol)
+            part = w * gp if name == "particle" else -w * np.conj(gp)
+            g = part if g is None else g + part
+            info[f"weight_{name}"] = w
+            info[f"t_{name}"] = _time.perf_counter() - t1
+        return ts, g, info
 
     def _fraction(self, op_site0, psi, eps_chain, v_chain, sector):
         n = self.n_bath + 1
@@ -380,6 +428,50 @@ class ImpuritySolver:
         gp = hybridization(self.ctx, CHAIN, np.concatenate([ap - e0, bp]), z)[0]
         gh = hybridization(self.ctx, CHAIN, np.concatenate([-(ah - e0), bh]), z)[0]
         return e0, gp + gh
+
+
+def green_time_many(baths, n_bath, U, device=0, workers=4, bond_dim=256, progress=None, **green_kw):
+    """G(t) (ImpuritySolver.green_time) of several independent impurity
+    problems (the reference's independent-solve parallelism,
+    ARCHITECTURE.md:2523-2525, on one device): `workers` host threads, each
+    with its own context (HIP stream and memory pool), take the problems in
+    turn, so that the kernels of different solves run concurrently on the
+    GPU.  baths: [(eps_chain, v_chain)]; progress(i, info), when given, is
+    called from the worker thread after problem i.  Returns [(t, G(t), info)]
+    in the order of `baths`."""
+    import threading
+    out = [None] * len(baths)
+    errors = []
+    nxt = [0]
+    lock = threading.Lock()
+
+    def work():
+        ctx = tk_hip.Context(device)
+        try:
+            solver = ImpuritySolver(ctx, n_bath, U, bond_dim=bond_dim)
+            while True:
+                with lock:
+                    if errors or nxt[0] >= len(baths):
+                        return
+                    i = nxt[0]
+                    nxt[0] += 1
+                out[i] = solver.green_time(*baths[i], **green_kw)
+                if progress is not None:
+                    progress(i, out[i][2])
+        except Exception as e:  # reported by the caller's thread
+            with lock:
+                errors.append(e)
+        finally:
+            ctx.close()
+
+    threads = [threading.Thread(target=work) for _ in range(max(1, min(int(workers), len(baths))))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return out
 
 
 # ---------------------------------------------------------------- bath fit

@@ -114,6 +114,28 @@ def test_ed_free_fermions():
     assert np.max(np.abs(g - g_free)) < 1e-12
 
 
+def test_ed_green_time_free_fermions_and_limits():
+    """U = 0: G^R_00(t) = -i [exp(-i h t)]_00 of the one-particle chain matrix
+    (independent of the many-body ED); the atomic limit -i (e^{-i U t / 2}
+    + e^{i U t / 2}) / 2; G^R(0) = -i (the anticommutator) for U = 4."""
+    rng = np.random.default_rng(6)
+    eps_imp, eps, v = 0.1, rng.uniform(-1, 1, 3), rng.uniform(0.3, 0.8, 3)
+    h = np.diag(np.concatenate([[eps_imp], eps]))
+    for k in range(3):
+        h[k, k + 1] = h[k + 1, k] = v[k]
+    t = np.linspace(0, 7, 29)
+    _, g = O.aim_green_time(eps_imp, 0.0, eps, v, 2, 2, t)
+    w, q = np.linalg.eigh(h)
+    free = -1j * (np.exp(-1j * np.multiply.outer(t, w)) @ (q[0] ** 2))
+    assert np.max(np.abs(g - free)) < 1e-12
+    U = 4.0
+    _, gu = O.aim_green_time(-U / 2, U, [], [], 1, 0, t)
+    _, gd = O.aim_green_time(-U / 2, U, [], [], 0, 1, t)
+    assert np.max(np.abs(0.5 * (gu + gd) + 1j * np.cos(U * t / 2))) < 1e-12
+    _, g4 = O.aim_green_time(-U / 2, U, eps, v, 2, 2, [0.0])
+    assert abs(g4[0] + 1j) < 1e-12
+
+
 def test_mlp_gradient_finite_difference():
     dims = [6, 5, 4, 3]
     rng = np.random.default_rng(7)
