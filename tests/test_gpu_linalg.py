@@ -212,6 +212,49 @@ def test_block_svd_degenerate_reconstructs(be):
         assert np.abs(u @ c - b).max() < 1e-13 * np.abs(b).max() * 10
 
 
+def _graded(rng, k, n_big):
+    """n_big values spread over 1 .. 3e-5 (above the first Gram level's line)
+    and the rest over 2.5e-5 .. 3e-15, kept away from the 1e-12 cut."""
+    big = 10.0 ** np.linspace(0, -4.4, n_big)
+    small = 10.0 ** rng.uniform(-14.5, -4.6, k - n_big)
+    small = np.where(np.abs(np.log10(small) + 12) < 0.5, small * 10, small)
+    return np.sort(np.concatenate([big, small]))[::-1]
+
+
+@pytest.mark.parametrize("flat", [False, True])
+def test_block_svd_large_blocks_match_lapack(be, flat):
+    """Blocks of 256 to 700 rows with graded spectra (few values above the
+    first Gram level's line, 3e-5 sigma_max) or flat ones: LAPACK's kept
+    counts, values to the accuracy of _sigma_ok, an orthonormal U with
+    C = U^T X, the kept subspace, and the first level's directions to 5e-8."""
+    print(f"case block svd large blocks flat={flat}", flush=True)
+    rng = np.random.default_rng(31 + flat)
+    blocks, sig = [], []
+    for (m, n, nbig) in [(500, 600, 20), (300, 280, 35), (700, 700, 12), (256, 300, 50)]:
+        k = min(m, n)
+        s = 10.0 ** rng.uniform(-1, 0, k) if flat else _graded(rng, k, nbig)
+        s = np.sort(s)[::-1]
+        blocks.append(_orth(rng, m, k) @ np.diag(s) @ _orth(rng, n, k).T)
+        sig.append(np.linalg.svd(blocks[-1], compute_uv=False))
+    out, nk, nd, err = be.block_svd_truncate(blocks, 1500, 1e-12)
+    keep, nk2, nd2, err2 = tk_hip.truncation_select(sig, 1500, 1e-12, 1, 1e-12, 1e-26)
+    assert [len(o[1]) for o in out] == keep and nk == nk2 and nd == nd2
+    smax = max(x[0] for x in sig)
+    for b, (u, s, c), sl in zip(blocks, out, sig):
+        k = len(s)
+        assert _sigma_ok(s, sl[:k], smax)
+        assert np.abs(u.T @ u - np.eye(k)).max() < 1e-10
+        assert np.abs(c - u.T @ b).max() <= 1e-12 * sl[0]
+        ul, _, _ = np.linalg.svd(b, full_matrices=False)
+        if k < len(sl) and sl[k - 1] - sl[k] > 1e-3 * sl[0]:
+            assert np.abs(u @ u.T - ul[:, :k] @ ul[:, :k].T).max() < 1e-8
+        # the level-one directions (sigma >= 3.2e-5 sigma_max) to working precision
+        k1 = int(np.sum(sl >= 3.2e-5 * smax))
+        if 0 < k1 <= k:
+            p1 = ul[:, :k1] @ ul[:, :k1].T
+            assert np.abs(p1 @ u[:, :k1] - u[:, :k1]).max() < 5e-8
+
+
 def test_eigh_lowest(be):
     print("case eigh_lowest", flush=True)
     # faer.rs:846-864
