@@ -1,6 +1,8 @@
 // Davidson eigensolver with device-resident FP64 Krylov vectors (see davidson.hpp).
 #include "davidson.hpp"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -301,11 +303,21 @@ struct PlanUser {
   HeffPlan* plan;
 };
 // The plan's own precision replays its graph; FP64 products of a
-// low-precision plan (the refinement) take the eager FP64 path.
+// low-precision plan (the refinement) take the eager FP64 path.  With
+// TK_HEFF_GRAPHS=0 in the environment every product is launched eagerly (the
+// same kernels and work lists), for tools that cannot follow graph launches.
+bool heff_graphs() {
+  static const bool on = [] {
+    const char* e = std::getenv("TK_HEFF_GRAPHS");
+    return e == nullptr || std::atoi(e) != 0;
+  }();
+  return on;
+}
 void plan_matvec(const double* x, double* y, int fp64, void* user) {
   HeffPlan* p = static_cast<PlanUser*>(user)->plan;
   if (fp64 && p->prec() != Prec::FP64) p->apply_fp64(x, y);
-  else p->apply_graph(x, y);
+  else if (heff_graphs()) p->apply_graph(x, y);
+  else p->apply(x, y);
 }
 }  // namespace
 
