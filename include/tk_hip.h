@@ -432,6 +432,24 @@ tk_status tk_dmrg_sweep(tk_dmrg* d, tk_sweep_stats* out);
  * already done are kept (the state stays a valid MPS); the engine must be
  * prepared again (tk_dmrg_prepare) before the next sweep. */
 tk_status tk_dmrg_set_cancel_flag(tk_dmrg* d, const int32_t* flag);
+/* Subspace expansion for tk_dmrg_sweep_single_site (the expansion_alpha /
+ * expansion_vectors of TdvpStabilizationConfig, ARCHITECTURE.md:1735-1790,
+ * applied to the one-site sweep, whose reference step sweep/mod.rs:451-509
+ * never grows a bond).  With alpha > 0 every centre move that a one-site
+ * solve follows adds alpha |theta| r / |r| to the two-site tensor
+ * theta = A_i B_{i+1}, r = H_eff theta with its theta component removed, and
+ * splits the result with the two-site truncation rule at
+ * min(schedule D, current D + max_new) (max_new <= 0: the schedule's D), so
+ * the bond takes up the directions H couples to.  The perturbation leaves the
+ * bases O(alpha) away from the ground state's (the energy settles about
+ * alpha^2 |E| above it); with adaptive != 0 (adaptive_expansion) alpha drops
+ * tenfold after every single-site sweep whose energy change is within
+ * 10 alpha^2 max(1, |E|), and expansion stops below 1e-10.  alpha = 0 (the
+ * default) restores the untruncated centre move; alpha < 0 or non-finite is
+ * TK_ERR_INVALID_ARG. */
+tk_status tk_dmrg_set_subspace_expansion(tk_dmrg* d, double alpha, int64_t max_new, int32_t adaptive);
+/* The expansion's current alpha (after the adaptive decreases). */
+tk_status tk_dmrg_subspace_expansion_alpha(tk_dmrg* d, double* alpha);
 /* dmrg_step_single_site (tk-dmrg/src/sweep/mod.rs:451-509) over a whole sweep.
  * The reference solves the one-site problem but neither writes the eigenvector
  * back nor moves the centre; here every site is replaced by the lowest

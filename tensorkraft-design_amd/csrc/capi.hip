@@ -940,6 +940,20 @@ tk_status tk_dmrg_set_cancel_flag(tk_dmrg* d, const int32_t* flag) {
   });
 }
 
+tk_status tk_dmrg_set_subspace_expansion(tk_dmrg* d, double alpha, int64_t max_new, int32_t adaptive) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d) throw tk::ArgError("null argument");
+    d->e->set_subspace_expansion(alpha, max_new, adaptive != 0);
+  });
+}
+
+tk_status tk_dmrg_subspace_expansion_alpha(tk_dmrg* d, double* alpha) {
+  return guard(d ? d->ctx : nullptr, [&] {
+    if (!d || !alpha) throw tk::ArgError("null argument");
+    *alpha = d->e->subspace_expansion_alpha();
+  });
+}
+
 tk_status tk_dmrg_apply_site_op(tk_dmrg* d, const double* op, const int32_t* dq, double* norm2) {
   return guard(d ? d->ctx : nullptr, [&] {
     if (!d || !op || !dq) throw tk::ArgError("null argument");

@@ -318,6 +318,18 @@ double DmrgEngine::evolve_one_site(int i, double alpha, int krylov_dim, double t
   return std::log(r.norm_out) - std::log(r.norm_in);
 }
 
+namespace {
+// x . y of two device vectors, on the host
+double dot_host(Context* ctx, const double* x, const double* y, int64_t n) {
+  DBuf<double> o(ctx, 1);
+  dot_many(ctx, x, n, 1, y, n, o.get());
+  double h = 0;
+  o.download(&h, 1);
+  ctx->sync();
+  return h;
+}
+}  // namespace
+
 StepStats DmrgEngine::step_single_site(int i, bool lr) {
   StepStats st;
   st.site = i;
@@ -340,16 +352,43 @@ StepStats DmrgEngine::step_single_site(int i, bool lr) {
   st.converged = dr.converged;
   st.residual = dr.residual;
   st.dim = psi.total;
-  // move the centre: untruncated split of the site with its neighbour
+  // move the centre: untruncated split of the site with its neighbour, or
+  // with subspace expansion a truncated split of the perturbed pair (only
+  // where a one-site solve follows: not into site 0 at the end of the sweep)
   const int j = lr ? i : i - 1;  // left site of the pair
   st.bond_dim = bonds_[lr ? i + 1 : i].total;
   if (lr ? i + 1 < n_ : i > 0) {
+    const bool expand = expand_alpha_ > 0 && (lr || i > 1);
     const BlockTable pair = make_block_table(cfg_.sym, {cfg_.mpo[j].phys, cfg_.mpo[j + 1].phys, bonds_[j], bonds_[j + 2]},
                                              {Dir::Out, Dir::Out, Dir::Out, Dir::In}, QN{});
     DBuf<double> th(ctx_, size_t(std::max<int64_t>(1, pair.total)));
     contract_two_site(j, pair, th.get());
     StepStats sp;
-    split_two_site(j, lr, pair, th.get(), INT64_MAX, 0.0, kLambdaZero, &sp);
+    if (expand) {
+      HeffPlan plan(ctx_, Prec::FP64, L_[j].t, L_[j].d.get(), R_[j + 2].t, R_[j + 2].d.get(), cfg_.mpo[j],
+                    cfg_.mpo[j + 1]);
+      if (plan.psi().total != pair.total || plan.psi().n_blocks() != pair.n_blocks())
+        throw std::logic_error("expansion: the plan's two-site table differs from the pair's");
+      DBuf<double> r(ctx_, size_t(std::max<int64_t>(1, pair.total)));
+      plan.apply_fp64(th.get(), r.get());
+      // r_perp = r - (theta.r / theta.theta) theta, theta += alpha |theta| r_perp / |r_perp|
+      const double tt = dot_host(ctx_, th.get(), th.get(), pair.total);
+      const double tr = dot_host(ctx_, th.get(), r.get(), pair.total);
+      if (tt > 0) {
+        axpy(ctx_, -tr / tt, th.get(), r.get(), pair.total);
+        const double rr = dot_host(ctx_, r.get(), r.get(), pair.total);
+        if (rr > 1e-30 * tt * std::max(1.0, tr * tr / (tt * tt)))
+          axpy(ctx_, expand_alpha_ * std::sqrt(tt / rr), r.get(), th.get(), pair.total);
+      }
+      const int64_t cur = bonds_[j + 1].total;
+      const int64_t sched =
+          cfg_.bond_schedule[std::min<size_t>(size_t(sweeps_), cfg_.bond_schedule.size() - 1)];
+      const int64_t cap = expand_max_new_ > 0 ? std::min(sched, cur + expand_max_new_) : sched;
+      const double floor = std::max(cfg_.svd_cutoff * cfg_.svd_cutoff, kLambdaZero);
+      split_two_site(j, lr, pair, th.get(), std::max(cap, cur), cfg_.svd_cutoff, floor, &sp);
+    } else {
+      split_two_site(j, lr, pair, th.get(), INT64_MAX, 0.0, kLambdaZero, &sp);
+    }
     st.truncation_error = sp.truncation_error;
     st.bond_dim = sp.bond_dim;
     st.svd_nmax = sp.svd_nmax;
@@ -367,6 +406,14 @@ StepStats DmrgEngine::step_single_site(int i, bool lr) {
     cur_->t_env += t4 - t3;
   }
   return st;
+}
+
+void DmrgEngine::set_subspace_expansion(double alpha, int64_t max_new, bool adaptive) {
+  if (!(alpha >= 0) || !std::isfinite(alpha)) throw ArgError("subspace expansion: alpha must be finite and >= 0");
+  expand_alpha_ = alpha;
+  expand_max_new_ = max_new;
+  expand_adaptive_ = adaptive;
+  expand_have_prev_ = false;
 }
 
 SweepStats DmrgEngine::sweep_single_site() {
@@ -396,6 +443,15 @@ SweepStats DmrgEngine::sweep_single_site() {
   }
   energy_ = steps_.back().energy;
   s.energy = energy_;
+  if (expand_alpha_ > 0 && expand_adaptive_) {
+    const double a2 = expand_alpha_ * expand_alpha_;
+    if (expand_have_prev_ && std::fabs(energy_ - expand_e_prev_) <= 10.0 * a2 * std::max(1.0, std::fabs(energy_))) {
+      expand_alpha_ *= 0.1;
+      if (expand_alpha_ < 1e-10) expand_alpha_ = 0.0;
+    }
+    expand_e_prev_ = energy_;
+    expand_have_prev_ = true;
+  }
   ++sweeps_;
   cur_ = nullptr;
   return s;

@@ -96,6 +96,22 @@ class DmrgEngine {
   // step (null clears the flag).  The state stays a valid MPS; prepare()
   // again before the next sweep.
   void set_cancel_flag(const int32_t* flag) { cancel_ = flag; }
+  // Subspace expansion of the single-site sweep (the expansion of
+  // TdvpStabilizationConfig, ARCHITECTURE.md:1735-1790: expansion_alpha,
+  // expansion_vectors).  With alpha > 0 every centre move that a one-site
+  // solve follows perturbs the two-site tensor theta = A_i B_{i+1} by
+  // alpha |theta| r_perp / |r_perp|, r_perp = H theta with its theta component
+  // removed, and splits it with the two-site rule at
+  // min(schedule D, current D + max_new) (max_new <= 0: the schedule's D):
+  // directions the Hamiltonian couples to enter the bond, which a plain
+  // single-site sweep can never grow.  alpha = 0 (the default) keeps the
+  // untruncated, bond-preserving centre move.  The perturbation leaves the
+  // bases of the bonds it passed O(alpha) away from the ground state's, so the
+  // energy settles ~alpha^2 |E| above it; with `adaptive` (the reference's
+  // adaptive_expansion) alpha drops tenfold after every sweep whose energy
+  // change is within 10 alpha^2 max(1, |E|), and expansion stops below 1e-10.
+  void set_subspace_expansion(double alpha, int64_t max_new, bool adaptive);
+  double subspace_expansion_alpha() const { return expand_alpha_; }
   double energy() const { return energy_; }
   int sweeps_done() const { return sweeps_; }
   const std::vector<QIndex>& bonds() const { return bonds_; }
@@ -171,6 +187,11 @@ class DmrgEngine {
   std::vector<StepStats> steps_;
   SweepStats* cur_ = nullptr;
   const int32_t* cancel_ = nullptr;
+  double expand_alpha_ = 0.0;
+  int64_t expand_max_new_ = 0;
+  bool expand_adaptive_ = true;
+  double expand_e_prev_ = 0.0;
+  bool expand_have_prev_ = false;
   void check_cancel();
 };
 

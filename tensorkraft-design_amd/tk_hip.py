@@ -150,6 +150,8 @@ def lib():
         "tk_regularized_svd_inverse": (I32, [P, I64, I64, I64, pD, P, P, D, P]),
         "tk_bond_schedule_warmup": (I32, [I64, I64, I64, I64, pI64, pI64]),
         "tk_dmrg_set_cancel_flag": (I32, [P, pI32]),
+        "tk_dmrg_set_subspace_expansion": (I32, [P, D, I64, I32]),
+        "tk_dmrg_subspace_expansion_alpha": (I32, [P, C.POINTER(C.c_double)]),
         "tk_davidson": (I32, [P, I64, MATVEC_FN, P, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_lanczos": (I32, [P, I64, MATVEC_FN, P, D, I32, I32, P, pD, pD, pI32]),
         "tk_exp_krylov": (I32, [P, I64, MATVEC_FN, P, P, D, I32, D, D, P, pI32]),
@@ -210,7 +212,7 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_block_gemm",
             "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_truncate_svd", "tk_truncation_select",
             "tk_block_svd_truncate", "tk_eigh_lowest", "tk_qr", "tk_regularized_svd_inverse",
-            "tk_bond_schedule_warmup", "tk_dmrg_set_cancel_flag", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
+            "tk_bond_schedule_warmup", "tk_dmrg_set_cancel_flag", "tk_dmrg_set_subspace_expansion", "tk_dmrg_subspace_expansion_alpha", "tk_davidson", "tk_lanczos", "tk_exp_krylov", "tk_dmrg_tdvp_step",
             "tk_dmrg_sweep_single_site", "tk_dmrg_apply_site_op", "tk_dmrg_overlap",
             "tk_ctx_timing", "tk_ctx_tune", "tk_tune_table", "tk_tune_save", "tk_tune_load", "tk_tune_clear",
             "tk_heff_tiling", "tk_heff_apply_graph",
@@ -1743,6 +1745,23 @@ class Dmrg:
         """Raise (or clear) the cancel flag; a running sweep stops before its
         next step with TK_ERR_CANCELLED.  Safe from another thread."""
         self.cancel_flag[0] = 1 if on else 0
+
+    def subspace_expansion(self, alpha=1e-4, max_new=0, adaptive=True):
+        """Subspace expansion of sweep_single_site (expansion_alpha,
+        expansion_vectors and adaptive_expansion of TdvpStabilizationConfig,
+        ARCHITECTURE.md:1735-1790): each centre move perturbs the two-site
+        tensor by alpha times the normalised part of H theta orthogonal to
+        theta and splits it with the two-site rule at min(schedule D,
+        D + max_new) (max_new <= 0: the schedule's D); adaptive lowers alpha
+        tenfold once a sweep's energy change is at its alpha^2 floor.
+        alpha = 0 turns it off."""
+        _check(self.ctx.L.tk_dmrg_set_subspace_expansion(self.h, float(alpha), int(max_new), 1 if adaptive else 0),
+               self.ctx.h)
+
+    def subspace_expansion_alpha(self):
+        a = C.c_double(0)
+        _check(self.ctx.L.tk_dmrg_subspace_expansion_alpha(self.h, C.byref(a)), self.ctx.h)
+        return a.value
 
     def close(self):
         if self.h and self.ctx.h:
