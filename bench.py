@@ -615,6 +615,85 @@ def bench_sweep_c4(args):
     return rec
 
 
+def bench_permute(args):
+    """Block-sparse data movement on the config-4 two-site tensor (legs
+    s_i, s_j (spin-1/2), a_L, a_R (the U(1) bond of 4096 states in 31
+    sectors), flux 0: 4.66 M FP64 entries): BlockSparseTensor::permute with
+    a_L <-> a_R (every block transposed through the LDS-staged tiles), the
+    permute (a_L, s_i, s_j, a_R) that keeps the innermost axis (gathered
+    rows), and fuse_legs of (s_i, s_j, a_L) (strided chunk copies), each timed
+    with HIP events around its kernels on the device-resident tensor (the
+    host-built work lists and their uploads excluded); a step is one call.
+    value = the transposing permute's algorithmic bandwidth (16 bytes per
+    entry).  Each result is checked by the inverse operation reproducing the
+    input."""
+    import ctypes as C
+
+    import numpy as np
+    import tk_hip
+
+    ctx = tk_hip.Context(0)
+    sym = tk_hip.SYM_U1
+    bond = tk_hip.synthetic_u1_bond(4096)
+    legs = [tk_hip.SPIN_PHYS, tk_hip.SPIN_PHYS, bond, bond]
+    dirs = [tk_hip.DIR_OUT, tk_hip.DIR_OUT, tk_hip.DIR_OUT, tk_hip.DIR_IN]
+    _, _, _, tot = tk_hip.block_table(sym, legs, dirs, (0, 0))
+    x = np.random.default_rng(7).standard_normal(tot)
+    dx, dy, dz = ctx.to_device(x), ctx.empty(tot), ctx.empty(tot)
+    qs, arr, d, f = tk_hip._leg_args(legs, dirs, (0, 0))
+    pI = C.POINTER(C.c_int32)
+
+    def permute(p, src, dst, lg, dr):
+        q2, a2, d2, f2 = tk_hip._leg_args(lg, dr, (0, 0))
+        pp = np.ascontiguousarray(p, dtype=np.int32)
+        tk_hip._check(ctx.L.tk_block_permute(ctx.h, sym, len(lg), a2, d2.ctypes.data_as(pI), f2.ctypes.data_as(pI),
+                                             pp.ctypes.data_as(pI), C.c_void_p(src.ptr), tot, C.c_void_p(dst.ptr),
+                                             tot), ctx.h)
+
+    res = {}
+    for name, perm in (("transpose_aL_aR", [0, 1, 3, 2]), ("keep_innermost", [2, 0, 1, 3])):
+        for _ in range(args.warmup):
+            permute(perm, dx, dy, legs, dirs)
+        ctx.timing(True)
+        for _ in range(args.steps):
+            permute(perm, dx, dy, legs, dirs)
+        tim = ctx.timing_read()
+        ctx.timing(False)
+        inv = [perm.index(k) for k in range(4)]
+        permute(inv, dy, dz, [legs[k] for k in perm], [dirs[k] for k in perm])
+        ok = bool(np.array_equal(dz.get(), x))
+        parts = {k: v for k, v in tim.items() if k in ("copy_tiles", "copy_rows", "permute_gather")}
+        us = 1e3 * sum(v["ms"] for v in parts.values()) / args.steps
+        res[name] = {"us": round(us, 2), "GB/s": round(16.0 * tot / (us * 1e-6) / 1e9, 1), "round_trip_exact": ok,
+                     "kernels": {k: {"us": round(1e3 * v["ms"] / args.steps, 2),
+                                     "GB/s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)} for k, v in parts.items()}}
+    for _ in range(args.warmup):
+        tk_hip._check(ctx.L.tk_block_fuse(ctx.h, sym, 4, arr, d.ctypes.data_as(pI), f.ctypes.data_as(pI), 0, 3,
+                                          C.c_void_p(dx.ptr), tot, C.c_void_p(dy.ptr), tot), ctx.h)
+    ctx.timing(True)
+    for _ in range(args.steps):
+        tk_hip._check(ctx.L.tk_block_fuse(ctx.h, sym, 4, arr, d.ctypes.data_as(pI), f.ctypes.data_as(pI), 0, 3,
+                                          C.c_void_p(dx.ptr), tot, C.c_void_p(dy.ptr), tot), ctx.h)
+    t = ctx.timing_read()["copy_rows"]
+    ctx.timing(False)
+    us = 1e3 * t["ms"] / args.steps
+    res["fuse_si_sj_aL"] = {"us": round(us, 2), "GB/s": round(t["bytes"] / args.steps / (us * 1e-6) / 1e9, 1)}
+    for b in (dx, dy, dz):
+        b.free()
+    ctx.close()
+    if not all(v.get("round_trip_exact", True) for v in res.values()):
+        raise SystemExit("bench.py: a permute round trip did not reproduce its input")
+    gbs = res["transpose_aL_aR"]["GB/s"]
+    return {"metric": "block-sparse permute bandwidth (a_L <-> a_R on the 4.66 M-entry config-4 two-site tensor)",
+            "value": gbs, "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(res["transpose_aL_aR"]["us"] / 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "block_permute_cylinder_psi_D4096", "entries": int(tot), "blocks": "U(1), 31 sectors"},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 4),
+                         "traffic": None, "kernel": "copy_tiles_kernel"},
+            "results": res, "cpu_baseline": None}
+
+
 def bench_impurity(args):
     """The DMFT impurity workload: args.steps independent impurity problems
     (impurity + 7 bath sites in chain geometry, U = 4, seeded baths), each
@@ -668,7 +747,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="sweep",
-                    choices=["sweep", "sweep-c4", "heff", "heff-c4", "copy", "tdvp", "sweep1", "impurity"])
+                    choices=["sweep", "sweep-c4", "heff", "heff-c4", "copy", "tdvp", "sweep1", "impurity", "permute"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--sites", type=int, default=64)
@@ -697,6 +776,10 @@ def main():
         args.steps = (4 if args.workload == "tdvp" else 1) if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         rec = bench_tdvp(args) if args.workload == "tdvp" else bench_sweep1(args)
+    elif args.workload == "permute":
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 3 if args.warmup is None else args.warmup
+        rec = bench_permute(args)
     elif args.workload == "impurity":
         args.steps = 32 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup

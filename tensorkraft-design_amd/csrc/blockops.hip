@@ -12,39 +12,6 @@ namespace tk {
 namespace {
 
 constexpr int kMaxRank = 8;
-constexpr int64_t kPermChunk = 4096;  // elements per work item
-
-// out block = in block with axes permuted: element e of the out block (row-major
-// over dims_out) reads sum_k idx_k * in_stride[k] of the in block.
-struct PermJob {
-  int64_t in_off, out_off, size;
-  int64_t dims_out[kMaxRank];
-  int64_t in_stride[kMaxRank];
-  int32_t rank, pad;
-};
-struct PermItem {
-  int64_t e0;
-  int32_t job, pad;
-};
-
-__global__ __launch_bounds__(256) void permute_kernel(const PermItem* __restrict__ items,
-                                                      const PermJob* __restrict__ jobs,
-                                                      const double* __restrict__ in, double* __restrict__ out) {
-  const PermItem it = items[blockIdx.x];
-  const PermJob jb = jobs[it.job];
-  const int64_t e1 = min(it.e0 + kPermChunk, jb.size);
-  for (int64_t e = it.e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    int64_t rest = e, src = 0;
-    for (int k = jb.rank - 1; k >= 0; --k) {
-      const int64_t d = jb.dims_out[k];
-      const int64_t i = rest % d;
-      rest /= d;
-      src += i * jb.in_stride[k];
-    }
-    out[jb.out_off + e] = in[jb.in_off + src];
-  }
-}
-
 void check_rank(const std::vector<QIndex>& legs, const std::vector<Dir>& dirs) {
   if (legs.empty() || legs.size() > size_t(kMaxRank) || dirs.size() != legs.size())
     throw ArgError("block op: rank outside 1..8 or legs/directions mismatch");
@@ -108,40 +75,78 @@ void block_permute(Context* ctx, Sym sym, const std::vector<QIndex>& legs, const
   const BlockTable tin = make_block_table(sym, legs, dirs, flux);
   const BlockTable tout = make_block_table(sym, pl, pd, flux);
   if (tin.total > in_n || tout.total > out_n) throw std::out_of_range("permute: buffers smaller than the tables");
-  std::vector<PermJob> jobs;
-  std::vector<PermItem> items;
+  // Every block becomes a set of 2-D copies over two of its axes, the other
+  // axes fixed (slabs).  When the input's innermost axis a is not the
+  // output's innermost axis b, each slab over (a, b) is a transpose
+  // out[a][b] = in[b][a] that the copy kernel stages through LDS in 32 x 32
+  // tiles; otherwise the innermost axis stays innermost and each slab over
+  // (r, innermost), r the longest other axis, is a copy of contiguous rows
+  // with fixed row strides on both sides.  Reads and writes are coalesced
+  // rows either way.
+  std::vector<CopyJob> cjobs;
   std::vector<int32_t> osecs(static_cast<size_t>(rank));
+  const int pa = [&] {  // output position of the input's innermost axis
+    for (int k = 0; k < rank; ++k)
+      if (perm[k] == rank - 1) return k;
+    return rank - 1;
+  }();
   for (int64_t b = 0; b < tin.n_blocks(); ++b) {
     for (int k = 0; k < rank; ++k) osecs[k] = tin.s(b, perm[k]);
     const int64_t ob = tout.find(osecs.data());
     if (ob < 0) throw std::logic_error("permute: block missing from the output table");
-    PermJob j{};
-    j.in_off = tin.offset[b];
-    j.out_off = tout.offset[ob];
-    j.size = tin.size(b);
-    j.rank = rank;
-    int64_t st[kMaxRank];
+    const int64_t in_off = tin.offset[b], out_off = tout.offset[ob], size = tin.size(b);
+    if (out_off + size > tout.total || in_off + size > tin.total)
+      throw std::out_of_range("permute: block outside its buffer");
+    int64_t st[kMaxRank], ost[kMaxRank], dout[kMaxRank], istr[kMaxRank];
     int64_t s = 1;
     for (int l = rank - 1; l >= 0; --l) {
       st[l] = s;
       s *= tin.dim(b, l);
     }
     for (int k = 0; k < rank; ++k) {
-      j.dims_out[k] = tin.dim(b, perm[k]);
-      j.in_stride[k] = st[perm[k]];
+      dout[k] = tin.dim(b, perm[k]);
+      istr[k] = st[perm[k]];
     }
-    if (j.out_off + j.size > tout.total || j.in_off + j.size > tin.total)
-      throw std::out_of_range("permute: block outside its buffer");
-    for (int64_t e0 = 0; e0 < j.size; e0 += kPermChunk) items.push_back({e0, int32_t(jobs.size()), 0});
-    jobs.push_back(j);
+    s = 1;
+    for (int k = rank - 1; k >= 0; --k) {
+      ost[k] = s;
+      s *= dout[k];
+    }
+    // the two copy axes (output positions) and the job shape
+    int ra, ca;
+    bool trans;
+    if (pa != rank - 1) {
+      ra = pa;  // rows of the output tile: the input's innermost axis
+      ca = rank - 1;
+      trans = true;
+    } else {
+      ca = rank - 1;
+      ra = -1;
+      for (int k = 0; k < rank - 1; ++k)
+        if (ra < 0 || dout[k] > dout[ra]) ra = k;
+      trans = false;
+    }
+    const int64_t m = ra >= 0 ? dout[ra] : 1, n = dout[ca];
+    const int64_t src_ld = trans ? istr[ca] : (ra >= 0 ? istr[ra] : n), dst_ld = ra >= 0 ? ost[ra] : n;
+    if (m > INT32_MAX || n > INT32_MAX || src_ld > INT32_MAX || dst_ld > INT32_MAX)
+      throw ArgError("permute: block too large for the copy kernel");
+    std::vector<int> other;
+    for (int k = 0; k < rank; ++k)
+      if (k != ra && k != ca) other.push_back(k);
+    const int64_t slabs = size / (m * n);
+    for (int64_t sl = 0; sl < slabs; ++sl) {
+      int64_t q = sl, bi = in_off, bo = out_off;
+      for (int o = int(other.size()) - 1; o >= 0; --o) {
+        const int k = other[size_t(o)];
+        const int64_t i = q % dout[k];
+        q /= dout[k];
+        bi += i * istr[k];
+        bo += i * ost[k];
+      }
+      cjobs.push_back({bi, bo, int32_t(m), int32_t(n), int32_t(src_ld), int32_t(dst_ld), trans ? 1 : 0, 0});
+    }
   }
-  if (items.empty()) return;
-  DBuf<PermJob> dj(ctx, jobs.size());
-  DBuf<PermItem> di(ctx, items.size());
-  dj.upload(jobs.data(), jobs.size());
-  di.upload(items.data(), items.size());
-  hipLaunchKernelGGL(permute_kernel, dim3(items.size()), dim3(256), 0, ctx->stream, di.get(), dj.get(), in, out);
-  TK_HIP(hipGetLastError());
+  if (!cjobs.empty()) launch_copies(ctx, cjobs, in, in_n, out, out_n);
   ctx->sync();
 }
 

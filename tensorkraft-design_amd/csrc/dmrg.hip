@@ -18,16 +18,65 @@
 namespace tk {
 namespace {
 
-__global__ void copy2d_kernel(const CopyJob* __restrict__ jobs, int n_jobs, const double* __restrict__ src,
-                              double* __restrict__ dst) {
-  for (int j = blockIdx.x; j < n_jobs; j += gridDim.x) {
-    const CopyJob c = jobs[j];
-    const int total = c.m * c.n;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-      const int i = e / c.n, k = e - i * c.n;
-      const double v = c.trans ? src[c.src + int64_t(k) * c.src_ld + i] : src[c.src + int64_t(i) * c.src_ld + k];
-      dst[c.dst + int64_t(i) * c.dst_ld + k] = v;
+// Work items of the strided copies: rows [r0, r1) x columns [c0, c1) of one
+// job (plain copies; a long row is cut into column chunks) or the 32 x 32
+// output tile at (r0, c0) (transposing copies).
+struct CopyItem {
+  int32_t job, r0, r1, c0, c1, pad;
+};
+constexpr int kCopyTile = 32;
+constexpr int kCopyItemElems = 8192;
+
+// Plain copies as coalesced row segments: a single row with all 256 threads
+// along it, several rows four at a time with 64 consecutive columns each.
+__global__ __launch_bounds__(256) void copy_rows_kernel(const CopyItem* __restrict__ items, int n_items,
+                                                        const CopyJob* __restrict__ jobs,
+                                                        const double* __restrict__ src, double* __restrict__ dst) {
+  const int ty = threadIdx.x >> 6, tx = threadIdx.x & 63;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const CopyItem ci = items[it];
+    const CopyJob c = jobs[ci.job];
+    if (ci.r1 - ci.r0 == 1) {
+      // four loads in flight per thread before their stores
+      const double* s = src + c.src + int64_t(ci.r0) * c.src_ld;
+      double* d = dst + c.dst + int64_t(ci.r0) * c.dst_ld;
+      int k = ci.c0 + int(threadIdx.x);
+      for (; k + 3 * 256 < ci.c1; k += 4 * 256) {
+        const double v0 = s[k], v1 = s[k + 256], v2 = s[k + 512], v3 = s[k + 768];
+        d[k] = v0;
+        d[k + 256] = v1;
+        d[k + 512] = v2;
+        d[k + 768] = v3;
+      }
+      for (; k < ci.c1; k += 256) d[k] = s[k];
+    } else {
+      for (int i = ci.r0 + ty; i < ci.r1; i += 4)
+        for (int k = ci.c0 + tx; k < ci.c1; k += 64)
+          dst[c.dst + int64_t(i) * c.dst_ld + k] = src[c.src + int64_t(i) * c.src_ld + k];
     }
+  }
+}
+
+// Transposing copies dst[i][k] = src[k][i]: a 32 x 32 tile is read along i
+// (coalesced in src), staged in LDS and written along k (coalesced in dst).
+__global__ __launch_bounds__(256) void copy_tiles_kernel(const CopyItem* __restrict__ items, int n_items,
+                                                         const CopyJob* __restrict__ jobs,
+                                                         const double* __restrict__ src, double* __restrict__ dst) {
+  __shared__ double t[kCopyTile][kCopyTile + 1];
+  const int tx = threadIdx.x & (kCopyTile - 1), ty = threadIdx.x >> 5;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const CopyItem ci = items[it];
+    const CopyJob c = jobs[ci.job];
+    for (int r = ty; r < kCopyTile; r += 8) {
+      const int k = ci.c0 + r, i = ci.r0 + tx;
+      if (k < c.n && i < c.m) t[r][tx] = src[c.src + int64_t(k) * c.src_ld + i];
+    }
+    __syncthreads();
+    for (int r = ty; r < kCopyTile; r += 8) {
+      const int i = ci.r0 + r, k = ci.c0 + tx;
+      if (i < c.m && k < c.n) dst[c.dst + int64_t(i) * c.dst_ld + k] = t[tx][r];
+    }
+    __syncthreads();
   }
 }
 
@@ -71,10 +120,44 @@ void launch_copies(Context* ctx, const std::vector<CopyJob>& jobs, const double*
     if (c.src < 0 || c.dst < 0 || s_end > src_n || d_end > dst_n || c.dst_ld < c.n)
       throw std::out_of_range("copy: job outside its buffers");
   }
+  std::vector<CopyItem> rows, tiles;
+  double row_elems = 0, tile_elems = 0;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const CopyJob& c = jobs[j];
+    (c.trans ? tile_elems : row_elems) += double(c.m) * double(c.n);
+    if (c.trans) {
+      for (int i0 = 0; i0 < c.m; i0 += kCopyTile)
+        for (int k0 = 0; k0 < c.n; k0 += kCopyTile) tiles.push_back({int32_t(j), i0, i0 + 1, k0, k0 + 1, 0});
+    } else if (c.n >= kCopyItemElems / 2) {
+      for (int r = 0; r < c.m; ++r)
+        for (int k0 = 0; k0 < c.n; k0 += kCopyItemElems / 2)
+          rows.push_back({int32_t(j), r, r + 1, k0, std::min(c.n, k0 + kCopyItemElems / 2), 0});
+    } else {
+      const int nr = std::max(1, kCopyItemElems / c.n);
+      for (int r0 = 0; r0 < c.m; r0 += nr) rows.push_back({int32_t(j), r0, std::min(c.m, r0 + nr), 0, c.n, 0});
+    }
+  }
   DBuf<CopyJob> d(ctx, jobs.size());
   d.upload(jobs.data(), jobs.size());
-  int grid = int(std::min<size_t>(jobs.size(), size_t(ctx->n_cu) * 8));
-  hipLaunchKernelGGL(copy2d_kernel, dim3(grid), dim3(256), 0, ctx->stream, d.get(), int(jobs.size()), src, dst);
+  DBuf<CopyItem> di(ctx, rows.size() + tiles.size());
+  if (!rows.empty()) di.upload(rows.data(), rows.size());
+  if (!tiles.empty())
+    TK_HIP(hipMemcpyAsync(di.get() + rows.size(), tiles.data(), tiles.size() * sizeof(CopyItem),
+                          hipMemcpyHostToDevice, ctx->stream));
+  const size_t cap = size_t(ctx->n_cu) * 8;
+  // algorithmic bytes: each element read once and written once
+  if (!rows.empty()) {
+    const int tk = ctx->timer.begin(ctx->stream);
+    hipLaunchKernelGGL(copy_rows_kernel, dim3(unsigned(std::min(rows.size(), cap))), dim3(256), 0, ctx->stream,
+                       di.get(), int(rows.size()), d.get(), src, dst);
+    ctx->timer.end(tk, ctx->stream, "copy_rows", 16.0 * row_elems, 0.0, 1);
+  }
+  if (!tiles.empty()) {
+    const int tk = ctx->timer.begin(ctx->stream);
+    hipLaunchKernelGGL(copy_tiles_kernel, dim3(unsigned(std::min(tiles.size(), cap))), dim3(256), 0, ctx->stream,
+                       di.get() + rows.size(), int(tiles.size()), d.get(), src, dst);
+    ctx->timer.end(tk, ctx->stream, "copy_tiles", 16.0 * tile_elems, 0.0, 1);
+  }
   TK_HIP(hipGetLastError());
   ctx->sync();
 }
