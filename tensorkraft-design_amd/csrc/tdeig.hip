@@ -14,6 +14,7 @@
 #include <string>
 
 #include "tdeig.hpp"
+#include "tdcoop.hpp"
 
 #include "contract.hpp"
 #include "kernels.hpp"
@@ -2299,10 +2300,43 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
     k0s[o] = stop_of(ns[o]);
     kmax0 = std::max(kmax0, k0s[o]);
   }
+  // column steps: one cooperative launch for the jobs whose rows fit one
+  // XCD's registers, one launch per column for the rest
+  const DevJob* djs = djd;
+  std::vector<int> k0step = k0s;
+  int kstep = kmax0;
+  DBuf<DevJob> dstep;
+  DBuf<double> rowbuf;
+  if (kmax0 > 0 && tridiag_coop_available(ctx_)) {
+    std::vector<CoopJob> cj;
+    for (size_t o = 0; o < jobs.size(); ++o) cj.push_back({dj[o].a_off, dj[o].w_off, ns[o], k0s[o]});
+    rowbuf.reset(ctx_, size_t(2 * std::max<int64_t>(w_n, 1)));
+    const int tc = ctx_->timer.begin(ctx_->stream);
+    double cbytes = 0;
+    int64_t csteps = 0;
+    const std::vector<int> done = tridiag_columns_coop(ctx_, cj, A, D, E, TAU, vp.get(), rowbuf.get(), w_n);
+    for (int o : done) {
+      // HBM: the matrix read once, the reflectors and the trailing block written once
+      const double n = ns[o], k0 = k0s[o];
+      cbytes += 8.0 * (n * n + k0 * (n - 1.0 - 0.5 * k0) + (n - k0) * (n - k0));
+      csteps = std::max<int64_t>(csteps, k0s[o]);
+      k0step[o] = 0;
+    }
+    ctx_->timer.end(tc, ctx_->stream, "tridiag_coop", cbytes, 0.0, csteps);
+    if (!done.empty()) {
+      std::vector<DevJob> ds(dj.begin(), dj.begin() + jobs.size());
+      for (int o : done) ds[o].pad = 0;
+      dstep.reset(ctx_, ds.size());
+      dstep.upload(ds.data(), ds.size());
+      djs = dstep.get();
+      kstep = 0;
+      for (size_t o = 0; o < jobs.size(); ++o) kstep = std::max(kstep, k0step[o]);
+    }
+  }
   const int tk = ctx_->timer.begin(ctx_->stream);
   double bytes = 0;
   int64_t launches = 0;
-  if (kmax0 > 0) {
+  if (kstep > 0) {
     static const int rows_override = [] {
       const char* e = std::getenv("TK_TRIDIAG_ROWS");
       return e ? std::atoi(e) : 0;
@@ -2311,12 +2345,12 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
       const char* e = std::getenv("TK_TRIDIAG_RW2_ROWS");
       return e ? int64_t(std::atoll(e)) : int64_t(4096);
     }();
-    for (int k = 0; k < kmax0; ++k) {
+    for (int k = 0; k < kstep; ++k) {
       // algorithmic traffic: every step reads and writes the trailing block
       int64_t rows = 0, grid4 = 0, grid8 = 0, grid16 = 0, grid8x2 = 0;
       int trail = 0;  // the largest n - k among the active jobs
       for (size_t o = 0; o < ns.size(); ++o)
-        if (k < k0s[o]) {
+        if (k < k0step[o]) {
           const int64_t r = ns[o] - k - 1;
           rows += r;
           bytes += 16.0 * double(r) * double(r);
@@ -2339,7 +2373,7 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
       const int64_t grid = rw == 2 ? grid8x2 : r == 16 ? grid16 : r == 8 ? grid8 : grid4;
       // the smallest register capacity that holds the trailing size
       const int cap = trail <= 512 ? 512 : trail <= 1024 ? 1024 : 2048;
-      launch_tdstep(r, rw, cap, dim3(unsigned(grid)), lds, ctx_->stream, djd, nj, k, A, D, E, TAU, vp.get(), w_n,
+      launch_tdstep(r, rw, cap, dim3(unsigned(grid)), lds, ctx_->stream, djs, nj, k, A, D, E, TAU, vp.get(), w_n,
                     nmax_);
       ++launches;
     }
