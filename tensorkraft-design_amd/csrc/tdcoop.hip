@@ -5,6 +5,7 @@
 // row i > k takes the update of step k-1, A22 -= v w^T + w v^T with
 // w = tau p - tau^2/2 (p.v) v, and contributes p_k[i] = (A22 v_k)_i.
 #include "tdcoop.hpp"
+#include "wave.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -31,39 +32,10 @@ struct CoopSlot {
   int32_t job, rank;
 };
 
-// One DPP step on a double: x + (x moved by dpp control `ctl` within each
-// row of 16 lanes).
-template <int ctl>
-__device__ __forceinline__ double dpp_add(double x) {
-  const int2 v = __builtin_bit_cast(int2, x);
-  int2 m;
-  m.x = __builtin_amdgcn_update_dpp(0, v.x, ctl, 0xF, 0xF, false);
-  m.y = __builtin_amdgcn_update_dpp(0, v.y, ctl, 0xF, 0xF, false);
-  return x + __builtin_bit_cast(double, m);
-}
-__device__ __forceinline__ double lane_value(double x, int l) {
-  const int2 v = __builtin_bit_cast(int2, x);
-  int2 r;
-  r.x = __builtin_amdgcn_readlane(v.x, l);
-  r.y = __builtin_amdgcn_readlane(v.y, l);
-  return __builtin_bit_cast(double, r);
-}
-// Wavefront sum, the same in every lane and in the same order in every
-// wavefront: within each row of 16 lanes by DPP (partners at xor 1, xor 2,
-// then the mirrored 8 and 16), then the four row sums read out in row order
-// (no LDS-crossbar permutes).
-__device__ __forceinline__ double wsum(double x) {
-  x = dpp_add<0xB1>(x);   // quad_perm [1,0,3,2]
-  x = dpp_add<0x4E>(x);   // quad_perm [2,3,0,1]
-  x = dpp_add<0x141>(x);  // row_half_mirror
-  x = dpp_add<0x140>(x);  // row_mirror
-  return ((lane_value(x, 0) + lane_value(x, 16)) + lane_value(x, 32)) + lane_value(x, 48);
-}
-
 // Workgroup sum in a fixed order (identical in every workgroup of a job);
 // `red` alternates between two halves so consecutive calls need one barrier.
 __device__ __forceinline__ double wg_sum(double x, double* red, int& parity) {
-  x = wsum(x);
+  x = wave_sum(x);
   double* r = red + parity * kCoopWaves;
   parity ^= 1;
   if ((threadIdx.x & 63) == 0) r[threadIdx.x >> 6] = x;
@@ -278,7 +250,7 @@ __global__ __launch_bounds__(kCoopNT) __attribute__((amdgpu_waves_per_eu(4))) vo
     for (int rr = 0; rr < RW; ++rr) {
       const int i = row[rr];
       if (i < n && i > k) {
-        const double sv = wsum(acc[rr]);
+        const double sv = wave_sum(acc[rr]);
         if (lane == 0) pdst[i] = sv;
         if (i == k + 1) {
 #pragma unroll

@@ -15,6 +15,7 @@
 
 #include "tdeig.hpp"
 #include "tdcoop.hpp"
+#include "wave.hpp"
 
 #include "contract.hpp"
 #include "kernels.hpp"
@@ -44,21 +45,6 @@ struct DevJob {
   int32_t pad;
 };
 
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
-__device__ __forceinline__ double wave_min(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
-  return x;
-}
-__device__ __forceinline__ double wave_max(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
-  return x;
-}
 
 // Sum over the workgroup; `red` holds 2 * (NT / 64) doubles used alternately,
 // so consecutive calls need no second barrier.

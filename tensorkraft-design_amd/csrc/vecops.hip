@@ -6,6 +6,7 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "wave.hpp"
 
 namespace tk {
 namespace {
@@ -13,11 +14,6 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMaxVec = 128;
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
 
 // partial[i * nchunks + c] = sum over chunk c of V_i . x for the kVecGroup
 // vectors of group blockIdx.y; chunks of kDotChunk elements, one barrier.

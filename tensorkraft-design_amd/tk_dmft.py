@@ -715,7 +715,7 @@ class DMFTLoop:
 
         for it in range(self.max_iterations):
             d_new, star_new, target, res = self.step(chain, star)
-            rec = {"iteration": it, "residual": res, "delta_new": d_new}
+            rec = {"iteration": it, "residual": res, "delta_new": d_new, "target": np.concatenate(target)}
             self.history.append(rec)
             self.checkpoint(it, chain, res < self.tol)
             if pending is not None:
@@ -739,11 +739,15 @@ class DMFTLoop:
         return chain, self.solver.n_solves - start, False
 
 
-def train_bath_proposal(ctx, runs, dims=(512, 256, 256, 14), epochs=4000, lr=1e-3, seed=0):
+def train_bath_proposal(ctx, runs, dims=(512, 256, 256, 14), epochs=20000, lr=1e-3, lr_final=1e-5, seed=0):
     """The learned bath proposal: an Mlp from Delta_new(i w_n) (every
     iteration of the given converged DMFTLoop runs) to the chain parameters of
     that run's self-consistent bath, inputs and targets standardised over the
-    training set.  runs: [(history, converged chain)].  Returns (network,
+    training set.  Full-batch Adam with the learning rate falling
+    geometrically from lr to lr_final over ten stages of epochs / 10
+    (lr_final=None keeps lr): the proposal has to land within a few 1e-4 of
+    the self-consistent bath to save solves, because the mixing restarts its
+    history from it.  runs: [(history, converged chain)].  Returns (network,
     proposal function for DMFTLoop.solve, loss history)."""
     xs, ts = [], []
     for hist, chain in runs:
@@ -756,7 +760,15 @@ def train_bath_proposal(ctx, runs, dims=(512, 256, 256, 14), epochs=4000, lr=1e-
     xm, xsd = x.mean(0), x.std(0) + 1e-8
     tm, tsd = t.mean(0), t.std(0) + 1e-8
     net = Mlp(ctx, list(dims), seed=seed)
-    loss = net.train(((x - xm) / xsd).astype(np.float32), ((t - tm) / tsd).astype(np.float32), epochs=epochs, lr=lr)
+    xn = ((x - xm) / xsd).astype(np.float32)
+    tn = ((t - tm) / tsd).astype(np.float32)
+    if lr_final is None:
+        loss = net.train(xn, tn, epochs=epochs, lr=lr)
+    else:
+        loss = []
+        for k in range(10):
+            rate = lr * (lr_final / lr) ** (k / 9.0)
+            loss += net.train(xn, tn, epochs=max(1, epochs // 10), lr=rate)
     nb = dims[-1] // 2
 
     def proposal(delta_new):

@@ -195,7 +195,8 @@ def test_dmft_loop_and_learned_proposal(ctx):
     Matsubara points): the self-consistency loop converges, and the network
     trained on the solver's own (Delta, converged bath) pairs from 16 other
     U values reaches the same baths in fewer impurity solves over five
-    held-out U values (a rejected proposal costs one judging solve)."""
+    held-out U values (a rejected proposal costs one judging solve): at
+    least a quarter fewer in total (measured: about half)."""
     print("case dmft loop + learned proposal", flush=True)
     runs = []
     for k in range(16):
@@ -204,8 +205,8 @@ def test_dmft_loop_and_learned_proposal(ctx):
         chain, n, ok = loop.solve()
         assert ok, f"U={U} did not converge"
         runs.append((loop.history, chain))
-    net, proposal, loss = D.train_bath_proposal(ctx, runs, epochs=4000)
-    assert loss[-1] < 0.01 * loss[0]
+    net, proposal, loss = D.train_bath_proposal(ctx, runs)
+    assert loss[-1] < 1e-4 * loss[0]
     tot_plain = tot_learned = 0
     for U in (3.9, 4.0, 4.1, 4.25, 4.4):
         plain = D.DMFTLoop(ctx, n_bath=7, U=U, tol=1e-5, max_iterations=60)
@@ -218,7 +219,7 @@ def test_dmft_loop_and_learned_proposal(ctx):
         tot_plain += na
         tot_learned += nb
     print(f"total: plain {tot_plain}, learned {tot_learned}", flush=True)
-    assert tot_learned < tot_plain
+    assert tot_learned < 0.75 * tot_plain
 
 
 def test_dmft_checkpoint_written_and_resumed(ctx, tmp_path):

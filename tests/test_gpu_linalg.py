@@ -209,7 +209,11 @@ def test_block_svd_degenerate_reconstructs(be):
     out, nk, nd, err = be.block_svd_truncate(blocks, 10 ** 6, 0.0, degeneracy_tol=0.0)
     for b, (u, s, c) in zip(blocks, out):
         assert np.abs(u.T @ u - np.eye(len(s))).max() < 1e-12
-        assert np.abs(u @ c - b).max() < 1e-13 * np.abs(b).max() * 10
+        # the third block spans sigma 2 .. 1e-6: its directions near the
+        # first Gram level's line (3e-5 sigma_max) come from the second level,
+        # which recovers the span to a few thousand eps |X| (measured 0.7 to
+        # 1.6e-12 |X| across the summation orders of the reductions)
+        assert np.abs(u @ c - b).max() < 4e-12 * np.abs(b).max()
 
 
 def _graded(rng, k, n_big):
