@@ -11,6 +11,10 @@ Workloads
   heff   one H_eff*psi on the U(1)xU(1) Hubbard microbenchmark bond
          (d=4, MPO bond 6, D=1024 on both sides, 145 sectors); a step is one
          matvec with inputs resident in HBM.
+  heff-c3  one H_eff*psi on the centre bonds of the L=64 Hubbard ground
+         state ramped to D=2048 (the timed sweeps' shapes), random data.
+  heff-d2048 the same construction with D=2048 on both sides (the bond
+         dimension of the timed sweeps); every precision timed.
   sweep-c4 two-site DMRG on the J1 Heisenberg 6 x 12 cylinder (U(1), Sz=0),
          D 256 -> 4096 over 5 sweeps, in FP64, FP32 and split-BF16 from the
          same seeded MPS; a step is the sweep at D=4096.
@@ -56,16 +60,37 @@ def cpu_baseline_heff(p, budget_s=10.0):
 
 HEFF_WORKLOADS = {
     "heff": ("heff_c2", "H_eff*psi time, Hubbard U(1)xU(1) d=4 w=6 D=1024 (single matvec)"),
+    "heff-d2048": ("heff_hubbard_uu_D2048", "H_eff*psi time, Hubbard U(1)xU(1) d=4 w=6 D=2048 (single matvec)"),
+    "heff-c3": ("heff_c3_hubbard_L64_D2048_centre",
+                "H_eff*psi time, Hubbard L=64 D=2048 ground state's centre bond (single matvec)"),
     "heff-c4": ("heff_c4_cylinder_6x12_D4096",
                 "H_eff*psi time, J1 Heisenberg 6x12 cylinder U(1) d=2 w=20 D=4096 (single matvec)"),
 }
 
 
-def heff_problem_for(tk_hip, workload):
-    if workload == "heff-c4":
+def heff_problem_for(tk_hip, workload, args=None, ctx=None):
+    if workload == "heff-c3":
+        # the bonds either side of the centre pair of the L=64 chain's ground
+        # state ramped to D=2048 (the timed sweeps' largest H_eff); random
+        # environments and psi of that block structure
+        def log(msg):
+            print(msg, file=sys.stderr, flush=True)
+
+        d, mpo, sym, _, _ = _ground_state(tk_hip, ctx, args, log)
+        n = args.sites
+        bl, br = d.bond(n // 2 - 1), d.bond(n // 2 + 1)
+        d.close()
+        log("centre bonds " + json.dumps({"left": bl, "right": br}))
+        p = tk_hip.heff_problem(bl, br, mpo[n // 2], seed=2048, sym=sym)
+        desc = {"bond_dim": sum(x for _, x in bl), "sectors": len(bl), "mpo_bond_dim": 6,
+                "bond": [n // 2 - 1, n // 2], "sector_dim_max": max(x for _, x in bl)}
+    elif workload == "heff-c4":
         p = tk_hip.c4_problem(seed=4096)
         desc = {"bond_dim": 4096, "sectors": len(p["bond_l"]), "mpo_bond_dim": int(p["W"].w.shape[3]),
                 "lattice": "cylinder 6 x 12, snake order", "bond": [p["site"], p["site"] + 1]}
+    elif workload == "heff-d2048":
+        p = tk_hip.c2_problem(seed=2048, total=2048)
+        desc = {"bond_dim": 2048, "sectors": len(p["bond_l"]), "mpo_bond_dim": 6}
     else:
         p = tk_hip.c2_problem(seed=2024)
         desc = {"bond_dim": 1024, "sectors": len(p["bond_l"]), "mpo_bond_dim": 6}
@@ -80,7 +105,8 @@ def bench_heff(args):
     import tk_hip
 
     ctx = tk_hip.Context(0)
-    p, desc = heff_problem_for(tk_hip, args.workload)
+    p, desc = heff_problem_for(tk_hip, args.workload, args, ctx)
+    ctx.tune(not args.no_tune)
     L, R, x = ctx.to_device(p["L"]), ctx.to_device(p["R"]), ctx.to_device(p["x"])
     y = ctx.empty(len(p["x"]))
     Wi, Wj = p["W"], p.get("Wj", p["W"])
@@ -747,7 +773,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="sweep",
-                    choices=["sweep", "sweep-c4", "heff", "heff-c4", "copy", "tdvp", "sweep1", "impurity", "permute"])
+                    choices=["sweep", "sweep-c4", "heff", "heff-d2048", "heff-c3", "heff-c4", "copy", "tdvp", "sweep1", "impurity", "permute"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--sites", type=int, default=64)
@@ -767,6 +793,8 @@ def main():
         args.steps = 3 if args.steps is None else args.steps
         rec = bench_copy(args)
     elif args.workload in HEFF_WORKLOADS:
+        args.dmax = 2048 if args.dmax is None else args.dmax
+        args.ramp = 7 if args.ramp is None else args.ramp
         args.steps = 200 if args.steps is None else args.steps
         args.warmup = 20 if args.warmup is None else args.warmup
         rec = bench_heff(args)

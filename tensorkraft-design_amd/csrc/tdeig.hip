@@ -2298,17 +2298,22 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
     for (size_t o = 0; o < jobs.size(); ++o) cj.push_back({dj[o].a_off, dj[o].w_off, ns[o], k0s[o]});
     rowbuf.reset(ctx_, size_t(2 * std::max<int64_t>(w_n, 1)));
     const int tc = ctx_->timer.begin(ctx_->stream);
-    double cbytes = 0;
+    double cbytes = 0, cflops = 0;
     int64_t csteps = 0;
     const std::vector<int> done = tridiag_columns_coop(ctx_, cj, A, D, E, TAU, vp.get(), rowbuf.get(), w_n);
     for (int o : done) {
-      // HBM: the matrix read once, the reflectors and the trailing block written once
+      // HBM: the matrix read once, the reflectors and the trailing block
+      // written once; arithmetic: per column step the rank-2 update and
+      // A22 v, 6 flops per trailing element
       const double n = ns[o], k0 = k0s[o];
       cbytes += 8.0 * (n * n + k0 * (n - 1.0 - 0.5 * k0) + (n - k0) * (n - k0));
+      for (int k = 0; k < k0s[o]; ++k) cflops += 6.0 * double(ns[o] - k - 1) * double(ns[o] - k - 1);
       csteps = std::max<int64_t>(csteps, k0s[o]);
       k0step[o] = 0;
     }
-    ctx_->timer.end(tc, ctx_->stream, "tridiag_coop", cbytes, 0.0, csteps);
+    // one "launch" per column step of the batch (the steps run in lock-step
+    // inside the cooperative launch)
+    ctx_->timer.end(tc, ctx_->stream, "tridiag_coop_fp64", cbytes, cflops, csteps);
     if (!done.empty()) {
       std::vector<DevJob> ds(dj.begin(), dj.begin() + jobs.size());
       for (int o : done) ds[o].pad = 0;

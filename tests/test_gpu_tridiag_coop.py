@@ -52,7 +52,7 @@ def test_eigh_coop_matches_per_column_and_lapack(be, ctx, n):
     t = ctx.timing_read()
     ctx.timing(False)
     # the cooperative kernel ran for every size it holds
-    assert ("tridiag_coop" in t and t["tridiag_coop"]["bytes"] > 0) == (n <= 1024)
+    assert ("tridiag_coop_fp64" in t and t["tridiag_coop_fp64"]["bytes"] > 0) == (n <= 1024)
     w0, v0 = _coop_off(lambda: be.eigh(a))
     ref = np.linalg.eigvalsh(a)
     amax = np.abs(a).max()

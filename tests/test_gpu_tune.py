@@ -74,6 +74,46 @@ def test_tuned_plan_matches_default_and_oracle(ctx, tmp_path, prec):
     tc.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_every_tiling_matches_oracle(tmp_path, prec):
+    """Each workgroup form, forced through a loaded cache entry for the
+    plan's stage signatures, gives the oracle's product: wavefront tiles,
+    32 x 128 and 64 x 64 (at two thresholds).  Split-BF16 has only the
+    wavefront and 32 x 128 forms and runs the 64 x 64 form's outputs as
+    wavefront tiles."""
+    print(f"case forced tilings {prec}", flush=True)
+    p = tk_hip.c2_problem(seed=2025)
+    ref = oracle.heff_apply(p, p["x"])
+    tol = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 2e-4}[prec]
+    tc = tk_hip.Context(0)
+    tc.tune(True)
+    pl, keep = _plan(tc, p, prec)
+    x, y = tc.to_device(p["x"]), tc.empty(len(p["x"]))
+    pl.apply(x, y)
+    sigs = sorted({e["signature"] for e in tk_hip.tune_table()})
+    pl.close()
+    path = os.path.join(str(tmp_path), "forced.txt")
+    try:
+        for kind, tmin in ((0, 48), (1, 48), (2, 32), (2, 64)):
+            with open(path, "w") as fh:
+                for s in sigs:
+                    fh.write(f"0 {s} {kind} {tmin} 1 1\n")
+            tk_hip.tune_clear()
+            tk_hip.tune_load(path)
+            q, keep_q = _plan(tc, p, prec)
+            assert q.tiling(1) == (kind, tmin) and q.tiling(3) == (kind, tmin)
+            yq = tc.empty(len(p["x"]))
+            q.apply(x, yq)
+            got = yq.get()
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            print(f"  kind {kind} tile_min {tmin}: rel err {err:.2e}", flush=True)
+            assert err <= tol
+            q.close()
+    finally:
+        tk_hip.tune_clear()
+        tc.close()
+
+
 def test_tuned_sweep_matches_untuned(ctx):
     """A DMRG run with tuning on reaches the untuned run's energies."""
     print("case tune dmrg", flush=True)
