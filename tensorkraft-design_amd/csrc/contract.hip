@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <string>
 
@@ -118,6 +119,21 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
+// The U consecutive elements p[0, U) of one lane (an operand whose inner
+// index is contiguous in memory): one or two 16-byte loads when the whole run
+// lies inside the block (live, base + U <= n), else element by element with
+// zeros past n.  Operand rows are only element-aligned; global loads on this
+// target do not need 16-byte alignment.
+template <typename T, int U>
+__device__ __forceinline__ void run_load(T (&v)[U], const T* __restrict__ p, bool live, int base, int n) {
+  if (live && base + U <= n) {
+    __builtin_memcpy(v, p, sizeof(v));
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = (live && base + u < n) ? p[u] : T(0);
+  }
+}
+
 // Inner-dimension chunks of 4*U: lane group lk supplies k = k0 + U*lk + u at
 // MFMA step u (a permutation of the k order inside the chunk, the same for A
 // and B), so each lane reads U consecutive k of a row-major A row.  The work
@@ -125,19 +141,16 @@ template <> struct Mfma<float> {
 // source) is a scalar load, and the sources of a term are taken two at a
 // time so that the operand loads of a chunk are all in flight before the
 // first FMA.
+// Items wave0, wave0 + nwaves, ... of the list, one per wavefront pass.
 template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
-__global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__ items, int n_items,
-                                                       const GOut* __restrict__ outs,
-                                                       const GTerm* __restrict__ terms,
-                                                       const GSrc* __restrict__ srcs,
-                                                       const TI* __restrict__ A, const TI* __restrict__ B,
-                                                       TO* __restrict__ C, double beta) {
+__device__ __forceinline__ void wave_items(const GItem* __restrict__ items, int n_items, int wave0, int nwaves,
+                                           const GOut* __restrict__ outs, const GTerm* __restrict__ terms,
+                                           const GSrc* __restrict__ srcs, const TI* __restrict__ A,
+                                           const TI* __restrict__ B, TO* __restrict__ C, double beta) {
   using M = Mfma<TI>;
   using acc_t = typename M::acc_t;
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
-  const int nwaves = gridDim.x * (blockDim.x >> 6);
-  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   for (int it = wave0; it < n_items; it += nwaves) {
     const GItem item = items[it];
     const GOut o = outs[item.out];
@@ -157,12 +170,18 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
       for (int k0 = 0; k0 < tm.k; k0 += 4 * U) {
         const int kb = k0 + U * lk;
         TI a0[U], a1[U], b0[U], b1[U];
+        if (TB) {
+          run_load<TI, U>(b0, Bp + int64_t(q0) * tm.ldb + kb, in_q0, kb, tm.k);
+          run_load<TI, U>(b1, Bp + int64_t(q1) * tm.ldb + kb, in_q1, kb, tm.k);
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int kk = kb + u;
           const bool kin = kk < tm.k;
-          b0[u] = (kin && in_q0) ? (TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0]) : TI(0);
-          b1[u] = (kin && in_q1) ? (TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1]) : TI(0);
+          if (!TB) {
+            b0[u] = (kin && in_q0) ? Bp[int64_t(kk) * tm.ldb + q0] : TI(0);
+            b1[u] = (kin && in_q1) ? Bp[int64_t(kk) * tm.ldb + q1] : TI(0);
+          }
           a0[u] = 0;
           a1[u] = 0;
         }
@@ -174,14 +193,21 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
           const TI* __restrict__ Apb = A + sb.a_off;
           const TI ca = TI(sa.coef), cb = two ? TI(sb.coef) : TI(0);
           TI xa0[U], xa1[U], xb0[U], xb1[U];
+          if (!TA) {
+            run_load<TI, U>(xa0, Apa + int64_t(r0) * sa.lda + kb, in_r0, kb, tm.k);
+            run_load<TI, U>(xa1, Apa + int64_t(r1) * sa.lda + kb, in_r1, kb, tm.k);
+            run_load<TI, U>(xb0, Apb + int64_t(r0) * sb.lda + kb, in_r0 && two, kb, tm.k);
+            run_load<TI, U>(xb1, Apb + int64_t(r1) * sb.lda + kb, in_r1 && two, kb, tm.k);
+          } else {
 #pragma unroll
-          for (int u = 0; u < U; ++u) {
-            const int kk = kb + u;
-            const bool kin = kk < tm.k;
-            xa0[u] = lda_at(Apa, sa.lda, r0, in_r0, kk, kin);
-            xa1[u] = lda_at(Apa, sa.lda, r1, in_r1, kk, kin);
-            xb0[u] = lda_at(Apb, sb.lda, r0, in_r0 && two, kk, kin);
-            xb1[u] = lda_at(Apb, sb.lda, r1, in_r1 && two, kk, kin);
+            for (int u = 0; u < U; ++u) {
+              const int kk = kb + u;
+              const bool kin = kk < tm.k;
+              xa0[u] = lda_at(Apa, sa.lda, r0, in_r0, kk, kin);
+              xa1[u] = lda_at(Apa, sa.lda, r1, in_r1, kk, kin);
+              xb0[u] = lda_at(Apb, sb.lda, r0, in_r0 && two, kk, kin);
+              xb1[u] = lda_at(Apb, sb.lda, r1, in_r1 && two, kk, kin);
+            }
           }
 #pragma unroll
           for (int u = 0; u < U; ++u) {
@@ -221,6 +247,18 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
   }
 }
 
+template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
+__global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__ items, int n_items,
+                                                       const GOut* __restrict__ outs,
+                                                       const GTerm* __restrict__ terms,
+                                                       const GSrc* __restrict__ srcs,
+                                                       const TI* __restrict__ A, const TI* __restrict__ B,
+                                                       TO* __restrict__ C, double beta) {
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  wave_items<TI, TO, TA, TB, TC, U>(items, n_items, wave0, nwaves, outs, terms, srcs, A, B, C, beta);
+}
+
 // Workgroup items: four wavefronts own a 32 x 128 output tile (wavefront w:
 // columns n0 + 32 w ..).  Per inner chunk the workgroup forms the
 // source-summed A tile (32 rows x 4U inner indices, sources in list order)
@@ -229,13 +267,13 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
 // operand (in H_eff stage 3 the MPO-weighted sum of T1 blocks) is read once
 // per four wavefront tiles.  Same k order and source order as the wavefront
 // kernel.
+// Items blk0, blk0 + nblk, ... of the list, one per workgroup pass (every
+// thread of the workgroup calls this).
 template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
-__global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restrict__ items, int n_items,
-                                                            const GOut* __restrict__ outs,
-                                                            const GTerm* __restrict__ terms,
-                                                            const GSrc* __restrict__ srcs,
-                                                            const TI* __restrict__ A, const TI* __restrict__ B,
-                                                            TO* __restrict__ C, double beta) {
+__device__ __forceinline__ void wide_items(const GItem* __restrict__ items, int n_items, int blk0, int nblk,
+                                           const GOut* __restrict__ outs, const GTerm* __restrict__ terms,
+                                           const GSrc* __restrict__ srcs, const TI* __restrict__ A,
+                                           const TI* __restrict__ B, TO* __restrict__ C, double beta) {
   using M = Mfma<TI>;
   using acc_t = typename M::acc_t;
   constexpr int KC = 4 * U;              // inner indices per chunk
@@ -244,7 +282,7 @@ __global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restr
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   int buf = 0;
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+  for (int it = blk0; it < n_items; it += nblk) {
     const GItem item = items[it];
     const GOut o = outs[item.out];
     const int nw0 = item.n0 + 32 * wv;  // this wavefront's first column
@@ -260,12 +298,17 @@ __global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restr
       for (int k0 = 0; k0 < tm.k; k0 += KC) {
         const int kb = k0 + U * lk;
         TI b0[U], b1[U];
+        if (TB) {
+          run_load<TI, U>(b0, Bp + int64_t(q0) * tm.ldb + kb, in_q0, kb, tm.k);
+          run_load<TI, U>(b1, Bp + int64_t(q1) * tm.ldb + kb, in_q1, kb, tm.k);
+        } else {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int kk = kb + u;
-          const bool kin = kk < tm.k;
-          b0[u] = (kin && in_q0) ? (TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0]) : TI(0);
-          b1[u] = (kin && in_q1) ? (TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1]) : TI(0);
+          for (int u = 0; u < U; ++u) {
+            const int kk = kb + u;
+            const bool kin = kk < tm.k;
+            b0[u] = (kin && in_q0) ? Bp[int64_t(kk) * tm.ldb + q0] : TI(0);
+            b1[u] = (kin && in_q1) ? Bp[int64_t(kk) * tm.ldb + q1] : TI(0);
+          }
         }
         TI av[PER];
 #pragma unroll
@@ -330,6 +373,39 @@ __global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restr
   }
 }
 
+
+template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
+__global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restrict__ items, int n_items,
+                                                            const GOut* __restrict__ outs,
+                                                            const GTerm* __restrict__ terms,
+                                                            const GSrc* __restrict__ srcs,
+                                                            const TI* __restrict__ A, const TI* __restrict__ B,
+                                                            TO* __restrict__ C, double beta) {
+  wide_items<TI, TO, TA, TB, TC, U>(items, n_items, blockIdx.x, gridDim.x, outs, terms, srcs, A, B, C, beta);
+}
+
+// Both item lists of a wide list in one launch: workgroups [0, wblocks) run
+// the 32 x 128 items, the others the wavefront items of the narrow outputs,
+// so the narrow outputs fill the compute units the workgroup items leave
+// idle instead of running as a launch of their own before them.  The two
+// lists write disjoint outputs; each item runs the same code as in its own
+// kernel, so results are those of the two launches.
+template <typename TI, typename TO, bool TA, bool TB, bool TC, int U>
+__global__ __launch_bounds__(256) void contract_fused_kernel(const GItem* __restrict__ witems, int n_witems,
+                                                             int wblocks, const GItem* __restrict__ items,
+                                                             int n_items, const GOut* __restrict__ outs,
+                                                             const GTerm* __restrict__ terms,
+                                                             const GSrc* __restrict__ srcs,
+                                                             const TI* __restrict__ A, const TI* __restrict__ B,
+                                                             TO* __restrict__ C, double beta) {
+  if (int(blockIdx.x) < wblocks) {
+    wide_items<TI, TO, TA, TB, TC, U>(witems, n_witems, blockIdx.x, wblocks, outs, terms, srcs, A, B, C, beta);
+  } else {
+    const int nwaves = (int(gridDim.x) - wblocks) * 4;
+    const int wave0 = __builtin_amdgcn_readfirstlane((int(blockIdx.x) - wblocks) * 4 + int(threadIdx.x >> 6));
+    wave_items<TI, TO, TA, TB, TC, U>(items, n_items, wave0, nwaves, outs, terms, srcs, A, B, C, beta);
+  }
+}
 
 // Workgroup items of 64 x 64: four wavefronts in a 2 x 2 arrangement, each
 // owning a 32 x 32 quarter (2 x 2 MFMA tiles).  Per inner chunk of KC = 4U
@@ -462,8 +538,31 @@ template <typename T> struct KUnroll;
 template <> struct KUnroll<double> { static constexpr int value = 4; };
 template <> struct KUnroll<float> { static constexpr int value = 8; };
 
+// Whether a wide list runs its two item lists as one launch (TK_CONTRACT_FUSE,
+// read per launch or graph build; default on, 0 turns it off).
+bool fuse_wide() {
+  const char* e = std::getenv("TK_CONTRACT_FUSE");
+  return !(e && e[0] == '0');
+}
+
 template <typename TI, typename TO>
 void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
+  if (L.n_witems > 0 && L.wg_kind == kWgWide && L.n_narrow > 0 && fuse_wide()) {
+    const int wblocks = std::min(L.n_witems, ctx->n_cu * 4);
+    const int nblocks = std::min((L.n_narrow + 3) / 4, ctx->n_cu * 8);
+#define TK_CASE(F)                                                                                     \
+  case F:                                                                                              \
+    hipLaunchKernelGGL((contract_fused_kernel<TI, TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC), KUnroll<TI>::value>), \
+                       dim3(wblocks + nblocks), dim3(256), 0, ctx->stream, L.witems, L.n_witems, wblocks, L.items, \
+                       L.n_narrow, L.outs, L.terms, L.srcs, A, B, C, L.beta);                          \
+    break;
+    switch (L.flags & 7) {
+      TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+    }
+#undef TK_CASE
+    TK_HIP(hipGetLastError());
+    return;
+  }
   // wavefront items: all of them, or only the narrow ones when the list has
   // workgroup items for its wide outputs
   const int n_wave = L.n_witems > 0 ? L.n_narrow : L.n_items;
@@ -528,6 +627,8 @@ KernelNode make_node(const void* f, dim3 grid, const GItem* items, int n, const 
   k.set(6, B);
   k.set(7, C);
   k.set(8, L.beta);
+  k.b_arg = 6;
+  k.c_arg = 7;
   k.bind();
   return k;
 }
@@ -536,8 +637,39 @@ KernelNode make_node(const void* f, dim3 grid, const GItem* items, int n, const 
 template <typename TI, typename TO>
 std::vector<KernelNode> nodes_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
   std::vector<KernelNode> out;
-  const int n_wave = L.n_witems > 0 ? L.n_narrow : L.n_items;
   const int F = L.flags & 7;
+  if (L.n_witems > 0 && L.wg_kind == kWgWide && L.n_narrow > 0 && fuse_wide()) {
+    const int wblocks = std::min(L.n_witems, ctx->n_cu * 4);
+    const int nblocks = std::min((L.n_narrow + 3) / 4, ctx->n_cu * 8);
+    const void* f = nullptr;
+#define TK_CASE(G) \
+  case G: f = reinterpret_cast<const void*>(&contract_fused_kernel<TI, TO, bool(G & kTransA), bool(G & kTransB), bool(G & kTransC), KUnroll<TI>::value>); break;
+    switch (F) { TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7) }
+#undef TK_CASE
+    KernelNode k;
+    k.func = f;
+    k.grid = dim3(wblocks + nblocks);
+    k.block = dim3(256);
+    k.nargs = 12;
+    k.set(0, L.witems);
+    k.set(1, L.n_witems);
+    k.set(2, wblocks);
+    k.set(3, L.items);
+    k.set(4, L.n_narrow);
+    k.set(5, L.outs);
+    k.set(6, L.terms);
+    k.set(7, L.srcs);
+    k.set(8, A);
+    k.set(9, B);
+    k.set(10, C);
+    k.set(11, L.beta);
+    k.b_arg = 9;
+    k.c_arg = 10;
+    k.bind();
+    out.push_back(k);
+    return out;
+  }
+  const int n_wave = L.n_witems > 0 ? L.n_narrow : L.n_items;
   if (n_wave > 0) {
     const int blocks = std::min((n_wave + 3) / 4, ctx->n_cu * 8);
     const void* f = nullptr;

@@ -422,13 +422,13 @@ void HeffPlan::build_graph(const double* x, double* y) {
   }
   for (const KernelNode& k : s1) {
     gnodes_.push_back(k);
-    gx_arg_.push_back(prec_ == Prec::FP64 ? 6 : -1);  // argument 6 is B = x
+    gx_arg_.push_back(prec_ == Prec::FP64 ? k.b_arg : -1);  // stage 1's B is x
     gy_arg_.push_back(-1);
   }
   for (const KernelNode& k : s3) {
     gnodes_.push_back(k);
     gx_arg_.push_back(-1);
-    gy_arg_.push_back(7);  // argument 7 is C = y
+    gy_arg_.push_back(k.c_arg);  // stage 3's C is y
   }
   for (KernelNode& k : gnodes_) k.bind();  // slots moved with the vector
   TK_HIP(hipGraphCreate(&graph_, 0));
