@@ -71,11 +71,14 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
   // the first matvec and residual are FP64; low precision takes over only
   // for a cold start (see DavidsonOpts::low_start_rel)
   bool low = false;
+  // corrections of a converging solve in low precision (DavidsonOpts::low_tail)
+  bool tail = false, tail_done = false;
 
   auto matvec = [&](const double* in, double* out) {
-    mv(in, out, low ? 0 : 1, user);
+    const bool lp = low || tail;
+    mv(in, out, lp ? 0 : 1, user);
     ++res.matvecs;
-    if (!low) ++res.matvecs_fp64;
+    if (!lp) ++res.matvecs_fp64;
   };
   // basis <- {v / |v|}, AV0 = A v0, H00
   auto start_from = [&](const double* v) {
@@ -134,6 +137,22 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     if (it == 0 && low_precision &&
         res.residual > opts.low_start_rel * std::max(1.0, std::fabs(res.eigenvalue)))
       low = true;  // far from converged: expand the basis with low-precision matvecs
+    if (tail) {
+      // a residual that stops falling has met the rounding of the
+      // low-precision products: refine in FP64 from the Ritz vector
+      if (res.residual < best * 0.7) {
+        best = res.residual;
+        stall = 0;
+      } else if (++stall >= 3 && res.residual >= opts.tol) {
+        tail = false;
+        tail_done = true;
+        ritz_vector(nsub);
+        start_from(w.y.get());
+        fill_f64(ctx, c, 1.0, 1);
+        nsub = 1;
+        continue;
+      }
+    }
     if (low) {
       if (res.residual < best * 0.7) {
         best = res.residual;
@@ -153,6 +172,10 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     } else if (res.residual < opts.tol) {
       res.converged = true;
       break;
+    } else if (low_precision && !tail && !tail_done && opts.low_tail > 0 && res.residual < opts.low_tail * opts.tol) {
+      tail = true;
+      best = res.residual;
+      stall = 0;
     }
     if (!can_grow) break;  // the basis spans the space: exact
     if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
@@ -162,8 +185,10 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
       TK_HIP(hipMemcpyAsync(host + 2, tn2, sizeof(double), hipMemcpyDeviceToHost, s));
       TK_HIP(hipStreamSynchronize(s));
       if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
-        if (low) {  // stagnated in low precision: refine in FP64
+        if (low || tail) {  // stagnated in low precision: refine in FP64
           low = false;
+          if (tail) tail_done = true;
+          tail = false;
           ritz_vector(nsub);
           start_from(w.y.get());
           fill_f64(ctx, c, 1.0, 1);
@@ -175,7 +200,15 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     }
     scale_by_inv_sqrt(ctx, slot, tn2, n);
     matvec(slot, AV + int64_t(nsub) * n);
-    dot_many(ctx, V, n, nsub + 1, AV + int64_t(nsub) * n, n, H + int64_t(nsub) * m);
+    if (tail) {
+      // H_jt = (A v_j) . t from the products held (A is symmetric), so the
+      // couplings of the new vector to the FP64 columns carry no
+      // low-precision rounding; only H_tt uses the new product
+      dot_many(ctx, AV, n, nsub, slot, n, H + int64_t(nsub) * m);
+      dot_many(ctx, slot, n, 1, AV + int64_t(nsub) * n, n, H + int64_t(nsub) * m + nsub);
+    } else {
+      dot_many(ctx, V, n, nsub + 1, AV + int64_t(nsub) * n, n, H + int64_t(nsub) * m);
+    }
     ++nsub;
     c_current = false;
   }

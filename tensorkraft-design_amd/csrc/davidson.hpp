@@ -11,11 +11,24 @@
 // wavefront kernel (Householder tridiagonalisation, multisection, inverse
 // iteration) on the device.
 //
-// Mixed precision: while the plan's precision is below FP64, new basis vectors
-// are multiplied with the low-precision kernels; once the residual stalls or
-// falls below `switch_rel * |theta|`, the solver restarts from the Ritz vector
-// with FP64 matvecs (FP64 residual refinement) and finishes with the FP64
-// residual test, so the returned eigenvalue is an FP64 Rayleigh quotient.
+// Mixed precision, with a plan precision below FP64, in two places:
+// - a cold start (first residual above low_start_rel * |theta|) multiplies new
+//   basis vectors with the low-precision kernels until the residual stalls or
+//   falls below switch_rel * |theta|, then restarts from the Ritz vector with
+//   FP64 matvecs (FP64 residual refinement);
+// - once the residual of an FP64 iteration is below low_tail * tol, the
+//   remaining corrections are multiplied in low precision without a restart.
+//   The projected matrix takes a new vector t's couplings to the earlier
+//   columns as (A v_j) . t from the products held (A is symmetric), not as
+//   v_j . (A t) from the rounded product, whose error (about eps |A|, eps the
+//   low precision's relative error) would exceed the true coupling of t to
+//   the start vector; only H_tt uses the rounded product.  The coefficient of
+//   t in the Ritz vector is then of the order of the residual over the gap,
+//   so the rounding moves the residual by about that coefficient times
+//   eps |A|: below tol for low_tail = 100 in FP32 (eps about 4e-7) up to
+//   |A| / gap of about 1e4.  A residual that stops falling (three iterations
+//   without a 0.7 reduction) restarts from the Ritz vector in FP64.
+// The residual test is on the Ritz residual of the basis held.
 #pragma once
 #include <cstdint>
 
@@ -34,6 +47,8 @@ struct DavidsonOpts {
   // when the first residual is above low_start_rel * |theta| (a warm start
   // already that close is refined in FP64 without a basis restart).
   double low_start_rel = 1e-4;
+  // low-precision corrections once ||r|| < low_tail * tol (0 or less: off)
+  double low_tail = 100;
 };
 
 struct DavidsonResult {

@@ -63,7 +63,7 @@ class tk_dmrg_config(C.Structure):
                 ("bond_schedule", C.POINTER(C.c_int64)), ("svd_cutoff", C.c_double), ("min_bond_dim", C.c_int64),
                 ("degeneracy_tol", C.c_double), ("davidson_tol", C.c_double), ("max_subspace", C.c_int32),
                 ("max_iter", C.c_int32), ("precondition", C.c_int32), ("switch_rel", C.c_double),
-                ("eigensolver", C.c_int32), ("max_krylov", C.c_int32)]
+                ("eigensolver", C.c_int32), ("max_krylov", C.c_int32), ("low_tail", C.c_double)]
 
 
 class tk_kernel_timing(C.Structure):
@@ -1713,7 +1713,7 @@ class Dmrg:
 
     def __init__(self, ctx, mpo_sites, sym, prec="fp64", schedule=(64,), svd_cutoff=1e-10, davidson_tol=1e-10,
                  max_subspace=60, max_iter=1000, precondition=True, switch_rel=1e-6, degeneracy_tol=1e-12,
-                 min_bond_dim=1, eigensolver="davidson", max_krylov=100):
+                 min_bond_dim=1, eigensolver="davidson", max_krylov=100, low_tail=100.0):
         self.ctx = ctx
         self.sym = sym
         self.n = len(mpo_sites)
@@ -1731,7 +1731,7 @@ class Dmrg:
         cfg = tk_dmrg_config(PREC[prec], sym, self.n, len(sched), sched.ctypes.data_as(C.POINTER(C.c_int64)),
                              svd_cutoff, min_bond_dim, degeneracy_tol, davidson_tol, max_subspace, max_iter,
                              1 if precondition else 0, switch_rel, {"davidson": 0, "lanczos": 1}[eigensolver],
-                             max_krylov)
+                             max_krylov, low_tail if low_tail else -1.0)
         h = C.c_void_p()
         _check(ctx.L.tk_dmrg_create(ctx.h, C.byref(cfg), arr, C.byref(h)), ctx.h)
         self.h = h
