@@ -302,6 +302,32 @@ SweepStats DmrgEngine::sweep() {
   return s;
 }
 
+namespace {
+bool same_layout(const BlockTable& a, const BlockTable& b) {
+  if (a.total != b.total || a.keys != b.keys || a.offset != b.offset || a.sec != b.sec || a.legs.size() != b.legs.size())
+    return false;
+  for (size_t k = 0; k < a.legs.size(); ++k)
+    if (a.legs[k].q != b.legs[k].q || a.legs[k].dim != b.legs[k].dim) return false;
+  return true;
+}
+}  // namespace
+
+HeffPlan& DmrgEngine::two_site_plan(int i) {
+  if (plan_cache_.size() < size_t(n_)) plan_cache_.resize(size_t(n_));
+  CachedPlan& c = plan_cache_[size_t(i)];
+  const Env& l = L_[i];
+  const Env& r = R_[i + 2];
+  if (c.plan && same_layout(c.l, l.t) && same_layout(c.r, r.t)) {
+    c.plan->rebind(l.d.get(), r.d.get());
+    return *c.plan;
+  }
+  c.plan.reset();  // release its device buffers before the new plan allocates
+  c.plan = std::make_unique<HeffPlan>(ctx_, cfg_.prec, l.t, l.d.get(), r.t, r.d.get(), cfg_.mpo[i], cfg_.mpo[i + 1]);
+  c.l = l.t;
+  c.r = r.t;
+  return *c.plan;
+}
+
 StepStats DmrgEngine::step(int i, bool lr) {
   StepStats st;
   st.site = i;
@@ -314,8 +340,7 @@ StepStats DmrgEngine::step(int i, bool lr) {
   const int64_t max_bond =
       cfg_.bond_schedule[std::min<size_t>(size_t(sweeps_), cfg_.bond_schedule.size() - 1)];
   double t0 = now();
-  HeffPlan plan(ctx_, cfg_.prec, L_[i].t, L_[i].d.get(), R_[i + 2].t, R_[i + 2].d.get(), cfg_.mpo[i],
-                cfg_.mpo[i + 1]);
+  HeffPlan& plan = two_site_plan(i);
   const BlockTable& psi = plan.psi();
   DBuf<double> x(ctx_, size_t(std::max<int64_t>(1, psi.total)));
   contract_two_site(i, psi, x.get());

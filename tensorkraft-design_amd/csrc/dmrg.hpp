@@ -15,6 +15,7 @@
 //   environments (bra, w, ket) with every allowed block, q(bra) = q(ket) - q(w)
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "davidson.hpp"
@@ -154,6 +155,10 @@ class DmrgEngine {
     DBuf<double> d;
   };
   StepStats step(int i, bool lr);
+  // The two-site H_eff plan of sites (i, i+1): the one cached for i when the
+  // environments' block layouts are those it was built for (rebound to the
+  // current environment buffers), else a new one that replaces it.
+  HeffPlan& two_site_plan(int i);
   // theta = A[i] A[i+1] into x (psi layout of `psi`)
   void contract_two_site(int i, const BlockTable& psi, double* x);
   // SVD of the two-site vector, truncation and write-back of A[i], A[i+1].
@@ -181,6 +186,11 @@ class DmrgEngine {
   std::vector<QIndex> bonds_;
   std::vector<Site> site_;
   std::vector<Env> L_, R_;
+  struct CachedPlan {
+    BlockTable l, r;  // layouts of the environments the plan was built for
+    std::unique_ptr<HeffPlan> plan;
+  };
+  std::vector<CachedPlan> plan_cache_;
   double energy_ = 0;
   int sweeps_ = 0;
   int centre_ = -1;  // site of the orthogonality centre (-1 before prepare)
