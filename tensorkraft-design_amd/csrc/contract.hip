@@ -714,21 +714,13 @@ struct Packing {
 
 size_t DeviceContractList::packed_bytes(const ContractList& h) { return Packing(h).total; }
 
-void DeviceContractList::upload_into(Context* ctx, const ContractList& h, char* base, size_t capacity) {
+void DeviceContractList::pack(const ContractList& h, char* dst, char* base) {
   const Packing pk(h);
-  if (pk.total > capacity)
-    throw ArgError("contraction list of " + std::to_string(pk.total) + " bytes exceeds the " +
-                   std::to_string(capacity) + "-byte buffer");
-  host.assign(pk.total, 0);
-  std::copy_n(reinterpret_cast<const char*>(h.items.data()), h.items.size() * sizeof(GItem),
-              host.data() + pk.o_items);
-  std::copy_n(reinterpret_cast<const char*>(h.witems.data()), h.witems.size() * sizeof(GItem),
-              host.data() + pk.o_witems);
-  std::copy_n(reinterpret_cast<const char*>(h.outs.data()), h.outs.size() * sizeof(GOut), host.data() + pk.o_outs);
-  std::copy_n(reinterpret_cast<const char*>(h.terms.data()), h.terms.size() * sizeof(GTerm),
-              host.data() + pk.o_terms);
-  std::copy_n(reinterpret_cast<const char*>(h.srcs.data()), h.srcs.size() * sizeof(GSrc), host.data() + pk.o_srcs);
-  if (pk.total) TK_HIP(hipMemcpyAsync(base, host.data(), pk.total, hipMemcpyHostToDevice, ctx->stream));
+  std::copy_n(reinterpret_cast<const char*>(h.items.data()), h.items.size() * sizeof(GItem), dst + pk.o_items);
+  std::copy_n(reinterpret_cast<const char*>(h.witems.data()), h.witems.size() * sizeof(GItem), dst + pk.o_witems);
+  std::copy_n(reinterpret_cast<const char*>(h.outs.data()), h.outs.size() * sizeof(GOut), dst + pk.o_outs);
+  std::copy_n(reinterpret_cast<const char*>(h.terms.data()), h.terms.size() * sizeof(GTerm), dst + pk.o_terms);
+  std::copy_n(reinterpret_cast<const char*>(h.srcs.data()), h.srcs.size() * sizeof(GSrc), dst + pk.o_srcs);
   items = reinterpret_cast<GItem*>(base + pk.o_items);
   witems = reinterpret_cast<GItem*>(base + pk.o_witems);
   outs = reinterpret_cast<GOut*>(base + pk.o_outs);
@@ -741,6 +733,36 @@ void DeviceContractList::upload_into(Context* ctx, const ContractList& h, char* 
   flags = h.flags;
   flops = h.flops;
   beta = h.beta;
+}
+
+void DeviceContractList::upload_into(Context* ctx, const ContractList& h, char* base, size_t capacity) {
+  const Packing pk(h);
+  if (pk.total > capacity)
+    throw ArgError("contraction list of " + std::to_string(pk.total) + " bytes exceeds the " +
+                   std::to_string(capacity) + "-byte buffer");
+  host.assign(pk.total, 0);
+  pack(h, host.data(), base);
+  ctx->upload(base, host.data(), pk.total);
+}
+
+int ListPack::add(ContractList&& l, int64_t a_n, int64_t b_n, int64_t c_n) {
+  if (l.outs.empty()) return -1;
+  l.finalize();
+  l.validate(a_n, b_n, c_n);
+  lists_.push_back(std::move(l));
+  return int(lists_.size()) - 1;
+}
+
+void ListPack::upload(Context* ctx) {
+  dev_.clear();
+  dev_.resize(lists_.size());
+  std::vector<size_t> off(lists_.size() + 1, 0);
+  for (size_t i = 0; i < lists_.size(); ++i) off[i + 1] = off[i] + DeviceContractList::packed_bytes(lists_[i]);
+  if (off.back() == 0) return;
+  buf_.reset(ctx, off.back());
+  host_.assign(off.back(), 0);
+  for (size_t i = 0; i < lists_.size(); ++i) dev_[i].pack(lists_[i], host_.data() + off[i], buf_.get() + off[i]);
+  ctx->upload(buf_.get(), host_.data(), off.back());
 }
 
 void DeviceContractList::upload(Context* ctx, const ContractList& h) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -37,6 +38,10 @@ struct CancelledError : std::runtime_error {
 
 // Size-class caching allocator: freed blocks are kept for reuse until the
 // context is destroyed, so the sweep's per-step buffers cost no hipMalloc.
+// A block freed on the host may be handed out again while kernels queued
+// before the free still use it: every use goes through the context's one
+// stream (launches and staged copies alike), so the new owner's work runs
+// after theirs and no synchronisation is needed at a free.
 class DevicePool {
  public:
   DevicePool() = default;
@@ -201,21 +206,49 @@ struct Context {
     }
     return scratch_p;
   }
+  // Pinned staging for host-to-device uploads (a context is driven by one
+  // host thread).  A copy out of pageable memory may hold the host until the
+  // stream reaches it, leaving the GPU idle while the host prepares the next
+  // launches; staged in pinned memory it is a DMA in stream order.  Slices
+  // are recycled once the stream has drained (sync(), or when the arena is
+  // full); an upload larger than the arena is copied directly.
+  static constexpr size_t kStageBytes = size_t(32) << 20;
+  char* stage_p = nullptr;
+  size_t stage_off = 0;
+  void upload(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    if (stage_p == nullptr || bytes > kStageBytes) {
+      TK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+      return;
+    }
+    if (stage_off + bytes > kStageBytes) {
+      TK_HIP(hipStreamSynchronize(stream));
+      stage_off = 0;
+    }
+    std::memcpy(stage_p + stage_off, src, bytes);
+    TK_HIP(hipMemcpyAsync(dst, stage_p + stage_off, bytes, hipMemcpyHostToDevice, stream));
+    stage_off += (bytes + 255) & ~size_t(255);
+  }
   explicit Context(int dev) : device(dev) {
     TK_HIP(hipSetDevice(dev));
     TK_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     hipDeviceProp_t prop;
     TK_HIP(hipGetDeviceProperties(&prop, dev));
     n_cu = prop.multiProcessorCount;
+    TK_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage_p), kStageBytes, hipHostMallocDefault));
   }
   ~Context() {
     if (stream) {
       (void)hipStreamSynchronize(stream);
       (void)hipStreamDestroy(stream);
     }
+    if (stage_p) (void)hipHostFree(stage_p);
     pool.release_all();
   }
-  void sync() { TK_HIP(hipStreamSynchronize(stream)); }
+  void sync() {
+    TK_HIP(hipStreamSynchronize(stream));
+    stage_off = 0;
+  }
 };
 
 // RAII device buffer drawn from the context pool.
@@ -251,7 +284,7 @@ class DBuf {
   size_t size() const { return n_; }
   void upload(const T* h, size_t n) {
     if (n > n_) throw ArgError("upload larger than buffer");
-    if (n) TK_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, ctx_->stream));
+    ctx_->upload(p_, h, n * sizeof(T));
   }
   void download(T* h, size_t n) const {
     if (n > n_) throw ArgError("download larger than buffer");

@@ -105,7 +105,6 @@ void run_list(Context* ctx, ContractList& cl, int64_t a_n, int64_t b_n, int64_t 
   DeviceContractList d;
   d.upload(ctx, cl);
   launch_contract(ctx, d, A, B, C);
-  ctx->sync();  // the list's host staging and device buffer end here
 }
 
 }  // namespace
@@ -142,8 +141,7 @@ void launch_copies(Context* ctx, const std::vector<CopyJob>& jobs, const double*
   DBuf<CopyItem> di(ctx, rows.size() + tiles.size());
   if (!rows.empty()) di.upload(rows.data(), rows.size());
   if (!tiles.empty())
-    TK_HIP(hipMemcpyAsync(di.get() + rows.size(), tiles.data(), tiles.size() * sizeof(CopyItem),
-                          hipMemcpyHostToDevice, ctx->stream));
+    ctx->upload(di.get() + rows.size(), tiles.data(), tiles.size() * sizeof(CopyItem));
   const size_t cap = size_t(ctx->n_cu) * 8;
   // algorithmic bytes: each element read once and written once
   if (!rows.empty()) {

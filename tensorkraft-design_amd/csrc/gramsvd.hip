@@ -19,7 +19,6 @@ void run(Context* ctx, ContractList& cl, int64_t a_n, int64_t b_n, int64_t c_n, 
   DeviceContractList d;
   d.upload(ctx, cl);
   launch_contract(ctx, d, A, B, C);
-  ctx->sync();
 }
 
 // G_b = X_b X_b^T for the selected blocks

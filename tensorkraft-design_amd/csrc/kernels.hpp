@@ -31,6 +31,28 @@ struct DeviceContractList {
   void upload_into(Context* ctx, const ContractList& h, char* dst, size_t capacity);
   // bytes the packed list of h takes
   static size_t packed_bytes(const ContractList& h);
+  // The packed bytes of h into host memory at dst, and the table pointers
+  // set for the same bytes at device address base (no transfer).
+  void pack(const ContractList& h, char* dst, char* base);
+};
+
+// Several lists in one device allocation, uploaded with one copy, for
+// phases that launch many small contractions in sequence.
+class ListPack {
+ public:
+  // Finalizes `l` and checks it against the element counts of the buffers it
+  // will run on; its index in the pack (-1 for a list without outputs, which
+  // launches nothing).
+  int add(ContractList&& l, int64_t a_n, int64_t b_n, int64_t c_n);
+  // One allocation and one host-to-device copy for every list added.
+  void upload(Context* ctx);
+  const DeviceContractList& operator[](int i) const { return dev_[size_t(i)]; }
+
+ private:
+  std::vector<ContractList> lists_;
+  std::vector<DeviceContractList> dev_;
+  std::vector<char> host_;
+  DBuf<char> buf_;
 };
 
 // One kernel launch held as a graph node: the function, its shape and its

@@ -425,8 +425,8 @@ std::vector<int> tridiag_columns_coop(Context* ctx, const std::vector<CoopJob>& 
     const size_t jb = dj.size() * sizeof(CoopJobDev);
     const size_t sb = slots.size() * sizeof(CoopSlot);
     meta.reset(ctx, jb + sb);
-    TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb, hipMemcpyHostToDevice, ctx->stream));
-    TK_HIP(hipMemcpyAsync(meta.get() + jb, slots.data(), sb, hipMemcpyHostToDevice, ctx->stream));
+    ctx->upload(meta.get(), dj.data(), jb);
+    ctx->upload(meta.get() + jb, slots.data(), sb);
     // arrival flags, then one slot counter per XCD
     const size_t nctr = (size_t(nflag) + size_t(nx)) * kCtrStride;
     ctr.reset(ctx, nctr);

@@ -1845,18 +1845,6 @@ struct VecWork {
   double orth_bytes = 0, orth_flops = 0, bt_bytes = 0, bt_flops = 0;
 };
 
-// Launches one list; the device and host copies of its tables are kept in
-// `keep` until the caller has synchronised.
-void run_list(Context* ctx, ContractList& cl, int64_t a_n, int64_t b_n, int64_t c_n, const double* A, const double* B,
-              double* C, std::vector<std::unique_ptr<DeviceContractList>>& keep) {
-  if (cl.outs.empty()) return;
-  cl.finalize();
-  cl.validate(a_n, b_n, c_n);
-  keep.push_back(std::make_unique<DeviceContractList>());
-  keep.back()->upload(ctx, cl);
-  launch_contract(ctx, *keep.back(), A, B, C);
-}
-
 // Z (output, ld n, the first k columns at a_off) = Q * zi, with Q the product
 // of the reflectors stored in the rows of A; blocks of kWyNb reflectors are
 // applied last block first, each as W = V^T Y (split over row segments) and
@@ -1891,11 +1879,13 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
     }
   }
   if (hb.empty()) return;
-  std::vector<std::unique_ptr<DeviceContractList>> keep;
   const int64_t st_n = int64_t(hb.size()) * kWyNb * kWyNb;
   DBuf<double> S(ctx, size_t(st_n)), T(ctx, size_t(st_n)), Ut(ctx, size_t(a_n));
   DBuf<WyBlock> wbd(ctx, wb.size());
   wbd.upload(wb.data(), wb.size());
+  // every list of the back-transformation, uploaded with one copy
+  ListPack pack;
+  int i_s, i_ut;
   {  // S = V^T V per block
     ContractList cl;
     cl.flags = kTransB;
@@ -1907,12 +1897,8 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
       cl.begin_term(v, j.n, j.n - j0 - 1);
       cl.add_src(v, j.n, 1.0);
     }
-    run_list(ctx, cl, a_n, a_n, st_n, A, A, S.get(), keep);
+    i_s = pack.add(std::move(cl), a_n, a_n, st_n);
   }
-  allow_lds(wy_t_kernel, kWyTLds);
-  hipLaunchKernelGGL(wy_t_kernel, dim3(unsigned(hb.size())), dim3(kWyNb), kWyTLds, ctx->stream, wbd.get(), S.get(),
-                     TAU, T.get());
-  TK_HIP(hipGetLastError());
   {  // U^T = T^T V^T per block, stored in the rows of Ut like V^T in A
     ContractList cl;
     cl.flags = kTransA;
@@ -1924,7 +1910,7 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
       cl.begin_term(v, j.n, nb);
       cl.add_src(wb[b].t_off, kWyNb, 1.0);
     }
-    run_list(ctx, cl, st_n, a_n, a_n, T.get(), A, Ut.get(), keep);
+    i_ut = pack.add(std::move(cl), st_n, a_n, a_n);
   }
   // application, last block of every job first
   size_t steps = 0;
@@ -1942,6 +1928,7 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
     w_max = std::max(w_max, w);
   }
   DBuf<double> Wb(ctx, size_t(std::max<int64_t>(w_max, 1)));
+  std::vector<std::pair<int, int>> step_lists(steps, {-1, -1});
   for (size_t st = 0; st < steps; ++st) {
     ContractList lw, lu;
     lw.flags = 0;
@@ -1967,10 +1954,23 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
         w += int64_t(nb) * k;
       }
     }
-    run_list(ctx, lw, a_n, z_n, w_max, A, Z, Wb.get(), keep);
-    run_list(ctx, lu, w_max, a_n, z_n, Wb.get(), Ut.get(), Z, keep);
+    step_lists[st].first = pack.add(std::move(lw), a_n, z_n, w_max);
+    step_lists[st].second = pack.add(std::move(lu), w_max, a_n, z_n);
   }
-  ctx->sync();  // the lists' device tables are released on return
+  pack.upload(ctx);
+  auto run = [&](int i, const double* A_, const double* B_, double* C_) {
+    if (i >= 0) launch_contract(ctx, pack[i], A_, B_, C_);
+  };
+  run(i_s, A, A, S.get());
+  allow_lds(wy_t_kernel, kWyTLds);
+  hipLaunchKernelGGL(wy_t_kernel, dim3(unsigned(hb.size())), dim3(kWyNb), kWyTLds, ctx->stream, wbd.get(), S.get(),
+                     TAU, T.get());
+  TK_HIP(hipGetLastError());
+  run(i_ut, T.get(), A, Ut.get());
+  for (size_t st = 0; st < steps; ++st) {
+    run(step_lists[st].first, A, Z, Wb.get());
+    run(step_lists[st].second, Wb.get(), Ut.get(), Z);
+  }
 }
 
 // Cholesky factor and inverse of the nb x nb Gram block G = Z_b^T Z_b of each
@@ -2112,7 +2112,6 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
     h_max = std::max(h_max, h);
   }
   DBuf<double> H(ctx, size_t(h_max));
-  std::vector<std::unique_ptr<DeviceContractList>> keep;
   // Cholesky QR scratch sized for the largest step: Gram blocks and R^-1
   // (nb x nb per job), T (n x nb per job); per step and job the block offset
   // (-1: none) and the success flag
@@ -2136,6 +2135,14 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
   DBuf<int64_t> cq_off(ctx, std::max<size_t>(cq_offs.size(), 1));
   if (!cq_offs.empty()) cq_off.upload(cq_offs.data(), cq_offs.size());
   DBuf<int> cq_ok(ctx, std::max<size_t>(size_t(steps) * nj, 1));
+  // every step's lists first, uploaded with one copy; then the launches
+  struct StepLists {
+    int h[2] = {-1, -1}, u2[2] = {-1, -1};
+    int g = -1, t = -1, g2 = -1, z = -1;
+    bool chol = false;
+  };
+  std::vector<StepLists> sl(static_cast<size_t>(steps));
+  ListPack pack;
   for (int st = 0; st < steps; ++st) {
     for (int pass = 0; pass < 2; ++pass) {
       ContractList lh, lu2;
@@ -2160,14 +2167,13 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
           h += int64_t(c0) * nb;
         }
       }
-      run_list(ctx, lh, z_n, z_n, h_max, Z, Z, H.get(), keep);
-      run_list(ctx, lu2, h_max, z_n, z_n, H.get(), Z, Z, keep);
+      sl[size_t(st)].h[pass] = pack.add(std::move(lh), z_n, z_n, h_max);
+      sl[size_t(st)].u2[pass] = pack.add(std::move(lu2), h_max, z_n, z_n);
     }
     // within the block: Cholesky QR twice through grouped contractions,
     // G = Z_b^T Z_b = R^T R, T = Z_b R^-1, then the same from T back into
     // Z_b; a block with a numerically dependent column keeps Z_b (R^-1 = I)
     // and is finished column by column by orth_kernel
-    const int* done = nullptr;
     if (kOrthBlock <= kCholNB) {
       int64_t gt = 0, tt = 0;
       ContractList lg, lt, lg2, lz;
@@ -2194,28 +2200,47 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
         tt += int64_t(j.n) * nb;
       }
       if (gt > 0) {
-        const int64_t* offs = cq_off.get() + int64_t(st) * nj;
-        int* okf = cq_ok.get() + int64_t(st) * nj;
-        const size_t lds = 2 * size_t(kCholNB) * (kCholNB + 1) * sizeof(double);
-        allow_lds(cholinv_kernel<kCholNB>, lds);
-        run_list(ctx, lg, z_n, z_n, cq_gt, Z, Z, cq_g.get(), keep);
-        hipLaunchKernelGGL(cholinv_kernel<kCholNB>, dim3(unsigned(nj)), dim3(256), lds, ctx->stream, djd, offs,
-                           cq_g.get(), cq_r.get(), norms.get(), st, kOrthBlock, 1, okf);
-        run_list(ctx, lt, z_n, cq_gt, cq_tt, Z, cq_r.get(), cq_t.get(), keep);
-        run_list(ctx, lg2, cq_tt, cq_tt, cq_gt, cq_t.get(), cq_t.get(), cq_g.get(), keep);
-        hipLaunchKernelGGL(cholinv_kernel<kCholNB>, dim3(unsigned(nj)), dim3(256), lds, ctx->stream, djd, offs,
-                           cq_g.get(), cq_r.get(), norms.get(), st, kOrthBlock, 0, okf);
-        run_list(ctx, lz, cq_tt, cq_gt, z_n, cq_t.get(), cq_r.get(), Z, keep);
-        TK_HIP(hipGetLastError());
-        done = okf;
+        StepLists& q = sl[size_t(st)];
+        q.chol = true;
+        q.g = pack.add(std::move(lg), z_n, z_n, cq_gt);
+        q.t = pack.add(std::move(lt), z_n, cq_gt, cq_tt);
+        q.g2 = pack.add(std::move(lg2), cq_tt, cq_tt, cq_gt);
+        q.z = pack.add(std::move(lz), cq_tt, cq_gt, z_n);
       }
+    }
+  }
+  pack.upload(ctx);
+  auto run = [&](int i, const double* A, const double* B, double* C) {
+    if (i >= 0) launch_contract(ctx, pack[i], A, B, C);
+  };
+  const size_t chol_lds = 2 * size_t(kCholNB) * (kCholNB + 1) * sizeof(double);
+  if (kOrthBlock <= kCholNB) allow_lds(cholinv_kernel<kCholNB>, chol_lds);
+  for (int st = 0; st < steps; ++st) {
+    const StepLists& q = sl[size_t(st)];
+    for (int pass = 0; pass < 2; ++pass) {
+      run(q.h[pass], Z, Z, H.get());
+      run(q.u2[pass], H.get(), Z, Z);
+    }
+    const int* done = nullptr;
+    if (q.chol) {
+      const int64_t* offs = cq_off.get() + int64_t(st) * nj;
+      int* okf = cq_ok.get() + int64_t(st) * nj;
+      run(q.g, Z, Z, cq_g.get());
+      hipLaunchKernelGGL(cholinv_kernel<kCholNB>, dim3(unsigned(nj)), dim3(256), chol_lds, ctx->stream, djd, offs,
+                         cq_g.get(), cq_r.get(), norms.get(), st, kOrthBlock, 1, okf);
+      run(q.t, Z, cq_r.get(), cq_t.get());
+      run(q.g2, cq_t.get(), cq_t.get(), cq_g.get());
+      hipLaunchKernelGGL(cholinv_kernel<kCholNB>, dim3(unsigned(nj)), dim3(256), chol_lds, ctx->stream, djd, offs,
+                         cq_g.get(), cq_r.get(), norms.get(), st, kOrthBlock, 0, okf);
+      run(q.z, cq_t.get(), cq_r.get(), Z);
+      TK_HIP(hipGetLastError());
+      done = okf;
     }
     if (nmax <= 1024)
       launch_orth_range<8>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get(), done);
     else
       launch_orth_range<4>(ctx, nmax, nj, djd, Z, lu, st, kOrthBlock, norms.get(), done);
   }
-  ctx->sync();  // the lists' tables and the scratch buffers are released on return
 }
 
 }  // namespace
@@ -2269,10 +2294,9 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   const size_t jb_bytes = dj.size() * sizeof(DevJob);
   const size_t tj_bytes = tj.size() * sizeof(TailJob);
   DBuf<char> meta(ctx_, jb_bytes + tj_bytes + items.size() * sizeof(BisItem));
-  TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb_bytes, hipMemcpyHostToDevice, ctx_->stream));
-  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes, tj.data(), tj_bytes, hipMemcpyHostToDevice, ctx_->stream));
-  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes + tj_bytes, items.data(), items.size() * sizeof(BisItem),
-                        hipMemcpyHostToDevice, ctx_->stream));
+  ctx_->upload(meta.get(), dj.data(), jb_bytes);
+  ctx_->upload(meta.get() + jb_bytes, tj.data(), tj_bytes);
+  ctx_->upload(meta.get() + jb_bytes + tj_bytes, items.data(), items.size() * sizeof(BisItem));
   const DevJob* djd = reinterpret_cast<const DevJob*>(meta.get());
   const TailJob* tjd = reinterpret_cast<const TailJob*>(meta.get() + jb_bytes);
   const BisItem* itd = reinterpret_cast<const BisItem*>(meta.get() + jb_bytes + tj_bytes);
@@ -2416,8 +2440,6 @@ void SymEigBatch::values(const std::vector<SymJob>& jobs, double* A, int64_t a_n
   TK_HIP(hipGetLastError());
   TK_HIP(hipMemcpyAsync(tn_.get(), tn_all.get() + jobs.size(), jobs.size() * sizeof(double),
                         hipMemcpyDeviceToDevice, ctx_->stream));
-  // keep the metadata alive until the kernels have consumed it
-  ctx_->sync();
 }
 
 void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z_n) {
@@ -2442,8 +2464,8 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   const size_t jb_bytes = dj.size() * sizeof(DevJob);
   const size_t vi_bytes = vitems.size() * sizeof(VecItem);
   DBuf<char> meta(ctx_, jb_bytes + vi_bytes);
-  TK_HIP(hipMemcpyAsync(meta.get(), dj.data(), jb_bytes, hipMemcpyHostToDevice, ctx_->stream));
-  TK_HIP(hipMemcpyAsync(meta.get() + jb_bytes, vitems.data(), vi_bytes, hipMemcpyHostToDevice, ctx_->stream));
+  ctx_->upload(meta.get(), dj.data(), jb_bytes);
+  ctx_->upload(meta.get() + jb_bytes, vitems.data(), vi_bytes);
   const DevJob* djd = reinterpret_cast<const DevJob*>(meta.get());
   const VecItem* vid = reinterpret_cast<const VecItem*>(meta.get() + jb_bytes);
   // per-job Gershgorin bounds indexed by the caller's job number
@@ -2473,7 +2495,6 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   const int tb = ctx_->timer.begin(ctx_->stream);
   wy_backtransform(ctx_, dj, djd, A_, a_n_, TAU, zi.get(), z_tot, Z, z_n);
   ctx_->timer.end(tb, ctx_->stream, "backtransform", work.bt_bytes, work.bt_flops, 1);
-  ctx_->sync();
 }
 
 void orthonormalize_columns(Context* ctx, const std::vector<OrthJob>& jobs, double* Z, int64_t z_n) {
