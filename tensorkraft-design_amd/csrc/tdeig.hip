@@ -1176,16 +1176,37 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
 // Column 2-norms of the n x k row-major Z of every job into norms (at w_off).
 __global__ __launch_bounds__(256) void colnorm_kernel(const DevJob* __restrict__ jobs, const double* __restrict__ Z,
                                                       double* __restrict__ norms) {
+  // 16 columns per workgroup, the rows split over 16 lanes per column (a
+  // row's 16 columns are one 128-byte segment); the 16 partial sums of a
+  // column are added in lane order
+  __shared__ double part[16][17];
   const DevJob jb = jobs[blockIdx.y];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= jb.k) return;
+  const int cl = threadIdx.x & 15, rl = threadIdx.x >> 4;
+  const int c = blockIdx.x * 16 + cl;
   const double* z = Z + jb.z_off;
-  double s = 0;
-  for (int i = 0; i < jb.n; ++i) {
-    const double x = z[int64_t(i) * jb.k + c];
-    s += x * x;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  if (c < jb.k) {
+    int i = rl;
+    for (; i + 48 < jb.n; i += 64) {
+      const double x0 = z[int64_t(i) * jb.k + c], x1 = z[int64_t(i + 16) * jb.k + c];
+      const double x2 = z[int64_t(i + 32) * jb.k + c], x3 = z[int64_t(i + 48) * jb.k + c];
+      s0 += x0 * x0;
+      s1 += x1 * x1;
+      s2 += x2 * x2;
+      s3 += x3 * x3;
+    }
+    for (; i < jb.n; i += 16) {
+      const double x = z[int64_t(i) * jb.k + c];
+      s0 += x * x;
+    }
   }
-  norms[jb.w_off + c] = sqrt(s);
+  part[rl][cl] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (threadIdx.x < 16 && c < jb.k) {
+    double t = 0;
+    for (int r = 0; r < 16; ++r) t += part[r][cl];
+    norms[jb.w_off + c] = sqrt(t);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1816,8 +1837,12 @@ __global__ __launch_bounds__(kWyNb) void wy_t_kernel(const WyBlock* __restrict__
   const double* tau = TAUg + b.tau_off;
   const int l = threadIdx.x, nb = b.nb;
   for (int m = 0; m < kWyNb; ++m) tl[l * (kWyNb + 1) + m] = 0.0;
+  // column i + 1 of S is requested while step i runs (its global load is
+  // the step's latency otherwise)
+  double snext = 0.0;
   for (int i = 0; i < nb; ++i) {
-    sc[l] = l < i ? s[int64_t(l) * kWyNb + i] : 0.0;
+    sc[l] = snext;
+    snext = (l < i + 1 && i + 1 < nb) ? s[int64_t(l) * kWyNb + i + 1] : 0.0;
     __syncthreads();
     if (l < i) {
       // four partial sums, so the LDS reads of the row are issued ahead
@@ -1997,7 +2022,10 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   const int nb = min(c_count, jb.k - c0);
   const int tid = threadIdx.x;
   constexpr int LD = NB + 1;
-  for (int q = tid; q < nb * nb; q += 256) L[(q / nb) * LD + q % nb] = G[off + q];
+  // 16 x 16 thread grid over the block (no index division in the loops)
+  const int tx = tid & 15, ty = tid >> 4;
+  for (int i = ty; i < nb; i += 16)
+    for (int j = tx; j < nb; j += 16) L[i * LD + j] = G[off + int64_t(i) * nb + j];
   if (tid == 0) good = test ? 1 : ok[blockIdx.x];
   __syncthreads();
   // Right-looking Cholesky with one barrier per column: every thread reads
@@ -2019,10 +2047,10 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
     }
     rprev = sqrt(dd);
     const double idd = 1.0 / dd;
-    const int rem = nb - a - 1;
-    for (int q = tid; q < rem * rem; q += 256) {
-      const int b = a + 1 + q / rem, c = a + 1 + q % rem;
-      if (c >= b) L[b * LD + c] -= (L[a * LD + b] * idd) * L[a * LD + c];
+    for (int b = a + 1 + ty; b < nb; b += 16) {
+      const double ub = L[a * LD + b] * idd;
+      for (int c = a + 1 + tx; c < nb; c += 16)
+        if (c >= b) L[b * LD + c] -= ub * L[a * LD + c];
     }
     __syncthreads();
   }
@@ -2049,10 +2077,9 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
     }
   }
   __syncthreads();
-  for (int q = tid; q < nb * nb; q += 256) {
-    const int i = q / nb, j = q % nb;
-    Rinv[off + q] = g ? (i <= j ? X[i * LD + j] : 0.0) : (i == j ? 1.0 : 0.0);
-  }
+  for (int i = ty; i < nb; i += 16)
+    for (int j = tx; j < nb; j += 16)
+      Rinv[off + int64_t(i) * nb + j] = g ? (i <= j ? X[i * LD + j] : 0.0) : (i == j ? 1.0 : 0.0);
   if (tid == 0) ok[blockIdx.x] = g ? 1 : 0;
 }
 
@@ -2096,7 +2123,7 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
   if (kmax == 0) return;
   const int kOrthBlock = orth_block();
   DBuf<double> norms(ctx, size_t(std::max<int64_t>(w_n, 1)));
-  hipLaunchKernelGGL(colnorm_kernel, dim3(unsigned((kmax + 255) / 256), unsigned(nj)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(colnorm_kernel, dim3(unsigned((kmax + 15) / 16), unsigned(nj)), dim3(256), 0, ctx->stream,
                      djd, Z, norms.get());
   TK_HIP(hipGetLastError());
   // block `step` of job j covers columns [first_j + step * kOrthBlock, ...)
