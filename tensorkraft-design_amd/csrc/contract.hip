@@ -119,6 +119,22 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
+// Item position of workgroup b among nb persistent workgroups.  The
+// dispatcher deals workgroups to the eight XCDs in turn (b, b + 8, ... share
+// one XCD and its L2), and neighbouring items of a list usually belong to
+// one output and share its operand rows and columns; so each XCD takes runs
+// of kXcdRun consecutive items, the runs dealt round-robin to the XCDs (the
+// list's longest-first order is kept at run granularity).  Workgroups past
+// the last whole round of runs keep their own index.  A bijection on
+// [0, nb); results do not depend on it.
+constexpr int kXcdRun = 16;
+__device__ __forceinline__ int xcd_pos(int b, int nb) {
+  const int whole = nb / (8 * kXcdRun) * (8 * kXcdRun);
+  if (b >= whole) return b;
+  const int x = b & 7, j = b >> 3;
+  return (j / kXcdRun) * (8 * kXcdRun) + x * kXcdRun + j % kXcdRun;
+}
+
 // The U consecutive elements p[0, U) of one lane (an operand whose inner
 // index is contiguous in memory): one or two 16-byte loads when the whole run
 // lies inside the block (live, base + U <= n), else element by element with
@@ -255,7 +271,8 @@ __global__ __launch_bounds__(256) void contract_kernel(const GItem* __restrict__
                                                        const TI* __restrict__ A, const TI* __restrict__ B,
                                                        TO* __restrict__ C, double beta) {
   const int nwaves = gridDim.x * (blockDim.x >> 6);
-  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int wave0 = __builtin_amdgcn_readfirstlane(xcd_pos(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                                                   (threadIdx.x >> 6));
   wave_items<TI, TO, TA, TB, TC, U>(items, n_items, wave0, nwaves, outs, terms, srcs, A, B, C, beta);
 }
 
@@ -381,7 +398,8 @@ __global__ __launch_bounds__(256) void contract_wide_kernel(const GItem* __restr
                                                             const GSrc* __restrict__ srcs,
                                                             const TI* __restrict__ A, const TI* __restrict__ B,
                                                             TO* __restrict__ C, double beta) {
-  wide_items<TI, TO, TA, TB, TC, U>(items, n_items, blockIdx.x, gridDim.x, outs, terms, srcs, A, B, C, beta);
+  wide_items<TI, TO, TA, TB, TC, U>(items, n_items, xcd_pos(blockIdx.x, gridDim.x), gridDim.x, outs, terms, srcs, A,
+                                    B, C, beta);
 }
 
 // Both item lists of a wide list in one launch: workgroups [0, wblocks) run
@@ -399,10 +417,13 @@ __global__ __launch_bounds__(256) void contract_fused_kernel(const GItem* __rest
                                                              const TI* __restrict__ A, const TI* __restrict__ B,
                                                              TO* __restrict__ C, double beta) {
   if (int(blockIdx.x) < wblocks) {
-    wide_items<TI, TO, TA, TB, TC, U>(witems, n_witems, blockIdx.x, wblocks, outs, terms, srcs, A, B, C, beta);
+    wide_items<TI, TO, TA, TB, TC, U>(witems, n_witems, xcd_pos(blockIdx.x, wblocks), wblocks, outs, terms, srcs, A,
+                                      B, C, beta);
   } else {
-    const int nwaves = (int(gridDim.x) - wblocks) * 4;
-    const int wave0 = __builtin_amdgcn_readfirstlane((int(blockIdx.x) - wblocks) * 4 + int(threadIdx.x >> 6));
+    const int nb = int(gridDim.x) - wblocks;
+    const int nwaves = nb * 4;
+    const int wave0 =
+        __builtin_amdgcn_readfirstlane(xcd_pos(int(blockIdx.x) - wblocks, nb) * 4 + int(threadIdx.x >> 6));
     wave_items<TI, TO, TA, TB, TC, U>(items, n_items, wave0, nwaves, outs, terms, srcs, A, B, C, beta);
   }
 }
@@ -445,7 +466,7 @@ __global__ __launch_bounds__(256, 2) void contract_tile64_kernel(const GItem* __
     a_slot[h] = ((a_r[h] >> 4) * U + a_k[h] % U) * 64 + (a_r[h] & 15) + 16 * (a_k[h] / U);
     b_slot[h] = ((b_c[h] >> 4) * U + b_k[h] % U) * 64 + (b_c[h] & 15) + 16 * (b_k[h] / U);
   }
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+  for (int it = xcd_pos(blockIdx.x, gridDim.x); it < n_items; it += gridDim.x) {
     const GItem item = items[it];
     const GOut o = outs[item.out];
     acc_t c00 = {0, 0, 0, 0}, c01 = c00, c10 = c00, c11 = c00;
