@@ -391,7 +391,7 @@ typedef struct {
   int32_t max_krylov;           /* Lanczos vectors per pass (<= 512; 0: 100) */
   double low_tail;              /* with prec below FP64, Davidson corrections are multiplied in
                                    that precision once the residual is below low_tail *
-                                   davidson_tol (0: 100; negative: never) */
+                                   davidson_tol (0: 1000; negative: never) */
 } tk_dmrg_config;
 enum { TK_EIG_DAVIDSON = 0, TK_EIG_LANCZOS = 1 };
 

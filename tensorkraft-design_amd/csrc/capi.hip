@@ -853,7 +853,7 @@ tk_status tk_dmrg_create(tk_ctx* ctx, const tk_dmrg_config* cfg, const tk_mpo_si
     c.davidson.max_iter = cfg->max_iter > 0 ? cfg->max_iter : 1000;
     c.davidson.precondition = cfg->precondition != 0;
     c.davidson.switch_rel = cfg->switch_rel > 0 ? cfg->switch_rel : 1e-6;
-    c.davidson.low_tail = cfg->low_tail == 0 ? 100.0 : cfg->low_tail;
+    c.davidson.low_tail = cfg->low_tail == 0 ? 1000.0 : cfg->low_tail;
     if (cfg->eigensolver != TK_EIG_DAVIDSON && cfg->eigensolver != TK_EIG_LANCZOS)
       throw tk::ArgError("unknown eigensolver");
     if (cfg->max_krylov > 512) throw tk::ArgError("max_krylov above 512");

@@ -25,8 +25,8 @@
 //   the start vector; only H_tt uses the rounded product.  The coefficient of
 //   t in the Ritz vector is then of the order of the residual over the gap,
 //   so the rounding moves the residual by about that coefficient times
-//   eps |A|: below tol for low_tail = 100 in FP32 (eps about 4e-7) up to
-//   |A| / gap of about 1e4.  A residual that stops falling (three iterations
+//   eps |A|: below tol for low_tail = 1000 in FP32 (eps about 4e-7) up to
+//   |A| / gap of about 2500, in split-BF16 (eps about 6e-6) up to about 150.  A residual that stops falling (three iterations
 //   without a 0.7 reduction) restarts from the Ritz vector in FP64.
 // The residual test is on the Ritz residual of the basis held.
 #pragma once
@@ -48,7 +48,7 @@ struct DavidsonOpts {
   // already that close is refined in FP64 without a basis restart).
   double low_start_rel = 1e-4;
   // low-precision corrections once ||r|| < low_tail * tol (0 or less: off)
-  double low_tail = 100;
+  double low_tail = 1000;
 };
 
 struct DavidsonResult {

@@ -1713,7 +1713,7 @@ class Dmrg:
 
     def __init__(self, ctx, mpo_sites, sym, prec="fp64", schedule=(64,), svd_cutoff=1e-10, davidson_tol=1e-10,
                  max_subspace=60, max_iter=1000, precondition=True, switch_rel=1e-6, degeneracy_tol=1e-12,
-                 min_bond_dim=1, eigensolver="davidson", max_krylov=100, low_tail=100.0):
+                 min_bond_dim=1, eigensolver="davidson", max_krylov=100, low_tail=1000.0):
         self.ctx = ctx
         self.sym = sym
         self.n = len(mpo_sites)
