@@ -648,6 +648,7 @@ KernelNode make_node(const void* f, dim3 grid, const GItem* items, int n, const 
   k.set(6, B);
   k.set(7, C);
   k.set(8, L.beta);
+  k.a_arg = 5;
   k.b_arg = 6;
   k.c_arg = 7;
   k.bind();
@@ -684,6 +685,7 @@ std::vector<KernelNode> nodes_typed(Context* ctx, const DeviceContractList& L, c
     k.set(9, B);
     k.set(10, C);
     k.set(11, L.beta);
+    k.a_arg = 8;
     k.b_arg = 9;
     k.c_arg = 10;
     k.bind();

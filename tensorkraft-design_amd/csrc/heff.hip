@@ -319,10 +319,31 @@ void HeffPlan::ensure_low() {
   low_ready_ = true;
 }
 
+void HeffPlan::patch_graph(const std::vector<int>& args, const void* to) {
+  for (size_t i = 0; i < gnodes_.size(); ++i) {
+    KernelNode& k = gnodes_[i];
+    if (args[i] < 0 || k.slot[args[i]] == reinterpret_cast<uint64_t>(to)) continue;
+    k.set(args[i], to);
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(k.func);
+    p.gridDim = k.grid;
+    p.blockDim = k.block;
+    p.sharedMemBytes = k.shmem;
+    p.kernelParams = k.args;
+    p.extra = nullptr;
+    TK_HIP(hipGraphExecKernelNodeSetParams(graph_exec_, ghandles_[i], &p));
+  }
+}
+
 void HeffPlan::rebind(const double* L, const double* R) {
   L_ = L;
   R_ = R;
-  drop_graph();  // its nodes point at the old environments
+  // a low-precision graph reads the FP32 copies, whose buffers stay; an FP64
+  // graph's L and R slots are re-pointed
+  if (graph_exec_ && prec_ == Prec::FP64) {
+    patch_graph(gl_arg_, L_);
+    patch_graph(gr_arg_, R_);
+  }
   if (low_ready_) {
     convert_f64_to_f32(ctx_, L_, Lf_.get(), l_total_);
     convert_f64_to_f32(ctx_, R_, Rf_.get(), r_total_);
@@ -404,6 +425,8 @@ void HeffPlan::drop_graph() {
   ghandles_.clear();
   gx_arg_.clear();
   gy_arg_.clear();
+  gl_arg_.clear();
+  gr_arg_.clear();
 }
 
 void HeffPlan::build_graph(const double* x, double* y) {
@@ -417,6 +440,8 @@ void HeffPlan::build_graph(const double* x, double* y) {
     gnodes_.push_back(convert_f64_to_f32_node(ctx_, x, xf_.get(), psi_.total));
     gx_arg_.push_back(0);
     gy_arg_.push_back(-1);
+    gl_arg_.push_back(-1);
+    gr_arg_.push_back(-1);
     s1 = contract_nodes(ctx_, d1_, Lf_.get(), xf_.get(), t1f_.get());
     s3 = contract_nodes(ctx_, d3_, t1f_.get(), Rf_.get(), y);
   }
@@ -424,11 +449,15 @@ void HeffPlan::build_graph(const double* x, double* y) {
     gnodes_.push_back(k);
     gx_arg_.push_back(prec_ == Prec::FP64 ? k.b_arg : -1);  // stage 1's B is x
     gy_arg_.push_back(-1);
+    gl_arg_.push_back(prec_ == Prec::FP64 ? k.a_arg : -1);  // and its A is L
+    gr_arg_.push_back(-1);
   }
   for (const KernelNode& k : s3) {
     gnodes_.push_back(k);
     gx_arg_.push_back(-1);
     gy_arg_.push_back(k.c_arg);  // stage 3's C is y
+    gl_arg_.push_back(-1);
+    gr_arg_.push_back(prec_ == Prec::FP64 ? k.b_arg : -1);  // and its B is R
   }
   for (KernelNode& k : gnodes_) k.bind();  // slots moved with the vector
   TK_HIP(hipGraphCreate(&graph_, 0));

@@ -128,6 +128,11 @@ class HeffPlan {
   std::vector<KernelNode> gnodes_;
   std::vector<hipGraphNode_t> ghandles_;
   std::vector<int> gx_arg_, gy_arg_;
+  // FP64 graphs: the slots holding L (stage 1's A) and R (stage 3's B), -1
+  // elsewhere; a rebind re-points them instead of rebuilding the graph
+  std::vector<int> gl_arg_, gr_arg_;
+  // set the slot args[i] of every graph node i (-1: none) to `to`
+  void patch_graph(const std::vector<int>& args, const void* to);
   void build_graph(const double* x, double* y);
   void drop_graph();
   void ensure_low();

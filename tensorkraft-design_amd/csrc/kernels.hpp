@@ -57,14 +57,14 @@ class ListPack {
 
 // One kernel launch held as a graph node: the function, its shape and its
 // argument values, each in an 8-byte slot that args[i] points to (an int
-// occupies the slot's low bytes).  b_arg and c_arg name the slots of a
-// contraction's B and C operands (-1 for other kernels).
+// occupies the slot's low bytes).  a_arg, b_arg and c_arg name the slots of
+// a contraction's A, B and C operands (-1 for other kernels).
 struct KernelNode {
   const void* func = nullptr;
   dim3 grid, block;
   unsigned shmem = 0;
   int nargs = 0;
-  int b_arg = -1, c_arg = -1;
+  int a_arg = -1, b_arg = -1, c_arg = -1;
   uint64_t slot[12] = {};
   void* args[12] = {};
   template <typename T>
@@ -79,8 +79,8 @@ struct KernelNode {
 };
 
 void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C);
-// The launches launch_contract would make, as graph nodes (their b_arg and
-// c_arg name the B and C slots).
+// The launches launch_contract would make, as graph nodes (their a_arg,
+// b_arg and c_arg name the A, B and C slots).
 std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const double* A, const double* B,
                                        double* C);
 std::vector<KernelNode> contract_nodes(Context* ctx, const DeviceContractList& L, const float* A, const float* B,
