@@ -4,8 +4,9 @@
 Workloads
   sweep  (default) two-site DMRG on the Hubbard chain L=64, t=1, U=4, half
          filling, Sz=0, U(1)xU(1): bond dimension 256 -> 2048 over 8 sweeps,
-         run in FP64 and in mixed precision from the same seeded MPS; a step
-         is one full sweep at D=2048.  The ramp sweeps (D < 2048) are never
+         run in FP64 and in mixed precision (by default split-BF16 products
+         with FP64 Krylov vectors, SVD and refinement; --mixed fp32 for FP32
+         MFMA) from the same seeded MPS; a step is one full sweep at D=2048.  The ramp sweeps (D < 2048) are never
          timed; --warmup untimed sweeps at D=2048 follow them, then --steps
          timed ones; value is the median timed mixed sweep.
   heff   one H_eff*psi on the U(1)xU(1) Hubbard microbenchmark bond
@@ -775,7 +776,7 @@ def main():
     ap.add_argument("--workload", default="sweep",
                     choices=["sweep", "sweep-c4", "heff", "heff-d2048", "heff-c3", "heff-c4", "copy", "tdvp", "sweep1", "impurity", "permute"])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32", "bf16x3"])
-    ap.add_argument("--mixed", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--mixed", default="bf16x3", choices=["fp32", "bf16x3"])
     ap.add_argument("--sites", type=int, default=64)
     ap.add_argument("--d0", type=int, default=256)
     ap.add_argument("--dmax", type=int, default=None)

@@ -347,8 +347,8 @@ tk_status tk_heff_psi_table(tk_heff_plan* plan, int64_t capacity, int64_t* n_blo
 tk_status tk_heff_apply(tk_heff_plan* plan, const double* x_dev, double* y_dev);
 /* The same product from a HIP graph of the apply's kernel launches (built at
  * the first call; later calls re-point its x and y and replay it).  The
- * Davidson and Lanczos solvers of the sweep multiply through this path.
- * Split-BF16 plans run tk_heff_apply. */
+ * Davidson and Lanczos solvers of the sweep multiply through this path, in
+ * every precision. */
 tk_status tk_heff_apply_graph(tk_heff_plan* plan, const double* x_dev, double* y_dev);
 /* The tiling stage 1 or 3 of the plan runs with (after its first apply when
  * the context tunes): wg_kind and tile_min as in tk_tune_entry. */

@@ -18,6 +18,7 @@
 #include "contract.hpp"
 #include "device.hpp"
 #include "kernels.hpp"
+#include "tile_util.hpp"
 
 namespace tk {
 
@@ -118,37 +119,6 @@ template <> struct Mfma<float> {
   // accumulator element r of lane group g holds row 4g + r
   static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
-
-// Item position of workgroup b among nb persistent workgroups.  The
-// dispatcher deals workgroups to the eight XCDs in turn (b, b + 8, ... share
-// one XCD and its L2), and neighbouring items of a list usually belong to
-// one output and share its operand rows and columns; so each XCD takes runs
-// of kXcdRun consecutive items, the runs dealt round-robin to the XCDs (the
-// list's longest-first order is kept at run granularity).  Workgroups past
-// the last whole round of runs keep their own index.  A bijection on
-// [0, nb); results do not depend on it.
-constexpr int kXcdRun = 16;
-__device__ __forceinline__ int xcd_pos(int b, int nb) {
-  const int whole = nb / (8 * kXcdRun) * (8 * kXcdRun);
-  if (b >= whole) return b;
-  const int x = b & 7, j = b >> 3;
-  return (j / kXcdRun) * (8 * kXcdRun) + x * kXcdRun + j % kXcdRun;
-}
-
-// The U consecutive elements p[0, U) of one lane (an operand whose inner
-// index is contiguous in memory): one or two 16-byte loads when the whole run
-// lies inside the block (live, base + U <= n), else element by element with
-// zeros past n.  Operand rows are only element-aligned; global loads on this
-// target do not need 16-byte alignment.
-template <typename T, int U>
-__device__ __forceinline__ void run_load(T (&v)[U], const T* __restrict__ p, bool live, int base, int n) {
-  if (live && base + U <= n) {
-    __builtin_memcpy(v, p, sizeof(v));
-  } else {
-#pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = (live && base + u < n) ? p[u] : T(0);
-  }
-}
 
 // Inner-dimension chunks of 4*U: lane group lk supplies k = k0 + U*lk + u at
 // MFMA step u (a permutation of the k order inside the chunk, the same for A
@@ -559,16 +529,10 @@ template <typename T> struct KUnroll;
 template <> struct KUnroll<double> { static constexpr int value = 4; };
 template <> struct KUnroll<float> { static constexpr int value = 8; };
 
-// Whether a wide list runs its two item lists as one launch (TK_CONTRACT_FUSE,
-// read per launch or graph build; default on, 0 turns it off).
-bool fuse_wide() {
-  const char* e = std::getenv("TK_CONTRACT_FUSE");
-  return !(e && e[0] == '0');
-}
 
 template <typename TI, typename TO>
 void launch_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
-  if (L.n_witems > 0 && L.wg_kind == kWgWide && L.n_narrow > 0 && fuse_wide()) {
+  if (L.n_witems > 0 && L.wg_kind == kWgWide && L.n_narrow > 0 && contract_fuse_wide()) {
     const int wblocks = std::min(L.n_witems, ctx->n_cu * 4);
     const int nblocks = std::min((L.n_narrow + 3) / 4, ctx->n_cu * 8);
 #define TK_CASE(F)                                                                                     \
@@ -660,7 +624,7 @@ template <typename TI, typename TO>
 std::vector<KernelNode> nodes_typed(Context* ctx, const DeviceContractList& L, const TI* A, const TI* B, TO* C) {
   std::vector<KernelNode> out;
   const int F = L.flags & 7;
-  if (L.n_witems > 0 && L.wg_kind == kWgWide && L.n_narrow > 0 && fuse_wide()) {
+  if (L.n_witems > 0 && L.wg_kind == kWgWide && L.n_narrow > 0 && contract_fuse_wide()) {
     const int wblocks = std::min(L.n_witems, ctx->n_cu * 4);
     const int nblocks = std::min((L.n_narrow + 3) / 4, ctx->n_cu * 8);
     const void* f = nullptr;
@@ -736,6 +700,11 @@ struct Packing {
 }  // namespace
 
 size_t DeviceContractList::packed_bytes(const ContractList& h) { return Packing(h).total; }
+
+bool contract_fuse_wide() {
+  const char* e = std::getenv("TK_CONTRACT_FUSE");
+  return !(e && e[0] == '0');
+}
 
 void DeviceContractList::pack(const ContractList& h, char* dst, char* base) {
   const Packing pk(h);

@@ -5,14 +5,17 @@
 // (hi*hi + hi*lo + lo*hi) with FP32 accumulation on gfx950's
 // v_mfma_f32_16x16x32_bf16.  The dropped lo*lo term bounds the relative error
 // of each product near 2^-16; the Krylov solver wins FP64 accuracy back by
-// FP64 residual refinement.  Work-list format and tiling are those of
-// contract.hip (one wavefront per 32x32 output tile, or four per 32x128
-// workgroup item for the wide outputs).
+// FP64 residual refinement.  Work-list format, tiling, the 16-byte operand
+// runs, the XCD item order and the one launch of a wide list's two item
+// lists are those of contract.hip (one wavefront per 32x32 output tile, or
+// four per 32x128 workgroup item for the wide outputs).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "kernels.hpp"
+#include "tile_util.hpp"
 
 namespace tk {
 namespace {
@@ -37,19 +40,14 @@ __device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& 
   }
 }
 
+// Items wave0, wave0 + nwaves, ... of the list, one per wavefront pass.
 template <typename TO, bool TA, bool TB, bool TC>
-__global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __restrict__ items, int n_items,
-                                                              const GOut* __restrict__ outs,
-                                                              const GTerm* __restrict__ terms,
-                                                              const GSrc* __restrict__ srcs,
-                                                              const float* __restrict__ A,
-                                                              const float* __restrict__ B, TO* __restrict__ C,
-                                                              double beta) {
+__device__ __forceinline__ void bf16_wave_items(const GItem* __restrict__ items, int n_items, int wave0, int nwaves,
+                                                const GOut* __restrict__ outs, const GTerm* __restrict__ terms,
+                                                const GSrc* __restrict__ srcs, const float* __restrict__ A,
+                                                const float* __restrict__ B, TO* __restrict__ C, double beta) {
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
-  const int nwaves = gridDim.x * (blockDim.x >> 6);
-  // wave-uniform item index: every descriptor is a scalar load
-  const int wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   for (int it = wave0; it < n_items; it += nwaves) {
     const GItem item = items[it];
     const GOut o = outs[item.out];
@@ -62,28 +60,41 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __res
       const GTerm tm = terms[t];
       const float* __restrict__ Bp = B + tm.b_off;
       for (int k0 = 0; k0 < tm.k; k0 += 32) {
+        const int kb = k0 + 8 * lk;
         float a0[8], a1[8], b0[8], b1[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) a0[i] = a1[i] = b0[i] = b1[i] = 0.f;
+        for (int i = 0; i < 8; ++i) a0[i] = a1[i] = 0.f;
         for (int s = tm.s_begin; s < tm.s_end; ++s) {
           const GSrc sr = srcs[s];
           const float* __restrict__ Ap = A + sr.a_off;
           const float cf = float(sr.coef);
+          float x0[8], x1[8];
+          if (!TA) {
+            run_load<float, 8>(x0, Ap + int64_t(r0) * sr.lda + kb, r0 < o.m, kb, tm.k);
+            run_load<float, 8>(x1, Ap + int64_t(r1) * sr.lda + kb, r1 < o.m, kb, tm.k);
+          } else {
 #pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            const int kk = k0 + 8 * lk + i;
-            if (kk < tm.k) {
-              if (r0 < o.m) a0[i] += cf * (TA ? Ap[int64_t(kk) * sr.lda + r0] : Ap[int64_t(r0) * sr.lda + kk]);
-              if (r1 < o.m) a1[i] += cf * (TA ? Ap[int64_t(kk) * sr.lda + r1] : Ap[int64_t(r1) * sr.lda + kk]);
+            for (int i = 0; i < 8; ++i) {
+              const int kk = kb + i;
+              x0[i] = (kk < tm.k && r0 < o.m) ? Ap[int64_t(kk) * sr.lda + r0] : 0.f;
+              x1[i] = (kk < tm.k && r1 < o.m) ? Ap[int64_t(kk) * sr.lda + r1] : 0.f;
             }
           }
-        }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int kk = k0 + 8 * lk + i;
-          if (kk < tm.k) {
-            if (q0 < o.n) b0[i] = TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0];
-            if (q1 < o.n) b1[i] = TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1];
+          for (int i = 0; i < 8; ++i) {
+            a0[i] += cf * x0[i];
+            a1[i] += cf * x1[i];
+          }
+        }
+        if (TB) {
+          run_load<float, 8>(b0, Bp + int64_t(q0) * tm.ldb + kb, q0 < o.n, kb, tm.k);
+          run_load<float, 8>(b1, Bp + int64_t(q1) * tm.ldb + kb, q1 < o.n, kb, tm.k);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int kk = kb + i;
+            b0[i] = (kk < tm.k && q0 < o.n) ? Bp[int64_t(kk) * tm.ldb + q0] : 0.f;
+            b1[i] = (kk < tm.k && q1 < o.n) ? Bp[int64_t(kk) * tm.ldb + q1] : 0.f;
           }
         }
         bf16x8 a0h, a0l, a1h, a1l, b0h, b0l, b1h, b1l;
@@ -120,24 +131,36 @@ __global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __res
   }
 }
 
+template <typename TO, bool TA, bool TB, bool TC>
+__global__ __launch_bounds__(256) void contract_bf16x3_kernel(const GItem* __restrict__ items, int n_items,
+                                                              const GOut* __restrict__ outs,
+                                                              const GTerm* __restrict__ terms,
+                                                              const GSrc* __restrict__ srcs,
+                                                              const float* __restrict__ A,
+                                                              const float* __restrict__ B, TO* __restrict__ C,
+                                                              double beta) {
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  // wave-uniform item index: every descriptor is a scalar load
+  const int wave0 = __builtin_amdgcn_readfirstlane(xcd_pos(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                                                   (threadIdx.x >> 6));
+  bf16_wave_items<TO, TA, TB, TC>(items, n_items, wave0, nwaves, outs, terms, srcs, A, B, C, beta);
+}
+
 // Workgroup items (32 x 128, one wavefront per 32 columns): the
 // source-summed A chunk (32 rows x 32 inner indices) is formed once per
 // workgroup in LDS (double-buffered, one barrier per chunk), as in the FP64 /
 // FP32 workgroup kernel; each wavefront splits it and its own B columns.
 template <typename TO, bool TA, bool TB, bool TC>
-__global__ __launch_bounds__(256) void contract_bf16x3_wide_kernel(const GItem* __restrict__ items, int n_items,
-                                                                   const GOut* __restrict__ outs,
-                                                                   const GTerm* __restrict__ terms,
-                                                                   const GSrc* __restrict__ srcs,
-                                                                   const float* __restrict__ A,
-                                                                   const float* __restrict__ B, TO* __restrict__ C,
-                                                                   double beta) {
+__device__ __forceinline__ void bf16_wide_items(const GItem* __restrict__ items, int n_items, int blk0, int nblk,
+                                                const GOut* __restrict__ outs, const GTerm* __restrict__ terms,
+                                                const GSrc* __restrict__ srcs, const float* __restrict__ A,
+                                                const float* __restrict__ B, TO* __restrict__ C, double beta) {
   constexpr int KC = 32, PER = 32 * KC / 256;
   __shared__ float As[2][32][KC + 1];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   int buf = 0;
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+  for (int it = blk0; it < n_items; it += nblk) {
     const GItem item = items[it];
     const GOut o = outs[item.out];
     const int nw0 = item.n0 + 32 * wv;
@@ -151,12 +174,18 @@ __global__ __launch_bounds__(256) void contract_bf16x3_wide_kernel(const GItem* 
       const float* __restrict__ Bp = B + tm.b_off;
       for (int k0 = 0; k0 < tm.k; k0 += KC) {
         float b0[8], b1[8];
+        const int kb = k0 + 8 * lk;
+        if (TB) {
+          run_load<float, 8>(b0, Bp + int64_t(q0) * tm.ldb + kb, live && q0 < o.n, kb, tm.k);
+          run_load<float, 8>(b1, Bp + int64_t(q1) * tm.ldb + kb, live && q1 < o.n, kb, tm.k);
+        } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int kk = k0 + 8 * lk + i;
-          const bool kin = live && kk < tm.k;
-          b0[i] = (kin && q0 < o.n) ? (TB ? Bp[int64_t(q0) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q0]) : 0.f;
-          b1[i] = (kin && q1 < o.n) ? (TB ? Bp[int64_t(q1) * tm.ldb + kk] : Bp[int64_t(kk) * tm.ldb + q1]) : 0.f;
+          for (int i = 0; i < 8; ++i) {
+            const int kk = kb + i;
+            const bool kin = live && kk < tm.k;
+            b0[i] = (kin && q0 < o.n) ? Bp[int64_t(kk) * tm.ldb + q0] : 0.f;
+            b1[i] = (kin && q1 < o.n) ? Bp[int64_t(kk) * tm.ldb + q1] : 0.f;
+          }
         }
         float av[PER];
 #pragma unroll
@@ -229,11 +258,60 @@ __global__ __launch_bounds__(256) void contract_bf16x3_wide_kernel(const GItem* 
   }
 }
 
+template <typename TO, bool TA, bool TB, bool TC>
+__global__ __launch_bounds__(256) void contract_bf16x3_wide_kernel(const GItem* __restrict__ items, int n_items,
+                                                                   const GOut* __restrict__ outs,
+                                                                   const GTerm* __restrict__ terms,
+                                                                   const GSrc* __restrict__ srcs,
+                                                                   const float* __restrict__ A,
+                                                                   const float* __restrict__ B, TO* __restrict__ C,
+                                                                   double beta) {
+  bf16_wide_items<TO, TA, TB, TC>(items, n_items, xcd_pos(blockIdx.x, gridDim.x), gridDim.x, outs, terms, srcs, A, B,
+                                  C, beta);
+}
+
+// Both item lists of a wide list in one launch, as contract_fused_kernel.
+template <typename TO, bool TA, bool TB, bool TC>
+__global__ __launch_bounds__(256) void contract_bf16x3_fused_kernel(const GItem* __restrict__ witems, int n_witems,
+                                                                    int wblocks, const GItem* __restrict__ items,
+                                                                    int n_items, const GOut* __restrict__ outs,
+                                                                    const GTerm* __restrict__ terms,
+                                                                    const GSrc* __restrict__ srcs,
+                                                                    const float* __restrict__ A,
+                                                                    const float* __restrict__ B, TO* __restrict__ C,
+                                                                    double beta) {
+  if (int(blockIdx.x) < wblocks) {
+    bf16_wide_items<TO, TA, TB, TC>(witems, n_witems, xcd_pos(blockIdx.x, wblocks), wblocks, outs, terms, srcs, A, B,
+                                    C, beta);
+  } else {
+    const int nb = int(gridDim.x) - wblocks;
+    const int wave0 =
+        __builtin_amdgcn_readfirstlane(xcd_pos(int(blockIdx.x) - wblocks, nb) * 4 + int(threadIdx.x >> 6));
+    bf16_wave_items<TO, TA, TB, TC>(items, n_items, wave0, nb * 4, outs, terms, srcs, A, B, C, beta);
+  }
+}
+
 template <typename TO>
 void launch_bf16(Context* ctx, const DeviceContractList& L, const float* A, const float* B, TO* C) {
   // 32 x 128 workgroup items have a split-BF16 form; 64 x 64 items run as
   // their 32 x 32 expansion on the wavefront kernel
   const bool wide = L.n_witems > 0 && L.wg_kind == kWgWide;
+  if (wide && L.n_narrow > 0 && contract_fuse_wide()) {
+    const int wblocks = std::min(L.n_witems, ctx->n_cu * 4);
+    const int nblocks = std::min((L.n_narrow + 3) / 4, ctx->n_cu * 8);
+#define TK_CASE(F)                                                                                         \
+  case F:                                                                                                  \
+    hipLaunchKernelGGL((contract_bf16x3_fused_kernel<TO, bool(F & kTransA), bool(F & kTransB), bool(F & kTransC)>), \
+                       dim3(wblocks + nblocks), dim3(256), 0, ctx->stream, L.witems, L.n_witems, wblocks, L.items, \
+                       L.n_narrow, L.outs, L.terms, L.srcs, A, B, C, L.beta);                           \
+    break;
+    switch (L.flags & 7) {
+      TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7)
+    }
+#undef TK_CASE
+    TK_HIP(hipGetLastError());
+    return;
+  }
   const int n_wave = wide ? L.n_narrow : L.n_items;
   if (n_wave > 0) {
     int blocks = std::min((n_wave + 3) / 4, ctx->n_cu * 8);
@@ -264,7 +342,98 @@ void launch_bf16(Context* ctx, const DeviceContractList& L, const float* A, cons
   TK_HIP(hipGetLastError());
 }
 
+template <typename TO>
+KernelNode bf16_node(const void* f, dim3 grid, const GItem* items, int n, const DeviceContractList& L, const float* A,
+                     const float* B, TO* C) {
+  KernelNode k;
+  k.func = f;
+  k.grid = grid;
+  k.block = dim3(256);
+  k.nargs = 9;
+  k.set(0, items);
+  k.set(1, n);
+  k.set(2, L.outs);
+  k.set(3, L.terms);
+  k.set(4, L.srcs);
+  k.set(5, A);
+  k.set(6, B);
+  k.set(7, C);
+  k.set(8, L.beta);
+  k.a_arg = 5;
+  k.b_arg = 6;
+  k.c_arg = 7;
+  k.bind();
+  return k;
+}
+
+// The launches of launch_bf16, as graph nodes.
+template <typename TO>
+std::vector<KernelNode> bf16_nodes(Context* ctx, const DeviceContractList& L, const float* A, const float* B, TO* C) {
+  std::vector<KernelNode> out;
+  const int F = L.flags & 7;
+  const bool wide = L.n_witems > 0 && L.wg_kind == kWgWide;
+  if (wide && L.n_narrow > 0 && contract_fuse_wide()) {
+    const int wblocks = std::min(L.n_witems, ctx->n_cu * 4);
+    const int nblocks = std::min((L.n_narrow + 3) / 4, ctx->n_cu * 8);
+    const void* f = nullptr;
+#define TK_CASE(G) \
+  case G: f = reinterpret_cast<const void*>(&contract_bf16x3_fused_kernel<TO, bool(G & kTransA), bool(G & kTransB), bool(G & kTransC)>); break;
+    switch (F) { TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7) }
+#undef TK_CASE
+    KernelNode k;
+    k.func = f;
+    k.grid = dim3(wblocks + nblocks);
+    k.block = dim3(256);
+    k.nargs = 12;
+    k.set(0, L.witems);
+    k.set(1, L.n_witems);
+    k.set(2, wblocks);
+    k.set(3, L.items);
+    k.set(4, L.n_narrow);
+    k.set(5, L.outs);
+    k.set(6, L.terms);
+    k.set(7, L.srcs);
+    k.set(8, A);
+    k.set(9, B);
+    k.set(10, C);
+    k.set(11, L.beta);
+    k.a_arg = 8;
+    k.b_arg = 9;
+    k.c_arg = 10;
+    k.bind();
+    out.push_back(k);
+    return out;
+  }
+  const int n_wave = wide ? L.n_narrow : L.n_items;
+  if (n_wave > 0) {
+    const void* f = nullptr;
+#define TK_CASE(G) \
+  case G: f = reinterpret_cast<const void*>(&contract_bf16x3_kernel<TO, bool(G & kTransA), bool(G & kTransB), bool(G & kTransC)>); break;
+    switch (F) { TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7) }
+#undef TK_CASE
+    out.push_back(bf16_node(f, dim3(std::min((n_wave + 3) / 4, ctx->n_cu * 8)), L.items, n_wave, L, A, B, C));
+  }
+  if (wide) {
+    const void* f = nullptr;
+#define TK_CASE(G) \
+  case G: f = reinterpret_cast<const void*>(&contract_bf16x3_wide_kernel<TO, bool(G & kTransA), bool(G & kTransB), bool(G & kTransC)>); break;
+    switch (F) { TK_CASE(0) TK_CASE(1) TK_CASE(2) TK_CASE(3) TK_CASE(4) TK_CASE(5) TK_CASE(6) TK_CASE(7) }
+#undef TK_CASE
+    out.push_back(bf16_node(f, dim3(std::min(L.n_witems, ctx->n_cu * 4)), L.witems, L.n_witems, L, A, B, C));
+  }
+  return out;
+}
+
 }  // namespace
+
+std::vector<KernelNode> contract_bf16x3_nodes(Context* ctx, const DeviceContractList& L, const float* A,
+                                              const float* B, float* C) {
+  return bf16_nodes<float>(ctx, L, A, B, C);
+}
+std::vector<KernelNode> contract_bf16x3_nodes(Context* ctx, const DeviceContractList& L, const float* A,
+                                              const float* B, double* C) {
+  return bf16_nodes<double>(ctx, L, A, B, C);
+}
 
 void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C) {
   launch_bf16<float>(ctx, L, A, B, C);

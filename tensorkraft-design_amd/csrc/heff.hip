@@ -442,8 +442,13 @@ void HeffPlan::build_graph(const double* x, double* y) {
     gy_arg_.push_back(-1);
     gl_arg_.push_back(-1);
     gr_arg_.push_back(-1);
-    s1 = contract_nodes(ctx_, d1_, Lf_.get(), xf_.get(), t1f_.get());
-    s3 = contract_nodes(ctx_, d3_, t1f_.get(), Rf_.get(), y);
+    if (prec_ == Prec::BF16X3) {
+      s1 = contract_bf16x3_nodes(ctx_, d1_, Lf_.get(), xf_.get(), t1f_.get());
+      s3 = contract_bf16x3_nodes(ctx_, d3_, t1f_.get(), Rf_.get(), y);
+    } else {
+      s1 = contract_nodes(ctx_, d1_, Lf_.get(), xf_.get(), t1f_.get());
+      s3 = contract_nodes(ctx_, d3_, t1f_.get(), Rf_.get(), y);
+    }
   }
   for (const KernelNode& k : s1) {
     gnodes_.push_back(k);
@@ -479,10 +484,6 @@ void HeffPlan::build_graph(const double* x, double* y) {
 }
 
 void HeffPlan::apply_graph(const double* x, double* y) {
-  if (prec_ == Prec::BF16X3) {
-    apply(x, y);
-    return;
-  }
   if (tune_pending_) tune(x, y);
   if (!graph_exec_) build_graph(x, y);
   for (size_t i = 0; i < gnodes_.size(); ++i) {
@@ -509,7 +510,10 @@ void HeffPlan::apply_graph(const double* x, double* y) {
   }
   const int tk = ctx_->timer.begin(ctx_->stream);
   TK_HIP(hipGraphLaunch(graph_exec_, ctx_->stream));
-  ctx_->timer.end(tk, ctx_->stream, prec_ == Prec::FP64 ? "heff_apply_graph_fp64" : "heff_apply_graph_fp32",
+  ctx_->timer.end(tk, ctx_->stream,
+                  prec_ == Prec::FP64 ? "heff_apply_graph_fp64"
+                  : prec_ == Prec::FP32 ? "heff_apply_graph_fp32"
+                                        : "heff_apply_graph_bf16x3",
                   stats_.s1_bytes + stats_.s3_bytes, stats_.s1_flops + stats_.s3_flops, 1);
 }
 

@@ -78,6 +78,9 @@ struct KernelNode {
   }
 };
 
+// Whether a wide list runs its workgroup and wavefront items as one launch
+// (TK_CONTRACT_FUSE, read per launch or graph build; default on, 0: two).
+bool contract_fuse_wide();
 void launch_contract(Context* ctx, const DeviceContractList& L, const double* A, const double* B, double* C);
 // The launches launch_contract would make, as graph nodes (their a_arg,
 // b_arg and c_arg name the A, B and C slots).
@@ -93,6 +96,12 @@ void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, 
 void launch_contract(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C);
 // split-BF16 (hi+lo, three products, FP32 accumulation) on FP32 operands
 void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, float* C);
+// The launches of launch_contract_bf16x3, as graph nodes (a_arg, b_arg and
+// c_arg set as for contract_nodes).
+std::vector<KernelNode> contract_bf16x3_nodes(Context* ctx, const DeviceContractList& L, const float* A,
+                                              const float* B, float* C);
+std::vector<KernelNode> contract_bf16x3_nodes(Context* ctx, const DeviceContractList& L, const float* A,
+                                              const float* B, double* C);
 void launch_contract_bf16x3(Context* ctx, const DeviceContractList& L, const float* A, const float* B, double* C);
 
 // 2-D strided copies (optionally transposed) in one launch (dmrg.hip).
