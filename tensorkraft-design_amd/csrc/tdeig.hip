@@ -1822,48 +1822,47 @@ struct WyBlock {
   int32_t nb, pad;
 };
 
-// T column by column in LDS: T[0:i, i] = -tau_i T[0:i, 0:i] S[0:i, i], one
-// thread per row l, the column S[:, i] staged in LDS.
+// T column by column in LDS: T[0:i, i] = -tau_i T[0:i, 0:i] S[0:i, i].  Four
+// lanes per row l (lane q sums the terms m = l + q, l + q + 4, ...; the quad
+// is closed by two butterfly steps in a fixed order), the column S[:, i]
+// staged in LDS; the next column's global loads are issued before this
+// column's sums.
+constexpr int kWyTLanes = 4;
 constexpr size_t kWyTLds = (size_t(kWyNb) * (kWyNb + 1) + kWyNb) * sizeof(double);
 
-__global__ __launch_bounds__(kWyNb) void wy_t_kernel(const WyBlock* __restrict__ blocks,
-                                                     const double* __restrict__ S, const double* __restrict__ TAUg,
-                                                     double* __restrict__ T) {
+__global__ __launch_bounds__(kWyNb * kWyTLanes) void wy_t_kernel(const WyBlock* __restrict__ blocks,
+                                                                 const double* __restrict__ S,
+                                                                 const double* __restrict__ TAUg,
+                                                                 double* __restrict__ T) {
   extern __shared__ double sm[];
-  double* tl = sm;                              // kWyNb x (kWyNb + 1)
-  double* sc = sm + kWyNb * (kWyNb + 1);        // column i of S
+  double* tl = sm;                        // kWyNb x (kWyNb + 1)
+  double* sc = sm + kWyNb * (kWyNb + 1);  // column i of S
   const WyBlock b = blocks[blockIdx.x];
   const double* s = S + b.t_off;
   const double* tau = TAUg + b.tau_off;
-  const int l = threadIdx.x, nb = b.nb;
-  for (int m = 0; m < kWyNb; ++m) tl[l * (kWyNb + 1) + m] = 0.0;
-  // column i + 1 of S is requested while step i runs (its global load is
-  // the step's latency otherwise)
+  const int l = threadIdx.x / kWyTLanes, q = threadIdx.x % kWyTLanes, nb = b.nb;
+  for (int m = q; m < kWyNb; m += kWyTLanes) tl[l * (kWyNb + 1) + m] = 0.0;
   double snext = 0.0;
   for (int i = 0; i < nb; ++i) {
-    sc[l] = snext;
-    snext = (l < i + 1 && i + 1 < nb) ? s[int64_t(l) * kWyNb + i + 1] : 0.0;
+    if (q == 0) sc[l] = snext;
+    snext = (q == 0 && l < i + 1 && i + 1 < nb) ? s[int64_t(l) * kWyNb + i + 1] : 0.0;
     __syncthreads();
+    // every lane takes part in the butterfly (uniform control flow)
+    double a = 0;
     if (l < i) {
-      // four partial sums, so the LDS reads of the row are issued ahead
       const double* tr = tl + l * (kWyNb + 1);
-      double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-      int m = l;
-      for (; m + 3 < i; m += 4) {
-        a0 += tr[m] * sc[m];
-        a1 += tr[m + 1] * sc[m + 1];
-        a2 += tr[m + 2] * sc[m + 2];
-        a3 += tr[m + 3] * sc[m + 3];
-      }
-      for (; m < i; ++m) a0 += tr[m] * sc[m];
-      tl[l * (kWyNb + 1) + i] = -tau[i] * ((a0 + a1) + (a2 + a3));
-    } else if (l == i) {
-      tl[l * (kWyNb + 1) + i] = tau[i];
+      for (int m = l + q; m < i; m += kWyTLanes) a += tr[m] * sc[m];
+    }
+    a += __shfl_xor(a, 1, 64);
+    a += __shfl_xor(a, 2, 64);
+    if (q == 0) {
+      if (l < i) tl[l * (kWyNb + 1) + i] = -tau[i] * a;
+      else if (l == i) tl[l * (kWyNb + 1) + i] = tau[i];
     }
     __syncthreads();
   }
   double* t = T + b.t_off;
-  for (int m = 0; m < kWyNb; ++m) t[int64_t(l) * kWyNb + m] = tl[l * (kWyNb + 1) + m];
+  for (int m = q; m < kWyNb; m += kWyTLanes) t[int64_t(l) * kWyNb + m] = tl[l * (kWyNb + 1) + m];
 }
 
 struct VecWork {
@@ -1988,7 +1987,8 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
   };
   run(i_s, A, A, S.get());
   allow_lds(wy_t_kernel, kWyTLds);
-  hipLaunchKernelGGL(wy_t_kernel, dim3(unsigned(hb.size())), dim3(kWyNb), kWyTLds, ctx->stream, wbd.get(), S.get(),
+  hipLaunchKernelGGL(wy_t_kernel, dim3(unsigned(hb.size())), dim3(kWyNb * kWyTLanes), kWyTLds, ctx->stream, wbd.get(),
+                     S.get(),
                      TAU, T.get());
   TK_HIP(hipGetLastError());
   run(i_ut, T.get(), A, Ut.get());
