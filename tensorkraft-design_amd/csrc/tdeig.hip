@@ -1998,6 +1998,11 @@ void wy_backtransform(Context* ctx, const std::vector<DevJob>& dj, const DevJob*
   }
 }
 
+// Rows per partial Gram block of the block orthonormalisation: G = Z_b^T Z_b
+// over n rows as ceil(n / kGramSeg) independent contractions of kGramSeg
+// rows (short inner chains), summed by cholinv_kernel when it loads G.
+constexpr int kGramSeg = 256;
+
 // Cholesky factor and inverse of the nb x nb Gram block G = Z_b^T Z_b of each
 // job (one workgroup per job, goff[job] < 0: no block): G = R^T R and
 // Rinv = R^-1 (upper triangular, nb x nb at goff).  With `test`, the block is
@@ -2015,6 +2020,7 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   double* L = sm;                 // NB x (NB + 1): G, then R in the upper triangle
   double* X = sm + NB * (NB + 1);  // NB x (NB + 1): R^-1
   __shared__ int good;
+  __shared__ double nrm[NB];
   const int64_t off = goff[blockIdx.x];
   if (off < 0) return;
   const DevJob jb = jobs[blockIdx.x];
@@ -2022,10 +2028,18 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   const int nb = min(c_count, jb.k - c0);
   const int tid = threadIdx.x;
   constexpr int LD = NB + 1;
-  // 16 x 16 thread grid over the block (no index division in the loops)
+  // 16 x 16 thread grid over the block (no index division in the loops); G
+  // arrives as one partial Gram block per segment of kGramSeg rows, summed
+  // here in segment order
   const int tx = tid & 15, ty = tid >> 4;
+  const int nseg = (jb.n + kGramSeg - 1) / kGramSeg;
   for (int i = ty; i < nb; i += 16)
-    for (int j = tx; j < nb; j += 16) L[i * LD + j] = G[off + int64_t(i) * nb + j];
+    for (int j = tx; j < nb; j += 16) {
+      double g = 0;
+      for (int sg = 0; sg < nseg; ++sg) g += G[off + (int64_t(sg) * nb + i) * nb + j];
+      L[i * LD + j] = g;
+    }
+  if (test && tid < nb) nrm[tid] = norms[jb.w_off + c0 + tid];
   if (tid == 0) good = test ? 1 : ok[blockIdx.x];
   __syncthreads();
   // Right-looking Cholesky with one barrier per column: every thread reads
@@ -2036,7 +2050,7 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   double rprev = 1.0;
   for (int a = 0; a < nb && gl; ++a) {
     const double dd = L[a * LD + a];
-    if (!(dd > 0) || (test && sqrt(dd) <= 0.1 * norms[jb.w_off + c0 + a])) {
+    if (!(dd > 0) || (test && sqrt(dd) <= 0.1 * nrm[a])) {
       gl = false;
       break;
     }
@@ -2152,7 +2166,7 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
       if (c0 >= j.k) continue;
       const int nb = std::min(kOrthBlock, j.k - c0);
       cq_offs[size_t(st) * nj + o] = gt;
-      gt += int64_t(nb) * nb;
+      gt += int64_t((j.n + kGramSeg - 1) / kGramSeg) * nb * nb;
       tt += int64_t(j.n) * nb;
     }
     cq_gt = std::max(cq_gt, gt);
@@ -2211,19 +2225,24 @@ void orth_blocked(Context* ctx, const std::vector<DevJob>& dj, const DevJob* djd
         const int c0 = j.pad + st * kOrthBlock;
         if (c0 >= j.k) continue;
         const int nb = std::min(kOrthBlock, j.k - c0);
-        lg.begin_out(gt, nb, nb, nb);
-        lg.begin_term(j.z_off + c0, j.k, j.n);
-        lg.add_src(j.z_off + c0, j.k, 1.0);
+        // partial Gram blocks per kGramSeg rows (summed by cholinv_kernel)
+        for (int r0 = 0, sg = 0; r0 < j.n; r0 += kGramSeg, ++sg) {
+          const int rows = std::min(kGramSeg, j.n - r0);
+          const int64_t gs = gt + int64_t(sg) * nb * nb;
+          lg.begin_out(gs, nb, nb, nb);
+          lg.begin_term(j.z_off + int64_t(r0) * j.k + c0, j.k, rows);
+          lg.add_src(j.z_off + int64_t(r0) * j.k + c0, j.k, 1.0);
+          lg2.begin_out(gs, nb, nb, nb);
+          lg2.begin_term(tt + int64_t(r0) * nb, nb, rows);
+          lg2.add_src(tt + int64_t(r0) * nb, nb, 1.0);
+        }
         lt.begin_out(tt, j.n, nb, nb);
         lt.begin_term(gt, nb, nb);
         lt.add_src(j.z_off + c0, j.k, 1.0);
-        lg2.begin_out(gt, nb, nb, nb);
-        lg2.begin_term(tt, nb, j.n);
-        lg2.add_src(tt, nb, 1.0);
         lz.begin_out(j.z_off + c0, j.n, nb, j.k);
         lz.begin_term(gt, nb, nb);
         lz.add_src(tt, nb, 1.0);
-        gt += int64_t(nb) * nb;
+        gt += int64_t((j.n + kGramSeg - 1) / kGramSeg) * nb * nb;
         tt += int64_t(j.n) * nb;
       }
       if (gt > 0) {
