@@ -2031,13 +2031,37 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   // 16 x 16 thread grid over the block (no index division in the loops); G
   // arrives as one partial Gram block per segment of kGramSeg rows, summed
   // here in segment order
+  // (the 16 elements of a thread are loaded together per segment: a
+  // per-element loop over the segments would wait on every load in turn)
   const int tx = tid & 15, ty = tid >> 4;
   const int nseg = (jb.n + kGramSeg - 1) / kGramSeg;
-  for (int i = ty; i < nb; i += 16)
-    for (int j = tx; j < nb; j += 16) {
-      double g = 0;
-      for (int sg = 0; sg < nseg; ++sg) g += G[off + (int64_t(sg) * nb + i) * nb + j];
-      L[i * LD + j] = g;
+  static_assert(NB <= 64, "four rows and columns per thread");
+  double gs[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) gs[a][b] = 0.0;
+  for (int sg = 0; sg < nseg; ++sg) {
+    const double* Gs = G + off + int64_t(sg) * nb * nb;
+    double v[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int i = ty + 16 * a, j = tx + 16 * b;
+        v[a][b] = (i < nb && j < nb) ? Gs[int64_t(i) * nb + j] : 0.0;
+      }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) gs[a][b] += v[a][b];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int i = ty + 16 * a, j = tx + 16 * b;
+      if (i < nb && j < nb) L[i * LD + j] = gs[a][b];
     }
   if (test && tid < nb) nrm[tid] = norms[jb.w_off + c0 + tid];
   if (tid == 0) good = test ? 1 : ok[blockIdx.x];
