@@ -2017,8 +2017,8 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
                                                       const double* __restrict__ norms, int step, int c_count,
                                                       int test, int* __restrict__ ok) {
   extern __shared__ double sm[];
-  double* L = sm;                 // NB x (NB + 1): G, then R in the upper triangle
-  double* X = sm + NB * (NB + 1);  // NB x (NB + 1): R^-1
+  double* L = sm;                 // NB x (NB + 1): G, then the rows of R unscaled
+  double* W = sm + NB * (NB + 1);  // NB x (NB + 1): R^-T with its rows unscaled
   __shared__ int good;
   __shared__ double nrm[NB];
   const int64_t off = goff[blockIdx.x];
@@ -2066,58 +2066,44 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
   if (test && tid < nb) nrm[tid] = norms[jb.w_off + c0 + tid];
   if (tid == 0) good = test ? 1 : ok[blockIdx.x];
   __syncthreads();
-  // Right-looking Cholesky with one barrier per column: every thread reads
-  // the pivot (the same value, so the same decision); step a updates the
-  // rows below a with row a unscaled (u_b u_c / d_aa) and scales row a - 1,
-  // which no thread reads any more, by the previous pivot's square root.
+  // Right-looking Cholesky G = R^T R with R^-1 formed in the same steps, one
+  // barrier per column.  Y = R^-T (lower triangular) solves R^T Y = I by
+  // rows: Y_a = (e_a - sum_{l<a} R_la Y_l) / R_aa.  W starts as I; step a
+  // reads the pivot d = R_aa^2 (every thread the same value, so the same
+  // decision) and, with u_b = L_ab / d for the rows b > a below it, updates
+  // the trailing block L_bc -= u_b L_ac (c >= b) and W_bc -= u_b W_ac (c <= a)
+  // with row a unscaled: R_ab Y_a = u_b W_a.  Row a of W is final then;
+  // Y_a = W_a / R_aa and R^-1 = Y^T, scaled by the stored pivots on the way
+  // out.  Row a is read and only rows below it written within a step.
+  __shared__ double piv[NB];
+  for (int i = ty; i < nb; i += 16)
+    for (int j = tx; j < nb; j += 16) W[i * LD + j] = i == j ? 1.0 : 0.0;
+  __syncthreads();
   bool gl = good != 0;
-  double rprev = 1.0;
   for (int a = 0; a < nb && gl; ++a) {
     const double dd = L[a * LD + a];
     if (!(dd > 0) || (test && sqrt(dd) <= 0.1 * nrm[a])) {
       gl = false;
       break;
     }
-    if (a > 0) {
-      const double ir = 1.0 / rprev;
-      for (int b = a + tid; b < nb; b += 256) L[(a - 1) * LD + b] *= ir;
-      if (tid == 0) L[(a - 1) * LD + (a - 1)] = rprev;
-    }
-    rprev = sqrt(dd);
+    if (tid == 0) piv[a] = sqrt(dd);
     const double idd = 1.0 / dd;
     for (int b = a + 1 + ty; b < nb; b += 16) {
       const double ub = L[a * LD + b] * idd;
-      for (int c = a + 1 + tx; c < nb; c += 16)
-        if (c >= b) L[b * LD + c] -= ub * L[a * LD + c];
+      for (int c = tx; c < nb; c += 16) {
+        if (c >= b)
+          L[b * LD + c] -= ub * L[a * LD + c];
+        else if (c <= a)
+          W[b * LD + c] -= ub * W[a * LD + c];
+      }
     }
     __syncthreads();
   }
-  if (gl && nb > 0 && tid == 0) L[(nb - 1) * LD + (nb - 1)] = rprev;  // the last row is its pivot alone
-  __syncthreads();
   const bool g = gl;
-  // column j of R^-1 by the four lanes 4j..4j+3 (one wavefront holds 16
-  // columns): X_jj = 1 / R_jj, X_ij = -(sum_{l=i+1..j} R_il X_lj) / R_ii, the
-  // sum split over the four lanes and closed by two butterfly steps; the
-  // loop over i is uniform so that every lane takes part in the shuffles
-  if (g) {
-    const int j = tid >> 2, q = tid & 3;
-    const bool live = j < nb;
-    if (live && q == 0) X[j * LD + j] = 1.0 / L[j * LD + j];
-    __builtin_amdgcn_wave_barrier();
-    for (int i = nb - 2; i >= 0; --i) {
-      double t = 0;
-      if (live && i < j)
-        for (int l = i + 1 + q; l <= j; l += 4) t += L[i * LD + l] * X[l * LD + j];
-      t += __shfl_xor(t, 1, 64);
-      t += __shfl_xor(t, 2, 64);
-      if (live && i < j && q == 0) X[i * LD + j] = -t / L[i * LD + i];
-      __builtin_amdgcn_wave_barrier();  // entry (i, j) is read by the column's other lanes next
-    }
-  }
   __syncthreads();
   for (int i = ty; i < nb; i += 16)
     for (int j = tx; j < nb; j += 16)
-      Rinv[off + int64_t(i) * nb + j] = g ? (i <= j ? X[i * LD + j] : 0.0) : (i == j ? 1.0 : 0.0);
+      Rinv[off + int64_t(i) * nb + j] = g ? (i <= j ? W[j * LD + i] / piv[j] : 0.0) : (i == j ? 1.0 : 0.0);
   if (tid == 0) ok[blockIdx.x] = g ? 1 : 0;
 }
 
