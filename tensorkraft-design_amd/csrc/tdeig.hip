@@ -24,7 +24,7 @@ namespace tk {
 namespace {
 
 constexpr double kEps = DBL_EPSILON;
-constexpr int kLuFields = 5;
+constexpr int kLuFields = 4;
 // lanes per eigenvalue in the multisection (TK_BIS_LANES = 4, 8 or 16 overrides, for tuning)
 int bis_lanes() {
   static const int l = [] {
@@ -39,7 +39,7 @@ struct DevJob {
   int64_t a_off;   // matrix / reflectors (and output columns in Z)
   int64_t w_off;   // eigenvalues, d, e, tau
   int64_t z_off;   // inverse-iteration vectors: n x k row-major (ld k)
-  int64_t lu_off;  // LU factors: kLuFields doubles per (row, vector), interleaved by vector
+  int64_t lu_off;  // LU factors: kLuFields doubles per (row, vector), row-major over (row, vector)
   int32_t n, k;
   int32_t job;     // index of the job in the caller's list
   int32_t pad;
@@ -964,27 +964,33 @@ __device__ int sturm_count_block(const double* d, const double* e, int a, int b,
   return cnt;
 }
 
-// One solve with the factors of T - lambda I stored by invit_kernel for
-// vector c (row i, field f at lu[(i * kLuFields + f) * k + c]): y <- U^-1 L^-1 P y
-// for the vector y[i * ys], in place, by the calling thread; returns max |y|.
+// Factors of T - lambda I for one vector, kLuFields = 4 doubles per (row i,
+// vector c) at lu[(i * k + c) * 4]: U0 (pivot), U1, U2 (superdiagonals) and
+// the multiplier.  Row i was interchanged with row i + 1 when U2 is nonzero
+// or -0.0; an unswapped row stores U2 = +0.0.
+__device__ __forceinline__ bool lu_swapped(double u2) { return u2 != 0.0 || signbit(u2); }
+
+// One solve with the factors of vector c stored by invit_kernel: y <- U^-1
+// L^-1 P y for the vector y[i * ys], in place, by the calling thread; returns
+// max |y|.
 __device__ double lu_solve_inplace(const double* __restrict__ lu, int k, int c, int n, double* y, int ys) {
-  auto F = [&](int i, int f) { return lu[(int64_t(i) * kLuFields + f) * k + c]; };
+  auto F = [&](int i) { return lu + (int64_t(i) * k + c) * kLuFields; };
   double cur = y[0];
   for (int i = 0; i + 1 < n; ++i) {
     double nxt = y[(i + 1) * ys];
-    if (F(i, 4) != 0.0) {
+    if (lu_swapped(F(i)[2])) {
       const double t = cur;
       cur = nxt;
       nxt = t;
     }
-    nxt -= F(i, 3) * cur;
+    nxt -= F(i)[3] * cur;
     y[i * ys] = cur;
     cur = nxt;
   }
   y[(n - 1) * ys] = cur;
   double z1 = 0, z2 = 0, zmax = 0;
   for (int i = n - 1; i >= 0; --i) {
-    const double zi = (y[i * ys] - F(i, 1) * z1 - F(i, 2) * z2) / F(i, 0);
+    const double zi = (y[i * ys] - F(i)[1] * z1 - F(i)[2] * z2) / F(i)[0];
     y[i * ys] = zi;
     zmax = fmax(zmax, fabs(zi));
     z2 = z1;
@@ -993,6 +999,12 @@ __device__ double lu_solve_inplace(const double* __restrict__ lu, int k, int c, 
   return zmax;
 }
 
+// The solves are serial in the rows, one thread per vector, and few vectors
+// run at once, so the loads' latency sets the time: every pass loads rows
+// in chunks of C, the factors of a row as two 16-byte loads, and there are
+// five passes over the rows: the factorisation runs the first forward solve
+// on the hashed start as it goes, and the last back substitution sums the
+// vector's 2-norm as it goes.
 __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ items, int n_items,
                                                    const DevJob* __restrict__ jobs, const double* __restrict__ Dg,
                                                    const double* __restrict__ Eg, const double* __restrict__ W,
@@ -1009,8 +1021,7 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
   const double lam = W[jb.w_off + c];
   double* lu = LU + jb.lu_off;
   double* z = Z + jb.z_off;
-  // row i of the factors: U0 (pivot), U1, U2 (superdiagonals), multiplier, swap flag
-  auto F = [&](int i, int f) -> double& { return lu[(int64_t(i) * kLuFields + f) * k + c]; };
+  auto F = [&](int i) { return reinterpret_cast<double2*>(lu + (int64_t(i) * k + c) * kLuFields); };
   auto Y = [&](int i) -> double& { return z[int64_t(i) * k + c]; };
   if (n == 1) {
     Y(0) = 1.0;
@@ -1054,7 +1065,12 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
     }
   }
   auto Ev = [&](int i) { return split && fabs(e[i]) <= tol_e ? 0.0 : e[i]; };
+  const uint32_t seed = uint32_t(c) * 2654435761u + uint32_t(jb.job);
+  auto start = [&](int i) { return i >= s0 && i < s1 ? hash_unit(seed, uint32_t(i)) : 0.0; };
+  // T - lambda I = P L U with partial pivoting, and L^-1 P applied to the
+  // hashed start on the way (cur: the running entry of the forward solve)
   double c0 = d[0] - lam, c1 = Ev(0);
+  double cur = start(0);
   for (int i0 = 0; i0 + 1 < n; i0 += C) {
     double dn[C], ei[C], en[C];
 #pragma unroll
@@ -1069,77 +1085,79 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
       const int i = i0 + u;
       if (i + 1 >= n) break;
       const double dd = dn[u] - lam;
+      const double yn = start(i + 1);
       if (fabs(c0) >= fabs(ei[u])) {
         c0 = guard(c0);
         const double m = fast_quot(ei[u], c0);
-        F(i, 0) = c0;
-        F(i, 1) = c1;
-        F(i, 2) = 0.0;
-        F(i, 3) = m;
-        F(i, 4) = 0.0;
+        F(i)[0] = make_double2(c0, c1);
+        F(i)[1] = make_double2(0.0, m);
         c0 = dd - m * c1;
         c1 = en[u];
+        Y(i) = cur;
+        cur = yn - m * cur;
       } else {
         // |c0| < |e_i|; below the normal range the hardware reciprocal of e_i
         // overflows, so that rare case takes the IEEE quotient (|m| < 1)
         const double m = fabs(ei[u]) >= DBL_MIN ? fast_quot(c0, ei[u]) : c0 / ei[u];
-        F(i, 0) = ei[u];
-        F(i, 1) = dd;
-        F(i, 2) = en[u];
-        F(i, 3) = m;
-        F(i, 4) = 1.0;
+        F(i)[0] = make_double2(ei[u], dd);
+        F(i)[1] = make_double2(en[u] == 0.0 ? -0.0 : en[u], m);
         c0 = c1 - m * dd;
         c1 = -m * en[u];
+        Y(i) = yn;
+        cur = cur - m * yn;
       }
     }
   }
-  F(n - 1, 0) = guard(c0);
-  F(n - 1, 1) = 0.0;
-  F(n - 1, 2) = 0.0;
-  for (int i = 0; i < n; ++i)
-    Y(i) = i >= s0 && i < s1 ? hash_unit(uint32_t(c) * 2654435761u + uint32_t(jb.job), uint32_t(i)) : 0.0;
-  double s = 1.0;
+  F(n - 1)[0] = make_double2(guard(c0), 0.0);
+  F(n - 1)[1] = make_double2(0.0, 0.0);
+  Y(n - 1) = cur;
+  double s = 1.0, sumsq = 0.0;
   for (int itn = 0; itn < 2; ++itn) {
-    // forward: row interchanges and multipliers applied to s * y
-    double cur = s * Y(0);
-    for (int i0 = 0; i0 + 1 < n; i0 += C) {
-      double yn[C], sw[C], ml[C];
+    if (itn > 0) {
+      // forward: row interchanges and multipliers applied to s * y
+      double cf = s * Y(0);
+      for (int i0 = 0; i0 + 1 < n; i0 += C) {
+        double yn[C], u2[C], ml[C];
 #pragma unroll
-      for (int u = 0; u < C; ++u) {
-        const int i = i0 + u;
-        const bool in = i + 1 < n;
-        yn[u] = in ? Y(i + 1) : 0.0;
-        sw[u] = in ? F(i, 4) : 0.0;
-        ml[u] = in ? F(i, 3) : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < C; ++u) {
-        const int i = i0 + u;
-        if (i + 1 >= n) break;
-        double nxt = s * yn[u];
-        if (sw[u] != 0.0) {
-          const double t = cur;
-          cur = nxt;
-          nxt = t;
+        for (int u = 0; u < C; ++u) {
+          const int i = i0 + u;
+          const bool in = i + 1 < n;
+          const double2 b = in ? F(i)[1] : make_double2(0.0, 0.0);
+          yn[u] = in ? Y(i + 1) : 0.0;
+          u2[u] = b.x;
+          ml[u] = b.y;
         }
-        nxt -= ml[u] * cur;
-        Y(i) = cur;
-        cur = nxt;
+#pragma unroll
+        for (int u = 0; u < C; ++u) {
+          const int i = i0 + u;
+          if (i + 1 >= n) break;
+          double nxt = s * yn[u];
+          if (lu_swapped(u2[u])) {
+            const double t = cf;
+            cf = nxt;
+            nxt = t;
+          }
+          nxt -= ml[u] * cf;
+          Y(i) = cf;
+          cf = nxt;
+        }
       }
+      Y(n - 1) = cf;
     }
-    Y(n - 1) = cur;
     // back substitution with the two superdiagonals
-    double z1 = 0, z2 = 0, zmax = 0;
+    double z1 = 0, z2 = 0, zmax = 0, ss = 0;
     for (int i1 = n - 1; i1 >= 0; i1 -= C) {
       double yv[C], u0[C], u1[C], u2[C];
 #pragma unroll
       for (int u = 0; u < C; ++u) {
         const int i = i1 - u;
         const bool in = i >= 0;
+        const double2 a = in ? F(i)[0] : make_double2(1.0, 0.0);
+        const double2 b = in ? F(i)[1] : make_double2(0.0, 0.0);
         yv[u] = in ? Y(i) : 0.0;
-        u0[u] = in ? F(i, 0) : 1.0;
-        u1[u] = in ? F(i, 1) : 0.0;
-        u2[u] = in ? F(i, 2) : 0.0;
+        u0[u] = a.x;
+        u1[u] = a.y;
+        u2[u] = b.x;
       }
 #pragma unroll
       for (int u = 0; u < C; ++u) {
@@ -1148,27 +1166,31 @@ __global__ __launch_bounds__(64) void invit_kernel(const VecItem* __restrict__ i
         const double zi = fast_quot(yv[u] - u1[u] * z1 - u2[u] * z2, u0[u]);
         Y(i) = zi;
         zmax = fmax(zmax, fabs(zi));
+        ss += zi * zi;
         z2 = z1;
         z1 = zi;
       }
     }
     s = zmax > 0 ? 1.0 / zmax : 1.0;
+    sumsq = ss;
   }
-  double nrm = 0;
-  for (int i0 = 0; i0 < n; i0 += C) {
-    double yv[C];
-#pragma unroll
-    for (int u = 0; u < C; ++u) yv[u] = i0 + u < n ? Y(i0 + u) * s : 0.0;
-#pragma unroll
-    for (int u = 0; u < C; ++u) nrm += yv[u] * yv[u];
+  // unit 2-norm; the sum of squares of the unscaled solve is exact enough
+  // unless it left the finite range, which a scaled pass then recomputes
+  double f;
+  if (sumsq <= 1e300) {
+    f = sumsq > 0 ? 1.0 / sqrt(sumsq) : 1.0;
+  } else {
+    double nrm = 0;
+    for (int i = 0; i < n; ++i) nrm += (Y(i) * s) * (Y(i) * s);
+    f = nrm > 0 ? s / sqrt(nrm) : 1.0;
   }
-  const double f = nrm > 0 ? s / sqrt(nrm) : 1.0;
-  for (int i0 = 0; i0 < n; i0 += C) {
-    double yv[C];
+  constexpr int C2 = 16;
+  for (int i0 = 0; i0 < n; i0 += C2) {
+    double yv[C2];
 #pragma unroll
-    for (int u = 0; u < C; ++u) yv[u] = i0 + u < n ? Y(i0 + u) : 0.0;
+    for (int u = 0; u < C2; ++u) yv[u] = i0 + u < n ? Y(i0 + u) : 0.0;
 #pragma unroll
-    for (int u = 0; u < C; ++u)
+    for (int u = 0; u < C2; ++u)
       if (i0 + u < n) Y(i0 + u) = yv[u] * f;
   }
 }
@@ -2534,8 +2556,10 @@ void SymEigBatch::vectors(const std::vector<int64_t>& keep, double* Z, int64_t z
   const int ti = ctx_->timer.begin(ctx_->stream);
   hipLaunchKernelGGL(invit_kernel, dim3((nv + 63) / 64), dim3(64), 0, ctx_->stream, vid, nv, djd, D, E, W_,
                      tn_.get(), lu.get(), zi.get());
-  // factors written once and read twice, the vector written and read 6 times
-  ctx_->timer.end(ti, ctx_->stream, "inverse_iteration", 8.0 * double(z_tot) * (3.0 * kLuFields + 6.0), 0.0, 1);
+  // per (row, vector): the four factor fields written once, read by two
+  // back substitutions and (two of them) by one forward solve; the vector
+  // written once and read and written by four passes
+  ctx_->timer.end(ti, ctx_->stream, "inverse_iteration", 8.0 * double(z_tot) * 23.0, 0.0, 1);
   TK_HIP(hipGetLastError());
   VecWork work;
   for (const DevJob& j : dj) {
