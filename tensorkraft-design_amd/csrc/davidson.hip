@@ -54,7 +54,7 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
   w.AV.reset(ctx, size_t(m) * n);
   w.H.reset(ctx, size_t(m) * m);
   w.c.reset(ctx, m);
-  w.h.reset(ctx, size_t(m) + 1);
+  w.h.reset(ctx, 2 * (size_t(m) + 1));
   w.scal.reset(ctx, 8);
   w.y.reset(ctx, n);
   w.ay.reset(ctx, n);
@@ -99,6 +99,27 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     subtract_projection(ctx, V, n, nsub, w.h.get(), slot, n, w.flag.get());
     dot_many(ctx, slot, n, 1, slot, n, tn2);
   };
+  // the same with V^T slot and slot.slot in h already (summed by the
+  // residual pass): the first pass also sums V^T of its result, which the
+  // second pass (when flagged) subtracts; each pass leaves |slot|^2 in *tn2
+  auto project_rest = [&](double* slot, int nsub) {
+    double* h2 = w.h.get() + (m + 1);
+    project_out_dots(ctx, V, n, nsub, w.h.get(), slot, n, h2, tn2);
+    reorth_flag(ctx, w.h.get(), nsub, w.flag.get());
+    project_out_dots(ctx, V, n, nsub, h2, slot, n, nullptr, tn2, w.flag.get());
+  };
+  // the Ritz residual (preconditioned by pre unless null) into t and its norm
+  // into *rn2; with `grow`, t is the new basis slot and is orthogonalised
+  // against V (for a small basis in two passes over V fewer)
+  auto residual_into = [&](double* t, int nsub, const double* pre, bool grow) {
+    if (grow && nsub <= fused_vec_max()) {
+      ritz_residual_dots(ctx, V, AV, n, nsub, c, theta, pre, t, n, rn2, w.h.get());
+      project_rest(t, nsub);
+    } else {
+      ritz_residual(ctx, V, AV, n, nsub, c, theta, pre, t, n, rn2);
+      if (grow) orthogonalise(t, nsub);
+    }
+  };
   // Ritz vector y = V c of the current projected solution
   auto ritz_vector = [&](int nsub) { combine(ctx, V, n, nsub, c, 0.0, w.y.get(), n); };
   start_from(x);
@@ -126,8 +147,7 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     // V and AV, formed before the one host sync of the iteration (discarded
     // when the iteration converges or switches precision)
     double* slot = V + int64_t(nsub) * n;
-    ritz_residual(ctx, V, AV, n, nsub, c, theta, dprec, can_grow ? slot : w.r.get(), n, rn2);
-    if (can_grow) orthogonalise(slot, nsub);
+    residual_into(can_grow ? slot : w.r.get(), nsub, dprec, can_grow);
     TK_HIP(hipMemcpyAsync(host, w.scal.get(), 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     TK_HIP(hipStreamSynchronize(s));
     TK_HIP(hipGetLastError());
@@ -180,8 +200,7 @@ DavidsonResult davidson_generic(Context* ctx, int64_t n, MatvecFn mv, void* user
     if (!can_grow) break;  // the basis spans the space: exact
     if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {
       // the preconditioned correction collapsed into the basis: raw residual
-      ritz_residual(ctx, V, AV, n, nsub, c, theta, nullptr, slot, n, rn2);
-      orthogonalise(slot, nsub);
+      residual_into(slot, nsub, nullptr, true);
       TK_HIP(hipMemcpyAsync(host + 2, tn2, sizeof(double), hipMemcpyDeviceToHost, s));
       TK_HIP(hipStreamSynchronize(s));
       if (std::sqrt(std::max(0.0, host[2])) < 1e-14) {

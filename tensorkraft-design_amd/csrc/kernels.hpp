@@ -136,6 +136,16 @@ void ritz_residual(Context* ctx, const double* V, const double* AV, int64_t ldv,
                    const double* theta, const double* diag, double* t, int64_t n, double* rn2);
 // *flag = 1 when a second Gram-Schmidt pass is needed: h[0..nvec) = V^T s and
 // h[nvec] = s.s from before the first pass, s.s - |h|^2 < s.s / 2
+// The largest basis of the two fused passes below.
+int fused_vec_max();
+// ritz_residual for nvec <= fused_vec_max() that also gives h[0..nvec) =
+// V^T t and h[nvec] = t.t of the t it writes.
+void ritz_residual_dots(Context* ctx, const double* V, const double* AV, int64_t ldv, int nvec, const double* c,
+                        const double* theta, const double* diag, double* t, int64_t n, double* rn2, double* h);
+// y <- y - V h for nvec <= fused_vec_max(), with h_new = V^T y (unless null)
+// and *nrm2 = y.y of the result; nothing when *active reads 0.
+void project_out_dots(Context* ctx, const double* V, int64_t ldv, int nvec, const double* h, double* y, int64_t n,
+                      double* h_new, double* nrm2, const int* active = nullptr);
 void reorth_flag(Context* ctx, const double* h, int nvec, int* flag);
 // x *= 1/sqrt(*nrm2) read on the device (no-op guard when the norm is zero)
 void scale_by_inv_sqrt(Context* ctx, double* x, const double* nrm2, int64_t n);

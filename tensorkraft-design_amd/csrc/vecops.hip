@@ -140,6 +140,137 @@ __global__ __launch_bounds__(kThreads) void ritz_residual_kernel(const double* _
   }
 }
 
+// The pass of ritz_residual_kernel for NV <= kFusedVec basis vectors,
+// summing on the way, with the basis values held in registers, the first
+// Gram-Schmidt pass's products of the t it writes: partial rows i < NV:
+// V_i . t, row NV: t . t, row NV + 1: r . r (grid columns each).  t and
+// r . r are those of ritz_residual_kernel (same order of every sum).
+constexpr int kFusedVec = 16;
+
+template <int NV>
+__global__ __launch_bounds__(kThreads) void ritz_residual_dots_kernel(
+    const double* __restrict__ V, const double* __restrict__ AV, int64_t ldv, const double* __restrict__ c,
+    const double* __restrict__ theta, const double* __restrict__ d, double* __restrict__ t, int64_t n,
+    double* __restrict__ partial, int grid) {
+  constexpr int NQ = NV + 2;
+  __shared__ double cs[NV];
+  __shared__ double red[kThreads / 64][NQ];
+  if (threadIdx.x < NV) cs[threadIdx.x] = c[threadIdx.x];
+  __syncthreads();
+  const double th = *theta;
+  double pd[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) pd[q] = 0.0;
+  for (int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x; j < n; j += int64_t(gridDim.x) * kThreads) {
+    double vv[NV], av[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      vv[i] = V[int64_t(i) * ldv + j];
+      av[i] = AV[int64_t(i) * ldv + j];
+    }
+    double y = 0.0, ay = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      y += cs[i] * vv[i];
+      ay += cs[i] * av[i];
+    }
+    const double r = ay - th * y;
+    pd[NV + 1] += r * r;
+    double tj = r;
+    if (d != nullptr) {
+      const double den = th - d[j];
+      tj = fabs(den) > 1e-14 ? r / den : r;
+    }
+    t[j] = tj;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) pd[i] += vv[i] * tj;
+    pd[NV] += tj * tj;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double x = wave_sum(pd[q]);
+    if (lane == 0) red[wave][q] = x;
+  }
+  __syncthreads();
+  const int q = threadIdx.x;
+  if (q < NQ) {
+    double x = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) x += red[w][q];
+    partial[int64_t(q) * grid + blockIdx.x] = x;
+  }
+}
+
+// y <- y - sum_i h_i V_i for NV <= kFusedVec vectors (the arithmetic of
+// combine_kernel with beta = 1, sign = -1), summing on the way the partial
+// rows i < NV: V_i . y_new (when `dots`) and row NV: y_new . y_new.
+// `active` as in dot_partial_kernel.
+template <int NV>
+__global__ __launch_bounds__(kThreads) void project_out_dots_kernel(const double* __restrict__ V, int64_t ldv,
+                                                                    const double* __restrict__ h,
+                                                                    double* __restrict__ y, int64_t n, int dots,
+                                                                    double* __restrict__ partial, int grid,
+                                                                    const int* __restrict__ active) {
+  if (active != nullptr && *active == 0) return;
+  constexpr int NQ = NV + 1;
+  __shared__ double hs[NV];
+  __shared__ double red[kThreads / 64][NQ];
+  if (threadIdx.x < NV) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double pd[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) pd[q] = 0.0;
+  for (int64_t j = int64_t(blockIdx.x) * kThreads + threadIdx.x; j < n; j += int64_t(gridDim.x) * kThreads) {
+    double vv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) vv[i] = V[int64_t(i) * ldv + j];
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s += hs[i] * vv[i];
+    const double yn = y[j] - s;
+    y[j] = yn;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) pd[i] += vv[i] * yn;
+    pd[NV] += yn * yn;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double x = wave_sum(pd[q]);
+    if (lane == 0) red[wave][q] = x;
+  }
+  __syncthreads();
+  const int q = threadIdx.x;
+  if ((dots && q < NV) || q == NV) {
+    double x = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) x += red[w][q];
+    partial[int64_t(q) * grid + blockIdx.x] = x;
+  }
+}
+
+// the launch of a fused pass for nvec = NV .. kFusedVec vectors
+template <int NV>
+void launch_ritz_dots(int nvec, dim3 grid, hipStream_t st, const double* V, const double* AV, int64_t ldv,
+                      const double* c, const double* theta, const double* d, double* t, int64_t n, double* part) {
+  if (nvec == NV)
+    hipLaunchKernelGGL(ritz_residual_dots_kernel<NV>, grid, dim3(kThreads), 0, st, V, AV, ldv, c, theta, d, t, n,
+                       part, int(grid.x));
+  else if constexpr (NV < kFusedVec)
+    launch_ritz_dots<NV + 1>(nvec, grid, st, V, AV, ldv, c, theta, d, t, n, part);
+}
+
+template <int NV>
+void launch_project_dots(int nvec, dim3 grid, hipStream_t st, const double* V, int64_t ldv, const double* h,
+                         double* y, int64_t n, int dots, double* part, const int* active) {
+  if (nvec == NV)
+    hipLaunchKernelGGL(project_out_dots_kernel<NV>, grid, dim3(kThreads), 0, st, V, ldv, h, y, n, dots, part,
+                       int(grid.x), active);
+  else if constexpr (NV < kFusedVec)
+    launch_project_dots<NV + 1>(nvec, grid, st, V, ldv, h, y, n, dots, part, active);
+}
+
 // Second Gram-Schmidt pass needed?  h[0..nvec) = V^T s and h[nvec] = s.s
 // before the first pass; |s - V h|^2 = s.s - |h|^2 for orthonormal V.  The
 // pass is repeated when the first one removed more than half of |s|^2.
@@ -239,6 +370,42 @@ void ritz_residual(Context* ctx, const double* V, const double* AV, int64_t ldv,
                      diag, t, n, part);
   hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(kThreads), 0, ctx->stream, part, grid, rn2,
                      static_cast<const int*>(nullptr));
+  TK_HIP(hipGetLastError());
+}
+
+int fused_vec_max() { return kFusedVec; }
+
+// Grid of the fused passes: one resident round (two workgroups per CU at
+// their register use), so that the per-workgroup reduction of their 17 or 18
+// sums is paid once per CU pair rather than once per 256 elements.
+int fused_grid(Context* ctx, int64_t n) {
+  const int64_t g = (n + kThreads - 1) / kThreads;
+  return int(std::max<int64_t>(1, std::min<int64_t>(g, int64_t(ctx->n_cu) * 2)));
+}
+
+void ritz_residual_dots(Context* ctx, const double* V, const double* AV, int64_t ldv, int nvec, const double* c,
+                        const double* theta, const double* diag, double* t, int64_t n, double* rn2, double* h) {
+  if (nvec <= 0 || nvec > kFusedVec) throw ArgError("ritz_residual_dots: vector count outside 1..16");
+  const int grid = fused_grid(ctx, n);
+  double* part = scratch(ctx, size_t(grid) * (nvec + 2));
+  launch_ritz_dots<1>(nvec, dim3(grid), ctx->stream, V, AV, ldv, c, theta, diag, t, n, part);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(nvec + 1), dim3(kThreads), 0, ctx->stream, part, grid, h,
+                     static_cast<const int*>(nullptr));
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(kThreads), 0, ctx->stream, part + int64_t(nvec + 1) * grid,
+                     grid, rn2, static_cast<const int*>(nullptr));
+  TK_HIP(hipGetLastError());
+}
+
+void project_out_dots(Context* ctx, const double* V, int64_t ldv, int nvec, const double* h, double* y, int64_t n,
+                      double* h_new, double* nrm2, const int* active) {
+  if (nvec <= 0 || nvec > kFusedVec) throw ArgError("project_out_dots: vector count outside 1..16");
+  const int grid = fused_grid(ctx, n);
+  double* part = scratch(ctx, size_t(grid) * (nvec + 1));
+  launch_project_dots<1>(nvec, dim3(grid), ctx->stream, V, ldv, h, y, n, h_new != nullptr ? 1 : 0, part, active);
+  if (h_new != nullptr)
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(nvec), dim3(kThreads), 0, ctx->stream, part, grid, h_new, active);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(kThreads), 0, ctx->stream, part + int64_t(nvec) * grid, grid,
+                     nrm2, active);
   TK_HIP(hipGetLastError());
 }
 
