@@ -15,54 +15,65 @@ constexpr int kThreads = 256;
 constexpr int kMaxVec = 128;
 
 
-// partial[i * nchunks + c] = sum over chunk c of V_i . x for the kVecGroup
-// vectors of group blockIdx.y; chunks of kDotChunk elements, one barrier.
+// partial[i * nchunks + c] = sum over chunk c of V_i . x for the NV vectors
+// i = i0 + NV blockIdx.y + q of a launch; chunks of kDotChunk elements, one
+// barrier.  The loads carry no branch (an element index beyond n reads
+// element n - 1 and meets x = 0), so each thread's loads are in flight
+// together.
 constexpr int kDotPer = 4;
 constexpr int kDotChunk = kThreads * kDotPer;
 constexpr int kVecGroup = 8;
 
 // `active` (device flag, may be null): when it reads 0 the launch does nothing.
-__global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __restrict__ V, int64_t ldv, int nvec,
+template <int NV>
+__global__ __launch_bounds__(kThreads) void dot_partial_kernel(const double* __restrict__ V, int64_t ldv, int i0,
                                                                const double* __restrict__ x, int64_t n,
                                                                double* __restrict__ partial, int nchunks,
                                                                const int* __restrict__ active) {
   if (active != nullptr && *active == 0) return;
-  __shared__ double red[kThreads / 64][kVecGroup];
+  __shared__ double red[kThreads / 64][NV];
   const int64_t base = int64_t(blockIdx.x) * kDotChunk + threadIdx.x;
-  const int i0 = blockIdx.y * kVecGroup;
-  const int nv = min(kVecGroup, nvec - i0);
+  const int ib = i0 + int(blockIdx.y) * NV;
   double xr[kDotPer];
+  int64_t at[kDotPer];
 #pragma unroll
   for (int j = 0; j < kDotPer; ++j) {
     const int64_t idx = base + int64_t(j) * kThreads;
+    at[j] = idx < n ? idx : n - 1;
     xr[j] = idx < n ? x[idx] : 0.0;
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double s[kVecGroup];
+  double s[NV];
 #pragma unroll
-  for (int q = 0; q < kVecGroup; ++q) {
+  for (int q = 0; q < NV; ++q) {
+    const double* __restrict__ v = V + int64_t(ib + q) * ldv;
     s[q] = 0.0;
-    if (q < nv) {
-      const double* __restrict__ v = V + int64_t(i0 + q) * ldv;
 #pragma unroll
-      for (int j = 0; j < kDotPer; ++j) {
-        const int64_t idx = base + int64_t(j) * kThreads;
-        if (idx < n) s[q] += v[idx] * xr[j];
-      }
-    }
+    for (int j = 0; j < kDotPer; ++j) s[q] += v[at[j]] * xr[j];
   }
 #pragma unroll
-  for (int q = 0; q < kVecGroup; ++q) {
+  for (int q = 0; q < NV; ++q) {
     const double t = wave_sum(s[q]);
     if (lane == 0) red[wave][q] = t;
   }
   __syncthreads();
-  if (threadIdx.x < nv) {
+  if (threadIdx.x < NV) {
     double t = 0.0;
 #pragma unroll
     for (int w = 0; w < kThreads / 64; ++w) t += red[w][threadIdx.x];
-    partial[int64_t(i0 + threadIdx.x) * nchunks + blockIdx.x] = t;
+    partial[int64_t(ib + threadIdx.x) * nchunks + blockIdx.x] = t;
   }
+}
+
+// one launch of the last nv < kVecGroup vectors, from vector i0
+template <int NV>
+void launch_dot_rest(int nv, int nchunks, hipStream_t st, const double* V, int64_t ldv, int i0, const double* x,
+                     int64_t n, double* part, const int* active) {
+  if (nv == NV)
+    hipLaunchKernelGGL(dot_partial_kernel<NV>, dim3(nchunks, 1), dim3(kThreads), 0, st, V, ldv, i0, x, n, part,
+                       nchunks, active);
+  else if constexpr (NV + 1 < kVecGroup)
+    launch_dot_rest<NV + 1>(nv, nchunks, st, V, ldv, i0, x, n, part, active);
 }
 
 __global__ __launch_bounds__(kThreads) void reduce_rows_kernel(const double* __restrict__ partial, int nchunks,
@@ -338,9 +349,11 @@ void dot_many(Context* ctx, const double* V, int64_t ldv, int nvec, const double
   if (nvec > kMaxVec) throw ArgError("dot_many: too many vectors");
   const int nchunks = int(std::max<int64_t>(1, (n + kDotChunk - 1) / kDotChunk));
   double* part = scratch(ctx, size_t(nchunks) * nvec);
-  const dim3 grid(nchunks, (nvec + kVecGroup - 1) / kVecGroup);
-  hipLaunchKernelGGL(dot_partial_kernel, grid, dim3(kThreads), 0, ctx->stream, V, ldv, nvec, x, n, part, nchunks,
-                     active);
+  const int full = nvec / kVecGroup, rest = nvec % kVecGroup;
+  if (full > 0)
+    hipLaunchKernelGGL(dot_partial_kernel<kVecGroup>, dim3(nchunks, full), dim3(kThreads), 0, ctx->stream, V, ldv, 0,
+                       x, n, part, nchunks, active);
+  if (rest > 0) launch_dot_rest<1>(rest, nchunks, ctx->stream, V, ldv, full * kVecGroup, x, n, part, active);
   hipLaunchKernelGGL(reduce_rows_kernel, dim3(nvec), dim3(kThreads), 0, ctx->stream, part, nchunks, out, active);
   TK_HIP(hipGetLastError());
 }
