@@ -1850,7 +1850,7 @@ struct WyBlock {
 // staged in LDS; the next column's global loads are issued before this
 // column's sums.
 constexpr int kWyTLanes = 4;
-constexpr size_t kWyTLds = (size_t(kWyNb) * (kWyNb + 1) + kWyNb) * sizeof(double);
+constexpr size_t kWyTLds = (size_t(kWyNb) * (kWyNb + 1) + 2 * kWyNb) * sizeof(double);
 
 __global__ __launch_bounds__(kWyNb * kWyTLanes) void wy_t_kernel(const WyBlock* __restrict__ blocks,
                                                                  const double* __restrict__ S,
@@ -1858,22 +1858,37 @@ __global__ __launch_bounds__(kWyNb * kWyTLanes) void wy_t_kernel(const WyBlock* 
                                                                  double* __restrict__ T) {
   extern __shared__ double sm[];
   double* tl = sm;                        // kWyNb x (kWyNb + 1)
-  double* sc = sm + kWyNb * (kWyNb + 1);  // column i of S
+  double* scb = sm + kWyNb * (kWyNb + 1);  // 2 x kWyNb: column i of S, double-buffered
   const WyBlock b = blocks[blockIdx.x];
   const double* s = S + b.t_off;
   const double* tau = TAUg + b.tau_off;
   const int l = threadIdx.x / kWyTLanes, q = threadIdx.x % kWyTLanes, nb = b.nb;
   for (int m = q; m < kWyNb; m += kWyTLanes) tl[l * (kWyNb + 1) + m] = 0.0;
   double snext = 0.0;
+  // Row l of T is written and read only by its own kWyTLanes lanes (one
+  // wavefront), so the one barrier per step publishes the shared column of
+  // S; the column alternates between two buffers, so that the next step's
+  // store cannot meet a read of this one.
   for (int i = 0; i < nb; ++i) {
+    double* sc = scb + (i & 1) * kWyNb;
     if (q == 0) sc[l] = snext;
     snext = (q == 0 && l < i + 1 && i + 1 < nb) ? s[int64_t(l) * kWyNb + i + 1] : 0.0;
     __syncthreads();
-    // every lane takes part in the butterfly (uniform control flow)
+    // every lane takes part in the butterfly (uniform control flow); four
+    // partial sums keep a lane's loads and products independent
     double a = 0;
     if (l < i) {
       const double* tr = tl + l * (kWyNb + 1);
-      for (int m = l + q; m < i; m += kWyTLanes) a += tr[m] * sc[m];
+      double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+      int m = l + q;
+      for (; m + 3 * kWyTLanes < i; m += 4 * kWyTLanes) {
+        a0 += tr[m] * sc[m];
+        a1 += tr[m + kWyTLanes] * sc[m + kWyTLanes];
+        a2 += tr[m + 2 * kWyTLanes] * sc[m + 2 * kWyTLanes];
+        a3 += tr[m + 3 * kWyTLanes] * sc[m + 3 * kWyTLanes];
+      }
+      for (; m < i; m += kWyTLanes) a0 += tr[m] * sc[m];
+      a = (a0 + a1) + (a2 + a3);
     }
     a += __shfl_xor(a, 1, 64);
     a += __shfl_xor(a, 2, 64);
@@ -1881,8 +1896,10 @@ __global__ __launch_bounds__(kWyNb * kWyTLanes) void wy_t_kernel(const WyBlock* 
       if (l < i) tl[l * (kWyNb + 1) + i] = -tau[i] * a;
       else if (l == i) tl[l * (kWyNb + 1) + i] = tau[i];
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
+  __syncthreads();
   double* t = T + b.t_off;
   for (int m = q; m < kWyNb; m += kWyTLanes) t[int64_t(l) * kWyNb + m] = tl[l * (kWyNb + 1) + m];
 }
