@@ -293,6 +293,12 @@ tk_status tk_qr(tk_ctx* ctx, int64_t m, int64_t n, const double* a_dev, double* 
  * on the device and s (k, host); delta > 0. */
 tk_status tk_regularized_svd_inverse(tk_ctx* ctx, int64_t m, int64_t n, int64_t k, const double* s_host,
                                      const double* u_dev, const double* vt_dev, double delta, double* out_dev);
+/* No counterpart in the reference: q[i] = a[i] / b[i] for device arrays of n
+ * doubles, formed as the eigensolver's Sturm counts and inverse iteration
+ * form their quotients (hardware reciprocal of b and one Newton step, then
+ * the product with a), for callers that check that quotient's error against
+ * IEEE division.  Synchronous. */
+tk_status tk_fast_quotient(tk_ctx* ctx, const double* a_dev, const double* b_dev, double* q_dev, int64_t n);
 /* BondDimensionSchedule::warmup (tk-dmrg/src/truncation/mod.rs:137-151):
  * round(d_init * r^i) (half away from zero), r = (d_max / d_init)^(1/n),
  * clamped to [1, d_max], i = 0..n (n + 1 values; n = 0 gives [d_max]).

@@ -11,6 +11,7 @@
 #include "dmft.hpp"
 #include "dmrg.hpp"
 #include "heff.hpp"
+#include "tdeig.hpp"
 #include "tdvp.hpp"
 #include "sym.hpp"
 #include "truncate.hpp"
@@ -608,6 +609,14 @@ tk_status tk_qr(tk_ctx* ctx, int64_t m, int64_t n, const double* a, double* q, d
   return guard(ctx, [&] {
     if (!ctx || !a || !q || !r) throw tk::ArgError("null argument");
     tk::qr(ctx->c.get(), m, n, a, q, r);
+  });
+}
+
+tk_status tk_fast_quotient(tk_ctx* ctx, const double* a, const double* b, double* q, int64_t n) {
+  return guard(ctx, [&] {
+    if (!ctx) throw tk::ArgError("null context");
+    if (n > 0 && (!a || !b || !q)) throw tk::ArgError("null operand");
+    tk::fast_quotient(ctx->c.get(), a, b, q, n);
   });
 }
 

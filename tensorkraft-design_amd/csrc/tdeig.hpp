@@ -42,6 +42,11 @@ struct SymJob {
   int32_t n;
 };
 
+// q[i] = a[i] / b[i] (device arrays of n) through the hardware reciprocal
+// and one Newton step, the quotient of the Sturm counts and of inverse
+// iteration; synchronous.
+void fast_quotient(Context* ctx, const double* a, const double* b, double* q, int64_t n);
+
 class SymEigBatch {
  public:
   explicit SymEigBatch(Context* ctx) : ctx_(ctx) {}

@@ -135,6 +135,7 @@ def lib():
         "tk_heff_time": (I32, [P, P, P, I32, pD, pD, pD]),
         "tk_gemm": (I32, [P, I64, I64, I64, D, P, I64, I64, P, I64, I64, D, P, I64, I64]),
         "tk_gemm_noalloc": (I32, [P, I64, I64, I64, D, P, I64, I64, P, I64, I64, D, P, I64, I64]),
+        "tk_fast_quotient": (I32, [P, P, P, P, I64]),
         "tk_block_gemm_table": (I32, [I32, C.POINTER(tk_block_matrix), C.POINTER(tk_block_matrix), I64, pI64, pU64,
                                       pI64, pI64]),
         "tk_block_gemm": (I32, [P, I32, C.POINTER(tk_block_matrix), C.POINTER(tk_block_matrix), P]),
@@ -208,7 +209,7 @@ EXPORTED = ["tk_version", "tk_ctx_create", "tk_ctx_destroy", "tk_ctx_sync", "tk_
             "tk_heff_plan_create", "tk_heff_plan_destroy", "tk_heff_psi_table", "tk_heff_apply",
             "tk_heff_apply_fp64", "tk_heff_diag", "tk_heff_stats", "tk_heff_stage_stats", "tk_heff_time",
             "tk_dmrg_create", "tk_dmrg_destroy", "tk_dmrg_set_state", "tk_dmrg_prepare", "tk_dmrg_sweep",
-            "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_gemm_noalloc", "tk_block_gemm_table",
+            "tk_dmrg_steps", "tk_dmrg_bond", "tk_dmrg_site", "tk_gemm", "tk_gemm_noalloc", "tk_fast_quotient", "tk_block_gemm_table",
             "tk_block_gemm",
             "tk_spmv", "tk_svd_truncated", "tk_eigh", "tk_truncate_svd", "tk_truncation_select",
             "tk_block_svd_truncate", "tk_eigh_lowest", "tk_qr", "tk_regularized_svd_inverse",

@@ -363,3 +363,47 @@ def test_cancel_flag(ctx):
         s = d.sweep()
     assert s["energy"] <= e_full + 1e-9
     d.close()
+
+
+def _ulps(q, ref):
+    """|q - ref| in units of the last place of ref (ref finite)."""
+    return np.abs(q - ref) / np.spacing(np.abs(ref))
+
+
+def test_fast_quotient_ulps(ctx):
+    """The quotient of the eigensolver's Sturm counts and inverse iteration
+    (hardware reciprocal of the divisor, one Newton step, product with the
+    numerator) against IEEE division (numpy, correctly rounded): operands
+    over the normal range, divisors down to the Sturm recurrence's floor
+    (pivmin >= DBL_MIN), and numerators whose quotient is subnormal."""
+    print("case fast quotient ulps", flush=True)
+    rng = np.random.default_rng(7)
+    tiny = np.finfo(np.float64).tiny
+    sgn = lambda k: rng.choice([-1.0, 1.0], k)
+    n = 200_000
+    a = rng.uniform(1, 2, n) * np.exp2(rng.integers(-500, 500, n)) * sgn(n)
+    b = rng.uniform(1, 2, n) * np.exp2(rng.integers(-500, 500, n)) * sgn(n)
+    m = 50_000
+    # divisors within a factor 1e3 of DBL_MIN, numerators keeping the quotient normal
+    b2 = rng.uniform(1, 1e3, m) * tiny * sgn(m)
+    a2 = rng.uniform(1, 2, m) * np.exp2(rng.integers(-1000, -30, m)) * sgn(m)
+    # quotients below DBL_MIN (subnormal results)
+    b3 = rng.uniform(1, 2, m) * sgn(m)
+    a3 = rng.uniform(0, 1, m) * tiny * sgn(m)
+    cases = {"normal": (a, b), "divisor near DBL_MIN": (a2, b2), "subnormal quotient": (a3, b3)}
+    worst = {}
+    for name, (x, y) in cases.items():
+        dx, dy, dq = ctx.to_device(x), ctx.to_device(y), ctx.empty(len(x))
+        tk_hip._check(ctx.L.tk_fast_quotient(ctx.h, tk_hip.C.c_void_p(dx.ptr), tk_hip.C.c_void_p(dy.ptr),
+                                             tk_hip.C.c_void_p(dq.ptr), len(x)), ctx.h)
+        q = dq.get()
+        for d in (dx, dy, dq):
+            d.free()
+        ref = x / y
+        assert np.all(np.isfinite(q)), name
+        worst[name] = float(_ulps(q, ref).max())
+    print("fast quotient, max ulps from IEEE division:", worst, flush=True)
+    # one Newton step on the hardware reciprocal: 18 / 15 / 5 ulps measured;
+    # the bounds leave a margin and fail by orders of magnitude without the step
+    assert worst["normal"] <= 32.0 and worst["divisor near DBL_MIN"] <= 32.0
+    assert worst["subnormal quotient"] <= 8.0
