@@ -2138,13 +2138,23 @@ __global__ __launch_bounds__(256) void cholinv_kernel(const DevJob* __restrict__
     }
     if (tid == 0) piv[a] = sqrt(dd);
     const double idd = 1.0 / dd;
-    for (int b = a + 1 + ty; b < nb; b += 16) {
-      const double ub = L[a * LD + b] * idd;
-      for (int c = tx; c < nb; c += 16) {
-        if (c >= b)
-          L[b * LD + c] -= ub * L[a * LD + c];
-        else if (c <= a)
-          W[b * LD + c] -= ub * W[a * LD + c];
+    // rows b = a + 1 + ty + 16 r, columns c = tx + 16 s: entry (b, c) is the
+    // trailing block's when c >= b and Y's otherwise (W_a is zero beyond
+    // column a, so Y's entries a < c < b stay as they are); chosen by address,
+    // so that a thread's loads are issued together rather than one per branch
+    double ub[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ub[r] = L[a * LD + min(a + 1 + ty + 16 * r, NB - 1)] * idd;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int b = a + 1 + ty + 16 * r;
+      const int bl = min(b, NB - 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = tx + 16 * q;
+        double* M = c >= b ? L : W;
+        const double v = M[bl * LD + c] - ub[r] * M[a * LD + c];
+        if (b < nb && c < nb) M[bl * LD + c] = v;
       }
     }
     __syncthreads();
